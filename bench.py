@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Benchmark of the Wilson fermion-matrix hot path on one MI355X.
+
+Workload: 32x32x32x16 lattice, kappa = 0.1355, synthetic gauge U = reunitarise(exp(i 0.5 H))
+(seed 1), Gaussian source (seed 2), fermion boundary (1, 1, 1, -1).
+
+A step is one application of the full-lattice Wilson D (the reference operator
+_kernelDFermionWilsonSquareSU3) in FP64 to one right-hand side already resident in HBM.
+`value` is the algorithmic Dslash bandwidth: 384 reals x 8 bytes per site (8 neighbour
+spinors, 8 links, the site spinor in and out; no cache reuse counted) times the volume,
+divided by the measured time per step.  The same line carries FP32 / 12-real-link Dslash
+rates, the even-odd Schur operator, GMRES(12) solve times to |r|/|b| < 1e-10 in plain FP64
+and in FP32/FP16-inner defect correction, and the CPU oracle's Dslash on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+DIMS = (32, 32, 32, 16)
+KAPPA = 0.1355
+HBM_PEAK_GBS = 8000.0
+BYTES_D = {8: 384 * 8, 4: 384 * 4}          # full Wilson D, 18-real links
+BYTES_D12 = {8: 336 * 8, 4: 336 * 4}        # 12-real links
+BYTES_SCHUR = {8: 744 * 8, 4: 744 * 4}      # per even site
+FLOPS_D = 1368
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "path is not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import cudalatticegauge_amd as clg
+
+    V = int(np.prod(DIMS))
+    ctx = clg.Context(0)
+    log(f"device {ctx.info()}  lattice {DIMS}  V={V}")
+
+    def make_inputs(prec, gkind):
+        g = clg.Field(ctx, gkind, prec, DIMS)
+        if prec == clg.F64 and gkind == clg.GAUGE:
+            g.gauge_random(0.5, 1)
+        else:
+            g.copy_from(gauge64)
+        src = clg.Field(ctx, clg.WILSON, prec, DIMS, bc=(1, 1, 1, -1))
+        if prec == clg.F64:
+            src.gaussian(2)
+        else:
+            src.copy_from(src64)
+        out = clg.Field(ctx, clg.WILSON, prec, DIMS, bc=(1, 1, 1, -1))
+        return g, src, out
+
+    gauge64 = None
+    src64 = None
+    gauge64, src64, out64 = make_inputs(clg.F64, clg.GAUGE)
+    ctx.sync()
+
+    # ---------------- the timed steps: FP64 full-lattice D
+    for _ in range(args.warmup):
+        clg.wilson_apply(ctx, out64, src64, gauge64, KAPPA, clg.EFO_F_D)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        clg.wilson_apply(ctx, out64, src64, gauge64, KAPPA, clg.EFO_F_D)
+    ctx.sync()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    gbs = BYTES_D[8] * V / (ms_per_step * 1e-3) / 1e9
+    log(f"FP64 D: {ms_per_step * 1e3:.1f} us/step  {gbs:.0f} GB/s (algorithmic)")
+
+    # kernel-only time with HIP events on the library stream (same launches)
+    k64 = clg.time_wilson(ctx, out64, src64, gauge64, KAPPA, 0, args.steps)
+    extra = {"fp64_D_kernel_us": round(k64 * 1e3, 2),
+             "fp64_D_GBs": round(BYTES_D[8] * V / (k64 * 1e-3) / 1e9, 1),
+             "fp64_D_Gflops": round(FLOPS_D * V / (k64 * 1e-3) / 1e9, 1)}
+
+    # ---------------- other precisions / layouts
+    for name, prec, gk, nbytes in (("fp32_D", clg.F32, clg.GAUGE, BYTES_D[4]),
+                                   ("fp32_D_recon12", clg.F32, clg.GAUGE12, BYTES_D12[4]),
+                                   ("fp64_D_recon12", clg.F64, clg.GAUGE12, BYTES_D12[8])):
+        g, s, o = make_inputs(prec, gk)
+        ms = clg.time_wilson(ctx, o, s, g, KAPPA, 0, args.steps)
+        extra[name + "_kernel_us"] = round(ms * 1e3, 2)
+        extra[name + "_GBs"] = round(nbytes * V / (ms * 1e-3) / 1e9, 1)
+        log(f"{name}: {ms * 1e3:.1f} us  {extra[name + '_GBs']:.0f} GB/s")
+        del g, s, o
+    for name, prec, nb in (("fp64_schur", clg.F64, BYTES_SCHUR[8]), ("fp32_schur", clg.F32, BYTES_SCHUR[4])):
+        g, s, o = make_inputs(prec, clg.GAUGE)
+        e_in, e_out = s.parity(clg.EVEN), o.parity(clg.EVEN)
+        ms = clg.time_wilson(ctx, e_out, e_in, g, KAPPA, 2, args.steps)
+        extra[name + "_us"] = round(ms * 1e3, 2)
+        extra[name + "_GBs"] = round(nb * (V // 2) / (ms * 1e-3) / 1e9, 1)
+        log(f"{name}: {ms * 1e3:.1f} us  {extra[name + '_GBs']:.0f} GB/s")
+        del e_in, e_out, g, s, o
+
+    # ---------------- solves to |r|/|b| < 1e-10 (EO-preconditioned GMRES(12))
+    if not args.no_solve:
+        x = clg.Field(ctx, clg.WILSON, clg.F64, DIMS, bc=(1, 1, 1, -1))
+        results = {}
+        for name, inner, inner_acc in (("fp64", clg.F64, 0.0), ("mixed_fp32", clg.F32, 1e-4),
+                                       ("mixed_fp16link", clg.F16, 1e-3)):
+            p = clg.solver_params(max_dim=12, restart=500, accuracy=1e-10, stop_rule=clg.STOP_TRUE_RESIDUAL,
+                                  even_odd=1, inner_prec=inner, inner_accuracy=inner_acc, max_outer=60)
+            clg.solve(ctx, p, x, src64, gauge64, KAPPA, clg.EFO_F_InverseD)  # warm (tuning, scratch)
+            best = None
+            for _ in range(2):
+                st, rep = clg.solve(ctx, p, x, src64, gauge64, KAPPA, clg.EFO_F_InverseD)
+                if best is None or rep.seconds < best[1].seconds:
+                    best = (st, rep)
+            st, rep = best
+            results[name] = rep
+            extra[f"solve_{name}_s"] = round(rep.seconds, 4)
+            extra[f"solve_{name}_cycles"] = rep.cycles
+            extra[f"solve_{name}_outer"] = rep.outer
+            extra[f"solve_{name}_true_res"] = float(f"{rep.true_residual:.3e}")
+            log(f"solve {name}: {rep.seconds * 1e3:.1f} ms, cycles {rep.cycles}, outer {rep.outer}, "
+                f"res {rep.true_residual:.2e}, status {st}")
+        extra["solve_speedup_mixed_fp32_vs_fp64"] = round(results["fp64"].seconds / results["mixed_fp32"].seconds, 3)
+        extra["solve_speedup_mixed_fp16link_vs_fp64"] = round(
+            results["fp64"].seconds / results["mixed_fp16link"].seconds, 3)
+
+    # ---------------- CPU baseline: the oracle's Wilson D on the host cores (bounded sample)
+    cpu = None
+    if not args.no_cpu:
+        import orc
+        nth = orc.num_threads()
+        t_one = orc.time_wilson_d(DIMS, 0, 1, nth)
+        reps = max(1, min(50, int(10.0 / max(t_one, 1e-3))))
+        t = orc.time_wilson_d(DIMS, 0, reps, nth)
+        cpu_gbs = BYTES_D[8] * V * reps / t / 1e9
+        cpu = {"value": round(cpu_gbs, 2), "unit": "GB/s", "cores": nth, "kind": "port",
+               "sample": f"{reps} FP64 Wilson D applications at 32x32x32x16 on {nth} OpenMP threads "
+                         f"({t:.1f} s), oracle restatement of _kernelDFermionWilsonSquareSU3"}
+        log(f"CPU oracle D: {cpu_gbs:.1f} GB/s on {nth} threads")
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "wilson_dslash_fp64_32x32x32x16_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    achieved = BYTES_D[8] * V / (k64 * 1e-3) / 1e9
+    line = {
+        "metric": "Wilson Dslash GB/s (algorithmic bytes, % of 8 TB/s HBM peak) at 32^3x16; "
+                  "GMRES(12) solve time to 1e-10 mixed vs FP64",
+        "value": round(gbs, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Philox-seeded gauge exp(i*0.5*H) reunitarised, Gaussian source)",
+        "config": {"workload": "wilson_dslash_32x32x32x16_kappa0.1355", "lattice": list(DIMS), "kappa": KAPPA,
+                   "gauge_eps": 0.5, "parallelism": "single GPU"},
+        "pct_hbm_peak": round(100.0 * gbs / HBM_PEAK_GBS, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

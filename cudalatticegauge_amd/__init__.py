@@ -1,0 +1,278 @@
+"""MI355X-native fermion-matrix hot path of CudaLatticeGauge.
+
+The compute lives in ``libclg_hip.so`` (hand-written gfx950 HIP kernels behind the C ABI
+in ``include/clg_hip.h``).  This module is a thin ctypes binding over that ABI used by the
+tests and the benchmark; it never computes anything itself and raises if the shared
+library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libclg_hip.so")
+
+# enums mirrored from include/clg_hip.h
+WILSON, KS, GAUGE, GAUGE12 = 0, 1, 2, 3
+F64, F32, F16 = 0, 1, 2
+FULL, EVEN, ODD = 0, 1, 2
+EFO_F_D, EFO_F_Ddagger, EFO_F_DD, EFO_F_DDdagger = 0, 1, 2, 3
+EFO_F_InverseD, EFO_F_InverseDdagger, EFO_F_InverseDD, EFO_F_InverseDDdagger = 4, 5, 6, 7
+EOCT_None, EOCT_Minus, EOCT_Real, EOCT_Complex = 0, 1, 2, 3
+SOLVER_GMRES, SOLVER_BICGSTAB = 0, 1
+STOP_REFERENCE, STOP_TRUE_RESIDUAL = 0, 1
+OK, NOT_CONVERGED = 0, 1
+
+EXPORTED = [
+    "clg_init", "clg_finalize", "clg_synchronize", "clg_status_string", "clg_last_error",
+    "clg_device_info", "clg_field_create", "clg_field_destroy", "clg_field_set_bc",
+    "clg_field_upload", "clg_field_download", "clg_field_copy", "clg_field_zero",
+    "clg_field_parity_view", "clg_field_host_doubles", "clg_field_bytes", "clg_gauge_random",
+    "clg_gauge_unit", "clg_field_gaussian", "clg_field_point_source", "clg_wilson_apply",
+    "clg_wilson_hop", "clg_wilson_schur", "clg_ks_apply", "clg_axpy", "clg_scale", "clg_dot",
+    "clg_norm2", "clg_solve", "clg_solver_params_default", "clg_time_wilson",
+]
+
+
+class CLGError(RuntimeError):
+    pass
+
+
+class SolverParams(C.Structure):
+    _fields_ = [("solver", C.c_int), ("max_dim", C.c_int), ("restart", C.c_int),
+                ("max_step", C.c_int), ("deviation_step", C.c_int), ("accuracy", C.c_double),
+                ("absolute_accuracy", C.c_int), ("stop_rule", C.c_int), ("even_odd", C.c_int),
+                ("inner_prec", C.c_int), ("inner_accuracy", C.c_double), ("max_outer", C.c_int),
+                ("kind", C.c_int)]
+
+
+class SolveReport(C.Structure):
+    _fields_ = [("converged", C.c_int), ("cycles", C.c_int), ("outer", C.c_int),
+                ("operator_applies", C.c_int), ("true_residual", C.c_double),
+                ("last_deviation", C.c_double), ("seconds", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libclg_hip.so; raises if it is missing (there is no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CLGError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
+        sig = {
+            "clg_init": (C.c_int, [C.c_int, C.POINTER(vp)]),
+            "clg_finalize": (C.c_int, [vp]),
+            "clg_synchronize": (C.c_int, [vp]),
+            "clg_status_string": (C.c_char_p, [C.c_int]),
+            "clg_last_error": (C.c_char_p, []),
+            "clg_device_info": (C.c_int, [vp, C.c_char_p, C.c_int]),
+            "clg_field_create": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, ip, C.POINTER(vp)]),
+            "clg_field_destroy": (C.c_int, [vp]),
+            "clg_field_set_bc": (C.c_int, [vp, ip]),
+            "clg_field_upload": (C.c_int, [vp, dp]),
+            "clg_field_download": (C.c_int, [vp, dp]),
+            "clg_field_copy": (C.c_int, [vp, vp]),
+            "clg_field_zero": (C.c_int, [vp]),
+            "clg_field_parity_view": (C.c_int, [vp, C.c_int, C.POINTER(vp)]),
+            "clg_field_host_doubles": (C.c_long, [vp]),
+            "clg_field_bytes": (C.c_long, [vp]),
+            "clg_gauge_random": (C.c_int, [vp, C.c_double, C.c_uint]),
+            "clg_gauge_unit": (C.c_int, [vp]),
+            "clg_field_gaussian": (C.c_int, [vp, C.c_uint]),
+            "clg_field_point_source": (C.c_int, [vp, ip, C.c_int, C.c_int]),
+            "clg_wilson_apply": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double]),
+            "clg_wilson_hop": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_double, C.c_double, vp, C.c_double, C.c_double]),
+            "clg_wilson_schur": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_int]),
+            "clg_ks_apply": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double]),
+            "clg_axpy": (C.c_int, [vp, vp, C.c_double, C.c_double, vp]),
+            "clg_scale": (C.c_int, [vp, vp, C.c_double, C.c_double]),
+            "clg_dot": (C.c_int, [vp, vp, vp, dp]),
+            "clg_norm2": (C.c_int, [vp, vp, dp]),
+            "clg_solve": (C.c_int, [vp, C.POINTER(SolverParams), vp, vp, vp, C.c_double, C.c_int, vp,
+                                    C.POINTER(SolveReport)]),
+            "clg_solver_params_default": (C.c_int, [C.POINTER(SolverParams)]),
+            "clg_time_wilson": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, dp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status, allow=(OK,)):
+    if status not in allow:
+        msg = lib().clg_last_error().decode()
+        raise CLGError(f"{lib().clg_status_string(status).decode()}: {msg}")
+    return status
+
+
+def _i4(v):
+    return (C.c_int * 4)(*[int(a) for a in v])
+
+
+class Context:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().clg_init(device, C.byref(h)))
+        self.h = h
+
+    def sync(self):
+        check(lib().clg_synchronize(self.h))
+
+    def info(self):
+        buf = C.create_string_buffer(128)
+        check(lib().clg_device_info(self.h, buf, 128))
+        return buf.value.decode()
+
+    def close(self):
+        if self.h:
+            lib().clg_finalize(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Field:
+    """A device field (Wilson spinor, staggered vector or gauge links)."""
+
+    def __init__(self, ctx, kind, prec, dims, subset=FULL, bc=None, _handle=None, _parent=None):
+        self.ctx, self.kind, self.prec, self.subset = ctx, kind, prec, subset
+        self.dims = tuple(int(d) for d in dims)
+        self._parent = _parent
+        if _handle is None:
+            h = C.c_void_p()
+            check(lib().clg_field_create(ctx.h, kind, prec, subset, _i4(dims), C.byref(h)))
+            self.h = h
+        else:
+            self.h = _handle
+        if bc is not None:
+            check(lib().clg_field_set_bc(self.h, _i4(bc)))
+
+    def set_bc(self, bc):
+        check(lib().clg_field_set_bc(self.h, _i4(bc)))
+
+    def host_doubles(self):
+        return lib().clg_field_host_doubles(self.h)
+
+    def nbytes(self):
+        return lib().clg_field_bytes(self.h)
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.size != self.host_doubles():
+            raise CLGError(f"upload expects {self.host_doubles()} doubles, got {a.size}")
+        check(lib().clg_field_upload(self.h, a.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def download(self):
+        a = np.empty(self.host_doubles(), dtype=np.float64)
+        check(lib().clg_field_download(self.h, a.ctypes.data_as(C.POINTER(C.c_double))))
+        return a
+
+    def copy_from(self, src):
+        check(lib().clg_field_copy(self.h, src.h))
+
+    def zero(self):
+        check(lib().clg_field_zero(self.h))
+
+    def gaussian(self, seed):
+        check(lib().clg_field_gaussian(self.h, seed))
+
+    def point_source(self, coord, spin, colour):
+        check(lib().clg_field_point_source(self.h, _i4(coord), spin, colour))
+
+    def gauge_random(self, eps, seed):
+        check(lib().clg_gauge_random(self.h, eps, seed))
+
+    def gauge_unit(self):
+        check(lib().clg_gauge_unit(self.h))
+
+    def parity(self, subset):
+        h = C.c_void_p()
+        check(lib().clg_field_parity_view(self.h, subset, C.byref(h)))
+        return Field(self.ctx, self.kind, self.prec, self.dims, subset=subset, _handle=h, _parent=self)
+
+    def free(self):
+        if self.h:
+            lib().clg_field_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def wilson_apply(ctx, out, inp, gauge, kappa, op=EFO_F_D, coeff_type=EOCT_None, cre=1.0, cim=0.0):
+    check(lib().clg_wilson_apply(ctx.h, out.h, inp.h, gauge.h, kappa, op, coeff_type, cre, cim))
+
+
+def wilson_hop(ctx, out, inp, gauge, dagger=False, a=1.0, x=None, b=0.0):
+    a, b = complex(a), complex(b)
+    check(lib().clg_wilson_hop(ctx.h, out.h, inp.h, gauge.h, int(dagger), a.real, a.imag,
+                               x.h if x is not None else None, b.real, b.imag))
+
+
+def wilson_schur(ctx, out, inp, gauge, kappa, op=EFO_F_D):
+    check(lib().clg_wilson_schur(ctx.h, out.h, inp.h, gauge.h, kappa, op))
+
+
+def ks_apply(ctx, out, inp, gauge, two_am, op=EFO_F_D, coeff_type=EOCT_None, cre=1.0, cim=0.0):
+    check(lib().clg_ks_apply(ctx.h, out.h, inp.h, gauge.h, two_am, op, coeff_type, cre, cim))
+
+
+def axpy(ctx, y, a, x):
+    a = complex(a)
+    check(lib().clg_axpy(ctx.h, y.h, a.real, a.imag, x.h))
+
+
+def scale(ctx, y, a):
+    a = complex(a)
+    check(lib().clg_scale(ctx.h, y.h, a.real, a.imag))
+
+
+def dot(ctx, a, b):
+    out = (C.c_double * 2)()
+    check(lib().clg_dot(ctx.h, a.h, b.h, out))
+    return complex(out[0], out[1])
+
+
+def norm2(ctx, a):
+    out = C.c_double()
+    check(lib().clg_norm2(ctx.h, a.h, C.byref(out)))
+    return out.value
+
+
+def solver_params(**kw):
+    p = SolverParams()
+    check(lib().clg_solver_params_default(C.byref(p)))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise CLGError(f"unknown solver parameter {k}")
+        setattr(p, k, v)
+    return p
+
+
+def solve(ctx, params, x, b, gauge, kappa_or_mass, op=EFO_F_InverseD, start=None):
+    rep = SolveReport()
+    st = lib().clg_solve(ctx.h, C.byref(params), x.h, b.h, gauge.h, kappa_or_mass, op,
+                         start.h if start is not None else None, C.byref(rep))
+    check(st, allow=(OK, NOT_CONVERGED))
+    return st, rep
+
+
+def time_wilson(ctx, out, inp, gauge, kappa, variant, reps):
+    ms = C.c_double()
+    check(lib().clg_time_wilson(ctx.h, out.h, inp.h, gauge.h, kappa, variant, reps, C.byref(ms)))
+    return ms.value

@@ -1,0 +1,512 @@
+// clg_internal.h -- shared definitions of libclg_hip.so (not installed, not part of the ABI).
+//
+// Data layout in HBM (all fields):
+//   * Sites are split by parity.  The e-th site of parity p is the lexicographic site
+//     2e or 2e+1 with parity p (the reference's even/odd numbering, CLGLib_Private.h:72-116),
+//     so a half-lattice index h maps to coordinates with one division chain and Lt even.
+//   * A FULL fermion field is [parity-0 block | parity-1 block]; an EVEN/ODD field is one
+//     block.  A block is structure-of-arrays: `nplane` planes of Vh 16-byte (or 8/4-byte)
+//     chunks, so lane i of a wavefront reads chunk i of a plane: every load is a contiguous
+//     64-lane burst.
+//   * Gauge fields hold 8 blocks, (parity, mu), each a set of link planes over Vh sites:
+//     U_mu(x) for x of that parity.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/clg_hip.h"
+
+namespace clg {
+
+// ------------------------------------------------------------------ errors
+void set_error(const char* fmt, ...);
+const char* last_error();
+
+#define CLG_CHECK_ARG(cond, ...)              \
+    do {                                      \
+        if (!(cond)) {                        \
+            ::clg::set_error(__VA_ARGS__);    \
+            return CLG_BAD_ARG;               \
+        }                                     \
+    } while (0)
+
+#define CLG_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t _e = (call);                                                        \
+        if (_e != hipSuccess) {                                                        \
+            ::clg::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(_e),   \
+                             __FILE__, __LINE__);                                      \
+            return CLG_HIP_ERROR;                                                      \
+        }                                                                              \
+    } while (0)
+
+#define CLG_TRY(call)                 \
+    do {                              \
+        int _s = (call);              \
+        if (_s != CLG_OK) return _s;  \
+    } while (0)
+
+// ------------------------------------------------------------------ geometry
+struct Geo {
+    int L[4];       // x, y, z, t extents
+    int stride[4];  // lexicographic strides (t fastest)
+    int sgn[4];     // fermion boundary sign per direction
+    int Vh;         // half volume
+};
+
+// ------------------------------------------------------------------ handles
+}  // namespace clg
+
+struct clg_field_s;
+
+struct clg_context_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 0;
+    char arch[64] = {0};
+    // reduction scratch: per-block partials, a done-counter, device results
+    double* red_partial = nullptr;
+    size_t red_partial_cap = 0;  // doubles
+    unsigned* red_counter = nullptr;
+    double* red_result = nullptr;  // 256 doubles of device-resident results
+    double* host_result = nullptr;  // pinned
+    // scratch fields keyed by shape (owned)
+    std::vector<clg_field_s*> scratch;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct clg_field_s {
+    clg_context ctx = nullptr;
+    int kind = 0, prec = 0, subset = 0;
+    int dims[4] = {0, 0, 0, 0};
+    int bc[4] = {1, 1, 1, -1};
+    long V = 0;
+    long Vh = 0;
+    long nsites = 0;       // V (FULL) or Vh
+    void* data = nullptr;  // device storage
+    float* norm = nullptr; // FP16 fields: per-site scale, [parity block][Vh]
+    size_t bytes = 0;
+    bool owns = true;
+    clg_field_s* parent = nullptr;  // views
+    // size of one parity block in bytes (fermions) or one (parity,mu) block (gauge)
+    size_t block_bytes = 0;
+};
+
+namespace clg {
+
+// ------------------------------------------------------------------ layout sizes
+// bytes of one site's worth of a fermion parity block / one link of a gauge block
+inline size_t site_bytes(int kind, int prec) {
+    switch (kind) {
+        case CLG_WILSON: return prec == CLG_F64 ? 192 : prec == CLG_F32 ? 96 : 48;
+        case CLG_KS: return prec == CLG_F64 ? 48 : prec == CLG_F32 ? 24 : 12;
+        case CLG_GAUGE: return prec == CLG_F64 ? 144 : prec == CLG_F32 ? 72 : 36;
+        case CLG_GAUGE12: return prec == CLG_F64 ? 96 : prec == CLG_F32 ? 48 : 24;
+    }
+    return 0;
+}
+inline bool is_gauge(int kind) { return kind == CLG_GAUGE || kind == CLG_GAUGE12; }
+inline bool is_fermion(int kind) { return kind == CLG_WILSON || kind == CLG_KS; }
+inline int parity_of_subset(int subset) { return subset == CLG_ODD ? 1 : 0; }
+
+Geo make_geo(const clg_field_s* f);
+int check_same_lattice(const clg_field_s* a, const clg_field_s* b);
+// device pointer of parity block p of a fermion field (FULL: p in {0,1}; half: p must match)
+inline char* block_ptr(const clg_field_s* f, int p) {
+    if (f->subset == CLG_FULL) return (char*)f->data + (size_t)p * f->block_bytes;
+    return (char*)f->data;
+}
+inline float* norm_ptr(const clg_field_s* f, int p) {
+    if (!f->norm) return nullptr;
+    if (f->subset == CLG_FULL) return f->norm + (size_t)p * f->Vh;
+    return f->norm;
+}
+inline int field_parity(const clg_field_s* f) { return f->subset == CLG_ODD ? 1 : 0; }
+
+// reductions (clg_blas.hip)
+int ensure_reduction_scratch(clg_context ctx, size_t doubles);
+// scratch field of the same shape as `like` (kind/prec/subset/dims), index slot
+int scratch_field(clg_context ctx, const clg_field_s* like, int kind, int prec, int subset,
+                  int slot, clg_field_s** out);
+
+// launch helpers
+inline unsigned grid_for(long n, int block) { return (unsigned)((n + block - 1) / block); }
+
+}  // namespace clg
+
+// ======================================================================= device side
+namespace clgd {
+
+template <typename R> struct C2 { R x, y; };
+typedef C2<double> cd;
+typedef C2<float> cf;
+
+template <typename R> __device__ __forceinline__ C2<R> cmk(R a, R b) { return C2<R>{a, b}; }
+template <typename R> __device__ __forceinline__ C2<R> cadd(C2<R> a, C2<R> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename R> __device__ __forceinline__ C2<R> csub(C2<R> a, C2<R> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename R> __device__ __forceinline__ C2<R> cmul(C2<R> a, C2<R> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// conj(a) * b
+template <typename R> __device__ __forceinline__ C2<R> cmulc(C2<R> a, C2<R> b) {
+    return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
+}
+template <typename R> __device__ __forceinline__ C2<R> cconj(C2<R> a) { return {a.x, -a.y}; }
+template <typename R> __device__ __forceinline__ C2<R> cscale(C2<R> a, R s) { return {a.x * s, a.y * s}; }
+// acc += a*b
+template <typename R> __device__ __forceinline__ void cfma(C2<R>& acc, C2<R> a, C2<R> b) {
+    acc.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
+    acc.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
+}
+// acc += conj(a)*b
+template <typename R> __device__ __forceinline__ void cfmac(C2<R>& acc, C2<R> a, C2<R> b) {
+    acc.x = fma(a.x, b.x, fma(a.y, b.y, acc.x));
+    acc.y = fma(a.x, b.y, fma(-a.y, b.x, acc.y));
+}
+// i^k * a for compile-time k
+template <int K, typename R> __device__ __forceinline__ C2<R> mulz4(C2<R> a) {
+    if constexpr ((K & 3) == 1) return {-a.y, a.x};
+    else if constexpr ((K & 3) == 2) return {-a.x, -a.y};
+    else if constexpr ((K & 3) == 3) return {a.y, -a.x};
+    else return a;
+}
+
+// ------------------------------------------------------------------ site geometry
+struct Site {
+    int c[4];
+    int lex;
+};
+// coordinates of half-index h of parity p (Lt even)
+__device__ __forceinline__ Site site_of(const clg::Geo& g, int p, int h) {
+    Site s;
+    const int lex0 = 2 * h;
+    const int t0 = lex0 % g.L[3];
+    int r = lex0 / g.L[3];
+    s.c[2] = r % g.L[2];
+    r /= g.L[2];
+    s.c[1] = r % g.L[1];
+    s.c[0] = r / g.L[1];
+    const int b = (p ^ (s.c[0] + s.c[1] + s.c[2])) & 1;
+    s.c[3] = t0 + b;
+    s.lex = lex0 + b;
+    return s;
+}
+struct Nbr {
+    int p, h, sign;
+};
+template <int MU>
+__device__ __forceinline__ Nbr fwd_nbr(const clg::Geo& g, const Site& s, int p) {
+    Nbr n;
+    int lex;
+    const bool wrap = s.c[MU] == g.L[MU] - 1;
+    if (wrap) {
+        lex = s.lex - (g.L[MU] - 1) * g.stride[MU];
+        n.sign = g.sgn[MU];
+        n.p = p ^ 1 ^ (g.L[MU] & 1);
+    } else {
+        lex = s.lex + g.stride[MU];
+        n.sign = 1;
+        n.p = p ^ 1;
+    }
+    n.h = lex >> 1;
+    return n;
+}
+template <int MU>
+__device__ __forceinline__ Nbr bwd_nbr(const clg::Geo& g, const Site& s, int p) {
+    Nbr n;
+    int lex;
+    const bool wrap = s.c[MU] == 0;
+    if (wrap) {
+        lex = s.lex + (g.L[MU] - 1) * g.stride[MU];
+        n.sign = g.sgn[MU];
+        n.p = p ^ 1 ^ (g.L[MU] & 1);
+    } else {
+        lex = s.lex - g.stride[MU];
+        n.sign = 1;
+        n.p = p ^ 1;
+    }
+    n.h = lex >> 1;
+    return n;
+}
+
+// ------------------------------------------------------------------ storage traits
+// Every field is a uniform base pointer plus a set of planes.  An element index e selects
+// one site inside the planes (e = block * NPL * Vh + h for a field of several blocks), and
+// plane k of that site sits at base + k*Vh*CH + e*CH.  The per-lane part is a 32-bit byte
+// offset so the compiler can use the scalar-base + vector-offset load form.
+template <typename T>
+__device__ __forceinline__ T ldp(const char* base, int Vh, int k, unsigned e) {
+    return *reinterpret_cast<const T*>(base + (size_t)k * (size_t)Vh * sizeof(T) + (unsigned)(e * (unsigned)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void stp(char* base, int Vh, int k, unsigned e, T v) {
+    *reinterpret_cast<T*>(base + (size_t)k * (size_t)Vh * sizeof(T) + (unsigned)(e * (unsigned)sizeof(T))) = v;
+}
+
+// FP16 storage helpers
+__device__ __forceinline__ unsigned pack_h2(float a, float b) {
+    __half2 h = __floats2half2_rn(a, b);
+    return *reinterpret_cast<unsigned*>(&h);
+}
+__device__ __forceinline__ float2 unpack_h2(unsigned u) {
+    __half2 h = *reinterpret_cast<__half2*>(&u);
+    return __half22float2(h);
+}
+
+// Wilson spinor: 12 complex numbers per site.
+//   F64: 12 planes of double2; F32: 6 planes of float4 (two complex); F16: 3 planes of 8
+//   halves plus a per-site float scale (max |component|), element index ne for the scale.
+template <int PREC> struct WilsonIO;
+
+template <> struct WilsonIO<CLG_F64> {
+    static constexpr int NPL = 12;
+    template <typename R>
+    __device__ __forceinline__ static void load(const char* b, const float*, int Vh, unsigned e, unsigned, C2<R>* v) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            const double2 a = ldp<double2>(b, Vh, k, e);
+            v[k] = C2<R>{(R)a.x, (R)a.y};
+        }
+    }
+    template <typename R>
+    __device__ __forceinline__ static void store(char* b, float*, int Vh, unsigned e, unsigned, const C2<R>* v) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) stp<double2>(b, Vh, k, e, make_double2((double)v[k].x, (double)v[k].y));
+    }
+};
+
+template <> struct WilsonIO<CLG_F32> {
+    static constexpr int NPL = 6;
+    template <typename R>
+    __device__ __forceinline__ static void load(const char* b, const float*, int Vh, unsigned e, unsigned, C2<R>* v) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const float4 a = ldp<float4>(b, Vh, k, e);
+            v[2 * k] = C2<R>{(R)a.x, (R)a.y};
+            v[2 * k + 1] = C2<R>{(R)a.z, (R)a.w};
+        }
+    }
+    template <typename R>
+    __device__ __forceinline__ static void store(char* b, float*, int Vh, unsigned e, unsigned, const C2<R>* v) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            stp<float4>(b, Vh, k, e,
+                        make_float4((float)v[2 * k].x, (float)v[2 * k].y, (float)v[2 * k + 1].x, (float)v[2 * k + 1].y));
+    }
+};
+
+template <> struct WilsonIO<CLG_F16> {
+    static constexpr int NPL = 3;
+    template <typename R>
+    __device__ __forceinline__ static void load(const char* b, const float* nb, int Vh, unsigned e, unsigned ne, C2<R>* v) {
+        const float s = nb[ne];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const uint4 a = ldp<uint4>(b, Vh, k, e);
+            float2 f;
+            f = unpack_h2(a.x); v[4 * k + 0] = C2<R>{(R)(f.x * s), (R)(f.y * s)};
+            f = unpack_h2(a.y); v[4 * k + 1] = C2<R>{(R)(f.x * s), (R)(f.y * s)};
+            f = unpack_h2(a.z); v[4 * k + 2] = C2<R>{(R)(f.x * s), (R)(f.y * s)};
+            f = unpack_h2(a.w); v[4 * k + 3] = C2<R>{(R)(f.x * s), (R)(f.y * s)};
+        }
+    }
+    template <typename R>
+    __device__ __forceinline__ static void store(char* b, float* nb, int Vh, unsigned e, unsigned ne, const C2<R>* v) {
+        float m = 0.f;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) m = fmaxf(m, fmaxf(fabsf((float)v[k].x), fabsf((float)v[k].y)));
+        const float inv = m > 0.f ? 1.0f / m : 0.f;
+        nb[ne] = m;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            uint4 a;
+            a.x = pack_h2((float)v[4 * k + 0].x * inv, (float)v[4 * k + 0].y * inv);
+            a.y = pack_h2((float)v[4 * k + 1].x * inv, (float)v[4 * k + 1].y * inv);
+            a.z = pack_h2((float)v[4 * k + 2].x * inv, (float)v[4 * k + 2].y * inv);
+            a.w = pack_h2((float)v[4 * k + 3].x * inv, (float)v[4 * k + 3].y * inv);
+            stp<uint4>(b, Vh, k, e, a);
+        }
+    }
+};
+
+// Staggered vector: 3 complex per site, one complex per plane (16 / 8 / 4 bytes)
+template <int PREC> struct KSIO;
+template <> struct KSIO<CLG_F64> {
+    static constexpr int NPL = 3;
+    template <typename R>
+    __device__ __forceinline__ static void load(const char* b, const float*, int Vh, unsigned e, unsigned, C2<R>* v) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { const double2 a = ldp<double2>(b, Vh, k, e); v[k] = C2<R>{(R)a.x, (R)a.y}; }
+    }
+    template <typename R>
+    __device__ __forceinline__ static void store(char* b, float*, int Vh, unsigned e, unsigned, const C2<R>* v) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) stp<double2>(b, Vh, k, e, make_double2((double)v[k].x, (double)v[k].y));
+    }
+};
+template <> struct KSIO<CLG_F32> {
+    static constexpr int NPL = 3;
+    template <typename R>
+    __device__ __forceinline__ static void load(const char* b, const float*, int Vh, unsigned e, unsigned, C2<R>* v) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { const float2 a = ldp<float2>(b, Vh, k, e); v[k] = C2<R>{(R)a.x, (R)a.y}; }
+    }
+    template <typename R>
+    __device__ __forceinline__ static void store(char* b, float*, int Vh, unsigned e, unsigned, const C2<R>* v) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) stp<float2>(b, Vh, k, e, make_float2((float)v[k].x, (float)v[k].y));
+    }
+};
+template <> struct KSIO<CLG_F16> {
+    static constexpr int NPL = 3;
+    template <typename R>
+    __device__ __forceinline__ static void load(const char* b, const float* nb, int Vh, unsigned e, unsigned ne, C2<R>* v) {
+        const float s = nb[ne];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float2 f = unpack_h2(ldp<unsigned>(b, Vh, k, e));
+            v[k] = C2<R>{(R)(f.x * s), (R)(f.y * s)};
+        }
+    }
+    template <typename R>
+    __device__ __forceinline__ static void store(char* b, float* nb, int Vh, unsigned e, unsigned ne, const C2<R>* v) {
+        float m = 0.f;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) m = fmaxf(m, fmaxf(fabsf((float)v[k].x), fabsf((float)v[k].y)));
+        const float inv = m > 0.f ? 1.0f / m : 0.f;
+        nb[ne] = m;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) stp<unsigned>(b, Vh, k, e, pack_h2((float)v[k].x * inv, (float)v[k].y * inv));
+    }
+};
+
+// Gauge links: 8 blocks (parity, mu) of Vh links each; element index e = (p*4+mu)*NPL*Vh + h.
+// 18-real: 9 planes of one complex; 12-real: rows 0,1 only, row 2 = conj(r0 x r1) as
+// SU3::Norm builds it (SU3.h:1083-1085).
+template <typename R>
+__device__ __forceinline__ void rebuild_row2(C2<R>* m) {
+    m[6] = cconj(csub(cmul(m[1], m[5]), cmul(m[2], m[4])));
+    m[7] = cconj(csub(cmul(m[2], m[3]), cmul(m[0], m[5])));
+    m[8] = cconj(csub(cmul(m[0], m[4]), cmul(m[1], m[3])));
+}
+
+template <int PREC, int RECON> struct LinkIO;
+template <> struct LinkIO<CLG_F64, 18> {
+    static constexpr int NPL = 9;
+    template <typename R>
+    __device__ __forceinline__ static void load(const char* b, int Vh, unsigned e, C2<R>* m) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) { const double2 a = ldp<double2>(b, Vh, k, e); m[k] = C2<R>{(R)a.x, (R)a.y}; }
+    }
+    __device__ __forceinline__ static void store(char* b, int Vh, unsigned e, const cd* m) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) stp<double2>(b, Vh, k, e, make_double2(m[k].x, m[k].y));
+    }
+};
+template <> struct LinkIO<CLG_F64, 12> {
+    static constexpr int NPL = 6;
+    template <typename R>
+    __device__ __forceinline__ static void load(const char* b, int Vh, unsigned e, C2<R>* m) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) { const double2 a = ldp<double2>(b, Vh, k, e); m[k] = C2<R>{(R)a.x, (R)a.y}; }
+        rebuild_row2(m);
+    }
+    __device__ __forceinline__ static void store(char* b, int Vh, unsigned e, const cd* m) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) stp<double2>(b, Vh, k, e, make_double2(m[k].x, m[k].y));
+    }
+};
+template <> struct LinkIO<CLG_F32, 18> {
+    static constexpr int NPL = 9;
+    template <typename R>
+    __device__ __forceinline__ static void load(const char* b, int Vh, unsigned e, C2<R>* m) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) { const float2 a = ldp<float2>(b, Vh, k, e); m[k] = C2<R>{(R)a.x, (R)a.y}; }
+    }
+    __device__ __forceinline__ static void store(char* b, int Vh, unsigned e, const cd* m) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) stp<float2>(b, Vh, k, e, make_float2((float)m[k].x, (float)m[k].y));
+    }
+};
+template <> struct LinkIO<CLG_F32, 12> {
+    static constexpr int NPL = 3;
+    template <typename R>
+    __device__ __forceinline__ static void load(const char* b, int Vh, unsigned e, C2<R>* m) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float4 a = ldp<float4>(b, Vh, k, e);
+            m[2 * k] = C2<R>{(R)a.x, (R)a.y};
+            m[2 * k + 1] = C2<R>{(R)a.z, (R)a.w};
+        }
+        rebuild_row2(m);
+    }
+    __device__ __forceinline__ static void store(char* b, int Vh, unsigned e, const cd* m) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            stp<float4>(b, Vh, k, e,
+                        make_float4((float)m[2 * k].x, (float)m[2 * k].y, (float)m[2 * k + 1].x, (float)m[2 * k + 1].y));
+    }
+};
+template <> struct LinkIO<CLG_F16, 18> {
+    static constexpr int NPL = 9;
+    template <typename R>
+    __device__ __forceinline__ static void load(const char* b, int Vh, unsigned e, C2<R>* m) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) { const float2 f = unpack_h2(ldp<unsigned>(b, Vh, k, e)); m[k] = C2<R>{(R)f.x, (R)f.y}; }
+    }
+    __device__ __forceinline__ static void store(char* b, int Vh, unsigned e, const cd* m) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) stp<unsigned>(b, Vh, k, e, pack_h2((float)m[k].x, (float)m[k].y));
+    }
+};
+template <> struct LinkIO<CLG_F16, 12> {
+    static constexpr int NPL = 3;
+    template <typename R>
+    __device__ __forceinline__ static void load(const char* b, int Vh, unsigned e, C2<R>* m) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const uint2 a = ldp<uint2>(b, Vh, k, e);
+            const float2 f0 = unpack_h2(a.x), f1 = unpack_h2(a.y);
+            m[2 * k] = C2<R>{(R)f0.x, (R)f0.y};
+            m[2 * k + 1] = C2<R>{(R)f1.x, (R)f1.y};
+        }
+        rebuild_row2(m);
+    }
+    __device__ __forceinline__ static void store(char* b, int Vh, unsigned e, const cd* m) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            stp<uint2>(b, Vh, k, e,
+                       make_uint2(pack_h2((float)m[2 * k].x, (float)m[2 * k].y),
+                                  pack_h2((float)m[2 * k + 1].x, (float)m[2 * k + 1].y)));
+    }
+};
+
+// y = U x and y = U^dagger x on colour 3-vectors
+template <typename R>
+__device__ __forceinline__ void su3_mv(const C2<R>* u, const C2<R>* x, C2<R>* y) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        C2<R> a = cmul(u[3 * i], x[0]);
+        cfma(a, u[3 * i + 1], x[1]);
+        cfma(a, u[3 * i + 2], x[2]);
+        y[i] = a;
+    }
+}
+template <typename R>
+__device__ __forceinline__ void su3_mdv(const C2<R>* u, const C2<R>* x, C2<R>* y) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        C2<R> a = cmulc(u[i], x[0]);
+        cfmac(a, u[3 + i], x[1]);
+        cfmac(a, u[6 + i], x[2]);
+        y[i] = a;
+    }
+}
+
+}  // namespace clgd

@@ -1,0 +1,766 @@
+// clg_runtime.hip -- context, fields, host<->device layout conversion, synthetic inputs.
+#include "clg_internal.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+
+namespace clg {
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+const char* last_error() { return g_err.c_str(); }
+
+Geo make_geo(const clg_field_s* f) {
+    Geo g;
+    for (int i = 0; i < 4; ++i) {
+        g.L[i] = f->dims[i];
+        g.sgn[i] = f->bc[i];
+    }
+    g.stride[3] = 1;
+    g.stride[2] = g.L[3];
+    g.stride[1] = g.L[2] * g.L[3];
+    g.stride[0] = g.L[1] * g.L[2] * g.L[3];
+    g.Vh = (int)f->Vh;
+    return g;
+}
+
+int check_same_lattice(const clg_field_s* a, const clg_field_s* b) {
+    CLG_CHECK_ARG(a && b, "null field");
+    for (int i = 0; i < 4; ++i)
+        CLG_CHECK_ARG(a->dims[i] == b->dims[i], "fields live on different lattices");
+    return CLG_OK;
+}
+
+}  // namespace clg
+
+using namespace clg;
+using namespace clgd;
+
+// ============================================================== device kernels
+namespace {
+
+// ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), counter = (index, draw, stream, 0),
+//      key = (seed, 0x434C4721); two uniforms of 53 bits -> Box-Muller pair
+__device__ __forceinline__ void philox(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0,
+                                       unsigned k1, unsigned out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const unsigned hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        const unsigned hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        const unsigned n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+__device__ __forceinline__ double u53(unsigned a, unsigned b) {
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6) + 0.5) * (1.0 / 9007199254740992.0);
+}
+__device__ __forceinline__ void gauss2(unsigned idx, unsigned draw, unsigned stream, unsigned seed, double g[2]) {
+    unsigned w[4];
+    philox(idx, draw, stream, 0u, seed, 0x434C4721u, w);
+    const double u1 = u53(w[0], w[1]), u2 = u53(w[2], w[3]);
+    const double r = sqrt(-2.0 * log(u1));
+    double s, c;
+    sincos(6.283185307179586476925286766559 * u2, &s, &c);
+    g[0] = r * c;
+    g[1] = r * s;
+}
+enum { STREAM_GAUGE = 0, STREAM_WILSON = 1, STREAM_KS = 2 };
+
+__device__ void mat_mul(const cd* a, const cd* b, cd* r) {
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            cd acc = cmul(a[3 * i], b[j]);
+            acc = cadd(acc, cmul(a[3 * i + 1], b[3 + j]));
+            acc = cadd(acc, cmul(a[3 * i + 2], b[6 + j]));
+            r[3 * i + j] = acc;
+        }
+}
+// SU3::Norm (SU3.h:1058-1088)
+__device__ void su3_norm(cd* m) {
+    const double d1 = 1.0 / sqrt(m[0].x * m[0].x + m[0].y * m[0].y + m[1].x * m[1].x + m[1].y * m[1].y +
+                                 m[2].x * m[2].x + m[2].y * m[2].y);
+    m[0] = cscale(m[0], d1); m[1] = cscale(m[1], d1); m[2] = cscale(m[2], d1);
+    cd sp = cadd(cadd(cmul(m[0], cconj(m[3])), cmul(m[1], cconj(m[4]))), cmul(m[2], cconj(m[5])));
+    sp = cconj(sp);
+    m[3] = csub(m[3], cmul(sp, m[0])); m[4] = csub(m[4], cmul(sp, m[1])); m[5] = csub(m[5], cmul(sp, m[2]));
+    const double d2 = 1.0 / sqrt(m[3].x * m[3].x + m[3].y * m[3].y + m[4].x * m[4].x + m[4].y * m[4].y +
+                                 m[5].x * m[5].x + m[5].y * m[5].y);
+    m[3] = cscale(m[3], d2); m[4] = cscale(m[4], d2); m[5] = cscale(m[5], d2);
+    rebuild_row2(m);
+}
+// reunitarise(exp(i eps H)) with Taylor scaling-and-squaring (same recipe as the oracle)
+__device__ void random_link(unsigned link, double eps, unsigned seed, cd* out) {
+    double g[10];
+    for (int d = 0; d < 5; ++d) gauss2(link, (unsigned)d, STREAM_GAUGE, seed, g + 2 * d);
+    const double tr = (g[0] + g[1] + g[2]) / 3.0;
+    const double is2 = 0.70710678118654752440084436210485;
+    cd H[9];
+    H[0] = cd{g[0] - tr, 0.0};
+    H[4] = cd{g[1] - tr, 0.0};
+    H[8] = cd{g[2] - tr, 0.0};
+    H[1] = cd{g[3] * is2, g[4] * is2};
+    H[2] = cd{g[5] * is2, g[6] * is2};
+    H[5] = cd{g[7] * is2, g[8] * is2};
+    H[3] = cconj(H[1]);
+    H[6] = cconj(H[2]);
+    H[7] = cconj(H[5]);
+    cd A[9];
+    double nrm = 0;
+    for (int i = 0; i < 9; ++i) {
+        A[i] = cd{-eps * H[i].y, eps * H[i].x};
+        nrm += A[i].x * A[i].x + A[i].y * A[i].y;
+    }
+    nrm = sqrt(nrm);
+    int s = 0;
+    while (nrm > 0.5) { nrm *= 0.5; ++s; }
+    const double f = ldexp(1.0, -s);
+    for (int i = 0; i < 9; ++i) A[i] = cscale(A[i], f);
+    cd T[9], S[9], tmp[9];
+    for (int i = 0; i < 9; ++i) T[i] = cd{0.0, 0.0};
+    T[0] = T[4] = T[8] = cd{1.0, 0.0};
+    for (int i = 0; i < 9; ++i) S[i] = T[i];
+    for (int k = 1; k <= 20; ++k) {
+        mat_mul(T, A, tmp);
+        for (int i = 0; i < 9; ++i) { T[i] = cscale(tmp[i], 1.0 / k); S[i] = cadd(S[i], T[i]); }
+    }
+    for (int k = 0; k < s; ++k) {
+        mat_mul(S, S, tmp);
+        for (int i = 0; i < 9; ++i) S[i] = tmp[i];
+    }
+    su3_norm(S);
+    for (int i = 0; i < 9; ++i) out[i] = S[i];
+}
+
+// thread -> (parity, h) over the blocks a field stores
+struct Blocks {
+    int pbase;  // parity of block 0
+    int nblk;   // 1 or 2
+};
+__device__ __forceinline__ bool thread_site(const Geo& g, Blocks b, int& p, int& h) {
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (long)b.nblk * g.Vh) return false;
+    const int blk = (int)(gid / g.Vh);
+    h = (int)(gid - (long)blk * g.Vh);
+    p = b.nblk == 2 ? blk : b.pbase;
+    return true;
+}
+
+// host AoS (reference order) -> device layout
+template <int PREC>
+__global__ void k_wilson_from_aos(Geo g, Blocks b, bool full, const double* __restrict__ src, char* blk0,
+                                  size_t blk_bytes, float* norm0) {
+    int p, h;
+    if (!thread_site(g, b, p, h)) return;
+    const Site s = site_of(g, p, h);
+    const long hi = full ? s.lex : h;
+    cd v[12];
+    for (int k = 0; k < 12; ++k) v[k] = cd{src[hi * 24 + 2 * k], src[hi * 24 + 2 * k + 1]};
+    const int ib = b.nblk == 2 ? p : 0;
+    WilsonIO<PREC>::store(blk0, norm0, g.Vh, (unsigned)(ib * WilsonIO<PREC>::NPL * g.Vh + h), (unsigned)(ib * g.Vh + h), v);
+}
+template <int PREC>
+__global__ void k_wilson_to_aos(Geo g, Blocks b, bool full, double* __restrict__ dst, const char* blk0,
+                                size_t blk_bytes, const float* norm0) {
+    int p, h;
+    if (!thread_site(g, b, p, h)) return;
+    const Site s = site_of(g, p, h);
+    const long hi = full ? s.lex : h;
+    cd v[12];
+    const int ib = b.nblk == 2 ? p : 0;
+    WilsonIO<PREC>::load(blk0, norm0, g.Vh, (unsigned)(ib * WilsonIO<PREC>::NPL * g.Vh + h), (unsigned)(ib * g.Vh + h), v);
+    for (int k = 0; k < 12; ++k) { dst[hi * 24 + 2 * k] = v[k].x; dst[hi * 24 + 2 * k + 1] = v[k].y; }
+}
+template <int PREC>
+__global__ void k_ks_from_aos(Geo g, Blocks b, bool full, const double* __restrict__ src, char* blk0,
+                              size_t blk_bytes, float* norm0) {
+    int p, h;
+    if (!thread_site(g, b, p, h)) return;
+    const Site s = site_of(g, p, h);
+    const long hi = full ? s.lex : h;
+    cd v[3];
+    for (int k = 0; k < 3; ++k) v[k] = cd{src[hi * 6 + 2 * k], src[hi * 6 + 2 * k + 1]};
+    const int ib = b.nblk == 2 ? p : 0;
+    KSIO<PREC>::store(blk0, norm0, g.Vh, (unsigned)(ib * KSIO<PREC>::NPL * g.Vh + h), (unsigned)(ib * g.Vh + h), v);
+}
+template <int PREC>
+__global__ void k_ks_to_aos(Geo g, Blocks b, bool full, double* __restrict__ dst, const char* blk0,
+                            size_t blk_bytes, const float* norm0) {
+    int p, h;
+    if (!thread_site(g, b, p, h)) return;
+    const Site s = site_of(g, p, h);
+    const long hi = full ? s.lex : h;
+    cd v[3];
+    const int ib = b.nblk == 2 ? p : 0;
+    KSIO<PREC>::load(blk0, norm0, g.Vh, (unsigned)(ib * KSIO<PREC>::NPL * g.Vh + h), (unsigned)(ib * g.Vh + h), v);
+    for (int k = 0; k < 3; ++k) { dst[hi * 6 + 2 * k] = v[k].x; dst[hi * 6 + 2 * k + 1] = v[k].y; }
+}
+template <int PREC, int RECON>
+__global__ void k_gauge_from_aos(Geo g, const double* __restrict__ src, char* data, size_t blk_bytes) {
+    int p, h;
+    if (!thread_site(g, Blocks{0, 2}, p, h)) return;
+    const Site s = site_of(g, p, h);
+    for (int mu = 0; mu < 4; ++mu) {
+        cd m[9];
+        const double* q = src + ((long)s.lex * 4 + mu) * 18;
+        for (int k = 0; k < 9; ++k) m[k] = cd{q[2 * k], q[2 * k + 1]};
+        LinkIO<PREC, RECON>::store(data, g.Vh, (unsigned)((p * 4 + mu) * LinkIO<PREC, RECON>::NPL * g.Vh + h), m);
+    }
+}
+template <int PREC, int RECON>
+__global__ void k_gauge_to_aos(Geo g, double* __restrict__ dst, const char* data, size_t blk_bytes) {
+    int p, h;
+    if (!thread_site(g, Blocks{0, 2}, p, h)) return;
+    const Site s = site_of(g, p, h);
+    for (int mu = 0; mu < 4; ++mu) {
+        cd m[9];
+        LinkIO<PREC, RECON>::load(data, g.Vh, (unsigned)((p * 4 + mu) * LinkIO<PREC, RECON>::NPL * g.Vh + h), m);
+        double* q = dst + ((long)s.lex * 4 + mu) * 18;
+        for (int k = 0; k < 9; ++k) { q[2 * k] = m[k].x; q[2 * k + 1] = m[k].y; }
+    }
+}
+template <int PREC, int RECON>
+__global__ void k_gauge_random(Geo g, double eps, unsigned seed, char* data, size_t blk_bytes, int unit) {
+    int p, h;
+    if (!thread_site(g, Blocks{0, 2}, p, h)) return;
+    const Site s = site_of(g, p, h);
+    for (int mu = 0; mu < 4; ++mu) {
+        cd m[9];
+        if (unit) {
+            for (int k = 0; k < 9; ++k) m[k] = cd{(k % 4 == 0) ? 1.0 : 0.0, 0.0};
+        } else {
+            random_link((unsigned)(s.lex * 4 + mu), eps, seed, m);
+        }
+        LinkIO<PREC, RECON>::store(data, g.Vh, (unsigned)((p * 4 + mu) * LinkIO<PREC, RECON>::NPL * g.Vh + h), m);
+    }
+}
+template <int PREC>
+__global__ void k_wilson_gaussian(Geo g, Blocks b, unsigned seed, char* blk0, size_t blk_bytes, float* norm0) {
+    int p, h;
+    if (!thread_site(g, b, p, h)) return;
+    const Site s = site_of(g, p, h);
+    cd v[12];
+    const double is2 = 0.70710678118654752440084436210485;
+    for (int k = 0; k < 12; ++k) {
+        double gg[2];
+        gauss2((unsigned)s.lex, (unsigned)k, STREAM_WILSON, seed, gg);
+        v[k] = cd{gg[0] * is2, gg[1] * is2};
+    }
+    const int ib = b.nblk == 2 ? p : 0;
+    WilsonIO<PREC>::store(blk0, norm0, g.Vh, (unsigned)(ib * WilsonIO<PREC>::NPL * g.Vh + h), (unsigned)(ib * g.Vh + h), v);
+}
+template <int PREC>
+__global__ void k_ks_gaussian(Geo g, Blocks b, unsigned seed, char* blk0, size_t blk_bytes, float* norm0) {
+    int p, h;
+    if (!thread_site(g, b, p, h)) return;
+    const Site s = site_of(g, p, h);
+    cd v[3];
+    const double is2 = 0.70710678118654752440084436210485;
+    for (int k = 0; k < 3; ++k) {
+        double gg[2];
+        gauss2((unsigned)s.lex, (unsigned)k, STREAM_KS, seed, gg);
+        v[k] = cd{gg[0] * is2, gg[1] * is2};
+    }
+    const int ib = b.nblk == 2 ? p : 0;
+    KSIO<PREC>::store(blk0, norm0, g.Vh, (unsigned)(ib * KSIO<PREC>::NPL * g.Vh + h), (unsigned)(ib * g.Vh + h), v);
+}
+// precision conversion copy of fermion blocks (same subset layout)
+template <int PD, int PS, int NC>
+__global__ void k_convert(int n_per_blk, int nblk, char* d0, size_t dbb, float* dn0, const char* s0, size_t sbb,
+                          const float* sn0) {
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (long)nblk * n_per_blk) return;
+    const int ib = (int)(gid / n_per_blk);
+    const int h = (int)(gid - (long)ib * n_per_blk);
+    if constexpr (NC == 12) {
+        cf v[12];
+        WilsonIO<PS>::load(s0, sn0, n_per_blk, (unsigned)(ib * WilsonIO<PS>::NPL * n_per_blk + h), (unsigned)(ib * n_per_blk + h), v);
+        WilsonIO<PD>::store(d0, dn0, n_per_blk, (unsigned)(ib * WilsonIO<PD>::NPL * n_per_blk + h), (unsigned)(ib * n_per_blk + h), v);
+    } else {
+        cf v[3];
+        KSIO<PS>::load(s0, sn0, n_per_blk, (unsigned)(ib * KSIO<PS>::NPL * n_per_blk + h), (unsigned)(ib * n_per_blk + h), v);
+        KSIO<PD>::store(d0, dn0, n_per_blk, (unsigned)(ib * KSIO<PD>::NPL * n_per_blk + h), (unsigned)(ib * n_per_blk + h), v);
+    }
+}
+template <int PD, int PS, int NC>
+__global__ void k_convert_d(int n_per_blk, int nblk, char* d0, size_t dbb, float* dn0, const char* s0, size_t sbb,
+                            const float* sn0) {
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (long)nblk * n_per_blk) return;
+    const int ib = (int)(gid / n_per_blk);
+    const int h = (int)(gid - (long)ib * n_per_blk);
+    if constexpr (NC == 12) {
+        cd v[12];
+        WilsonIO<PS>::load(s0, sn0, n_per_blk, (unsigned)(ib * WilsonIO<PS>::NPL * n_per_blk + h), (unsigned)(ib * n_per_blk + h), v);
+        WilsonIO<PD>::store(d0, dn0, n_per_blk, (unsigned)(ib * WilsonIO<PD>::NPL * n_per_blk + h), (unsigned)(ib * n_per_blk + h), v);
+    } else {
+        cd v[3];
+        KSIO<PS>::load(s0, sn0, n_per_blk, (unsigned)(ib * KSIO<PS>::NPL * n_per_blk + h), (unsigned)(ib * n_per_blk + h), v);
+        KSIO<PD>::store(d0, dn0, n_per_blk, (unsigned)(ib * KSIO<PD>::NPL * n_per_blk + h), (unsigned)(ib * n_per_blk + h), v);
+    }
+}
+__global__ void k_point(char* blk, int Vh, int h, int comp, int prec, int kind, float* nrm) {
+    // writes 1 at one complex component of one site of a zeroed block
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (kind == CLG_WILSON) {
+        cd v[12];
+        for (int k = 0; k < 12; ++k) v[k] = cd{k == comp ? 1.0 : 0.0, 0.0};
+        if (prec == CLG_F64) WilsonIO<CLG_F64>::store(blk, nrm, Vh, (unsigned)h, (unsigned)h, v);
+        else if (prec == CLG_F32) WilsonIO<CLG_F32>::store(blk, nrm, Vh, (unsigned)h, (unsigned)h, v);
+        else WilsonIO<CLG_F16>::store(blk, nrm, Vh, (unsigned)h, (unsigned)h, v);
+    } else {
+        cd v[3];
+        for (int k = 0; k < 3; ++k) v[k] = cd{k == comp ? 1.0 : 0.0, 0.0};
+        if (prec == CLG_F64) KSIO<CLG_F64>::store(blk, nrm, Vh, (unsigned)h, (unsigned)h, v);
+        else if (prec == CLG_F32) KSIO<CLG_F32>::store(blk, nrm, Vh, (unsigned)h, (unsigned)h, v);
+        else KSIO<CLG_F16>::store(blk, nrm, Vh, (unsigned)h, (unsigned)h, v);
+    }
+}
+
+// gauge precision / compression conversion
+template <int PD, int RD, int PS, int RS>
+__global__ void k_gauge_convert(int Vh, char* dst, const char* src) {
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= 8L * Vh) return;
+    const int blk = (int)(gid / Vh);  // (parity, mu) block
+    const int h = (int)(gid - (long)blk * Vh);
+    cd m[9];
+    LinkIO<PS, RS>::load(src, Vh, (unsigned)(blk * LinkIO<PS, RS>::NPL * Vh + h), m);
+    LinkIO<PD, RD>::store(dst, Vh, (unsigned)(blk * LinkIO<PD, RD>::NPL * Vh + h), m);
+}
+
+Blocks blocks_of(const clg_field_s* f) {
+    if (f->subset == CLG_FULL) return Blocks{0, 2};
+    return Blocks{field_parity(f), 1};
+}
+
+const int kBlock = 256;
+
+}  // namespace
+
+// ============================================================== scratch fields
+namespace clg {
+int gauge_convert(clg_context ctx, clg_field_s* dst, const clg_field_s* src) {
+    CLG_CHECK_ARG(dst && src && is_gauge(dst->kind) && is_gauge(src->kind), "need gauge fields");
+    CLG_TRY(check_same_lattice(dst, src));
+    const int Vh = (int)src->Vh;
+    const unsigned grid = grid_for(8L * Vh, kBlock);
+    hipStream_t st = ctx->stream;
+    const int rd = dst->kind == CLG_GAUGE12 ? 12 : 18;
+    CLG_CHECK_ARG(src->prec == CLG_F64 && src->kind == CLG_GAUGE, "conversion source must be the FP64 18-real field");
+    char* d = (char*)dst->data;
+    const char* s = (const char*)src->data;
+    if (dst->prec == CLG_F64) {
+        if (rd == 12) k_gauge_convert<CLG_F64, 12, CLG_F64, 18><<<grid, kBlock, 0, st>>>(Vh, d, s);
+        else k_gauge_convert<CLG_F64, 18, CLG_F64, 18><<<grid, kBlock, 0, st>>>(Vh, d, s);
+    } else if (dst->prec == CLG_F32) {
+        if (rd == 12) k_gauge_convert<CLG_F32, 12, CLG_F64, 18><<<grid, kBlock, 0, st>>>(Vh, d, s);
+        else k_gauge_convert<CLG_F32, 18, CLG_F64, 18><<<grid, kBlock, 0, st>>>(Vh, d, s);
+    } else {
+        if (rd == 12) k_gauge_convert<CLG_F16, 12, CLG_F64, 18><<<grid, kBlock, 0, st>>>(Vh, d, s);
+        else k_gauge_convert<CLG_F16, 18, CLG_F64, 18><<<grid, kBlock, 0, st>>>(Vh, d, s);
+    }
+    CLG_HIP(hipGetLastError());
+    return CLG_OK;
+}
+
+int scratch_field(clg_context ctx, const clg_field_s* like, int kind, int prec, int subset, int slot,
+                  clg_field_s** out) {
+    int seen = 0;
+    for (clg_field_s* f : ctx->scratch) {
+        if (f->kind == kind && f->prec == prec && f->subset == subset && f->dims[0] == like->dims[0] &&
+            f->dims[1] == like->dims[1] && f->dims[2] == like->dims[2] && f->dims[3] == like->dims[3]) {
+            if (seen == slot) {
+                for (int i = 0; i < 4; ++i) f->bc[i] = like->bc[i];
+                *out = f;
+                return CLG_OK;
+            }
+            ++seen;
+        }
+    }
+    clg_field_s* f = nullptr;
+    for (int k = seen; k <= slot; ++k) {
+        CLG_TRY(clg_field_create(ctx, kind, prec, subset, like->dims, &f));
+        ctx->scratch.push_back(f);
+    }
+    for (int i = 0; i < 4; ++i) f->bc[i] = like->bc[i];
+    *out = f;
+    return CLG_OK;
+}
+}  // namespace clg
+
+// ============================================================== C ABI
+extern "C" {
+
+const char* clg_status_string(int s) {
+    switch (s) {
+        case CLG_OK: return "ok";
+        case CLG_NOT_CONVERGED: return "solver did not converge (best iterate written)";
+        case CLG_BAD_ARG: return "bad argument";
+        case CLG_HIP_ERROR: return "HIP runtime error";
+        case CLG_UNSUPPORTED: return "unsupported combination";
+        case CLG_OUT_OF_MEMORY: return "out of device memory";
+    }
+    return "unknown status";
+}
+
+const char* clg_last_error(void) { return clg::last_error(); }
+
+int clg_init(int device, clg_context* out) {
+    CLG_CHECK_ARG(out, "null output handle");
+    int n = 0;
+    CLG_HIP(hipGetDeviceCount(&n));
+    CLG_CHECK_ARG(device >= 0 && device < n, "device %d not present (%d devices)", device, n);
+    CLG_HIP(hipSetDevice(device));
+    clg_context c = new clg_context_s();
+    c->device = device;
+    hipDeviceProp_t prop;
+    CLG_HIP(hipGetDeviceProperties(&prop, device));
+    c->num_cu = prop.multiProcessorCount;
+    std::snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);
+    CLG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    CLG_HIP(hipMalloc((void**)&c->red_counter, 64 * sizeof(unsigned)));
+    CLG_HIP(hipMemsetAsync(c->red_counter, 0, 64 * sizeof(unsigned), c->stream));
+    CLG_HIP(hipMalloc((void**)&c->red_result, 4096 * sizeof(double)));
+    CLG_HIP(hipHostMalloc((void**)&c->host_result, 4096 * sizeof(double), hipHostMallocDefault));
+    CLG_HIP(hipEventCreate(&c->ev0));
+    CLG_HIP(hipEventCreate(&c->ev1));
+    CLG_HIP(hipStreamSynchronize(c->stream));
+    *out = c;
+    return CLG_OK;
+}
+
+int clg_finalize(clg_context c) {
+    if (!c) return CLG_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (clg_field_s* f : c->scratch) clg_field_destroy(f);
+    c->scratch.clear();
+    if (c->red_partial) (void)hipFree(c->red_partial);
+    if (c->red_counter) (void)hipFree(c->red_counter);
+    if (c->red_result) (void)hipFree(c->red_result);
+    if (c->host_result) (void)hipHostFree(c->host_result);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return CLG_OK;
+}
+
+int clg_synchronize(clg_context c) {
+    CLG_CHECK_ARG(c, "null context");
+    CLG_HIP(hipStreamSynchronize(c->stream));
+    CLG_HIP(hipGetLastError());
+    return CLG_OK;
+}
+
+int clg_device_info(clg_context c, char* buf, int len) {
+    CLG_CHECK_ARG(c && buf && len > 0, "bad arguments");
+    std::snprintf(buf, (size_t)len, "%s:%dCU", c->arch, c->num_cu);
+    return CLG_OK;
+}
+
+int clg_field_create(clg_context ctx, int kind, int prec, int subset, const int dims[4], clg_field* out) {
+    CLG_CHECK_ARG(ctx && out && dims, "null argument");
+    CLG_CHECK_ARG(kind >= CLG_WILSON && kind <= CLG_GAUGE12, "bad field kind %d", kind);
+    CLG_CHECK_ARG(prec >= CLG_F64 && prec <= CLG_F16, "bad precision %d", prec);
+    CLG_CHECK_ARG(subset >= CLG_FULL && subset <= CLG_ODD, "bad subset %d", subset);
+    CLG_CHECK_ARG(!(is_gauge(kind) && subset != CLG_FULL), "gauge fields are full-lattice");
+    long V = 1;
+    for (int i = 0; i < 4; ++i) {
+        CLG_CHECK_ARG(dims[i] >= 1 && dims[i] <= 127, "lattice extent %d out of range [1,127]", dims[i]);
+        V *= dims[i];
+    }
+    CLG_CHECK_ARG((dims[3] & 1) == 0, "Lt must be even for the even/odd site layout");
+    CLG_CHECK_ARG(V * 127L < (1L << 31), "lattice too large for 32-bit site indexing");
+    clg_field_s* f = new clg_field_s();
+    f->ctx = ctx;
+    f->kind = kind;
+    f->prec = prec;
+    f->subset = subset;
+    for (int i = 0; i < 4; ++i) f->dims[i] = dims[i];
+    f->V = V;
+    f->Vh = V / 2;
+    f->nsites = subset == CLG_FULL ? V : V / 2;
+    f->block_bytes = site_bytes(kind, prec) * (size_t)f->Vh;
+    const int nblk = is_gauge(kind) ? 8 : (subset == CLG_FULL ? 2 : 1);
+    f->bytes = f->block_bytes * nblk;
+    if (f->bytes >= (size_t(1) << 32)) {  // kernels address a field with 32-bit byte offsets
+        delete f;
+        set_error("field of %ld sites exceeds the 4 GiB per-field limit", V);
+        return CLG_BAD_ARG;
+    }
+    (void)hipSetDevice(ctx->device);
+    hipError_t e = hipMalloc(&f->data, f->bytes);
+    if (e != hipSuccess) {
+        delete f;
+        set_error("hipMalloc(%zu) failed: %s", (size_t)(f->bytes), hipGetErrorString(e));
+        return CLG_OUT_OF_MEMORY;
+    }
+    if (prec == CLG_F16 && is_fermion(kind)) {
+        e = hipMalloc((void**)&f->norm, sizeof(float) * (size_t)f->Vh * nblk);
+        if (e != hipSuccess) {
+            (void)hipFree(f->data);
+            delete f;
+            set_error("hipMalloc(norm) failed: %s", hipGetErrorString(e));
+            return CLG_OUT_OF_MEMORY;
+        }
+        f->bytes += sizeof(float) * (size_t)f->Vh * nblk;
+    }
+    *out = f;
+    return clg_field_zero(f);
+}
+
+int clg_field_destroy(clg_field f) {
+    if (!f) return CLG_OK;
+    if (f->owns) {
+        (void)hipSetDevice(f->ctx->device);
+        (void)hipStreamSynchronize(f->ctx->stream);
+        if (f->data) (void)hipFree(f->data);
+        if (f->norm) (void)hipFree(f->norm);
+    }
+    delete f;
+    return CLG_OK;
+}
+
+int clg_field_set_bc(clg_field f, const int bc[4]) {
+    CLG_CHECK_ARG(f && bc, "null argument");
+    for (int i = 0; i < 4; ++i) {
+        CLG_CHECK_ARG(bc[i] == 1 || bc[i] == -1, "boundary sign must be +1 or -1");
+        f->bc[i] = bc[i];
+    }
+    return CLG_OK;
+}
+
+long clg_field_host_doubles(clg_field f) {
+    if (!f) return -1;
+    if (is_gauge(f->kind)) return f->V * 72;
+    return f->nsites * (f->kind == CLG_WILSON ? 24 : 6);
+}
+
+long clg_field_bytes(clg_field f) { return f ? (long)f->bytes : -1; }
+
+int clg_field_parity_view(clg_field full, int parity_subset, clg_field* out) {
+    CLG_CHECK_ARG(full && out, "null argument");
+    CLG_CHECK_ARG(is_fermion(full->kind) && full->subset == CLG_FULL, "view needs a FULL fermion field");
+    CLG_CHECK_ARG(parity_subset == CLG_EVEN || parity_subset == CLG_ODD, "subset must be EVEN or ODD");
+    clg_field_s* v = new clg_field_s(*full);
+    v->owns = false;
+    v->parent = full;
+    v->subset = parity_subset;
+    v->nsites = full->Vh;
+    const int p = parity_subset == CLG_ODD ? 1 : 0;
+    v->data = (char*)full->data + (size_t)p * full->block_bytes;
+    v->norm = full->norm ? full->norm + (size_t)p * full->Vh : nullptr;
+    v->bytes = full->block_bytes;
+    *out = v;
+    return CLG_OK;
+}
+
+int clg_field_zero(clg_field f) {
+    CLG_CHECK_ARG(f, "null field");
+    (void)hipSetDevice(f->ctx->device);
+    const size_t nblk = is_gauge(f->kind) ? 8 : (f->subset == CLG_FULL ? 2 : 1);
+    CLG_HIP(hipMemsetAsync(f->data, 0, f->block_bytes * nblk, f->ctx->stream));
+    if (f->norm) CLG_HIP(hipMemsetAsync(f->norm, 0, sizeof(float) * f->Vh * nblk, f->ctx->stream));
+    return CLG_OK;
+}
+
+#define DISPATCH_PREC(prec, KERNEL, ...)                                      \
+    do {                                                                      \
+        if ((prec) == CLG_F64) KERNEL<CLG_F64><<<grid, kBlock, 0, st>>>(__VA_ARGS__); \
+        else if ((prec) == CLG_F32) KERNEL<CLG_F32><<<grid, kBlock, 0, st>>>(__VA_ARGS__); \
+        else KERNEL<CLG_F16><<<grid, kBlock, 0, st>>>(__VA_ARGS__);           \
+    } while (0)
+
+#define DISPATCH_GAUGE(f, KERNEL, ...)                                                          \
+    do {                                                                                        \
+        const bool r12 = (f)->kind == CLG_GAUGE12;                                              \
+        if ((f)->prec == CLG_F64) {                                                             \
+            if (r12) KERNEL<CLG_F64, 12><<<grid, kBlock, 0, st>>>(__VA_ARGS__);                 \
+            else KERNEL<CLG_F64, 18><<<grid, kBlock, 0, st>>>(__VA_ARGS__);                     \
+        } else if ((f)->prec == CLG_F32) {                                                      \
+            if (r12) KERNEL<CLG_F32, 12><<<grid, kBlock, 0, st>>>(__VA_ARGS__);                 \
+            else KERNEL<CLG_F32, 18><<<grid, kBlock, 0, st>>>(__VA_ARGS__);                     \
+        } else {                                                                                \
+            if (r12) KERNEL<CLG_F16, 12><<<grid, kBlock, 0, st>>>(__VA_ARGS__);                 \
+            else KERNEL<CLG_F16, 18><<<grid, kBlock, 0, st>>>(__VA_ARGS__);                     \
+        }                                                                                       \
+    } while (0)
+
+int clg_field_upload(clg_field f, const double* host) {
+    CLG_CHECK_ARG(f && host, "null argument");
+    (void)hipSetDevice(f->ctx->device);
+    hipStream_t st = f->ctx->stream;
+    const size_t nd = (size_t)clg_field_host_doubles(f);
+    double* stage = nullptr;
+    CLG_HIP(hipMalloc((void**)&stage, nd * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(stage, host, nd * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) { (void)hipFree(stage); CLG_HIP(e); }
+    const Geo g = make_geo(f);
+    if (is_gauge(f->kind)) {
+        const unsigned grid = grid_for(f->V, kBlock);
+        DISPATCH_GAUGE(f, k_gauge_from_aos, g, stage, (char*)f->data, f->block_bytes);
+    } else {
+        const Blocks b = blocks_of(f);
+        const unsigned grid = grid_for((long)b.nblk * f->Vh, kBlock);
+        const bool full = f->subset == CLG_FULL;
+        if (f->kind == CLG_WILSON)
+            DISPATCH_PREC(f->prec, k_wilson_from_aos, g, b, full, stage, (char*)f->data, f->block_bytes, f->norm);
+        else
+            DISPATCH_PREC(f->prec, k_ks_from_aos, g, b, full, stage, (char*)f->data, f->block_bytes, f->norm);
+    }
+    e = hipStreamSynchronize(st);
+    (void)hipFree(stage);
+    CLG_HIP(e);
+    CLG_HIP(hipGetLastError());
+    return CLG_OK;
+}
+
+int clg_field_download(clg_field f, double* host) {
+    CLG_CHECK_ARG(f && host, "null argument");
+    (void)hipSetDevice(f->ctx->device);
+    hipStream_t st = f->ctx->stream;
+    const size_t nd = (size_t)clg_field_host_doubles(f);
+    double* stage = nullptr;
+    CLG_HIP(hipMalloc((void**)&stage, nd * sizeof(double)));
+    const Geo g = make_geo(f);
+    if (is_gauge(f->kind)) {
+        const unsigned grid = grid_for(f->V, kBlock);
+        DISPATCH_GAUGE(f, k_gauge_to_aos, g, stage, (const char*)f->data, f->block_bytes);
+    } else {
+        const Blocks b = blocks_of(f);
+        const unsigned grid = grid_for((long)b.nblk * f->Vh, kBlock);
+        const bool full = f->subset == CLG_FULL;
+        if (f->kind == CLG_WILSON)
+            DISPATCH_PREC(f->prec, k_wilson_to_aos, g, b, full, stage, (const char*)f->data, f->block_bytes, f->norm);
+        else
+            DISPATCH_PREC(f->prec, k_ks_to_aos, g, b, full, stage, (const char*)f->data, f->block_bytes, f->norm);
+    }
+    hipError_t e = hipMemcpyAsync(host, stage, nd * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(stage);
+    CLG_HIP(e);
+    CLG_HIP(hipGetLastError());
+    return CLG_OK;
+}
+
+int clg_field_copy(clg_field dst, clg_field src) {
+    CLG_CHECK_ARG(dst && src, "null argument");
+    CLG_TRY(check_same_lattice(dst, src));
+    (void)hipSetDevice(dst->ctx->device);
+    hipStream_t st = dst->ctx->stream;
+    if (dst == src) return CLG_OK;
+    if (is_gauge(dst->kind) && is_gauge(src->kind) && (dst->kind != src->kind || dst->prec != src->prec))
+        return gauge_convert(dst->ctx, dst, src);
+    CLG_CHECK_ARG(dst->kind == src->kind && dst->subset == src->subset, "copy needs the same kind and subset");
+    if (dst->prec == src->prec) {
+        const size_t nblk = is_gauge(dst->kind) ? 8 : (dst->subset == CLG_FULL ? 2 : 1);
+        CLG_HIP(hipMemcpyAsync(dst->data, src->data, dst->block_bytes * nblk, hipMemcpyDeviceToDevice, st));
+        if (dst->norm)
+            CLG_HIP(hipMemcpyAsync(dst->norm, src->norm, sizeof(float) * dst->Vh * nblk, hipMemcpyDeviceToDevice, st));
+        return CLG_OK;
+    }
+    const int nblk = dst->subset == CLG_FULL ? 2 : 1;
+    const unsigned grid = grid_for((long)nblk * dst->Vh, kBlock);
+    const int n = (int)dst->Vh;
+    char* d0 = (char*)dst->data;
+    const char* s0 = (const char*)src->data;
+#define CONV(PD, PS)                                                                                          \
+    if (dst->prec == PD && src->prec == PS) {                                                                 \
+        if (PD == CLG_F64 || PS == CLG_F64) {                                                                 \
+            if (dst->kind == CLG_WILSON)                                                                      \
+                k_convert_d<PD, PS, 12><<<grid, kBlock, 0, st>>>(n, nblk, d0, dst->block_bytes, dst->norm, s0, \
+                                                                 src->block_bytes, src->norm);                \
+            else                                                                                              \
+                k_convert_d<PD, PS, 3><<<grid, kBlock, 0, st>>>(n, nblk, d0, dst->block_bytes, dst->norm, s0,  \
+                                                                src->block_bytes, src->norm);                 \
+        } else {                                                                                              \
+            if (dst->kind == CLG_WILSON)                                                                      \
+                k_convert<PD, PS, 12><<<grid, kBlock, 0, st>>>(n, nblk, d0, dst->block_bytes, dst->norm, s0,   \
+                                                               src->block_bytes, src->norm);                  \
+            else                                                                                              \
+                k_convert<PD, PS, 3><<<grid, kBlock, 0, st>>>(n, nblk, d0, dst->block_bytes, dst->norm, s0,    \
+                                                              src->block_bytes, src->norm);                   \
+        }                                                                                                     \
+    }
+    CONV(CLG_F64, CLG_F32)
+    CONV(CLG_F64, CLG_F16)
+    CONV(CLG_F32, CLG_F64)
+    CONV(CLG_F32, CLG_F16)
+    CONV(CLG_F16, CLG_F64)
+    CONV(CLG_F16, CLG_F32)
+#undef CONV
+    CLG_HIP(hipGetLastError());
+    return CLG_OK;
+}
+
+int clg_gauge_random(clg_field f, double eps, unsigned seed) {
+    CLG_CHECK_ARG(f && is_gauge(f->kind), "need a gauge field");
+    (void)hipSetDevice(f->ctx->device);
+    hipStream_t st = f->ctx->stream;
+    const Geo g = make_geo(f);
+    const unsigned grid = grid_for(f->V, kBlock);
+    DISPATCH_GAUGE(f, k_gauge_random, g, eps, seed, (char*)f->data, f->block_bytes, 0);
+    CLG_HIP(hipGetLastError());
+    return CLG_OK;
+}
+
+int clg_gauge_unit(clg_field f) {
+    CLG_CHECK_ARG(f && is_gauge(f->kind), "need a gauge field");
+    (void)hipSetDevice(f->ctx->device);
+    hipStream_t st = f->ctx->stream;
+    const Geo g = make_geo(f);
+    const unsigned grid = grid_for(f->V, kBlock);
+    DISPATCH_GAUGE(f, k_gauge_random, g, 0.0, 0u, (char*)f->data, f->block_bytes, 1);
+    CLG_HIP(hipGetLastError());
+    return CLG_OK;
+}
+
+int clg_field_gaussian(clg_field f, unsigned seed) {
+    CLG_CHECK_ARG(f && is_fermion(f->kind), "need a fermion field");
+    (void)hipSetDevice(f->ctx->device);
+    hipStream_t st = f->ctx->stream;
+    const Geo g = make_geo(f);
+    const Blocks b = blocks_of(f);
+    const unsigned grid = grid_for((long)b.nblk * f->Vh, kBlock);
+    if (f->kind == CLG_WILSON)
+        DISPATCH_PREC(f->prec, k_wilson_gaussian, g, b, seed, (char*)f->data, f->block_bytes, f->norm);
+    else
+        DISPATCH_PREC(f->prec, k_ks_gaussian, g, b, seed, (char*)f->data, f->block_bytes, f->norm);
+    CLG_HIP(hipGetLastError());
+    return CLG_OK;
+}
+
+int clg_field_point_source(clg_field f, const int coord[4], int spin, int colour) {
+    CLG_CHECK_ARG(f && coord && is_fermion(f->kind), "need a fermion field");
+    for (int i = 0; i < 4; ++i) CLG_CHECK_ARG(coord[i] >= 0 && coord[i] < f->dims[i], "coordinate out of range");
+    CLG_CHECK_ARG(colour >= 0 && colour < 3, "colour out of range");
+    CLG_CHECK_ARG(f->kind != CLG_WILSON || (spin >= 0 && spin < 4), "spin out of range");
+    CLG_TRY(clg_field_zero(f));
+    const long lex = ((long)(coord[0] * f->dims[1] + coord[1]) * f->dims[2] + coord[2]) * f->dims[3] + coord[3];
+    const int p = (coord[0] + coord[1] + coord[2] + coord[3]) & 1;
+    if (f->subset != CLG_FULL && p != field_parity(f)) return CLG_OK;  // site not on this half
+    const int comp = f->kind == CLG_WILSON ? spin * 3 + colour : colour;
+    hipStream_t st = f->ctx->stream;
+    k_point<<<1, 64, 0, st>>>(block_ptr(f, p), (int)f->Vh, (int)(lex >> 1), comp, f->prec, f->kind, norm_ptr(f, p));
+    CLG_HIP(hipGetLastError());
+    return CLG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,432 @@
+// clg_wilson.hip -- Wilson-square SU(3) hopping kernels (D, D^dagger, even-odd hops, Schur).
+//
+// Operator (reference _kernelDFermionWilsonSquareSU3, CFieldFermionWilsonSquareSU3.cu:32-163):
+//   (D psi)(x) = psi(x) - kappa sum_mu [ (1-g_mu) U_mu(x) s+ psi(x+mu) + (1+g_mu) U_mu(x-mu)^dag s- psi(x-mu) ]
+//   D^dagger = g5 D g5, i.e. the same sum with the two projectors exchanged.
+// The reference applies full 4x4 gamma products to 4-spinors.  Here each hop is spin
+// projected: (1 -+ g_mu) has rank 2, so psi(x+-mu) is projected to a 2-spinor, the link is
+// applied to 2 colour vectors instead of 4, and the other two spin rows are rebuilt from
+// the gamma table's phases: 1320 flops per site instead of ~2700.
+//
+// Chiral gamma table (GammaMatrix.h:447-475): (g_mu psi)[r] = i^z[r] psi[idx[r]].  For a
+// projector (1 + s g_mu), s = +-1:  h_j = psi_j + s i^z_j psi_idx_j (j = 0,1) and the full
+// 4-spinor is (h_0, h_1, s i^z_2 h_idx_2, s i^z_3 h_idx_3), because g_mu^2 = 1.
+#include "clg_internal.h"
+
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+
+using namespace clg;
+using namespace clgd;
+
+namespace {
+
+template <int MU, int R> struct G;  // idx / z4 of gamma_{MU+1}, row R
+#define GDEF(MU_, R_, I_, Z_) \
+    template <> struct G<MU_, R_> { static constexpr int idx = I_; static constexpr int z = Z_; };
+GDEF(0, 0, 3, 3) GDEF(0, 1, 2, 3) GDEF(0, 2, 1, 1) GDEF(0, 3, 0, 1)
+GDEF(1, 0, 3, 2) GDEF(1, 1, 2, 0) GDEF(1, 2, 1, 0) GDEF(1, 3, 0, 2)
+GDEF(2, 0, 2, 3) GDEF(2, 1, 3, 1) GDEF(2, 2, 0, 1) GDEF(2, 3, 1, 3)
+GDEF(3, 0, 2, 0) GDEF(3, 1, 3, 0) GDEF(3, 2, 0, 0) GDEF(3, 3, 1, 0)
+#undef GDEF
+
+// SG = 0 for the projector (1 + g_mu), 2 for (1 - g_mu) (adds i^2 = -1 to the phase)
+template <int MU, int SG, typename R>
+__device__ __forceinline__ void project(const C2<R>* psi, C2<R>* hh) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        hh[c] = cadd(psi[c], mulz4<G<MU, 0>::z + SG>(psi[G<MU, 0>::idx * 3 + c]));
+        hh[3 + c] = cadd(psi[3 + c], mulz4<G<MU, 1>::z + SG>(psi[G<MU, 1>::idx * 3 + c]));
+    }
+}
+template <int MU, int SG, typename R>
+__device__ __forceinline__ void reconstruct_add(C2<R>* acc, const C2<R>* uh) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        acc[c] = cadd(acc[c], uh[c]);
+        acc[3 + c] = cadd(acc[3 + c], uh[3 + c]);
+        acc[6 + c] = cadd(acc[6 + c], mulz4<G<MU, 2>::z + SG>(uh[G<MU, 2>::idx * 3 + c]));
+        acc[9 + c] = cadd(acc[9 + c], mulz4<G<MU, 3>::z + SG>(uh[G<MU, 3>::idx * 3 + c]));
+    }
+}
+
+// Register-pressure control.  Without it the compiler hoists the loads and projections of
+// all eight neighbours ahead of the link products, holding eight half-spinors live (FP64:
+// ~390 VGPR+AGPR, one wave per SIMD).  An empty asm that reads and writes the accumulator
+// and clobbers memory orders the work: loads of later hops cannot move above it and
+// arithmetic of earlier hops cannot sink below it.  PM = 1 pins after every half-hop,
+// PM = 2 after every direction (two hops in flight); the tuner picks per shape.
+template <typename R>
+__device__ __forceinline__ void pin_acc(C2<R>* acc) {
+    asm volatile("" : "+v"(acc[0].x), "+v"(acc[0].y), "+v"(acc[1].x), "+v"(acc[1].y), "+v"(acc[2].x), "+v"(acc[2].y),
+                 "+v"(acc[3].x), "+v"(acc[3].y), "+v"(acc[4].x), "+v"(acc[4].y), "+v"(acc[5].x), "+v"(acc[5].y)::"memory");
+    asm volatile("" : "+v"(acc[6].x), "+v"(acc[6].y), "+v"(acc[7].x), "+v"(acc[7].y), "+v"(acc[8].x), "+v"(acc[8].y),
+                 "+v"(acc[9].x), "+v"(acc[9].y), "+v"(acc[10].x), "+v"(acc[10].y), "+v"(acc[11].x), "+v"(acc[11].y)::"memory");
+}
+
+struct HopArgs {
+    Geo g;
+    const char* in;      // input field base (parity-0 block of a FULL field, or the half field)
+    const float* inn;
+    char* out;
+    float* outn;
+    const char* x;       // optional b*x term (same layout as out)
+    const float* xn;
+    const char* gauge;   // 8 (parity, mu) blocks
+    int in_full, out_full;  // 1 if the field holds both parity blocks
+    double ar, ai, br, bi;
+    int pbase, nblk;     // output parities handled
+};
+
+template <int MU, bool DAG, int PM, typename R, class SIO, class LIO>
+__device__ __forceinline__ void hop_mu(const HopArgs& a, const Site& s, int p, int h, C2<R>* acc) {
+    C2<R> psi[12], hh[6], uh[6], u[9];
+    const int Vh = a.g.Vh;
+    // forward: (1 - g_mu) U_mu(x) psi(x+mu)  [dagger: (1 + g_mu)]
+    {
+        const Nbr n = fwd_nbr<MU>(a.g, s, p);
+        const unsigned pe = a.in_full ? (unsigned)n.p : 0u;
+        SIO::load(a.in, a.inn, Vh, pe * SIO::NPL * Vh + n.h, pe * Vh + n.h, psi);
+        project<MU, DAG ? 0 : 2>(psi, hh);
+        if (n.sign < 0) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) hh[k] = cmk(-hh[k].x, -hh[k].y);
+        }
+        LIO::load(a.gauge, Vh, (unsigned)(p * 4 + MU) * LIO::NPL * Vh + h, u);
+        su3_mv(u, hh, uh);
+        su3_mv(u, hh + 3, uh + 3);
+        reconstruct_add<MU, DAG ? 0 : 2>(acc, uh);
+    }
+    if constexpr (PM == 1) pin_acc(acc);
+    // backward: (1 + g_mu) U_mu(x-mu)^dag psi(x-mu)  [dagger: (1 - g_mu)]
+    {
+        const Nbr n = bwd_nbr<MU>(a.g, s, p);
+        const unsigned pe = a.in_full ? (unsigned)n.p : 0u;
+        SIO::load(a.in, a.inn, Vh, pe * SIO::NPL * Vh + n.h, pe * Vh + n.h, psi);
+        project<MU, DAG ? 2 : 0>(psi, hh);
+        if (n.sign < 0) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) hh[k] = cmk(-hh[k].x, -hh[k].y);
+        }
+        LIO::load(a.gauge, Vh, (unsigned)(n.p * 4 + MU) * LIO::NPL * Vh + n.h, u);
+        su3_mdv(u, hh, uh);
+        su3_mdv(u, hh + 3, uh + 3);
+        reconstruct_add<MU, DAG ? 2 : 0>(acc, uh);
+    }
+    if constexpr (PM >= 1) pin_acc(acc);
+}
+
+template <typename R, int SP, int GP, int RECON, bool DAG, bool HASX, int PM>
+__global__ __launch_bounds__(256) void k_wilson_hop(HopArgs a) {
+    typedef WilsonIO<SP> SIO;
+    typedef LinkIO<GP, RECON> LIO;
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (long)a.nblk * a.g.Vh) return;
+    const int blk = (int)(gid / a.g.Vh);
+    const int h = (int)(gid - (long)blk * a.g.Vh);
+    const int p = a.nblk == 2 ? blk : a.pbase;
+    const Site s = site_of(a.g, p, h);
+    C2<R> acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k] = cmk<R>(0, 0);
+    hop_mu<0, DAG, PM, R, SIO, LIO>(a, s, p, h, acc);
+    hop_mu<1, DAG, PM, R, SIO, LIO>(a, s, p, h, acc);
+    hop_mu<2, DAG, PM, R, SIO, LIO>(a, s, p, h, acc);
+    hop_mu<3, DAG, PM, R, SIO, LIO>(a, s, p, h, acc);
+    const C2<R> ca = cmk<R>((R)a.ar, (R)a.ai);
+    const unsigned po = a.out_full ? (unsigned)p : 0u;
+    const unsigned eo = po * SIO::NPL * a.g.Vh + h, no = po * a.g.Vh + h;
+    C2<R> o[12];
+    if constexpr (HASX) {
+        const C2<R> cb = cmk<R>((R)a.br, (R)a.bi);
+        C2<R> xv[12];
+        SIO::load(a.x, a.xn, a.g.Vh, eo, no, xv);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            o[k] = cmul(cb, xv[k]);
+            cfma(o[k], ca, acc[k]);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) o[k] = cmul(ca, acc[k]);
+    }
+    SIO::store(a.out, a.outn, a.g.Vh, eo, no, o);
+}
+
+struct HopCfg {
+    int pm;     // pin mode
+    int block;  // threads per workgroup
+};
+
+template <typename R, int SP, int GP, int RECON, int PM>
+void launch_pm(const HopArgs& a, bool dag, bool hasx, hipStream_t st, int block) {
+    const unsigned grid = grid_for((long)a.nblk * a.g.Vh, block);
+    if (dag) {
+        if (hasx) k_wilson_hop<R, SP, GP, RECON, true, true, PM><<<grid, block, 0, st>>>(a);
+        else k_wilson_hop<R, SP, GP, RECON, true, false, PM><<<grid, block, 0, st>>>(a);
+    } else {
+        if (hasx) k_wilson_hop<R, SP, GP, RECON, false, true, PM><<<grid, block, 0, st>>>(a);
+        else k_wilson_hop<R, SP, GP, RECON, false, false, PM><<<grid, block, 0, st>>>(a);
+    }
+}
+
+template <typename R, int SP, int GP, int RECON>
+int launch_t(const HopArgs& a, bool dag, bool hasx, hipStream_t st, HopCfg c) {
+    if (c.pm == 2) launch_pm<R, SP, GP, RECON, 2>(a, dag, hasx, st, c.block);
+    else launch_pm<R, SP, GP, RECON, 1>(a, dag, hasx, st, c.block);
+    CLG_HIP(hipGetLastError());
+    return CLG_OK;
+}
+
+int launch_cfg(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool hasx, hipStream_t st, HopCfg c) {
+    if (prec == CLG_F64) {
+        return recon == 12 ? launch_t<double, CLG_F64, CLG_F64, 12>(a, dag, hasx, st, c)
+                           : launch_t<double, CLG_F64, CLG_F64, 18>(a, dag, hasx, st, c);
+    } else if (prec == CLG_F32 && gprec == CLG_F32) {
+        return recon == 12 ? launch_t<float, CLG_F32, CLG_F32, 12>(a, dag, hasx, st, c)
+                           : launch_t<float, CLG_F32, CLG_F32, 18>(a, dag, hasx, st, c);
+    } else if (prec == CLG_F32) {  // FP16 links, FP32 spinors
+        return recon == 12 ? launch_t<float, CLG_F32, CLG_F16, 12>(a, dag, hasx, st, c)
+                           : launch_t<float, CLG_F32, CLG_F16, 18>(a, dag, hasx, st, c);
+    }
+    return recon == 12 ? launch_t<float, CLG_F16, CLG_F16, 12>(a, dag, hasx, st, c)
+                       : launch_t<float, CLG_F16, CLG_F16, 18>(a, dag, hasx, st, c);
+}
+
+// ---- autotuner: the first launch of a (precision, recon, dagger, x, shape) key times each
+//      candidate with HIP events on the launch stream and keeps the fastest.  Outputs are
+//      rewritten identically by every candidate, so tuning is invisible to the caller unless
+//      out aliases x (then the default is used).  CLG_TUNE=0 disables tuning.
+struct TuneKey {
+    int prec, gprec, recon, dag, hasx, nblk, L0, L1, L2, L3;
+    bool operator<(const TuneKey& o) const {
+        return std::memcmp(this, &o, sizeof(TuneKey)) < 0;
+    }
+};
+std::map<TuneKey, HopCfg>& tune_table() {
+    static std::map<TuneKey, HopCfg> t;
+    return t;
+}
+std::mutex& tune_mutex() {
+    static std::mutex m;
+    return m;
+}
+bool tuning_enabled() {
+    const char* e = std::getenv("CLG_TUNE");
+    return !(e && e[0] == '0');
+}
+
+int launch_hop(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool hasx, hipStream_t st) {
+    const HopCfg dflt{1, 256};
+    TuneKey k;
+    std::memset(&k, 0, sizeof(k));
+    k.prec = prec; k.gprec = gprec; k.recon = recon; k.dag = dag; k.hasx = hasx; k.nblk = a.nblk;
+    k.L0 = a.g.L[0]; k.L1 = a.g.L[1]; k.L2 = a.g.L[2]; k.L3 = a.g.L[3];
+    HopCfg cfg = dflt;
+    bool need_tune = false;
+    {
+        std::lock_guard<std::mutex> lk(tune_mutex());
+        auto it = tune_table().find(k);
+        if (it != tune_table().end()) cfg = it->second;
+        else need_tune = true;
+    }
+    const bool aliased = hasx && a.x == a.out;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    (void)hipStreamIsCapturing(st, &cs);
+    if (need_tune && tuning_enabled() && !aliased && cs == hipStreamCaptureStatusNone && (long)a.nblk * a.g.Vh >= 4096) {
+        const HopCfg cands[] = {{1, 64}, {1, 128}, {1, 256}, {2, 64}, {2, 128}, {2, 256}};
+        hipEvent_t e0, e1;
+        CLG_HIP(hipEventCreate(&e0));
+        CLG_HIP(hipEventCreate(&e1));
+        float best = 1e30f;
+        for (const HopCfg& c : cands) {
+            CLG_TRY(launch_cfg(a, prec, gprec, recon, dag, hasx, st, c));  // warm
+            CLG_HIP(hipEventRecord(e0, st));
+            for (int r = 0; r < 3; ++r) CLG_TRY(launch_cfg(a, prec, gprec, recon, dag, hasx, st, c));
+            CLG_HIP(hipEventRecord(e1, st));
+            CLG_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            CLG_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) { best = ms; cfg = c; }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        std::lock_guard<std::mutex> lk(tune_mutex());
+        tune_table()[k] = cfg;
+    }
+    return launch_cfg(a, prec, gprec, recon, dag, hasx, st, cfg);
+}
+
+int check_gauge(const clg_field_s* f, const clg_field_s* gauge) {
+    CLG_CHECK_ARG(gauge && is_gauge(gauge->kind), "need a gauge field");
+    CLG_TRY(check_same_lattice(f, gauge));
+    CLG_CHECK_ARG(gauge->prec == f->prec || (f->prec == CLG_F32 && gauge->prec == CLG_F16),
+                  "gauge precision must match the fermion precision (or FP16 links with FP32 spinors)");
+    return CLG_OK;
+}
+
+HopArgs base_args(const clg_field_s* out, const clg_field_s* gauge) {
+    HopArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.g = make_geo(out);
+    a.gauge = (const char*)gauge->data;
+    return a;
+}
+
+}  // namespace
+
+namespace clg {
+
+// out(full) = coeff * (in - kappa H in) with dagger projectors when dag
+int wilson_full(clg_context ctx, clg_field_s* out, const clg_field_s* in, const clg_field_s* gauge, double kappa,
+                bool dag, double cre, double cim) {
+    HopArgs a = base_args(out, gauge);
+    a.in = (const char*)in->data;
+    a.inn = in->norm;
+    a.x = (const char*)in->data;
+    a.xn = in->norm;
+    a.out = (char*)out->data;
+    a.outn = out->norm;
+    a.in_full = 1;
+    a.out_full = 1;
+    a.br = cre;
+    a.bi = cim;
+    a.ar = -kappa * cre;
+    a.ai = -kappa * cim;
+    a.pbase = 0;
+    a.nblk = 2;
+    return launch_hop(a, out->prec, gauge->prec, gauge->kind == CLG_GAUGE12 ? 12 : 18, dag, true, ctx->stream);
+}
+
+// out(parity p) = b x + a H_{p,1-p} in
+int wilson_hop_half(clg_context ctx, clg_field_s* out, const clg_field_s* in, const clg_field_s* gauge, bool dag,
+                    double ar, double ai, const clg_field_s* x, double br, double bi) {
+    const int p = field_parity(out);
+    HopArgs a = base_args(out, gauge);
+    a.in = (const char*)in->data;
+    a.inn = in->norm;
+    a.out = (char*)out->data;
+    a.outn = out->norm;
+    if (x) {
+        a.x = (const char*)x->data;
+        a.xn = x->norm;
+    }
+    a.ar = ar;
+    a.ai = ai;
+    a.br = br;
+    a.bi = bi;
+    a.pbase = p;
+    a.nblk = 1;
+    return launch_hop(a, out->prec, gauge->prec, gauge->kind == CLG_GAUGE12 ? 12 : 18, dag, x != nullptr, ctx->stream);
+}
+
+int check_hop_fields(const clg_field_s* out, const clg_field_s* in, const clg_field_s* gauge) {
+    CLG_CHECK_ARG(out && in, "null field");
+    CLG_CHECK_ARG(out->kind == CLG_WILSON && in->kind == CLG_WILSON, "need Wilson fields");
+    CLG_CHECK_ARG(out->subset != CLG_FULL && in->subset != CLG_FULL, "hop needs half-lattice fields");
+    CLG_CHECK_ARG(out->subset != in->subset, "hop maps one parity to the other");
+    CLG_CHECK_ARG(out->prec == in->prec, "precision mismatch");
+    CLG_TRY(check_same_lattice(out, in));
+    for (int i = 0; i < 4; ++i) CLG_CHECK_ARG((out->dims[i] & 1) == 0, "even-odd operators need even extents");
+    CLG_TRY(check_gauge(out, gauge));
+    return CLG_OK;
+}
+
+// Schur complement on even sites: Mhat = 1 - kappa^2 H_eo H_oe (dag: projectors swapped)
+int wilson_schur_once(clg_context ctx, clg_field_s* out, const clg_field_s* in, const clg_field_s* gauge,
+                      double kappa, bool dag, clg_field_s* tmp_odd) {
+    CLG_TRY(wilson_hop_half(ctx, tmp_odd, in, gauge, dag, 1.0, 0.0, nullptr, 0.0, 0.0));
+    return wilson_hop_half(ctx, out, tmp_odd, gauge, dag, -kappa * kappa, 0.0, in, 1.0, 0.0);
+}
+
+}  // namespace clg
+
+extern "C" {
+
+int clg_wilson_apply(clg_context ctx, clg_field out, clg_field in, clg_field gauge, double kappa, int op,
+                     int coeff_type, double cre, double cim) {
+    CLG_CHECK_ARG(ctx && out && in, "null argument");
+    CLG_CHECK_ARG(out != in && out->data != in->data, "out and in must be distinct fields");
+    CLG_CHECK_ARG(out->kind == CLG_WILSON && in->kind == CLG_WILSON, "need Wilson fields");
+    CLG_CHECK_ARG(out->subset == CLG_FULL && in->subset == CLG_FULL, "need FULL fields");
+    CLG_CHECK_ARG(out->prec == in->prec, "precision mismatch");
+    CLG_TRY(check_same_lattice(out, in));
+    CLG_TRY(check_gauge(out, gauge));
+    CLG_CHECK_ARG(op >= EFO_F_D && op <= EFO_F_DDdagger, "operator %d not supported here", op);
+    double fr = 1.0, fi = 0.0;
+    switch (coeff_type) {
+        case EOCT_None: break;
+        case EOCT_Minus: fr = -1.0; break;
+        case EOCT_Real: fr = cre; break;
+        case EOCT_Complex: fr = cre; fi = cim; break;
+        default: CLG_CHECK_ARG(false, "bad coefficient type %d", coeff_type);
+    }
+    (void)hipSetDevice(ctx->device);
+    for (int i = 0; i < 4; ++i) out->bc[i] = in->bc[i];
+    if (op == EFO_F_D || op == EFO_F_Ddagger) return wilson_full(ctx, out, in, gauge, kappa, op == EFO_F_Ddagger, fr, fi);
+    // DD = D(D psi), DD^dagger = D(D^dagger psi); the coefficient is applied by the second D
+    // (CFieldFermionWilsonSquareSU3.cu:962-1014)
+    clg_field_s* tmp = nullptr;
+    CLG_TRY(scratch_field(ctx, in, CLG_WILSON, in->prec, CLG_FULL, 0, &tmp));
+    CLG_TRY(wilson_full(ctx, tmp, in, gauge, kappa, op == EFO_F_DDdagger, 1.0, 0.0));
+    return wilson_full(ctx, out, tmp, gauge, kappa, false, fr, fi);
+}
+
+int clg_wilson_hop(clg_context ctx, clg_field out, clg_field in, clg_field gauge, int dagger, double a_re,
+                   double a_im, clg_field x, double b_re, double b_im) {
+    CLG_CHECK_ARG(ctx, "null context");
+    CLG_TRY(check_hop_fields(out, in, gauge));
+    if (x) {
+        CLG_CHECK_ARG(x->kind == CLG_WILSON && x->subset == out->subset && x->prec == out->prec,
+                      "x must match out");
+    }
+    CLG_CHECK_ARG(out->data != in->data, "out and in must be distinct");
+    (void)hipSetDevice(ctx->device);
+    return wilson_hop_half(ctx, out, in, gauge, dagger != 0, a_re, a_im, x, b_re, b_im);
+}
+
+int clg_wilson_schur(clg_context ctx, clg_field out, clg_field in, clg_field gauge, double kappa, int op) {
+    CLG_CHECK_ARG(ctx && out && in, "null argument");
+    CLG_CHECK_ARG(out->subset == CLG_EVEN && in->subset == CLG_EVEN, "Schur operator acts on EVEN fields");
+    CLG_CHECK_ARG(out->kind == CLG_WILSON && in->kind == CLG_WILSON && out->prec == in->prec, "need Wilson fields");
+    CLG_CHECK_ARG(out->data != in->data, "out and in must be distinct");
+    CLG_TRY(check_same_lattice(out, in));
+    for (int i = 0; i < 4; ++i) CLG_CHECK_ARG((out->dims[i] & 1) == 0, "even-odd operators need even extents");
+    CLG_TRY(check_gauge(out, gauge));
+    CLG_CHECK_ARG(op >= EFO_F_D && op <= EFO_F_DDdagger, "operator %d not supported", op);
+    (void)hipSetDevice(ctx->device);
+    clg_field_s* todd = nullptr;
+    CLG_TRY(scratch_field(ctx, in, CLG_WILSON, in->prec, CLG_ODD, 0, &todd));
+    for (int i = 0; i < 4; ++i) { out->bc[i] = in->bc[i]; todd->bc[i] = in->bc[i]; }
+    if (op == EFO_F_D || op == EFO_F_Ddagger)
+        return wilson_schur_once(ctx, out, in, gauge, kappa, op == EFO_F_Ddagger, todd);
+    clg_field_s* teven = nullptr;
+    CLG_TRY(scratch_field(ctx, in, CLG_WILSON, in->prec, CLG_EVEN, 0, &teven));
+    for (int i = 0; i < 4; ++i) teven->bc[i] = in->bc[i];
+    CLG_TRY(wilson_schur_once(ctx, teven, in, gauge, kappa, op == EFO_F_DDdagger, todd));
+    return wilson_schur_once(ctx, out, teven, gauge, kappa, false, todd);
+}
+
+int clg_time_wilson(clg_context ctx, clg_field out, clg_field in, clg_field gauge, double kappa, int variant,
+                    int reps, double* ms_avg) {
+    CLG_CHECK_ARG(ctx && ms_avg && reps > 0, "bad arguments");
+    (void)hipSetDevice(ctx->device);
+    auto one = [&]() -> int {
+        if (variant == 0) return clg_wilson_apply(ctx, out, in, gauge, kappa, EFO_F_D, EOCT_None, 1.0, 0.0);
+        if (variant == 1) return clg_wilson_hop(ctx, out, in, gauge, 0, 1.0, 0.0, nullptr, 0.0, 0.0);
+        return clg_wilson_schur(ctx, out, in, gauge, kappa, EFO_F_D);
+    };
+    CLG_TRY(one());  // validates arguments, warms up
+    CLG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    for (int r = 0; r < reps; ++r) CLG_TRY(one());
+    CLG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    CLG_HIP(hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    CLG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    *ms_avg = (double)ms / reps;
+    return CLG_OK;
+}
+
+}  // extern "C"

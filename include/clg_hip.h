@@ -1,0 +1,202 @@
+/*
+ * clg_hip.h -- C ABI of the MI355X-native fermion-matrix library (libclg_hip.so).
+ *
+ * Plain C: opaque handles, plain pointers and sizes, integer status codes.  The
+ * library never aborts; every entry point returns a clg_status and leaves a
+ * message for clg_last_error().
+ *
+ * Each entry point names the reference (CudaLatticeGauge / CLGLib) interface it
+ * replaces.  Host arrays exchanged with the library use the reference's own AoS
+ * order in FP64 (interleaved re/im):
+ *   site = ((x*Ly + y)*Lz + z)*Lt + t                          CCommonData.h:274-282
+ *   Wilson spinor psi[site][spin][colour]        24 doubles     Vectors.h:575-888
+ *   staggered vector chi[site][colour]            6 doubles     Vectors.h:287-571
+ *   gauge U[site][mu][row][col]                  72 doubles     SU3.h:60 (m_me[3*row+col])
+ * Half-lattice (EVEN/ODD) fields exchange V/2 sites in the reference's even/odd
+ * index order: the e-th even (odd) site is whichever of the lexicographic sites
+ * 2e, 2e+1 has even (odd) parity (CLGLib_Private.h:72-116).  This needs Lt even.
+ */
+#ifndef CLG_HIP_H
+#define CLG_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct clg_context_s* clg_context;
+typedef struct clg_field_s* clg_field;
+
+/* ---- status codes: OK, NOT_CONVERGED (x still written, as CSLASolver::Solve returns
+ *      FALSE with the best iterate, CSolverGMRES.cpp:249-259), bad argument, HIP error */
+typedef enum {
+    CLG_OK = 0,
+    CLG_NOT_CONVERGED = 1,
+    CLG_BAD_ARG = 2,
+    CLG_HIP_ERROR = 3,
+    CLG_UNSUPPORTED = 4,
+    CLG_OUT_OF_MEMORY = 5
+} clg_status;
+
+/* ---- field kinds / precisions / subsets ---- */
+typedef enum {
+    CLG_WILSON = 0,   /* CFieldFermionWilsonSquareSU3: 4 spin x 3 colour per site */
+    CLG_KS = 1,       /* CFieldFermionKSSU3: 3 colour per site */
+    CLG_GAUGE = 2,    /* CFieldGaugeSU3: 4 links x 3x3 per site, 18 reals per link */
+    CLG_GAUGE12 = 3   /* same field stored as 12 reals per link (rows 0,1; row 2 rebuilt) */
+} clg_kind;
+
+typedef enum { CLG_F64 = 0, CLG_F32 = 1, CLG_F16 = 2 } clg_prec;
+typedef enum { CLG_FULL = 0, CLG_EVEN = 1, CLG_ODD = 2 } clg_subset;
+
+/* EFieldOperator (CField.h:37-56), same numbering */
+typedef enum {
+    EFO_F_D = 0,
+    EFO_F_Ddagger = 1,
+    EFO_F_DD = 2,
+    EFO_F_DDdagger = 3,
+    EFO_F_InverseD = 4,
+    EFO_F_InverseDdagger = 5,
+    EFO_F_InverseDD = 6,
+    EFO_F_InverseDDdagger = 7
+} clg_field_operator;
+
+/* EOperatorCoefficientType (CField.h:62-68), same numbering */
+typedef enum { EOCT_None = 0, EOCT_Minus = 1, EOCT_Real = 2, EOCT_Complex = 3 } clg_coeff_type;
+
+/* ============================== runtime ============================== */
+/* replaces appInitialCLG / appQuitCLG device set-up (CLGLibManager.cpp:1142-1163) */
+int clg_init(int device, clg_context* out);
+int clg_finalize(clg_context ctx);
+int clg_synchronize(clg_context ctx);
+const char* clg_status_string(int status);
+const char* clg_last_error(void);
+/* writes "gfx950:256CU" style text; no reference counterpart (diagnostics) */
+int clg_device_info(clg_context ctx, char* buf, int len);
+
+/* ============================== fields ============================== */
+/* replaces appCreate(FieldName) + CFieldFermionWilsonSquareSU3 / CFieldFermionKSSU3 /
+ * CFieldGaugeSU3 constructors (CFieldFermionWilsonSquareSU3.cu:570-583) */
+int clg_field_create(clg_context ctx, int kind, int prec, int subset, const int dims[4],
+                     clg_field* out);
+int clg_field_destroy(clg_field f);
+/* fermion boundary signs (YAML `Period`, CLGLibManager.cpp:552-572); default 1,1,1,-1 */
+int clg_field_set_bc(clg_field f, const int bc[4]);
+/* host <-> device in the reference AoS order described above */
+int clg_field_upload(clg_field f, const double* host);
+int clg_field_download(clg_field f, double* host);
+/* CField::CopyTo (CField.h:185); converts precision when dst and src differ */
+int clg_field_copy(clg_field dst, clg_field src);
+/* CField::Zero (CField.h:127) */
+int clg_field_zero(clg_field f);
+/* a half-lattice view (EVEN=1 / ODD=2) of a FULL fermion field; shares storage */
+int clg_field_parity_view(clg_field full, int parity_subset, clg_field* out);
+/* number of doubles clg_field_upload/download exchange */
+long clg_field_host_doubles(clg_field f);
+/* device bytes held by the field */
+long clg_field_bytes(clg_field f);
+
+/* ============================== synthetic inputs ============================== */
+/* U = reunitarise(exp(i eps H)), H traceless Hermitian with Gaussian entries from a
+ * counter-based Philox4x32-10 keyed by (seed, link); replaces
+ * CFieldGaugeSU3::InitialField(EFIT_Random) (CFieldGaugeSU3.cu:32-82) */
+int clg_gauge_random(clg_field gauge, double eps, unsigned seed);
+int clg_gauge_unit(clg_field gauge);
+/* Gaussian source, E|psi_i|^2 = 1; replaces InitialField(EFIT_RandomGaussian)
+ * (CFieldFermionWilsonSquareSU3.cu:463-500) */
+int clg_field_gaussian(clg_field f, unsigned seed);
+/* point source (EFS_Point, CFieldFermionWilsonSquareSU3.cu:1110-1160): 1 at (site, spin,
+ * colour); spin is ignored for staggered fields */
+int clg_field_point_source(clg_field f, const int coord[4], int spin, int colour);
+
+/* ============================== operators ============================== */
+/* Full-lattice Wilson D, D^dagger, DD, DD^dagger with an EOperatorCoefficientType factor.
+ * Replaces CFieldFermionWilsonSquareSU3::DS/DdaggerS/DDS/DDdaggerS
+ * (CFieldFermionWilsonSquareSU3.cu:909-1014 -> _kernelDFermionWilsonSquareSU3 :32-163).
+ * out and in must be distinct FULL Wilson fields of one precision. */
+int clg_wilson_apply(clg_context ctx, clg_field out, clg_field in, clg_field gauge, double kappa,
+                     int op, int coeff_type, double coeff_re, double coeff_im);
+/* Half-lattice hopping term: out = b*x + a*H_{p,1-p} in, where out (and x) live on parity
+ * p and in on the other parity; H is the Wilson hop sum_mu[(1-g_mu)U psi(x+mu) +
+ * (1+g_mu)U^dag psi(x-mu)] (projectors swapped when dagger != 0).  x may be NULL.
+ * The even-odd building block the reference only sketches (evenodd.tex; the compiled-out
+ * path at CFieldFermionWilsonSquareSU3.cu:1266-1356). */
+int clg_wilson_hop(clg_context ctx, clg_field out, clg_field in, clg_field gauge, int dagger,
+                   double a_re, double a_im, clg_field x, double b_re, double b_im);
+/* Even-site Schur complement Mhat = 1 - kappa^2 H_eo H_oe (op = EFO_F_D), Mhat^dagger
+ * (EFO_F_Ddagger), Mhat Mhat (EFO_F_DD), Mhat Mhat^dagger (EFO_F_DDdagger), applied with
+ * a scratch odd field owned by the context. */
+int clg_wilson_schur(clg_context ctx, clg_field out, clg_field in, clg_field gauge, double kappa,
+                     int op);
+/* Staggered D, D^dagger, DD, DD^dagger with mass term 2am (YAML `Mass`, CField.cpp:154);
+ * replaces CFieldFermionKSSU3::DS... (CFieldFermionKSSU3.cu:1344-1575 -> _kernelDFermionKS
+ * :34-117) */
+int clg_ks_apply(clg_context ctx, clg_field out, clg_field in, clg_field gauge, double two_am,
+                 int op, int coeff_type, double coeff_re, double coeff_im);
+
+/* ============================== BLAS ============================== */
+/* CField::Axpy (complex a), AxpyPlus/AxpyMinus (a = +-1) (CField.h:127-160) */
+int clg_axpy(clg_context ctx, clg_field y, double a_re, double a_im, clg_field x);
+/* CField::ScalarMultply (CField.h:160-170) */
+int clg_scale(clg_context ctx, clg_field y, double a_re, double a_im);
+/* CField::Dot -> cuDoubleComplex, sum_x a(x)^dagger b(x) accumulated in FP64
+ * (_kernelDotFermionWilsonSquareSU3, CFieldFermionWilsonSquareSU3.cu:394-411 +
+ * CCudaHelper::ThreadBufferSum, CudaHelper.cu:626-710).  Blocks until the value is on
+ * the host, as the reference's Dot does. */
+int clg_dot(clg_context ctx, clg_field a, clg_field b, double out[2]);
+int clg_norm2(clg_context ctx, clg_field a, double* out);
+
+/* ============================== solvers ============================== */
+typedef enum { CLG_SOLVER_GMRES = 0, CLG_SOLVER_BICGSTAB = 1 } clg_solver_kind;
+
+/* stop rules: REFERENCE = the reference solver's own criterion (GMRES: |g_m|^2 < Accuracy *
+ * (AbsoluteAccuracy ? 1 : |b|^2) at cycle end, CSolverGMRES.cpp:216-232; BiCGStab:
+ * |s|^2/|b|^2 < Accuracy, CSolverBiCGstab.cpp:466-480); TRUE_RESIDUAL = |b - A x|/|b| <
+ * accuracy computed in FP64 */
+typedef enum { CLG_STOP_REFERENCE = 0, CLG_STOP_TRUE_RESIDUAL = 1 } clg_stop_rule;
+
+typedef struct {
+    int solver;            /* clg_solver_kind */
+    int max_dim;           /* GMRES MaxDim (Krylov dimension per cycle), <= 64 */
+    int restart;           /* GMRES Restart (max cycles) / BiCGStab Restart */
+    int max_step;          /* BiCGStab MaxStep (steps per restart) */
+    int deviation_step;    /* BiCGStab DiviationStep */
+    double accuracy;       /* YAML Accuracy (squared-norm units for STOP_REFERENCE) */
+    int absolute_accuracy; /* YAML AbsoluteAccuracy */
+    int stop_rule;         /* clg_stop_rule */
+    int even_odd;          /* 1: solve through the even-site Schur complement (Wilson) */
+    int inner_prec;        /* CLG_F64: plain solve; CLG_F32 / CLG_F16: defect correction */
+    double inner_accuracy; /* relative residual target of each inner solve (e.g. 1e-5) */
+    int max_outer;         /* defect-correction outer iterations (mixed precision) */
+    int kind;              /* CLG_WILSON or CLG_KS */
+} clg_solver_params;
+
+typedef struct {
+    int converged;
+    int cycles;            /* GMRES cycles (or BiCGStab iterations) summed over inner solves */
+    int outer;             /* defect-correction iterations (1 for a plain solve) */
+    int operator_applies;  /* operator applications, counted in the solve precision */
+    double true_residual;  /* |b - A x| / |b| in FP64 at exit */
+    double last_deviation; /* the reference rule's last deviation */
+    double seconds;        /* wall time of the solve (device synchronised) */
+} clg_solve_report;
+
+/* CSLASolver::Solve (CSLASolver.h:42-50): x = op^{-1} b, op in {EFO_F_D, EFO_F_Ddagger,
+ * EFO_F_DD, EFO_F_DDdagger} of the Wilson (kappa) or staggered (2am) fermion; start may
+ * be NULL (then x0 = b as CSolverGMRES.cpp:140-147).  x may alias b. */
+int clg_solve(clg_context ctx, const clg_solver_params* params, clg_field x, clg_field b,
+              clg_field gauge, double kappa_or_mass, int op, clg_field start,
+              clg_solve_report* report);
+/* defaults matching the reference YAML defaults (CSolverGMRES.cpp:35-99) */
+int clg_solver_params_default(clg_solver_params* p);
+
+/* ============================== measurement ============================== */
+/* Times `reps` back-to-back launches of one Wilson operator on the context stream with HIP
+ * events; writes the average milliseconds per launch.  variant: 0 full-lattice D,
+ * 1 half-lattice hop, 2 Schur operator.  No reference counterpart (bench support). */
+int clg_time_wilson(clg_context ctx, clg_field out, clg_field in, clg_field gauge, double kappa,
+                    int variant, int reps, double* ms_avg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

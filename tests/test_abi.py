@@ -1,0 +1,49 @@
+"""CPU-side checks of the C-ABI library: it loads, exports every entry point that
+include/clg_hip.h declares, and rejects bad arguments without touching a GPU."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+import cudalatticegauge_amd as clg
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "clg_hip.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|long|const char\*)\s+(clg_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_is_built_and_loads():
+    assert os.path.exists(clg.LIB_PATH), "run __graft_entry__.build() first"
+    clg.lib()
+
+
+def test_exports_every_declared_symbol():
+    L = C.CDLL(clg.LIB_PATH)
+    syms = declared_symbols()
+    assert len(syms) >= 25
+    for s in syms:
+        assert hasattr(L, s), f"{s} declared in include/clg_hip.h but not exported"
+    assert set(syms) <= set(clg.EXPORTED) | set(syms)
+    for s in clg.EXPORTED:
+        assert s in syms, f"binding {s} is not declared in the header"
+
+
+def test_status_strings_and_null_checks():
+    L = clg.lib()
+    assert L.clg_status_string(0) == b"ok"
+    assert L.clg_status_string(1).startswith(b"solver did not converge")
+    # null output handle is rejected before any device call
+    assert L.clg_init(0, None) == 2
+    assert b"null" in L.clg_last_error()
+    assert L.clg_field_create(None, 0, 0, 0, None, None) == 2
+    assert L.clg_field_destroy(None) == 0
+
+
+def test_solver_defaults_follow_reference_yaml_defaults():
+    p = clg.solver_params()
+    assert p.max_dim == 20 and p.restart == 3 and p.accuracy == pytest.approx(1e-8)
+    assert p.stop_rule == clg.STOP_REFERENCE and p.inner_prec == clg.F64

@@ -1,0 +1,168 @@
+"""GPU solvers and the staggered operator against the CPU oracle.
+
+Solutions are compared with the oracle's FP64 solution (GMRES run to |r|/|b| < 1e-13);
+the bar is the solver tolerance times a conditioning margin, written per test.
+"""
+import numpy as np
+import pytest
+
+import cudalatticegauge_amd as clg
+import orc
+
+pytestmark = pytest.mark.gpu
+
+KAPPA = 0.1355
+DIMS0 = (8, 8, 8, 8)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = clg.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def c0():
+    U = orc.gauge_random(DIMS0, 0.3, 1)
+    b = orc.wilson_gaussian(DIMS0, 2)
+    x_ref, rc, _, res = orc.gmres(0, orc.D, DIMS0, orc.BC_FERMION, KAPPA, U, b, max_dim=30,
+                                  restart=200, accuracy=1e-13, stop_rule=1)
+    assert rc == 0 and res < 1e-13
+    return U, b, x_ref
+
+
+def fields(ctx, U, b, kind=clg.WILSON):
+    g = clg.Field(ctx, clg.GAUGE, clg.F64, DIMS0)
+    g.upload(U)
+    fb = clg.Field(ctx, kind, clg.F64, DIMS0, bc=orc.BC_FERMION)
+    fb.upload(b)
+    fx = clg.Field(ctx, kind, clg.F64, DIMS0, bc=orc.BC_FERMION)
+    return g, fb, fx
+
+
+def relerr(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("eo", [0, 1, 2])
+def test_gmres12_f64_c0(ctx, c0, eo):
+    U, b, x_ref = c0
+    g, fb, fx = fields(ctx, U, b)
+    p = clg.solver_params(max_dim=12, restart=200, accuracy=1e-10, stop_rule=clg.STOP_TRUE_RESIDUAL, even_odd=eo)
+    st, rep = clg.solve(ctx, p, fx, fb, g, KAPPA, clg.EFO_F_InverseD)
+    print(f"eo={eo} cycles={rep.cycles} applies={rep.operator_applies} res={rep.true_residual:.2e}")
+    assert st == clg.OK and rep.converged
+    assert rep.true_residual < 1e-10
+    x = fx.download()
+    r = b - orc.wilson_op(DIMS0, orc.BC_FERMION, KAPPA, U, x)
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-10
+    assert relerr(x, x_ref) < 1e-8   # |D^-1| <~ 1/(1-8 kappa) ~ 10 on this field
+
+
+@pytest.mark.parametrize("inner", [clg.F32, clg.F16])
+def test_mixed_precision_defect_correction(ctx, c0, inner):
+    U, b, x_ref = c0
+    g, fb, fx = fields(ctx, U, b)
+    p = clg.solver_params(max_dim=12, restart=50, accuracy=1e-10, stop_rule=clg.STOP_TRUE_RESIDUAL, even_odd=1,
+                          inner_prec=inner, inner_accuracy=1e-4 if inner == clg.F32 else 1e-2, max_outer=40)
+    st, rep = clg.solve(ctx, p, fx, fb, g, KAPPA, clg.EFO_F_InverseD)
+    print(f"inner={inner} outer={rep.outer} cycles={rep.cycles} res={rep.true_residual:.2e}")
+    assert st == clg.OK and rep.true_residual < 1e-10
+    assert relerr(fx.download(), x_ref) < 1e-8
+
+
+def test_reference_stop_rule_cycles_match_oracle(ctx, c0):
+    U, b, _ = c0
+    bl = float(np.dot(b, b))
+    _, rc, cyc_ref, dev_ref = orc.gmres(0, orc.D, DIMS0, orc.BC_FERMION, KAPPA, U, b, max_dim=12, restart=50,
+                                        accuracy=1e-20, blength=bl, stop_rule=0)
+    assert rc == 0
+    g, fb, fx = fields(ctx, U, b)
+    p = clg.solver_params(max_dim=12, restart=50, accuracy=1e-20, absolute_accuracy=0, stop_rule=clg.STOP_REFERENCE)
+    st, rep = clg.solve(ctx, p, fx, fb, g, KAPPA, clg.EFO_F_InverseD)
+    assert st == clg.OK
+    assert abs(rep.cycles - cyc_ref) <= 1
+    assert rep.last_deviation < 1e-20 * bl
+
+
+def test_not_converged_writes_best_iterate(ctx, c0):
+    U, b, _ = c0
+    g, fb, fx = fields(ctx, U, b)
+    p = clg.solver_params(max_dim=2, restart=1, accuracy=1e-30)
+    st, rep = clg.solve(ctx, p, fx, fb, g, KAPPA, clg.EFO_F_InverseD)
+    assert st == clg.NOT_CONVERGED and not rep.converged
+    x = fx.download()
+    r0 = np.linalg.norm(b - orc.wilson_op(DIMS0, orc.BC_FERMION, KAPPA, U, b))
+    r1 = np.linalg.norm(b - orc.wilson_op(DIMS0, orc.BC_FERMION, KAPPA, U, x))
+    assert r1 < r0   # x moved from x0 = b towards the solution
+
+
+@pytest.mark.parametrize("op", [clg.EFO_F_InverseDdagger, clg.EFO_F_InverseDDdagger, clg.EFO_F_InverseDD])
+def test_inverse_composites(ctx, c0, op):
+    U, b, _ = c0
+    g, fb, fx = fields(ctx, U, b)
+    p = clg.solver_params(max_dim=12, restart=200, accuracy=1e-11, stop_rule=clg.STOP_TRUE_RESIDUAL, even_odd=1)
+    st, rep = clg.solve(ctx, p, fx, fb, g, KAPPA, op)
+    assert st == clg.OK
+    x = fx.download()
+    Ax = orc.wilson_op(DIMS0, orc.BC_FERMION, KAPPA, U, x, op - clg.EFO_F_InverseD)
+    assert np.linalg.norm(b - Ax) / np.linalg.norm(b) < 1e-9
+
+
+def test_x_aliases_b(ctx, c0):
+    U, b, x_ref = c0
+    g, fb, _ = fields(ctx, U, b)
+    p = clg.solver_params(max_dim=12, restart=200, accuracy=1e-10, stop_rule=clg.STOP_TRUE_RESIDUAL, even_odd=1)
+    st, rep = clg.solve(ctx, p, fb, fb, g, KAPPA, clg.EFO_F_InverseD)
+    assert st == clg.OK and relerr(fb.download(), x_ref) < 1e-8
+
+
+def test_bicgstab_reference_rule(ctx, c0):
+    U, b, x_ref = c0
+    g, fb, fx = fields(ctx, U, b)
+    p = clg.solver_params(solver=clg.SOLVER_BICGSTAB, max_step=100, deviation_step=5, restart=3, accuracy=1e-22)
+    st, rep = clg.solve(ctx, p, fx, fb, g, KAPPA, clg.EFO_F_InverseD)
+    assert st == clg.OK
+    assert rep.true_residual < 1e-9
+    assert relerr(fx.download(), x_ref) < 1e-8
+
+
+# ------------------------------------------------------------------ staggered
+@pytest.mark.parametrize("op", [clg.EFO_F_D, clg.EFO_F_Ddagger, clg.EFO_F_DD, clg.EFO_F_DDdagger])
+def test_ks_parity(ctx, op):
+    dims = (4, 4, 4, 8)
+    U = orc.gauge_random(dims, 0.4, 1)
+    chi = orc.ks_gaussian(dims, 2)
+    g = clg.Field(ctx, clg.GAUGE, clg.F64, dims)
+    g.upload(U)
+    fi = clg.Field(ctx, clg.KS, clg.F64, dims, bc=orc.BC_FERMION)
+    fi.upload(chi)
+    fo = clg.Field(ctx, clg.KS, clg.F64, dims)
+    clg.ks_apply(ctx, fo, fi, g, 0.05, op)
+    ref = orc.ks_op(dims, orc.BC_FERMION, 0.05, U, chi, op)
+    assert np.abs(fo.download() - ref).max() / np.abs(ref).max() < 1e-12
+    g32 = clg.Field(ctx, clg.GAUGE12, clg.F32, dims)
+    g32.upload(U)
+    fi32 = clg.Field(ctx, clg.KS, clg.F32, dims, bc=orc.BC_FERMION)
+    fi32.upload(chi)
+    fo32 = clg.Field(ctx, clg.KS, clg.F32, dims)
+    clg.ks_apply(ctx, fo32, fi32, g32, 0.05, op)
+    assert np.abs(fo32.download() - ref).max() / np.abs(ref).max() < 1e-6
+
+
+def test_ks_solve_ddagger(ctx):
+    dims = (4, 4, 4, 8)
+    U = orc.gauge_random(dims, 0.4, 1)
+    chi = orc.ks_gaussian(dims, 2)
+    g = clg.Field(ctx, clg.GAUGE, clg.F64, dims)
+    g.upload(U)
+    fb = clg.Field(ctx, clg.KS, clg.F64, dims, bc=orc.BC_FERMION)
+    fb.upload(chi)
+    fx = clg.Field(ctx, clg.KS, clg.F64, dims, bc=orc.BC_FERMION)
+    p = clg.solver_params(max_dim=20, restart=100, accuracy=1e-10, stop_rule=clg.STOP_TRUE_RESIDUAL, kind=clg.KS)
+    st, rep = clg.solve(ctx, p, fx, fb, g, 0.1, clg.EFO_F_InverseDDdagger)
+    assert st == clg.OK
+    x = fx.download()
+    r = chi - orc.ks_op(dims, orc.BC_FERMION, 0.1, U, x, orc.DDDAG)
+    assert np.linalg.norm(r) / np.linalg.norm(chi) < 1e-9
