@@ -42,6 +42,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="only run the FP64 D steps plus 5 in-place scalings of one FP64 Wilson field "
+                         "(a read+write pass of known size used to calibrate rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -86,6 +89,14 @@ def main():
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     gbs = BYTES_D[8] * V / (ms_per_step * 1e-3) / 1e9
     log(f"FP64 D: {ms_per_step * 1e3:.1f} us/step  {gbs:.0f} GB/s (algorithmic)")
+    if args.profile:
+        nbytes = out64.nbytes()
+        for _ in range(5):
+            clg.scale(ctx, out64, 1.0)
+        ctx.sync()
+        log(f"profile mode: 5 scalings of a {nbytes} byte field (read {nbytes} B, write {nbytes} B each)")
+        ctx.close()
+        return
 
     # kernel-only time with HIP events on the library stream (same launches)
     k64 = clg.time_wilson(ctx, out64, src64, gauge64, KAPPA, 0, args.steps)

@@ -7,6 +7,7 @@ library has not been built.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -122,6 +123,7 @@ class Context:
         h = C.c_void_p()
         check(lib().clg_init(device, C.byref(h)))
         self.h = h
+        self._fields = weakref.WeakSet()
 
     def sync(self):
         check(lib().clg_synchronize(self.h))
@@ -132,9 +134,22 @@ class Context:
         return buf.value.decode()
 
     def close(self):
+        """Frees every field of this context (views first), then the context itself."""
         if self.h:
+            fields = list(self._fields)
+            for f in fields:
+                if f._parent is not None:
+                    f.free()
+            for f in fields:
+                f.free()
             lib().clg_finalize(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def __enter__(self):
         return self
@@ -156,6 +171,7 @@ class Field:
             self.h = h
         else:
             self.h = _handle
+        ctx._fields.add(self)
         if bc is not None:
             check(lib().clg_field_set_bc(self.h, _i4(bc)))
 
@@ -203,9 +219,9 @@ class Field:
         return Field(self.ctx, self.kind, self.prec, self.dims, subset=subset, _handle=h, _parent=self)
 
     def free(self):
-        if self.h:
+        if self.h and self.ctx.h:
             lib().clg_field_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

@@ -253,6 +253,11 @@ int launch_hop(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool 
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+        const char* verbose = std::getenv("CLG_TUNE_VERBOSE");
+        if (verbose && verbose[0] == '1')
+            std::fprintf(stderr, "[clg tune] wilson hop prec=%d gprec=%d recon=%d dag=%d x=%d nblk=%d L=%dx%dx%dx%d -> pin=%d block=%d (%.1f us)\n",
+                         prec, gprec, recon, (int)dag, (int)hasx, a.nblk, a.g.L[0], a.g.L[1], a.g.L[2], a.g.L[3],
+                         cfg.pm, cfg.block, best * 1e3 / 3);
         std::lock_guard<std::mutex> lk(tune_mutex());
         tune_table()[k] = cfg;
     }

