@@ -1,12 +1,15 @@
 // clg_blas.hip -- level-1 BLAS and FP64 reductions on fermion fields.
 //
 // F64 and F32 fermion fields are, whatever their site layout, one contiguous array of
-// complex numbers (double2 / float2), so the element-wise kernels run over that flat array.
-// Reductions accumulate in FP64 per lane, reduce across the 64-lane wavefront with
-// shuffles, across the workgroup through LDS, and across workgroups with a last-block pass;
-// the result is written to device memory, so the solvers never wait on the host for a dot
-// product (the reference's Dot does ~20 launches and a blocking copy, CudaHelper.cu:626-710).
+// complex numbers, so the element-wise kernels run over that flat array in 16-byte packets
+// (one FP64 complex or two FP32 complex per lane and load).  Reductions accumulate in FP64
+// per lane, reduce across the 64-lane wavefront with shuffles, across the workgroup through
+// LDS, and across workgroups with a last-block pass; the result is written to device memory,
+// so the solvers never wait on the host for a dot product (the reference's Dot does ~20
+// launches and a blocking copy, CudaHelper.cu:626-710).
 #include "clg_internal.h"
+
+#include <algorithm>
 
 using namespace clg;
 using namespace clgd;
@@ -14,7 +17,7 @@ using namespace clgd;
 namespace {
 
 const int kRedBlock = 256;
-const int kMaxRedGrid = 1024;
+const int kMaxRedGrid = 2048;
 const int kMaxK = 16;  // vectors per multi-dot / multi-axpy launch
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -23,14 +26,12 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-__device__ __forceinline__ double load_coherent(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
-// Reduce NR per-lane values over the grid; the last workgroup to finish writes out[0..NR).
+// Reduce NR per-lane values over the grid; the last workgroup to finish writes out[0..NR)
+// and returns true on its thread 0.
 template <int NR>
-__device__ void reduce_finish(double (&v)[NR], double* partial, unsigned* counter, double* out) {
-    __shared__ double sm[kRedBlock / 64][NR > 0 ? NR : 1];
+__device__ bool reduce_finish(double (&v)[NR], double* partial, unsigned* counter, double* out) {
+    __shared__ double sm[kRedBlock / 64][NR];
     __shared__ int is_last;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
@@ -52,14 +53,16 @@ __device__ void reduce_finish(double (&v)[NR], double* partial, unsigned* counte
         is_last = (t == gridDim.x - 1);
     }
     __syncthreads();
-    if (!is_last) return;
+    if (!is_last) return false;
+    // acquire: the fence invalidates this CU's vector L1, so the plain loads below see every
+    // block's partials (each published by a release fence before its counter increment)
     __threadfence();
     double acc[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = 0;
     for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
 #pragma unroll
-        for (int r = 0; r < NR; ++r) acc[r] += load_coherent(&partial[(size_t)b * NR + r]);
+        for (int r = 0; r < NR; ++r) acc[r] += partial[(size_t)b * NR + r];
     }
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = wave_sum(acc[r]);
@@ -76,135 +79,249 @@ __device__ void reduce_finish(double (&v)[NR], double* partial, unsigned* counte
             out[r] = s;
         }
         *counter = 0u;
+        return true;
+    }
+    return false;
+}
+
+// ---- 16-byte packets: NC complex numbers per lane and load, values handled as doubles
+enum { PK_D = 0, PK_F4 = 1, PK_F2 = 2 };
+template <int P> struct Pk;
+template <> struct Pk<PK_D> {
+    typedef double2 T;
+    static constexpr int NC = 1;
+    __device__ static void ld(const T* p, long i, double* v) {
+        const double2 a = p[i];
+        v[0] = a.x; v[1] = a.y;
+    }
+    __device__ static void st(T* p, long i, double* v) { p[i] = make_double2(v[0], v[1]); }
+};
+template <> struct Pk<PK_F4> {
+    typedef float4 T;
+    static constexpr int NC = 2;
+    __device__ static void ld(const T* p, long i, double* v) {
+        const float4 a = p[i];
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    }
+    // stores the FP32-rounded value and leaves it in v (so norms see what was stored)
+    __device__ static void st(T* p, long i, double* v) {
+        const float4 a = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+        p[i] = a;
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    }
+};
+template <> struct Pk<PK_F2> {
+    typedef float2 T;
+    static constexpr int NC = 1;
+    __device__ static void ld(const T* p, long i, double* v) {
+        const float2 a = p[i];
+        v[0] = a.x; v[1] = a.y;
+    }
+    __device__ static void st(T* p, long i, double* v) {
+        const float2 a = make_float2((float)v[0], (float)v[1]);
+        p[i] = a;
+        v[0] = a.x; v[1] = a.y;
+    }
+};
+
+#define GRID_STRIDE(i, n) for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (long)gridDim.x * blockDim.x)
+// four independent packets per lane per trip, so every wave keeps four loads in flight
+template <typename F>
+__device__ __forceinline__ void stride4(long n, F&& body) {
+    const long st = (long)gridDim.x * blockDim.x;
+    for (long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += 4 * st) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const long i = i0 + u * st;
+            if (i < n) body(i);
+        }
     }
 }
 
-template <typename T> struct Elem;
-template <> struct Elem<double2> {
-    __device__ static double2 ld(const double2* p, long i) { return p[i]; }
-    __device__ static void st(double2* p, long i, double x, double y) { p[i] = make_double2(x, y); }
-};
-template <> struct Elem<float2> {
-    __device__ static double2 ld(const float2* p, long i) {
-        const float2 a = p[i];
-        return make_double2((double)a.x, (double)a.y);
-    }
-    __device__ static void st(float2* p, long i, double x, double y) { p[i] = make_float2((float)x, (float)y); }
-};
-
-// y = a x + b y   (a, b complex, host scalars)
-// READY = false: b == 0, y is only written (so stale NaN/Inf in y never leaks into the result)
-template <typename T, bool READY>
-__global__ void k_axpby(long n, double ar, double ai, const T* __restrict__ x, double br, double bi, T* y) {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const double2 xv = Elem<T>::ld(x, i);
-        const double2 yv = READY ? Elem<T>::ld(y, i) : make_double2(0.0, 0.0);
-        const double rx = ar * xv.x - ai * xv.y + br * yv.x - bi * yv.y;
-        const double ry = ar * xv.y + ai * xv.x + br * yv.y + bi * yv.x;
-        Elem<T>::st(y, i, rx, ry);
-    }
+// y = a x + b y   (a, b complex, host scalars).  READY = false: b == 0 and y is only written,
+// so stale NaN/Inf in y never leaks into the result.
+template <int P, bool READY>
+__global__ void k_axpby(long n, double ar, double ai, const typename Pk<P>::T* __restrict__ x, double br, double bi,
+                        typename Pk<P>::T* y) {
+    constexpr int NC = Pk<P>::NC;
+    stride4(n, [&](long i) {
+        double xv[2 * NC], yv[2 * NC], o[2 * NC];
+        Pk<P>::ld(x, i, xv);
+        if (READY) Pk<P>::ld(y, i, yv);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            o[2 * c] = ar * xv[2 * c] - ai * xv[2 * c + 1];
+            o[2 * c + 1] = ar * xv[2 * c + 1] + ai * xv[2 * c];
+            if (READY) {
+                o[2 * c] += br * yv[2 * c] - bi * yv[2 * c + 1];
+                o[2 * c + 1] += br * yv[2 * c + 1] + bi * yv[2 * c];
+            }
+        }
+        Pk<P>::st(y, i, o);
+    });
 }
 // y += s * c x with c read from device memory (complex), s = +-1
-template <typename T>
-__global__ void k_axpy_dev(long n, const double* __restrict__ c, double s, const T* __restrict__ x, T* y) {
-    const double ar = s * c[0], ai = s * c[1];
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const double2 xv = Elem<T>::ld(x, i), yv = Elem<T>::ld(y, i);
-        Elem<T>::st(y, i, yv.x + ar * xv.x - ai * xv.y, yv.y + ar * xv.y + ai * xv.x);
-    }
+template <int P>
+__global__ void k_axpy_dev(long n, const double* __restrict__ cc, double s, const typename Pk<P>::T* __restrict__ x,
+                           typename Pk<P>::T* y) {
+    constexpr int NC = Pk<P>::NC;
+    const double ar = s * cc[0], ai = s * cc[1];
+    stride4(n, [&](long i) {
+        double xv[2 * NC], yv[2 * NC];
+        Pk<P>::ld(x, i, xv);
+        Pk<P>::ld(y, i, yv);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            yv[2 * c] += ar * xv[2 * c] - ai * xv[2 * c + 1];
+            yv[2 * c + 1] += ar * xv[2 * c + 1] + ai * xv[2 * c];
+        }
+        Pk<P>::st(y, i, yv);
+    });
 }
 // dst = src / sqrt(nrm2[0])   (nrm2 on device; zero-safe)
-template <typename T>
-__global__ void k_scale_inv_sqrt(long n, const double* __restrict__ nrm2, const T* __restrict__ src, T* dst) {
+template <int P>
+__global__ void k_scale_inv_sqrt(long n, const double* __restrict__ nrm2, const typename Pk<P>::T* __restrict__ src,
+                                 typename Pk<P>::T* dst) {
+    constexpr int NC = Pk<P>::NC;
     const double v = nrm2[0];
     const double f = v > 0 ? 1.0 / sqrt(v) : 0.0;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const double2 a = Elem<T>::ld(src, i);
-        Elem<T>::st(dst, i, a.x * f, a.y * f);
-    }
+    stride4(n, [&](long i) {
+        double a[2 * NC];
+        Pk<P>::ld(src, i, a);
+#pragma unroll
+        for (int c = 0; c < 2 * NC; ++c) a[c] *= f;
+        Pk<P>::st(dst, i, a);
+    });
+}
+// b = conj(a)
+template <int P>
+__global__ void k_conj(long n, const typename Pk<P>::T* __restrict__ a, typename Pk<P>::T* b) {
+    constexpr int NC = Pk<P>::NC;
+    stride4(n, [&](long i) {
+        double v[2 * NC];
+        Pk<P>::ld(a, i, v);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v[2 * c + 1] = -v[2 * c + 1];
+        Pk<P>::st(b, i, v);
+    });
 }
 // out = sum conj(a) b  (2 doubles)
-template <typename T>
-__global__ __launch_bounds__(kRedBlock) void k_dot(long n, const T* __restrict__ a, const T* __restrict__ b,
-                                                    double* partial, unsigned* counter, double* out) {
+template <int P>
+__global__ __launch_bounds__(kRedBlock) void k_dot(long n, const typename Pk<P>::T* __restrict__ a,
+                                                    const typename Pk<P>::T* __restrict__ b, double* partial,
+                                                    unsigned* counter, double* out) {
+    constexpr int NC = Pk<P>::NC;
     double v[2] = {0, 0};
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const double2 x = Elem<T>::ld(a, i), y = Elem<T>::ld(b, i);
-        v[0] += x.x * y.x + x.y * y.y;
-        v[1] += x.x * y.y - x.y * y.x;
-    }
+    stride4(n, [&](long i) {
+        double x[2 * NC], y[2 * NC];
+        Pk<P>::ld(a, i, x);
+        Pk<P>::ld(b, i, y);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            v[0] += x[2 * c] * y[2 * c] + x[2 * c + 1] * y[2 * c + 1];
+            v[1] += x[2 * c] * y[2 * c + 1] - x[2 * c + 1] * y[2 * c];
+        }
+    });
     reduce_finish<2>(v, partial, counter, out);
 }
-template <typename T>
-__global__ __launch_bounds__(kRedBlock) void k_norm2(long n, const T* __restrict__ a, double* partial,
+template <int P>
+__global__ __launch_bounds__(kRedBlock) void k_norm2(long n, const typename Pk<P>::T* __restrict__ a, double* partial,
                                                       unsigned* counter, double* out) {
+    constexpr int NC = Pk<P>::NC;
     double v[1] = {0};
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const double2 x = Elem<T>::ld(a, i);
-        v[0] += x.x * x.x + x.y * x.y;
-    }
+    stride4(n, [&](long i) {
+        double x[2 * NC];
+        Pk<P>::ld(a, i, x);
+#pragma unroll
+        for (int c = 0; c < 2 * NC; ++c) v[0] += x[c] * x[c];
+    });
     reduce_finish<1>(v, partial, counter, out);
 }
 
-template <typename T> struct Ptrs {
-    const T* p[kMaxK];
+template <int P> struct Ptrs {
+    const typename Pk<P>::T* p[kMaxK];
 };
-// out[2k..2k+1] = sum conj(V_k) w  for k < K
-template <typename T, int K>
-__global__ __launch_bounds__(kRedBlock) void k_multidot(long n, Ptrs<T> V, const T* __restrict__ w, double* partial,
-                                                         unsigned* counter, double* out) {
-    double v[2 * K];
+// out[2k..2k+1] = sum conj(V_k) w  for k < K; WN: also |w|^2 -> wnorm[0].
+// cond (nullable): when *cond == 0 the whole grid exits at once (a skipped pass).
+template <int P, int K, bool WN>
+__global__ __launch_bounds__(kRedBlock) void k_multidot(long n, Ptrs<P> V, const typename Pk<P>::T* __restrict__ w,
+                                                         double* partial, unsigned* counter, double* tmp, double* out,
+                                                         double* wnorm, const int* cond) {
+    constexpr int NC = Pk<P>::NC;
+    if (cond && *cond == 0) return;
+    double v[2 * K + 1];
 #pragma unroll
-    for (int k = 0; k < 2 * K; ++k) v[k] = 0;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const double2 y = Elem<T>::ld(w, i);
+    for (int k = 0; k < 2 * K + 1; ++k) v[k] = 0;
+    GRID_STRIDE(i, n) {
+        double y[2 * NC];
+        Pk<P>::ld(w, i, y);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const double2 x = Elem<T>::ld(V.p[k], i);
-            v[2 * k] += x.x * y.x + x.y * y.y;
-            v[2 * k + 1] += x.x * y.y - x.y * y.x;
+            double x[2 * NC];
+            Pk<P>::ld(V.p[k], i, x);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                v[2 * k] += x[2 * c] * y[2 * c] + x[2 * c + 1] * y[2 * c + 1];
+                v[2 * k + 1] += x[2 * c] * y[2 * c + 1] - x[2 * c + 1] * y[2 * c];
+            }
+        }
+        if constexpr (WN) {
+#pragma unroll
+            for (int c = 0; c < 2 * NC; ++c) v[2 * K] += y[c] * y[c];
         }
     }
-    reduce_finish<2 * K>(v, partial, counter, out);
+    if (reduce_finish<2 * K + 1>(v, partial, counter, tmp)) {
+        for (int k = 0; k < 2 * K; ++k) out[k] = tmp[k];
+        if constexpr (WN) wnorm[0] = tmp[2 * K];
+    }
 }
-// w += s * sum_k c_k V_k  (c on device, 2K doubles); optionally |w|^2 -> nrm_out
-template <typename T, int K, bool NORM>
-__global__ __launch_bounds__(kRedBlock) void k_multiaxpy(long n, Ptrs<T> V, const double* __restrict__ c, double s,
-                                                          T* w, double* partial, unsigned* counter, double* nrm_out) {
+// w += s * sum_k c_k V_k  (c on device, 2K doubles); NORM: |w|^2 of the stored result ->
+// nrm_out[0] and, if flag_out != null, *flag_out = (|w|^2 < eta * ref[0]) (the Gram-Schmidt
+// reorthogonalisation test).  cond (nullable): skip the pass when *cond == 0.
+template <int P, int K, bool NORM>
+__global__ __launch_bounds__(kRedBlock) void k_multiaxpy(long n, Ptrs<P> V, const double* __restrict__ cc, double s,
+                                                          typename Pk<P>::T* w, double* partial, unsigned* counter,
+                                                          double* nrm_out, const int* cond, const double* ref,
+                                                          double eta, int* flag_out) {
+    constexpr int NC = Pk<P>::NC;
+    if (cond && *cond == 0) return;
     double cr[K], ci[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) { cr[k] = s * c[2 * k]; ci[k] = s * c[2 * k + 1]; }
+    for (int k = 0; k < K; ++k) { cr[k] = s * cc[2 * k]; ci[k] = s * cc[2 * k + 1]; }
     double v[1] = {0};
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        double2 y = Elem<T>::ld(w, i);
+    GRID_STRIDE(i, n) {
+        double y[2 * NC];
+        Pk<P>::ld(w, i, y);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const double2 x = Elem<T>::ld(V.p[k], i);
-            y.x += cr[k] * x.x - ci[k] * x.y;
-            y.y += cr[k] * x.y + ci[k] * x.x;
+            double x[2 * NC];
+            Pk<P>::ld(V.p[k], i, x);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                y[2 * c] += cr[k] * x[2 * c] - ci[k] * x[2 * c + 1];
+                y[2 * c + 1] += cr[k] * x[2 * c + 1] + ci[k] * x[2 * c];
+            }
         }
-        Elem<T>::st(w, i, y.x, y.y);
+        Pk<P>::st(w, i, y);
         if constexpr (NORM) {
-            const double2 q = Elem<T>::ld(w, i);  // norm of the stored (rounded) value
-            v[0] += q.x * q.x + q.y * q.y;
+#pragma unroll
+            for (int c = 0; c < 2 * NC; ++c) v[0] += y[c] * y[c];
         }
     }
-    if constexpr (NORM) reduce_finish<1>(v, partial, counter, nrm_out);
+    if constexpr (NORM) {
+        if (reduce_finish<1>(v, partial, counter, nrm_out) && flag_out) *flag_out = nrm_out[0] < eta * ref[0] ? 1 : 0;
+    }
 }
 
 unsigned red_grid(clg_context ctx, long n) {
     long g = (n + kRedBlock * 4 - 1) / (kRedBlock * 4);
     const long cap = std::min<long>(kMaxRedGrid, (long)ctx->num_cu * 4);
-    if (g > cap) g = cap;
-    if (g < 1) g = 1;
-    return (unsigned)g;
+    return (unsigned)std::max<long>(1, std::min(g, cap));
 }
 unsigned ew_grid(clg_context ctx, long n) {
     long g = (n + 255) / 256;
     const long cap = (long)ctx->num_cu * 16;
-    if (g > cap) g = cap;
-    if (g < 1) g = 1;
-    return (unsigned)g;
+    return (unsigned)std::max<long>(1, std::min(g, cap));
 }
 
 }  // namespace
@@ -228,6 +345,15 @@ long n_complex(const clg_field_s* f) {
     return nblk * f->Vh * (f->kind == CLG_WILSON ? 12 : 3);
 }
 
+// packet kind of a field and its packet count
+static int pk_of(const clg_field_s* f, long& npk) {
+    const long n = n_complex(f);
+    if (f->prec == CLG_F64) { npk = n; return PK_D; }
+    if ((n & 1) == 0) { npk = n / 2; return PK_F4; }
+    npk = n;
+    return PK_F2;
+}
+
 static int check_flat(const clg_field_s* a) {
     CLG_CHECK_ARG(a && is_fermion(a->kind), "need a fermion field");
     CLG_CHECK_ARG(a->prec != CLG_F16, "BLAS on FP16-storage fields is not supported; convert first");
@@ -241,84 +367,93 @@ static int check_pair(const clg_field_s* a, const clg_field_s* b) {
     return check_same_lattice(a, b);
 }
 
+#define PK_DISPATCH(pk, MACRO)                  \
+    switch (pk) {                               \
+        case PK_D: MACRO(PK_D); break;          \
+        case PK_F4: MACRO(PK_F4); break;        \
+        default: MACRO(PK_F2); break;           \
+    }
+
 int blas_axpby(clg_context ctx, clg_field_s* y, double ar, double ai, const clg_field_s* x, double br, double bi) {
     CLG_TRY(check_pair(y, x));
-    const long n = n_complex(y);
+    long n;
+    const int pk = pk_of(y, n);
     const unsigned g = ew_grid(ctx, n);
     const bool ready = !(br == 0.0 && bi == 0.0);
-    if (y->prec == CLG_F64) {
-        if (ready) k_axpby<double2, true><<<g, 256, 0, ctx->stream>>>(n, ar, ai, (const double2*)x->data, br, bi, (double2*)y->data);
-        else k_axpby<double2, false><<<g, 256, 0, ctx->stream>>>(n, ar, ai, (const double2*)x->data, br, bi, (double2*)y->data);
-    } else {
-        if (ready) k_axpby<float2, true><<<g, 256, 0, ctx->stream>>>(n, ar, ai, (const float2*)x->data, br, bi, (float2*)y->data);
-        else k_axpby<float2, false><<<g, 256, 0, ctx->stream>>>(n, ar, ai, (const float2*)x->data, br, bi, (float2*)y->data);
-    }
+#define L(P)                                                                                                       \
+    if (ready)                                                                                                     \
+        k_axpby<P, true><<<g, 256, 0, ctx->stream>>>(n, ar, ai, (const Pk<P>::T*)x->data, br, bi, (Pk<P>::T*)y->data); \
+    else                                                                                                           \
+        k_axpby<P, false><<<g, 256, 0, ctx->stream>>>(n, ar, ai, (const Pk<P>::T*)x->data, br, bi, (Pk<P>::T*)y->data);
+    PK_DISPATCH(pk, L)
+#undef L
     CLG_HIP(hipGetLastError());
     return CLG_OK;
 }
 
 int blas_axpy_dev(clg_context ctx, clg_field_s* y, const double* c, double s, const clg_field_s* x) {
     CLG_TRY(check_pair(y, x));
-    const long n = n_complex(y);
+    long n;
+    const int pk = pk_of(y, n);
     const unsigned g = ew_grid(ctx, n);
-    if (y->prec == CLG_F64)
-        k_axpy_dev<double2><<<g, 256, 0, ctx->stream>>>(n, c, s, (const double2*)x->data, (double2*)y->data);
-    else
-        k_axpy_dev<float2><<<g, 256, 0, ctx->stream>>>(n, c, s, (const float2*)x->data, (float2*)y->data);
+#define L(P) k_axpy_dev<P><<<g, 256, 0, ctx->stream>>>(n, c, s, (const Pk<P>::T*)x->data, (Pk<P>::T*)y->data);
+    PK_DISPATCH(pk, L)
+#undef L
     CLG_HIP(hipGetLastError());
     return CLG_OK;
 }
 
 int blas_scale_inv_sqrt(clg_context ctx, clg_field_s* dst, const clg_field_s* src, const double* nrm2) {
     CLG_TRY(check_pair(dst, src));
-    const long n = n_complex(dst);
+    long n;
+    const int pk = pk_of(dst, n);
     const unsigned g = ew_grid(ctx, n);
-    if (dst->prec == CLG_F64)
-        k_scale_inv_sqrt<double2><<<g, 256, 0, ctx->stream>>>(n, nrm2, (const double2*)src->data, (double2*)dst->data);
-    else
-        k_scale_inv_sqrt<float2><<<g, 256, 0, ctx->stream>>>(n, nrm2, (const float2*)src->data, (float2*)dst->data);
+#define L(P) k_scale_inv_sqrt<P><<<g, 256, 0, ctx->stream>>>(n, nrm2, (const Pk<P>::T*)src->data, (Pk<P>::T*)dst->data);
+    PK_DISPATCH(pk, L)
+#undef L
+    CLG_HIP(hipGetLastError());
+    return CLG_OK;
+}
+
+int blas_conj(clg_context ctx, clg_field_s* dst, const clg_field_s* src) {
+    CLG_TRY(check_pair(dst, src));
+    long n;
+    const int pk = pk_of(dst, n);
+    const unsigned g = ew_grid(ctx, n);
+#define L(P) k_conj<P><<<g, 256, 0, ctx->stream>>>(n, (const Pk<P>::T*)src->data, (Pk<P>::T*)dst->data);
+    PK_DISPATCH(pk, L)
+#undef L
     CLG_HIP(hipGetLastError());
     return CLG_OK;
 }
 
 int blas_dot_dev(clg_context ctx, const clg_field_s* a, const clg_field_s* b, double* out) {
     CLG_TRY(check_pair(a, b));
-    CLG_TRY(ensure_reduction_scratch(ctx, (size_t)kMaxRedGrid * 2 * kMaxK));
-    const long n = n_complex(a);
+    CLG_TRY(ensure_reduction_scratch(ctx, (size_t)kMaxRedGrid * (2 * kMaxK + 1)));
+    long n;
+    const int pk = pk_of(a, n);
     const unsigned g = red_grid(ctx, n);
-    if (a->prec == CLG_F64)
-        k_dot<double2><<<g, kRedBlock, 0, ctx->stream>>>(n, (const double2*)a->data, (const double2*)b->data,
-                                                         ctx->red_partial, ctx->red_counter, out);
-    else
-        k_dot<float2><<<g, kRedBlock, 0, ctx->stream>>>(n, (const float2*)a->data, (const float2*)b->data,
-                                                        ctx->red_partial, ctx->red_counter, out);
+#define L(P)                                                                                                   \
+    k_dot<P><<<g, kRedBlock, 0, ctx->stream>>>(n, (const Pk<P>::T*)a->data, (const Pk<P>::T*)b->data, ctx->red_partial, \
+                                               ctx->red_counter, out);
+    PK_DISPATCH(pk, L)
+#undef L
     CLG_HIP(hipGetLastError());
     return CLG_OK;
 }
 
 int blas_norm2_dev(clg_context ctx, const clg_field_s* a, double* out) {
     CLG_TRY(check_flat(a));
-    CLG_TRY(ensure_reduction_scratch(ctx, (size_t)kMaxRedGrid * 2 * kMaxK));
-    const long n = n_complex(a);
+    CLG_TRY(ensure_reduction_scratch(ctx, (size_t)kMaxRedGrid * (2 * kMaxK + 1)));
+    long n;
+    const int pk = pk_of(a, n);
     const unsigned g = red_grid(ctx, n);
-    if (a->prec == CLG_F64)
-        k_norm2<double2><<<g, kRedBlock, 0, ctx->stream>>>(n, (const double2*)a->data, ctx->red_partial,
-                                                           ctx->red_counter, out);
-    else
-        k_norm2<float2><<<g, kRedBlock, 0, ctx->stream>>>(n, (const float2*)a->data, ctx->red_partial,
-                                                          ctx->red_counter, out);
+#define L(P) \
+    k_norm2<P><<<g, kRedBlock, 0, ctx->stream>>>(n, (const Pk<P>::T*)a->data, ctx->red_partial, ctx->red_counter, out);
+    PK_DISPATCH(pk, L)
+#undef L
     CLG_HIP(hipGetLastError());
     return CLG_OK;
-}
-
-template <typename T, int K>
-static void multidot_k(clg_context ctx, long n, const Ptrs<T>& P, const T* w, double* out) {
-    k_multidot<T, K><<<red_grid(ctx, n), kRedBlock, 0, ctx->stream>>>(n, P, w, ctx->red_partial, ctx->red_counter, out);
-}
-template <typename T, bool NORM, int K>
-static void multiaxpy_k(clg_context ctx, long n, const Ptrs<T>& P, const double* c, double s, T* w, double* nrm) {
-    const unsigned g = NORM ? red_grid(ctx, n) : ew_grid(ctx, n);
-    k_multiaxpy<T, K, NORM><<<g, kRedBlock, 0, ctx->stream>>>(n, P, c, s, w, ctx->red_partial, ctx->red_counter, nrm);
 }
 
 #define SWITCH_K(K, CALL)                                         \
@@ -333,75 +468,88 @@ static void multiaxpy_k(clg_context ctx, long n, const Ptrs<T>& P, const double*
         case 15: CALL(15); break; default: CALL(16); break;       \
     }
 
-// out[2k], out[2k+1] = <V_k, w>, k < nv (chunks of 16 vectors per launch)
-int blas_multidot_dev(clg_context ctx, clg_field_s* const* V, int nv, const clg_field_s* w, double* out) {
-    CLG_CHECK_ARG(nv >= 1, "need at least one vector");
-    for (int k = 0; k < nv; ++k) CLG_TRY(check_pair(V[k], w));
-    CLG_TRY(ensure_reduction_scratch(ctx, (size_t)kMaxRedGrid * 2 * kMaxK));
-    const long n = n_complex(w);
+template <int P>
+static int multidot_p(clg_context ctx, clg_field_s* const* V, int nv, const clg_field_s* w, double* out, double* wnorm,
+                      const int* cond, long n) {
+    const unsigned g = red_grid(ctx, n);
+    double* tmp = ctx->red_result + 2048;  // staging of the 2K+1 sums
     for (int k0 = 0; k0 < nv; k0 += kMaxK) {
         const int kk = std::min(kMaxK, nv - k0);
-        if (w->prec == CLG_F64) {
-            Ptrs<double2> P;
-            for (int k = 0; k < kk; ++k) P.p[k] = (const double2*)V[k0 + k]->data;
-            const double2* wp = (const double2*)w->data;
-#define CALL_D(KK) multidot_k<double2, KK>(ctx, n, P, wp, out + 2 * k0)
-            SWITCH_K(kk, CALL_D)
-#undef CALL_D
+        Ptrs<P> Pp;
+        for (int k = 0; k < kk; ++k) Pp.p[k] = (const typename Pk<P>::T*)V[k0 + k]->data;
+        const typename Pk<P>::T* wp = (const typename Pk<P>::T*)w->data;
+        double* wn = k0 == 0 ? wnorm : nullptr;
+        double* o = out + 2 * k0;
+#define CALL(KK)                                                                                                   \
+    if (wn)                                                                                                        \
+        k_multidot<P, KK, true><<<g, kRedBlock, 0, ctx->stream>>>(n, Pp, wp, ctx->red_partial, ctx->red_counter, tmp, o, \
+                                                                  wn, cond);                                       \
+    else                                                                                                           \
+        k_multidot<P, KK, false><<<g, kRedBlock, 0, ctx->stream>>>(n, Pp, wp, ctx->red_partial, ctx->red_counter, tmp,  \
+                                                                   o, nullptr, cond);
+        SWITCH_K(kk, CALL)
+#undef CALL
+        CLG_HIP(hipGetLastError());
+    }
+    return CLG_OK;
+}
+
+// out[2k], out[2k+1] = <V_k, w>, k < nv (chunks of 16 vectors per launch); wnorm (nullable)
+// receives |w|^2; cond (nullable, device int) skips the work when zero.
+int blas_multidot_dev(clg_context ctx, clg_field_s* const* V, int nv, const clg_field_s* w, double* out, double* wnorm,
+                      const int* cond) {
+    CLG_CHECK_ARG(nv >= 1, "need at least one vector");
+    for (int k = 0; k < nv; ++k) CLG_TRY(check_pair(V[k], w));
+    CLG_TRY(ensure_reduction_scratch(ctx, (size_t)kMaxRedGrid * (2 * kMaxK + 1)));
+    long n;
+    const int pk = pk_of(w, n);
+    if (pk == PK_D) return multidot_p<PK_D>(ctx, V, nv, w, out, wnorm, cond, n);
+    if (pk == PK_F4) return multidot_p<PK_F4>(ctx, V, nv, w, out, wnorm, cond, n);
+    return multidot_p<PK_F2>(ctx, V, nv, w, out, wnorm, cond, n);
+}
+
+template <int P>
+static int multiaxpy_p(clg_context ctx, clg_field_s* w, clg_field_s* const* V, int nv, const double* c, double s,
+                       double* nrm, const int* cond, const double* ref, double eta, int* flag, long n) {
+    for (int k0 = 0; k0 < nv; k0 += kMaxK) {
+        const int kk = std::min(kMaxK, nv - k0);
+        const bool want = k0 + kk >= nv && nrm != nullptr;
+        Ptrs<P> Pp;
+        for (int k = 0; k < kk; ++k) Pp.p[k] = (const typename Pk<P>::T*)V[k0 + k]->data;
+        typename Pk<P>::T* wp = (typename Pk<P>::T*)w->data;
+        const double* ck = c + 2 * k0;
+        if (want) {
+            const unsigned g = red_grid(ctx, n);
+#define CALL(KK)                                                                                                       \
+    k_multiaxpy<P, KK, true><<<g, kRedBlock, 0, ctx->stream>>>(n, Pp, ck, s, wp, ctx->red_partial, ctx->red_counter, nrm, \
+                                                               cond, ref, eta, flag);
+            SWITCH_K(kk, CALL)
+#undef CALL
         } else {
-            Ptrs<float2> P;
-            for (int k = 0; k < kk; ++k) P.p[k] = (const float2*)V[k0 + k]->data;
-            const float2* wp = (const float2*)w->data;
-#define CALL_F(KK) multidot_k<float2, KK>(ctx, n, P, wp, out + 2 * k0)
-            SWITCH_K(kk, CALL_F)
-#undef CALL_F
+            const unsigned g = ew_grid(ctx, n);
+#define CALL(KK)                                                                                                 \
+    k_multiaxpy<P, KK, false><<<g, kRedBlock, 0, ctx->stream>>>(n, Pp, ck, s, wp, ctx->red_partial, ctx->red_counter, \
+                                                                nullptr, cond, nullptr, 0.0, nullptr);
+            SWITCH_K(kk, CALL)
+#undef CALL
         }
         CLG_HIP(hipGetLastError());
     }
     return CLG_OK;
 }
 
-// w += s * sum_k c_k V_k ; if nrm != null, |w|^2 of the result -> nrm[0]
+// w += s * sum_k c_k V_k ; if nrm != null, |w|^2 of the result -> nrm[0] and, if flag != null,
+// *flag = (|w|^2 < eta * ref[0]).  cond (nullable) skips the work when *cond == 0.
 int blas_multiaxpy_dev(clg_context ctx, clg_field_s* w, clg_field_s* const* V, int nv, const double* c, double s,
-                       double* nrm) {
+                       double* nrm, const int* cond, const double* ref, double eta, int* flag) {
     CLG_CHECK_ARG(nv >= 1, "need at least one vector");
     for (int k = 0; k < nv; ++k) CLG_TRY(check_pair(V[k], w));
-    CLG_TRY(ensure_reduction_scratch(ctx, (size_t)kMaxRedGrid * 2 * kMaxK));
-    const long n = n_complex(w);
-    for (int k0 = 0; k0 < nv; k0 += kMaxK) {
-        const int kk = std::min(kMaxK, nv - k0);
-        const bool last = k0 + kk >= nv;
-        const bool want = last && nrm != nullptr;
-        if (w->prec == CLG_F64) {
-            Ptrs<double2> P;
-            for (int k = 0; k < kk; ++k) P.p[k] = (const double2*)V[k0 + k]->data;
-            double2* wp = (double2*)w->data;
-            if (want) {
-#define CALL_DN(KK) multiaxpy_k<double2, true, KK>(ctx, n, P, c + 2 * k0, s, wp, nrm)
-                SWITCH_K(kk, CALL_DN)
-#undef CALL_DN
-            } else {
-#define CALL_D0(KK) multiaxpy_k<double2, false, KK>(ctx, n, P, c + 2 * k0, s, wp, nullptr)
-                SWITCH_K(kk, CALL_D0)
-#undef CALL_D0
-            }
-        } else {
-            Ptrs<float2> P;
-            for (int k = 0; k < kk; ++k) P.p[k] = (const float2*)V[k0 + k]->data;
-            float2* wp = (float2*)w->data;
-            if (want) {
-#define CALL_FN(KK) multiaxpy_k<float2, true, KK>(ctx, n, P, c + 2 * k0, s, wp, nrm)
-                SWITCH_K(kk, CALL_FN)
-#undef CALL_FN
-            } else {
-#define CALL_F0(KK) multiaxpy_k<float2, false, KK>(ctx, n, P, c + 2 * k0, s, wp, nullptr)
-                SWITCH_K(kk, CALL_F0)
-#undef CALL_F0
-            }
-        }
-        CLG_HIP(hipGetLastError());
-    }
-    return CLG_OK;
+    CLG_TRY(ensure_reduction_scratch(ctx, (size_t)kMaxRedGrid * (2 * kMaxK + 1)));
+    long n;
+    const int pk = pk_of(w, n);
+    if (pk == PK_D) return multiaxpy_p<PK_D>(ctx, w, V, nv, c, s, nrm, cond, ref, eta, flag, n);
+    if (pk == PK_F4) return multiaxpy_p<PK_F4>(ctx, w, V, nv, c, s, nrm, cond, ref, eta, flag, n);
+    return multiaxpy_p<PK_F2>(ctx, w, V, nv, c, s, nrm, cond, ref, eta, flag, n);
 }
 
 // copies `count` doubles from device to the context's pinned buffer and waits
@@ -410,6 +558,23 @@ int fetch_doubles(clg_context ctx, const double* dev, int count, double* host) {
     CLG_HIP(hipMemcpyAsync(ctx->host_result, dev, sizeof(double) * count, hipMemcpyDeviceToHost, ctx->stream));
     CLG_HIP(hipStreamSynchronize(ctx->stream));
     for (int i = 0; i < count; ++i) host[i] = ctx->host_result[i];
+    return CLG_OK;
+}
+
+// persistent device workspace of the context (grow-only, zero-filled on growth)
+int workspace(clg_context ctx, size_t doubles, double** out) {
+    if (ctx->ws_cap < doubles) {
+        if (ctx->ws) {
+            CLG_HIP(hipStreamSynchronize(ctx->stream));
+            (void)hipFree(ctx->ws);
+            ctx->ws = nullptr;
+            ctx->ws_cap = 0;
+        }
+        CLG_HIP(hipMalloc((void**)&ctx->ws, doubles * sizeof(double)));
+        CLG_HIP(hipMemsetAsync(ctx->ws, 0, doubles * sizeof(double), ctx->stream));
+        ctx->ws_cap = doubles;
+    }
+    *out = ctx->ws;
     return CLG_OK;
 }
 

@@ -76,6 +76,9 @@ struct clg_context_s {
     unsigned* red_counter = nullptr;
     double* red_result = nullptr;  // 256 doubles of device-resident results
     double* host_result = nullptr;  // pinned
+    // solver workspace (Hessenberg, Givens, scalars), grow-only
+    double* ws = nullptr;
+    size_t ws_cap = 0;
     // scratch fields keyed by shape (owned)
     std::vector<clg_field_s*> scratch;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -137,6 +140,12 @@ int scratch_field(clg_context ctx, const clg_field_s* like, int kind, int prec, 
 
 // launch helpers
 inline unsigned grid_for(long n, int block) { return (unsigned)((n + block - 1) / block); }
+// grid for clgd::map_site: nblk parity blocks of Vh sites, optional parity interleave and XCD padding
+inline unsigned sched_grid(long Vh, int nblk, int block, int xcd, int pinter) {
+    long g = (nblk == 2 && pinter) ? 2 * ((Vh + block - 1) / block) : (nblk * Vh + block - 1) / block;
+    if (xcd > 1) g = (g + xcd - 1) / xcd * xcd;
+    return (unsigned)g;
+}
 
 }  // namespace clg
 
@@ -233,6 +242,32 @@ __device__ __forceinline__ Nbr bwd_nbr(const clg::Geo& g, const Site& s, int p) 
     }
     n.h = lex >> 1;
     return n;
+}
+
+// ------------------------------------------------------------------ work decomposition
+// Workgroups are handed to the 8 XCDs round-robin (wg -> XCD wg % 8), and each XCD has its
+// own L2.  With xcd = 8 the hardware id is remapped so that XCD k sweeps the contiguous
+// logical range [k*n/8, (k+1)*n/8): neighbouring sites, whose spinors and links are shared,
+// then meet in one L2.  With pinter = 1 a two-parity launch alternates parities block by
+// block, so a link's forward user (parity p) and backward user (parity 1-p) are co-resident.
+// The grid must be padded to a multiple of xcd (sched_grid) for the remap to be a bijection.
+__device__ __forceinline__ bool map_site(int Vh, int nblk, int pbase, int xcd, int pinter, int& p, int& h) {
+    unsigned wg = blockIdx.x;
+    if (xcd > 1) {
+        const unsigned per = gridDim.x / (unsigned)xcd;
+        wg = (wg % (unsigned)xcd) * per + wg / (unsigned)xcd;
+    }
+    if (nblk == 2 && pinter) {
+        p = (int)(wg & 1u);
+        h = (int)((wg >> 1) * blockDim.x + threadIdx.x);
+        return h < Vh;
+    }
+    const long gid = (long)wg * blockDim.x + threadIdx.x;
+    if (gid >= (long)nblk * Vh) return false;
+    const int blk = (int)(gid / Vh);
+    h = (int)(gid - (long)blk * Vh);
+    p = nblk == 2 ? blk : pbase;
+    return true;
 }
 
 // ------------------------------------------------------------------ storage traits

@@ -455,6 +455,7 @@ int clg_finalize(clg_context c) {
     if (c->red_counter) (void)hipFree(c->red_counter);
     if (c->red_result) (void)hipFree(c->red_result);
     if (c->host_result) (void)hipHostFree(c->host_result);
+    if (c->ws) (void)hipFree(c->ws);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     (void)hipStreamDestroy(c->stream);

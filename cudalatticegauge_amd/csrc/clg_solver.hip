@@ -8,9 +8,10 @@
 //   * BiCGStab: rho~ = conj(r0) (Saad), check |s|^2/|b|^2 every DiviationStep steps, restart
 //     from the last x on breakdown.
 // What differs, by design: the Arnoldi step orthogonalises with classical Gram-Schmidt
-// applied twice (two fused multi-dot / multi-axpy passes) instead of modified Gram-Schmidt,
-// and every Hessenberg entry, Givens rotation and step length stays in FP64 device memory;
-// the host reads one scalar per cycle.
+// (one multi-dot pass that also yields |w|^2, one multi-axpy pass that yields |w'|^2), and a
+// second pass runs only when the device-side test |w'|^2 < |w|^2/2 asks for it, instead of
+// modified Gram-Schmidt's j+1 dependent dot/axpy pairs.  Every Hessenberg entry, Givens
+// rotation and step length stays in FP64 device memory; the host reads one scalar per cycle.
 #include "clg_internal.h"
 
 #include <chrono>
@@ -27,9 +28,13 @@ int blas_axpy_dev(clg_context, clg_field_s*, const double*, double, const clg_fi
 int blas_scale_inv_sqrt(clg_context, clg_field_s*, const clg_field_s*, const double*);
 int blas_dot_dev(clg_context, const clg_field_s*, const clg_field_s*, double*);
 int blas_norm2_dev(clg_context, const clg_field_s*, double*);
-int blas_multidot_dev(clg_context, clg_field_s* const*, int, const clg_field_s*, double*);
-int blas_multiaxpy_dev(clg_context, clg_field_s*, clg_field_s* const*, int, const double*, double, double*);
+int blas_multidot_dev(clg_context, clg_field_s* const*, int, const clg_field_s*, double*, double*, const int*);
+int blas_multiaxpy_dev(clg_context, clg_field_s*, clg_field_s* const*, int, const double*, double, double*, const int*,
+                       const double*, double, int*);
 int fetch_doubles(clg_context, const double*, int, double*);
+int workspace(clg_context, size_t, double**);
+int blas_conj(clg_context, clg_field_s*, const clg_field_s*);
+int wilson_full_hop(clg_context, clg_field_s*, const clg_field_s*, const clg_field_s*, double, bool);
 int wilson_full(clg_context, clg_field_s*, const clg_field_s*, const clg_field_s*, double, bool, double, double);
 int wilson_hop_half(clg_context, clg_field_s*, const clg_field_s*, const clg_field_s*, bool, double, double,
                     const clg_field_s*, double, double);
@@ -42,15 +47,27 @@ namespace {
 
 // ------------------------------------------------------------------ small device kernels
 // Hessenberg column j (stored column-major, ld = m+1): add the second Gram-Schmidt pass
-// and set the sub-diagonal to |w|.
-__global__ void k_hess_update(double* H, int ld, int j, const double* c2, const double* wn2) {
+// and set the sub-diagonal to |w|.  shifted: the Arnoldi step ran on K = 1 - A, whose
+// Krylov space is A's; then column j of A's Hessenberg is e_j - (column j of K's).
+__global__ void k_hess_update(double* H, int ld, int j, const double* c2, const double* wn2, const int* reorth,
+                              int shifted) {
     const int i = threadIdx.x;
     if (i <= j) {
-        H[2 * (j * ld + i)] += c2[2 * i];
-        H[2 * (j * ld + i) + 1] += c2[2 * i + 1];
+        double re = H[2 * (j * ld + i)], im = H[2 * (j * ld + i) + 1];
+        if (*reorth) {
+            re += c2[2 * i];
+            im += c2[2 * i + 1];
+        }
+        if (shifted) {
+            re = (i == j ? 1.0 : 0.0) - re;
+            im = -im;
+        }
+        H[2 * (j * ld + i)] = re;
+        H[2 * (j * ld + i) + 1] = im;
     }
     if (i == 0) {
-        H[2 * (j * ld + j + 1)] = sqrt(wn2[0]);
+        const double s = sqrt(wn2[0]);
+        H[2 * (j * ld + j + 1)] = shifted ? -s : s;
         H[2 * (j * ld + j + 1) + 1] = 0.0;
     }
 }
@@ -133,13 +150,6 @@ __global__ void k_bicg_p(long n, const double* __restrict__ s, const T* __restri
         p[i].y = r[i].y + (br * qy + bi * qx);
     }
 }
-template <typename T>
-__global__ void k_conj(long n, const T* __restrict__ a, T* b) {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        b[i].x = a[i].x;
-        b[i].y = -a[i].y;
-    }
-}
 
 // ------------------------------------------------------------------ operators
 struct LinOp {
@@ -184,18 +194,18 @@ struct LinOp {
         CLG_TRY(ks_full(ctx, tmp, in, gauge, param, op == EFO_F_DDdagger, 1.0, 0.0));
         return ks_full(ctx, out, tmp, gauge, param, false, 1.0, 0.0);
     }
-};
 
-// workspace of device doubles owned by one solve
-struct DevBuf {
-    double* p = nullptr;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
-    int alloc(size_t n) {
-        CLG_HIP(hipMalloc((void**)&p, n * sizeof(double)));
-        CLG_HIP(hipMemset(p, 0, n * sizeof(double)));
-        return CLG_OK;
+    // Wilson D and the even-site Schur complement are 1 - K with K = kappa H or kappa^2 H_eo H_oe.
+    bool shiftable() const { return kind == CLG_WILSON && (op == EFO_F_D || op == EFO_F_Ddagger); }
+    // out = K in = in - A in, without forming the difference
+    int apply_k(clg_field_s* out, const clg_field_s* in) {
+        ++applies;
+        const bool dag = op == EFO_F_Ddagger;
+        if (!schur) return wilson_full_hop(ctx, out, in, gauge, param, dag);
+        clg_field_s* todd = nullptr;
+        CLG_TRY(scratch_field(ctx, in, CLG_WILSON, in->prec, CLG_ODD, 1, &todd));
+        CLG_TRY(wilson_hop_half(ctx, todd, in, gauge, dag, 1.0, 0.0, nullptr, 0.0, 0.0));
+        return wilson_hop_half(ctx, out, todd, gauge, dag, param * param, 0.0, nullptr, 0.0, 0.0);
     }
 };
 
@@ -219,15 +229,17 @@ int gmres(clg_context ctx, LinOp& A, clg_field_s* x, const clg_field_s* b, int m
     clg_field_s* w = V[m];
     const int ld = m + 1;
     // device workspace: H, c2, g, y, scalars
-    DevBuf ws;
     const size_t offH = 0, offC = offH + 2 * (size_t)ld * m, offG = offC + 2 * (size_t)m,
                  offY = offG + 2 * (size_t)(m + 1), offS = offY + 2 * (size_t)m;
-    CLG_TRY(ws.alloc(offS + 16));
-    double* H = ws.p + offH;
-    double* c2 = ws.p + offC;
-    double* g = ws.p + offG;
-    double* y = ws.p + offY;
-    double* sc = ws.p + offS;  // sc[0] |r|^2, sc[1] |w|^2, sc[2..3] deviation
+    double* wsp = nullptr;
+    CLG_TRY(workspace(ctx, offS + 16, &wsp));
+    double* H = wsp + offH;
+    double* c2 = wsp + offC;
+    double* g = wsp + offG;
+    double* y = wsp + offY;
+    double* sc = wsp + offS;  // sc[0] |r|^2, sc[1] |w|^2, sc[2..3] deviation, sc[4] |w|^2 before GS
+    int* reorth = reinterpret_cast<int*>(wsp + offS + 8);
+    const bool shifted = A.shiftable();
     out = GmresOut();
     for (int cyc = 0; cyc < max_cycles; ++cyc) {
         // r = b - A x
@@ -248,18 +260,21 @@ int gmres(clg_context ctx, LinOp& A, clg_field_s* x, const clg_field_s* b, int m
         CLG_TRY(blas_scale_inv_sqrt(ctx, V[0], V[0], sc));
         ++out.cycles;
         for (int j = 0; j < m; ++j) {
-            CLG_TRY(A.apply(w, V[j]));
+            if (shifted) CLG_TRY(A.apply_k(w, V[j]));
+            else CLG_TRY(A.apply(w, V[j]));
             double* Hj = H + 2 * (size_t)j * ld;
-            CLG_TRY(blas_multidot_dev(ctx, V.data(), j + 1, w, Hj));
-            CLG_TRY(blas_multiaxpy_dev(ctx, w, V.data(), j + 1, Hj, -1.0, nullptr));
-            CLG_TRY(blas_multidot_dev(ctx, V.data(), j + 1, w, c2));
-            CLG_TRY(blas_multiaxpy_dev(ctx, w, V.data(), j + 1, c2, -1.0, sc + 1));
-            k_hess_update<<<1, 64, 0, ctx->stream>>>(H, ld, j, c2, sc + 1);
+            // classical Gram-Schmidt, repeated only when the first pass cancelled more than half
+            // of |w|^2 (|w'|^2 < |w|^2 / 2, the Daniel-Gragg-Kaufman-Stewart test)
+            CLG_TRY(blas_multidot_dev(ctx, V.data(), j + 1, w, Hj, sc + 4, nullptr));
+            CLG_TRY(blas_multiaxpy_dev(ctx, w, V.data(), j + 1, Hj, -1.0, sc + 1, nullptr, sc + 4, 0.5, reorth));
+            CLG_TRY(blas_multidot_dev(ctx, V.data(), j + 1, w, c2, nullptr, reorth));
+            CLG_TRY(blas_multiaxpy_dev(ctx, w, V.data(), j + 1, c2, -1.0, sc + 1, reorth, nullptr, 0.0, nullptr));
+            k_hess_update<<<1, 64, 0, ctx->stream>>>(H, ld, j, c2, sc + 1, reorth, shifted ? 1 : 0);
             if (j < m - 1) CLG_TRY(blas_scale_inv_sqrt(ctx, V[j + 1], w, sc + 1));
         }
         k_gmres_lsq<<<1, 64, 0, ctx->stream>>>(H, ld, m, sc, g, y, sc + 2);
         CLG_HIP(hipGetLastError());
-        CLG_TRY(blas_multiaxpy_dev(ctx, x, V.data(), m, y, 1.0, nullptr));
+        CLG_TRY(blas_multiaxpy_dev(ctx, x, V.data(), m, y, 1.0, nullptr, nullptr, nullptr, 0.0, nullptr));
         x_is_zero = false;
         double dev[2];
         CLG_TRY(fetch_doubles(ctx, sc + 2, 2, dev));
@@ -291,9 +306,8 @@ int bicgstab(clg_context ctx, LinOp& A, clg_field_s* x, const clg_field_s* b, in
     CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 43, &Rh));
     CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 44, &S));
     CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 45, &T));
-    DevBuf ws;
-    CLG_TRY(ws.alloc(32));
-    double* s = ws.p;
+    double* s = nullptr;
+    CLG_TRY(workspace(ctx, 32, &s));
     const long n = (b->subset == CLG_FULL ? 2 : 1) * b->Vh * (b->kind == CLG_WILSON ? 12 : 3);
     const unsigned g = (unsigned)std::min<long>((n + 255) / 256, (long)ctx->num_cu * 16);
     iters = 0;
@@ -304,8 +318,7 @@ int bicgstab(clg_context ctx, LinOp& A, clg_field_s* x, const clg_field_s* b, in
         CLG_TRY(A.apply(T, R));
         CLG_TRY(blas_axpby(ctx, R, 1.0, 0.0, b, 0.0, 0.0));
         CLG_TRY(blas_axpby(ctx, R, -1.0, 0.0, T, 1.0, 0.0));
-        if (b->prec == CLG_F64) k_conj<double2><<<g, 256, 0, ctx->stream>>>(n, (const double2*)R->data, (double2*)Rh->data);
-        else k_conj<float2><<<g, 256, 0, ctx->stream>>>(n, (const float2*)R->data, (float2*)Rh->data);
+        CLG_TRY(blas_conj(ctx, Rh, R));
         CLG_TRY(blas_axpby(ctx, P, 1.0, 0.0, R, 0.0, 0.0));
         bool restart = false;
         for (int j = 0; j < step_count * dev_check; ++j) {

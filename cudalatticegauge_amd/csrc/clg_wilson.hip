@@ -78,6 +78,7 @@ struct HopArgs {
     int in_full, out_full;  // 1 if the field holds both parity blocks
     double ar, ai, br, bi;
     int pbase, nblk;     // output parities handled
+    int xcd, pinter;     // work decomposition (clgd::map_site)
 };
 
 template <int MU, bool DAG, int PM, typename R, class SIO, class LIO>
@@ -122,11 +123,8 @@ template <typename R, int SP, int GP, int RECON, bool DAG, bool HASX, int PM>
 __global__ __launch_bounds__(256) void k_wilson_hop(HopArgs a) {
     typedef WilsonIO<SP> SIO;
     typedef LinkIO<GP, RECON> LIO;
-    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= (long)a.nblk * a.g.Vh) return;
-    const int blk = (int)(gid / a.g.Vh);
-    const int h = (int)(gid - (long)blk * a.g.Vh);
-    const int p = a.nblk == 2 ? blk : a.pbase;
+    int p, h;
+    if (!map_site(a.g.Vh, a.nblk, a.pbase, a.xcd, a.pinter, p, h)) return;
     const Site s = site_of(a.g, p, h);
     C2<R> acc[12];
 #pragma unroll
@@ -158,11 +156,15 @@ __global__ __launch_bounds__(256) void k_wilson_hop(HopArgs a) {
 struct HopCfg {
     int pm;     // pin mode
     int block;  // threads per workgroup
+    int xcd;    // 0: hardware order, 8: XCD-contiguous slabs (+ parity interleave for full D)
 };
 
 template <typename R, int SP, int GP, int RECON, int PM>
-void launch_pm(const HopArgs& a, bool dag, bool hasx, hipStream_t st, int block) {
-    const unsigned grid = grid_for((long)a.nblk * a.g.Vh, block);
+void launch_pm(const HopArgs& a0, bool dag, bool hasx, hipStream_t st, int block, int xcd) {
+    HopArgs a = a0;
+    a.xcd = xcd;
+    a.pinter = xcd > 1 ? 1 : 0;
+    const unsigned grid = sched_grid(a.g.Vh, a.nblk, block, a.xcd, a.pinter);
     if (dag) {
         if (hasx) k_wilson_hop<R, SP, GP, RECON, true, true, PM><<<grid, block, 0, st>>>(a);
         else k_wilson_hop<R, SP, GP, RECON, true, false, PM><<<grid, block, 0, st>>>(a);
@@ -174,8 +176,8 @@ void launch_pm(const HopArgs& a, bool dag, bool hasx, hipStream_t st, int block)
 
 template <typename R, int SP, int GP, int RECON>
 int launch_t(const HopArgs& a, bool dag, bool hasx, hipStream_t st, HopCfg c) {
-    if (c.pm == 2) launch_pm<R, SP, GP, RECON, 2>(a, dag, hasx, st, c.block);
-    else launch_pm<R, SP, GP, RECON, 1>(a, dag, hasx, st, c.block);
+    if (c.pm == 2) launch_pm<R, SP, GP, RECON, 2>(a, dag, hasx, st, c.block, c.xcd);
+    else launch_pm<R, SP, GP, RECON, 1>(a, dag, hasx, st, c.block, c.xcd);
     CLG_HIP(hipGetLastError());
     return CLG_OK;
 }
@@ -219,7 +221,7 @@ bool tuning_enabled() {
 }
 
 int launch_hop(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool hasx, hipStream_t st) {
-    const HopCfg dflt{1, 256};
+    const HopCfg dflt{1, 256, 8};
     TuneKey k;
     std::memset(&k, 0, sizeof(k));
     k.prec = prec; k.gprec = gprec; k.recon = recon; k.dag = dag; k.hasx = hasx; k.nblk = a.nblk;
@@ -236,7 +238,8 @@ int launch_hop(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool 
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     (void)hipStreamIsCapturing(st, &cs);
     if (need_tune && tuning_enabled() && !aliased && cs == hipStreamCaptureStatusNone && (long)a.nblk * a.g.Vh >= 4096) {
-        const HopCfg cands[] = {{1, 64}, {1, 128}, {1, 256}, {2, 64}, {2, 128}, {2, 256}};
+        const HopCfg cands[] = {{1, 64, 0}, {1, 128, 0}, {1, 256, 0}, {2, 64, 0}, {2, 128, 0}, {2, 256, 0},
+                                {1, 64, 8}, {1, 128, 8}, {1, 256, 8}, {2, 64, 8}, {2, 128, 8}, {2, 256, 8}};
         hipEvent_t e0, e1;
         CLG_HIP(hipEventCreate(&e0));
         CLG_HIP(hipEventCreate(&e1));
@@ -255,9 +258,9 @@ int launch_hop(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool 
         (void)hipEventDestroy(e1);
         const char* verbose = std::getenv("CLG_TUNE_VERBOSE");
         if (verbose && verbose[0] == '1')
-            std::fprintf(stderr, "[clg tune] wilson hop prec=%d gprec=%d recon=%d dag=%d x=%d nblk=%d L=%dx%dx%dx%d -> pin=%d block=%d (%.1f us)\n",
+            std::fprintf(stderr, "[clg tune] wilson hop prec=%d gprec=%d recon=%d dag=%d x=%d nblk=%d L=%dx%dx%dx%d -> pin=%d block=%d xcd=%d (%.1f us)\n",
                          prec, gprec, recon, (int)dag, (int)hasx, a.nblk, a.g.L[0], a.g.L[1], a.g.L[2], a.g.L[3],
-                         cfg.pm, cfg.block, best * 1e3 / 3);
+                         cfg.pm, cfg.block, cfg.xcd, best * 1e3 / 3);
         std::lock_guard<std::mutex> lk(tune_mutex());
         tune_table()[k] = cfg;
     }
@@ -303,6 +306,22 @@ int wilson_full(clg_context ctx, clg_field_s* out, const clg_field_s* in, const 
     a.pbase = 0;
     a.nblk = 2;
     return launch_hop(a, out->prec, gauge->prec, gauge->kind == CLG_GAUGE12 ? 12 : 18, dag, true, ctx->stream);
+}
+
+// out(full) = a H in (no diagonal term): the K of D = 1 - kappa H when a = kappa
+int wilson_full_hop(clg_context ctx, clg_field_s* out, const clg_field_s* in, const clg_field_s* gauge, double a_re,
+                    bool dag) {
+    HopArgs a = base_args(out, gauge);
+    a.in = (const char*)in->data;
+    a.inn = in->norm;
+    a.out = (char*)out->data;
+    a.outn = out->norm;
+    a.in_full = 1;
+    a.out_full = 1;
+    a.ar = a_re;
+    a.pbase = 0;
+    a.nblk = 2;
+    return launch_hop(a, out->prec, gauge->prec, gauge->kind == CLG_GAUGE12 ? 12 : 18, dag, false, ctx->stream);
 }
 
 // out(parity p) = b x + a H_{p,1-p} in

@@ -129,7 +129,7 @@ def test_bicgstab_reference_rule(ctx, c0):
     p = clg.solver_params(solver=clg.SOLVER_BICGSTAB, max_step=100, deviation_step=5, restart=3, accuracy=1e-16)
     st, rep = clg.solve(ctx, p, fx, fb, g, KAPPA, clg.EFO_F_InverseD)
     assert st == clg.OK
-    assert abs(rep.cycles - it_o) <= 5          # same algorithm, rounding differs
+    assert abs(rep.cycles - it_o) <= 10         # same algorithm; reduction order differs
     assert rep.true_residual < 1e-7
     assert relerr(fx.download(), x_ref) < 1e-6
     assert relerr(fx.download(), x_o) < 1e-6
