@@ -22,6 +22,7 @@ EFO_F_D, EFO_F_Ddagger, EFO_F_DD, EFO_F_DDdagger = 0, 1, 2, 3
 EFO_F_InverseD, EFO_F_InverseDdagger, EFO_F_InverseDD, EFO_F_InverseDDdagger = 4, 5, 6, 7
 EOCT_None, EOCT_Minus, EOCT_Real, EOCT_Complex = 0, 1, 2, 3
 SOLVER_GMRES, SOLVER_BICGSTAB = 0, 1
+MS_FOM, MS_BICGSTAB = 0, 1
 STOP_REFERENCE, STOP_TRUE_RESIDUAL = 0, 1
 OK, NOT_CONVERGED = 0, 1
 
@@ -33,6 +34,7 @@ EXPORTED = [
     "clg_gauge_unit", "clg_field_gaussian", "clg_field_point_source", "clg_wilson_apply",
     "clg_wilson_hop", "clg_wilson_schur", "clg_ks_apply", "clg_axpy", "clg_scale", "clg_dot",
     "clg_norm2", "clg_solve", "clg_solver_params_default", "clg_time_wilson",
+    "clg_multishift_solve", "clg_rational_apply",
 ]
 
 
@@ -98,6 +100,10 @@ def lib():
                                     C.POINTER(SolveReport)]),
             "clg_solver_params_default": (C.c_int, [C.POINTER(SolverParams)]),
             "clg_time_wilson": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, dp]),
+            "clg_multishift_solve": (C.c_int, [vp, C.POINTER(SolverParams), C.c_int, C.c_int, C.POINTER(vp), dp,
+                                               C.c_int, vp, vp, C.c_double, C.c_int, C.POINTER(SolveReport)]),
+            "clg_rational_apply": (C.c_int, [vp, C.POINTER(SolverParams), C.c_int, C.c_int, vp, vp, vp, C.c_double,
+                                             C.c_int, dp, C.c_int, C.POINTER(SolveReport)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -292,3 +298,28 @@ def time_wilson(ctx, out, inp, gauge, kappa, variant, reps):
     ms = C.c_double()
     check(lib().clg_time_wilson(ctx.h, out.h, inp.h, gauge.h, kappa, variant, reps, C.byref(ms)))
     return ms.value
+
+
+def multishift_solve(ctx, params, xs, shifts, b, gauge, kappa_or_mass, op, kind=MS_FOM, prec=F64):
+    """xs[k] = (A + shifts[k])^-1 b for all k (A = op of the params.kind fermion)."""
+    sh = np.ascontiguousarray([[complex(s).real, complex(s).imag] for s in shifts], dtype=np.float64).ravel()
+    arr = (C.c_void_p * len(xs))(*[x.h.value if isinstance(x.h, C.c_void_p) else x.h for x in xs])
+    rep = SolveReport()
+    st = lib().clg_multishift_solve(ctx.h, C.byref(params), kind, prec, arr,
+                                    sh.ctypes.data_as(C.POINTER(C.c_double)), len(xs), b.h, gauge.h,
+                                    kappa_or_mass, op, C.byref(rep))
+    check(st, allow=(OK, NOT_CONVERGED))
+    return st, rep
+
+
+def rational_apply(ctx, params, out, inp, gauge, kappa_or_mass, op, coeffs, kind=MS_FOM, prec=F64):
+    """out = c inp + sum_k a_k (A + b_k)^-1 inp with coeffs = [c, a_1..a_n, b_1..b_n]."""
+    cf = np.ascontiguousarray(coeffs, dtype=np.float64)
+    degree = (cf.size - 1) // 2
+    if cf.size != 2 * degree + 1:
+        raise CLGError("coefficients must be [c, a_1..a_n, b_1..b_n]")
+    rep = SolveReport()
+    st = lib().clg_rational_apply(ctx.h, C.byref(params), kind, prec, out.h, inp.h, gauge.h, kappa_or_mass, op,
+                                  cf.ctypes.data_as(C.POINTER(C.c_double)), degree, C.byref(rep))
+    check(st, allow=(OK, NOT_CONVERGED))
+    return st, rep

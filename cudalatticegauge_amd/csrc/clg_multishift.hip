@@ -1,0 +1,606 @@
+// clg_multishift.hip -- multi-shift Krylov solvers (x_n = (A + c_n)^-1 b for all shifts at
+// once) and the rational-approximation driver of RHMC.
+//
+// Reference algorithms kept:
+//   * CMultiShiftFOM::Solve (CMultiShiftFOM.cpp:97-270): one Arnoldi basis per restart shared
+//     by all shifts, per-shift FOM solve (H + c_n) y = beta0_n e1, restart from v_m with
+//     beta0_n <- -h_{m,m-1} y_{m-1}; stop when max_n |beta0_n| < Accuracy * (1 or |b|^2).
+//   * CMultiShiftBiCGStab::Solve (CMultiShiftBiCGStab.cpp:73-270): shifted BiCGStab with the
+//     zeta / beta / alpha / chi / rho recurrences per shift, checked every DiviationStep steps:
+//     max(|s|, |s_n|) < Accuracy * (1 or |b|).
+//   * CFieldFermion::RationalApproximation (CField.cpp:114-148): out = c in + sum_i a_i x_i with
+//     x_i = (A + b_i)^-1 in and coefficients [c, a_1..a_n, b_1..b_n] (CRationalApproximation.h).
+// What differs: the per-shift vector updates of a step run as one batched kernel over all
+// active shifts (the shared vectors are read once per step, not once per shift), converged
+// shifts are compacted out of the launches, the FOM small systems are solved on device, and
+// the shift sums accumulate in FP64 whatever the storage precision of the solutions.
+#include "clg_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+using namespace clg;
+using namespace clgd;
+
+namespace clg {
+int blas_axpby(clg_context, clg_field_s*, double, double, const clg_field_s*, double, double);
+int blas_dot_dev(clg_context, const clg_field_s*, const clg_field_s*, double*);
+int blas_norm2_dev(clg_context, const clg_field_s*, double*);
+int blas_conj(clg_context, clg_field_s*, const clg_field_s*);
+int blas_multidot_dev(clg_context, clg_field_s* const*, int, const clg_field_s*, double*, double*, const int*);
+int blas_multiaxpy_dev(clg_context, clg_field_s*, clg_field_s* const*, int, const double*, double, double*, const int*,
+                       const double*, double, int*);
+int blas_scale_inv_sqrt(clg_context, clg_field_s*, const clg_field_s*, const double*);
+int fetch_doubles(clg_context, const double*, int, double*);
+int workspace(clg_context, size_t, double**);
+long n_complex(const clg_field_s*);
+int linop_apply(clg_context ctx, int kind, int op, const clg_field_s* gauge, double param, clg_field_s* out,
+                const clg_field_s* in);
+int gauge_convert(clg_context, clg_field_s*, const clg_field_s*);
+}  // namespace clg
+
+namespace {
+
+const int kMaxShift = 32;
+typedef C2<double> cx;
+
+// ---- FP64 arithmetic over 16-byte packets of F64 or F32 storage
+template <int P> struct MS;
+template <> struct MS<CLG_F64> {
+    typedef double2 T;
+    static constexpr int NC = 1;
+    __device__ static void ld(const T* p, long i, double* v) { const double2 a = p[i]; v[0] = a.x; v[1] = a.y; }
+    __device__ static void st(T* p, long i, const double* v) { p[i] = make_double2(v[0], v[1]); }
+};
+template <> struct MS<CLG_F32> {
+    typedef float4 T;
+    static constexpr int NC = 2;
+    __device__ static void ld(const T* p, long i, double* v) {
+        const float4 a = p[i];
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    }
+    __device__ static void st(T* p, long i, const double* v) {
+        p[i] = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+    }
+};
+
+struct ShiftPtrs {
+    void* x[kMaxShift];
+    void* s[kMaxShift];
+};
+struct VPtrs {
+    const void* v[64];
+};
+template <int NC>
+__device__ __forceinline__ void cmad(double* acc, const double* c, const double* v) {
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        acc[2 * q] += c[0] * v[2 * q] - c[1] * v[2 * q + 1];
+        acc[2 * q + 1] += c[0] * v[2 * q + 1] + c[1] * v[2 * q];
+    }
+}
+
+// shifted BiCGStab, first update of a step, for every active shift k:
+//   x_k += -beta_k s_k + (chi rho zeta)_k w ;  s_k += -(chi rho zeta / beta)_k w + (chi rho zeta_old / beta)_k r
+// coef[8k .. 8k+7] = {-beta, crz, -crz/beta, crzo/beta} as complex pairs (from the old s_k)
+template <int P>
+__global__ void k_msbicg_update1(long n, int ns, ShiftPtrs sp, const double* __restrict__ coef,
+                                 const typename MS<P>::T* __restrict__ w, const typename MS<P>::T* __restrict__ r) {
+    constexpr int NC = MS<P>::NC;
+    __shared__ double cs[8 * kMaxShift];
+    for (int t = threadIdx.x; t < 8 * ns; t += blockDim.x) cs[t] = coef[t];
+    __syncthreads();
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        double wv[2 * NC], rv[2 * NC];
+        MS<P>::ld(w, i, wv);
+        MS<P>::ld(r, i, rv);
+        for (int k = 0; k < ns; ++k) {
+            const double* c = cs + 8 * k;
+            typename MS<P>::T* xp = (typename MS<P>::T*)sp.x[k];
+            typename MS<P>::T* spp = (typename MS<P>::T*)sp.s[k];
+            double xv[2 * NC], sv[2 * NC], sn[2 * NC];
+            MS<P>::ld(xp, i, xv);
+            MS<P>::ld(spp, i, sv);
+            cmad<NC>(xv, c + 0, sv);
+            cmad<NC>(xv, c + 2, wv);
+#pragma unroll
+            for (int q = 0; q < 2 * NC; ++q) sn[q] = sv[q];
+            cmad<NC>(sn, c + 4, wv);
+            cmad<NC>(sn, c + 6, rv);
+            MS<P>::st(xp, i, xv);
+            MS<P>::st(spp, i, sn);
+        }
+    }
+}
+// second update: s_k = alpha_k s_k + (zeta rho)_k r ; coef[4k .. 4k+3]
+template <int P>
+__global__ void k_msbicg_update2(long n, int ns, ShiftPtrs sp, const double* __restrict__ coef,
+                                 const typename MS<P>::T* __restrict__ r) {
+    constexpr int NC = MS<P>::NC;
+    __shared__ double cs[4 * kMaxShift];
+    for (int t = threadIdx.x; t < 4 * ns; t += blockDim.x) cs[t] = coef[t];
+    __syncthreads();
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        double rv[2 * NC];
+        MS<P>::ld(r, i, rv);
+        for (int k = 0; k < ns; ++k) {
+            typename MS<P>::T* spp = (typename MS<P>::T*)sp.s[k];
+            double sv[2 * NC], o[2 * NC];
+            MS<P>::ld(spp, i, sv);
+#pragma unroll
+            for (int q = 0; q < 2 * NC; ++q) o[q] = 0;
+            cmad<NC>(o, cs + 4 * k, sv);
+            cmad<NC>(o, cs + 4 * k + 2, rv);
+            MS<P>::st(spp, i, o);
+        }
+    }
+}
+// MS-FOM: x_k += sum_j y[k][j] V_j for every active shift k (y[k*2m + 2j])
+template <int P>
+__global__ void k_msfom_update(long n, int ns, int m, ShiftPtrs sp, VPtrs vp, const double* __restrict__ y) {
+    constexpr int NC = MS<P>::NC;
+    extern __shared__ double ys[];
+    for (int t = threadIdx.x; t < ns * 2 * m; t += blockDim.x) ys[t] = y[t];
+    __syncthreads();
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        for (int k = 0; k < ns; ++k) {
+            double acc[2 * NC];
+            MS<P>::ld((const typename MS<P>::T*)sp.x[k], i, acc);
+            for (int j = 0; j < m; ++j) {
+                double v[2 * NC];
+                MS<P>::ld((const typename MS<P>::T*)vp.v[j], i, v);
+                cmad<NC>(acc, ys + (size_t)k * 2 * m + 2 * j, v);
+            }
+            MS<P>::st((typename MS<P>::T*)sp.x[k], i, acc);
+        }
+    }
+}
+// Hessenberg column j: second Gram-Schmidt pass and the sub-diagonal |w|
+__global__ void k_ms_hess(double* H, int ld, int j, const double* c2, const double* wn2, const int* reorth) {
+    const int i = threadIdx.x;
+    if (i <= j && *reorth) {
+        H[2 * (j * ld + i)] += c2[2 * i];
+        H[2 * (j * ld + i) + 1] += c2[2 * i + 1];
+    }
+    if (i == 0) {
+        H[2 * (j * ld + j + 1)] = sqrt(wn2[0]);
+        H[2 * (j * ld + j + 1) + 1] = 0.0;
+    }
+}
+// per-shift FOM solve (CMultiShiftFOM RotateHSolveY, CMultiShiftFOM.cpp:272-330): thread k copies
+// H (m x m, column-major, ld = m+1) into its scratch, adds c_k on the diagonal, applies m-1
+// Givens rotations to g = beta0_k e1, back-substitutes, then beta0_k <- -h_{m,m-1} y_{m-1}.
+__global__ void k_msfom_lsq(int ns, int m, const double* __restrict__ H, const double* __restrict__ shifts,
+                            double* Hc_all, double* g_all, double* y, double* beta0, double* errs) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ns) return;
+    const int ld = m + 1;
+    cx* Hc = reinterpret_cast<cx*>(Hc_all + (size_t)k * 2 * m * m);
+    cx* G = reinterpret_cast<cx*>(g_all + (size_t)k * 2 * m);
+    cx* Y = reinterpret_cast<cx*>(y + (size_t)k * 2 * m);
+    const cx* Hs = reinterpret_cast<const cx*>(H);
+    auto h = [&](int r, int col) -> cx& { return Hc[col * m + r]; };
+    for (int col = 0; col < m; ++col)
+        for (int r = 0; r < m; ++r) h(r, col) = Hs[col * ld + r];
+    for (int d = 0; d < m; ++d) h(d, d) = cadd(h(d, d), cx{shifts[2 * k], shifts[2 * k + 1]});
+    for (int d = 0; d < m; ++d) G[d] = cx{0.0, 0.0};
+    G[0] = cx{beta0[2 * k], beta0[2 * k + 1]};
+    for (int i = 0; i < m - 1; ++i) {
+        const cx hii = h(i, i), hi1 = h(i + 1, i);
+        const double den = 1.0 / sqrt(hii.x * hii.x + hii.y * hii.y + hi1.x * hi1.x + hi1.y * hi1.y);
+        const cx c = cscale(hii, den), s = cscale(hi1, den);
+        const cx ch = cconj(c), sh = cconj(s);
+        for (int j = i; j < m; ++j) {
+            const cx hij = h(i, j), h1j = h(i + 1, j);
+            h(i, j) = cadd(cmul(ch, hij), cmul(sh, h1j));
+            h(i + 1, j) = csub(cmul(c, h1j), cmul(s, hij));
+        }
+        const cx mg = cx{-G[i].x, -G[i].y};
+        G[i] = cmul(ch, G[i]);
+        G[i + 1] = cmul(s, mg);
+    }
+    for (int i = m - 1; i >= 0; --i) {
+        cx acc = G[i];
+        for (int j = i + 1; j < m; ++j) acc = csub(acc, cmul(h(i, j), Y[j]));
+        const cx d = h(i, i);
+        const double dd = d.x * d.x + d.y * d.y;
+        Y[i] = cx{(acc.x * d.x + acc.y * d.y) / dd, (acc.y * d.x - acc.x * d.y) / dd};
+    }
+    const double hm = Hs[(m - 1) * ld + m].x;  // h_{m, m-1}, real and positive
+    beta0[2 * k] = -hm * Y[m - 1].x;
+    beta0[2 * k + 1] = -hm * Y[m - 1].y;
+    errs[k] = sqrt(beta0[2 * k] * beta0[2 * k] + beta0[2 * k + 1] * beta0[2 * k + 1]);
+}
+
+unsigned ew_grid(clg_context ctx, long n) {
+    long g = (n + 255) / 256;
+    return (unsigned)std::max<long>(1, std::min<long>(g, (long)ctx->num_cu * 8));
+}
+// 16-byte packets of a fermion field (F64: one complex; F32: two complex, so n must be even)
+long n_packets(const clg_field_s* f) {
+    const long n = n_complex(f);
+    return f->prec == CLG_F64 ? n : n / 2;
+}
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+inline cx cm(cx a, cx b) { return cx{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+inline cx cdv(cx a, cx b) {
+    const double d = b.x * b.x + b.y * b.y;
+    return cx{(a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d};
+}
+inline cx csb(cx a, cx b) { return cx{a.x - b.x, a.y - b.y}; }
+inline cx cad(cx a, cx b) { return cx{a.x + b.x, a.y + b.y}; }
+
+struct OpRef {
+    int kind, op;
+    const clg_field_s* gauge;
+    double param;
+    int applies = 0;
+    int apply(clg_context ctx, clg_field_s* out, const clg_field_s* in) {
+        ++applies;
+        return linop_apply(ctx, kind, op, gauge, param, out, in);
+    }
+};
+
+// ------------------------------------------------------------------ MS-FOM
+int msfom(clg_context ctx, OpRef& A, std::vector<clg_field_s*>& X, const std::vector<cx>& shifts,
+          const clg_field_s* b, int m, int restart, double accuracy, double blen, int& cycles, double& maxerr) {
+    const int ns = (int)shifts.size();
+    CLG_CHECK_ARG(m >= 2 && m <= 63, "MS-FOM MaxDim must be in [2, 63]");
+    // V[0..m-1] basis, V[m] = w, vnext = v_m seeding the next cycle
+    std::vector<clg_field_s*> V(m + 1);
+    for (int k = 0; k <= m; ++k) CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 60 + k, &V[k]));
+    clg_field_s* vnext = nullptr;
+    CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 59, &vnext));
+    clg_field_s* w = V[m];
+    const int ld = m + 1;
+    const size_t oH = 0, oHc = oH + 2 * (size_t)ld * m, oG = oHc + 2 * (size_t)m * m * ns, oY = oG + 2 * (size_t)m * ns,
+                 oB = oY + 2 * (size_t)m * ns, oE = oB + 2 * (size_t)ns, oSh = oE + ns, oS = oSh + 2 * (size_t)ns,
+                 oC = oS + 16, tot = oC + 2 * (size_t)m;
+    double* ws = nullptr;
+    CLG_TRY(workspace(ctx, tot, &ws));
+    double* H = ws + oH;
+    double* sc = ws + oS;
+    int* reorth = reinterpret_cast<int*>(ws + oS + 8);
+    std::vector<double> hb(2 * ns), hsh(2 * ns);
+    for (int k = 0; k < ns; ++k) { hsh[2 * k] = shifts[k].x; hsh[2 * k + 1] = shifts[k].y; }
+    for (auto x : X) CLG_TRY(clg_field_zero(x));
+    std::vector<int> active(ns, 1);
+    cycles = 0;
+    maxerr = 0;
+    // v0 = b / |b|, beta0_k = |b|
+    CLG_TRY(blas_norm2_dev(ctx, b, sc));
+    CLG_TRY(blas_scale_inv_sqrt(ctx, V[0], b, sc));
+    double bn2;
+    CLG_TRY(fetch_doubles(ctx, sc, 1, &bn2));
+    for (int k = 0; k < ns; ++k) { hb[2 * k] = std::sqrt(bn2); hb[2 * k + 1] = 0.0; }
+    for (int cyc = 0; cyc < restart; ++cyc) {
+        ++cycles;
+        for (int j = 0; j < m; ++j) {
+            CLG_TRY(A.apply(ctx, w, V[j]));
+            double* Hj = H + 2 * (size_t)j * ld;
+            CLG_TRY(blas_multidot_dev(ctx, V.data(), j + 1, w, Hj, sc + 4, nullptr));
+            CLG_TRY(blas_multiaxpy_dev(ctx, w, V.data(), j + 1, Hj, -1.0, sc + 1, nullptr, sc + 4, 0.5, reorth));
+            CLG_TRY(blas_multidot_dev(ctx, V.data(), j + 1, w, ws + oC, nullptr, reorth));
+            CLG_TRY(blas_multiaxpy_dev(ctx, w, V.data(), j + 1, ws + oC, -1.0, sc + 1, reorth, nullptr, 0.0, nullptr));
+            k_ms_hess<<<1, 64, 0, ctx->stream>>>(H, ld, j, ws + oC, sc + 1, reorth);
+            CLG_TRY(blas_scale_inv_sqrt(ctx, j < m - 1 ? V[j + 1] : vnext, w, sc + 1));
+        }
+        std::vector<int> idx;
+        for (int k = 0; k < ns; ++k)
+            if (active[k]) idx.push_back(k);
+        const int na = (int)idx.size();
+        if (na == 0) break;
+        std::vector<double> bsh(2 * na), bb(2 * na);
+        for (int q = 0; q < na; ++q) {
+            bsh[2 * q] = hsh[2 * idx[q]]; bsh[2 * q + 1] = hsh[2 * idx[q] + 1];
+            bb[2 * q] = hb[2 * idx[q]]; bb[2 * q + 1] = hb[2 * idx[q] + 1];
+        }
+        CLG_HIP(hipMemcpyAsync(ws + oSh, bsh.data(), sizeof(double) * 2 * na, hipMemcpyHostToDevice, ctx->stream));
+        CLG_HIP(hipMemcpyAsync(ws + oB, bb.data(), sizeof(double) * 2 * na, hipMemcpyHostToDevice, ctx->stream));
+        k_msfom_lsq<<<(na + 63) / 64, 64, 0, ctx->stream>>>(na, m, H, ws + oSh, ws + oHc, ws + oG, ws + oY, ws + oB,
+                                                             ws + oE);
+        CLG_HIP(hipGetLastError());
+        ShiftPtrs sp;
+        VPtrs vp;
+        for (int q = 0; q < na; ++q) { sp.x[q] = X[idx[q]]->data; sp.s[q] = nullptr; }
+        for (int j = 0; j < m; ++j) vp.v[j] = V[j]->data;
+        const long np = n_packets(b);
+        const unsigned g = ew_grid(ctx, np);
+        const size_t shm = sizeof(double) * 2 * (size_t)m * na;
+        if (b->prec == CLG_F64) k_msfom_update<CLG_F64><<<g, 256, shm, ctx->stream>>>(np, na, m, sp, vp, ws + oY);
+        else k_msfom_update<CLG_F32><<<g, 256, shm, ctx->stream>>>(np, na, m, sp, vp, ws + oY);
+        CLG_HIP(hipGetLastError());
+        std::vector<double> e(na), bnew(2 * na);
+        CLG_TRY(fetch_doubles(ctx, ws + oE, na, e.data()));
+        CLG_TRY(fetch_doubles(ctx, ws + oB, 2 * na, bnew.data()));
+        maxerr = 0;
+        for (int q = 0; q < na; ++q) {
+            const int k = idx[q];
+            hb[2 * k] = bnew[2 * q];
+            hb[2 * k + 1] = bnew[2 * q + 1];
+            maxerr = std::max(maxerr, e[q]);
+            if (e[q] < accuracy * blen) active[k] = 0;
+        }
+        std::swap(V[0], vnext);  // v_m seeds the next cycle
+        if (maxerr < accuracy * blen) return CLG_OK;
+    }
+    return CLG_NOT_CONVERGED;
+}
+
+// ------------------------------------------------------------------ shifted BiCGStab
+int msbicgstab(clg_context ctx, OpRef& A, std::vector<clg_field_s*>& X, const std::vector<cx>& cn,
+               const clg_field_s* b, int step_count, int dev_check, double accuracy, double blen, int& iters,
+               double& maxerr) {
+    const int ns = (int)cn.size();
+    std::vector<clg_field_s*> S(ns);
+    for (int k = 0; k < ns; ++k) CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 130 + k, &S[k]));
+    clg_field_s *R, *W, *W0, *Sv, *SA, *WA;
+    CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 120, &R));
+    CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 121, &W));
+    CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 122, &W0));
+    CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 123, &Sv));
+    CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 124, &SA));
+    CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 125, &WA));
+    double* ws = nullptr;
+    CLG_TRY(workspace(ctx, 64 + 8 * (size_t)kMaxShift, &ws));
+    double* d = ws;              // dot results
+    double* coef = ws + 32;      // per-shift coefficients
+    std::vector<cx> betas(ns, cx{1, 0}), zeta(ns, cx{1, 0}), zetaold(ns, cx{1, 0}), rho(ns, cx{1, 0}), chis(ns),
+        alphas(ns);
+    std::vector<double> sl(ns, 1.0);
+    for (int k = 0; k < ns; ++k) {
+        CLG_TRY(clg_field_zero(X[k]));
+        CLG_TRY(blas_axpby(ctx, S[k], 1.0, 0.0, b, 0.0, 0.0));
+    }
+    CLG_TRY(blas_axpby(ctx, R, 1.0, 0.0, b, 0.0, 0.0));
+    CLG_TRY(blas_axpby(ctx, Sv, 1.0, 0.0, b, 0.0, 0.0));
+    CLG_TRY(blas_conj(ctx, W0, b));  // w0 = b^dagger (conjugated field)
+    auto fetch_c = [&](int off, cx& z) -> int {
+        double t[2];
+        CLG_TRY(fetch_doubles(ctx, d + off, 2, t));
+        z = cx{t[0], t[1]};
+        return CLG_OK;
+    };
+    cx delta, phi, tmp;
+    CLG_TRY(blas_dot_dev(ctx, W0, R, d + 0));
+    CLG_TRY(fetch_c(0, delta));
+    CLG_TRY(A.apply(ctx, SA, Sv));
+    CLG_TRY(blas_dot_dev(ctx, W0, SA, d + 2));
+    CLG_TRY(fetch_c(2, tmp));
+    phi = cdv(tmp, delta);
+    cx beta{0, 0}, alpha{0, 0};
+    const long np = n_packets(b);
+    const unsigned g = ew_grid(ctx, np);
+    iters = 0;
+    maxerr = 0;
+    bool done = false;
+    std::vector<double> hc(8 * kMaxShift);
+    for (int i = 0; i < step_count * dev_check; ++i) {
+        ++iters;
+        const cx newbeta = cdv(cx{-1, 0}, phi);
+        for (int k = 0; k < ns; ++k) {
+            if (sl[k] < accuracy * blen) continue;
+            if (i == 0) {
+                zeta[k] = cdv(cx{1, 0}, csb(cx{1, 0}, cm(cn[k], newbeta)));
+                betas[k] = cm(newbeta, zeta[k]);
+            } else {
+                const cx zeta_nm1 = zetaold[k];
+                zetaold[k] = zeta[k];
+                const cx zb = cm(zeta_nm1, beta);
+                zeta[k] = cdv(cm(zetaold[k], zb), cad(cm(alpha, cm(newbeta, csb(zeta_nm1, zetaold[k]))),
+                                                      cm(zb, csb(cx{1, 0}, cm(cn[k], newbeta)))));
+                betas[k] = cdv(cm(newbeta, zeta[k]), zetaold[k]);
+            }
+        }
+        beta = newbeta;
+        // w = r + beta sa ; wa = A w ; chi = (wa . w) / (wa . wa)
+        CLG_TRY(blas_axpby(ctx, W, 1.0, 0.0, R, 0.0, 0.0));
+        CLG_TRY(blas_axpby(ctx, W, beta.x, beta.y, SA, 1.0, 0.0));
+        CLG_TRY(A.apply(ctx, WA, W));
+        CLG_TRY(blas_dot_dev(ctx, WA, W, d + 4));
+        CLG_TRY(blas_norm2_dev(ctx, WA, d + 6));
+        double t3[4];
+        CLG_TRY(fetch_doubles(ctx, d + 4, 3, t3));
+        const cx chi = cx{t3[0] / t3[2], t3[1] / t3[2]};
+        // batched per-shift update 1 over the active shifts
+        std::vector<int> idx;
+        for (int k = 0; k < ns; ++k)
+            if (!(sl[k] < accuracy * blen)) idx.push_back(k);
+        const int na = (int)idx.size();
+        std::vector<cx> dinoms(ns);
+        for (int q = 0; q < na; ++q) {
+            const int k = idx[q];
+            const cx dinom = cad(cx{1, 0}, cm(cn[k], chi));
+            dinoms[k] = dinom;
+            chis[k] = cdv(chi, dinom);
+            const cx chirho = cm(chis[k], rho[k]);
+            const cx crz = cm(chirho, zeta[k]);
+            const cx crzb = cdv(crz, betas[k]);
+            const cx crzob = cdv(cm(chirho, zetaold[k]), betas[k]);
+            const double c8[8] = {-betas[k].x, -betas[k].y, crz.x, crz.y, -crzb.x, -crzb.y, crzob.x, crzob.y};
+            std::memcpy(&hc[8 * q], c8, sizeof(c8));
+        }
+        ShiftPtrs sp;
+        for (int q = 0; q < na; ++q) { sp.x[q] = X[idx[q]]->data; sp.s[q] = S[idx[q]]->data; }
+        if (na > 0) {
+            CLG_HIP(hipMemcpyAsync(coef, hc.data(), sizeof(double) * 8 * na, hipMemcpyHostToDevice, ctx->stream));
+            if (b->prec == CLG_F64)
+                k_msbicg_update1<CLG_F64><<<g, 256, 0, ctx->stream>>>(np, na, sp, coef, (const double2*)W->data,
+                                                                       (const double2*)R->data);
+            else
+                k_msbicg_update1<CLG_F32><<<g, 256, 0, ctx->stream>>>(np, na, sp, coef, (const float4*)W->data,
+                                                                       (const float4*)R->data);
+            CLG_HIP(hipGetLastError());
+        }
+        for (int q = 0; q < na; ++q) rho[idx[q]] = cdv(rho[idx[q]], dinoms[idx[q]]);
+        // r = w - chi wa ; delta' = w0 . r ; alpha = beta (-delta') / (chi delta)
+        CLG_TRY(blas_axpby(ctx, R, 1.0, 0.0, W, 0.0, 0.0));
+        CLG_TRY(blas_axpby(ctx, R, -chi.x, -chi.y, WA, 1.0, 0.0));
+        CLG_TRY(blas_dot_dev(ctx, W0, R, d + 8));
+        cx newdelta;
+        CLG_TRY(fetch_c(8, newdelta));
+        alpha = cdv(cm(beta, cx{-newdelta.x, -newdelta.y}), cm(chi, delta));
+        delta = newdelta;
+        for (int q = 0; q < na; ++q) {
+            const int k = idx[q];
+            alphas[k] = cdv(cm(cm(alpha, zeta[k]), betas[k]), cm(zetaold[k], beta));
+            const cx zr = cm(zeta[k], rho[k]);
+            const double c4[4] = {alphas[k].x, alphas[k].y, zr.x, zr.y};
+            std::memcpy(&hc[4 * q], c4, sizeof(c4));
+        }
+        if (na > 0) {
+            CLG_HIP(hipMemcpyAsync(coef, hc.data(), sizeof(double) * 4 * na, hipMemcpyHostToDevice, ctx->stream));
+            if (b->prec == CLG_F64)
+                k_msbicg_update2<CLG_F64><<<g, 256, 0, ctx->stream>>>(np, na, sp, coef, (const double2*)R->data);
+            else
+                k_msbicg_update2<CLG_F32><<<g, 256, 0, ctx->stream>>>(np, na, sp, coef, (const float4*)R->data);
+            CLG_HIP(hipGetLastError());
+        }
+        // s = alpha (s - chi sa) + r ; sa = A s ; phi = (w0 . sa) / delta
+        CLG_TRY(blas_axpby(ctx, Sv, -chi.x, -chi.y, SA, 1.0, 0.0));
+        CLG_TRY(blas_axpby(ctx, Sv, 1.0, 0.0, R, alpha.x, alpha.y));
+        CLG_TRY(A.apply(ctx, SA, Sv));
+        CLG_TRY(blas_dot_dev(ctx, W0, SA, d + 10));
+        CLG_TRY(fetch_c(10, tmp));
+        phi = cdv(tmp, delta);
+        if ((i + 1) % dev_check == 0) {
+            CLG_TRY(blas_norm2_dev(ctx, Sv, d + 12));
+            double s2;
+            CLG_TRY(fetch_doubles(ctx, d + 12, 1, &s2));
+            maxerr = std::sqrt(s2);
+            for (int q = 0; q < na; ++q) {
+                const int k = idx[q];
+                CLG_TRY(blas_norm2_dev(ctx, S[k], d + 14));
+                double t;
+                CLG_TRY(fetch_doubles(ctx, d + 14, 1, &t));
+                sl[k] = std::sqrt(t);
+                maxerr = std::max(maxerr, sl[k]);
+            }
+            if (maxerr < accuracy * blen) { done = true; break; }
+        }
+    }
+    return done ? CLG_OK : CLG_NOT_CONVERGED;
+}
+
+enum { MS_FOM = 0, MS_BICGSTAB = 1 };
+
+int run_multishift(clg_context ctx, const clg_solver_params* p, int kind_solver, int prec, std::vector<clg_field_s*>& X,
+                   const std::vector<cx>& shifts, const clg_field_s* b, const clg_field_s* gauge, double param, int op,
+                   clg_solve_report* rep) {
+    const double t0 = now_s();
+    // working precision: FP64 as given, or FP32 copies of b / gauge / solutions
+    const clg_field_s* bw = b;
+    const clg_field_s* gw = gauge;
+    std::vector<clg_field_s*> Xw = X;
+    if (prec == CLG_F32) {
+        clg_field_s *b32, *g32;
+        CLG_TRY(scratch_field(ctx, b, b->kind, CLG_F32, b->subset, 150, &b32));
+        CLG_TRY(clg_field_copy(b32, const_cast<clg_field_s*>(b)));
+        CLG_TRY(scratch_field(ctx, gauge, CLG_GAUGE, CLG_F32, CLG_FULL, 0, &g32));
+        CLG_TRY(gauge_convert(ctx, g32, gauge));
+        for (size_t k = 0; k < X.size(); ++k)
+            CLG_TRY(scratch_field(ctx, b, b->kind, CLG_F32, b->subset, 160 + (int)k, &Xw[k]));
+        bw = b32;
+        gw = g32;
+    }
+    OpRef A{p->kind, op, gw, param};
+    double bn2 = 1.0;
+    if (!p->absolute_accuracy) {
+        CLG_TRY(blas_norm2_dev(ctx, b, ctx->red_result + 24));
+        CLG_TRY(fetch_doubles(ctx, ctx->red_result + 24, 1, &bn2));
+    }
+    int it = 0;
+    double err = 0;
+    int st;
+    if (kind_solver == MS_FOM) {
+        // FOM compares |beta0| with Accuracy * (AbsoluteAccuracy ? 1 : |b|^2) (CMultiShiftFOM.cpp:119-124)
+        st = msfom(ctx, A, Xw, shifts, bw, p->max_dim, p->restart, p->accuracy, p->absolute_accuracy ? 1.0 : bn2, it,
+                   err);
+    } else {
+        // BiCGStab compares |s| with Accuracy * (AbsoluteAccuracy ? 1 : |b|)
+        st = msbicgstab(ctx, A, Xw, shifts, bw, p->max_step, p->deviation_step, p->accuracy,
+                        p->absolute_accuracy ? 1.0 : std::sqrt(bn2), it, err);
+    }
+    if (st != CLG_OK && st != CLG_NOT_CONVERGED) return st;
+    if (prec == CLG_F32)
+        for (size_t k = 0; k < X.size(); ++k) CLG_TRY(clg_field_copy(X[k], Xw[k]));
+    CLG_HIP(hipStreamSynchronize(ctx->stream));
+    if (rep) {
+        std::memset(rep, 0, sizeof(*rep));
+        rep->converged = st == CLG_OK;
+        rep->cycles = it;
+        rep->outer = 1;
+        rep->operator_applies = A.applies;
+        rep->last_deviation = err;
+        rep->true_residual = -1.0;
+        rep->seconds = now_s() - t0;
+    }
+    return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+int clg_multishift_solve(clg_context ctx, const clg_solver_params* p, int kind_solver, int prec, clg_field* xs,
+                         const double* shifts, int nshift, clg_field b, clg_field gauge, double param, int op,
+                         clg_solve_report* rep) {
+    CLG_CHECK_ARG(ctx && p && xs && shifts && b && gauge, "null argument");
+    CLG_CHECK_ARG(nshift >= 1 && nshift <= kMaxShift, "nshift must be in [1, %d]", kMaxShift);
+    CLG_CHECK_ARG(kind_solver == MS_FOM || kind_solver == MS_BICGSTAB, "unknown multi-shift solver %d", kind_solver);
+    CLG_CHECK_ARG(prec == CLG_F64 || prec == CLG_F32, "multi-shift precision must be FP64 or FP32");
+    CLG_CHECK_ARG(b->kind == p->kind && b->subset == CLG_FULL && b->prec == CLG_F64, "b must be a FULL FP64 field of params.kind");
+    CLG_CHECK_ARG(gauge->prec == CLG_F64 && gauge->kind == CLG_GAUGE, "gauge must be the FP64 18-real field");
+    CLG_CHECK_ARG(op >= EFO_F_D && op <= EFO_F_DDdagger, "operator %d not supported", op);
+    std::vector<clg_field_s*> X(nshift);
+    std::vector<cx> sh(nshift);
+    for (int k = 0; k < nshift; ++k) {
+        CLG_CHECK_ARG(xs[k] && xs[k]->kind == b->kind && xs[k]->subset == CLG_FULL && xs[k]->prec == CLG_F64,
+                      "solution %d must match b", k);
+        CLG_CHECK_ARG(xs[k]->data != b->data, "solutions must not alias b (CMultiShiftSolver.h:35)");
+        CLG_TRY(check_same_lattice(xs[k], b));
+        X[k] = xs[k];
+        sh[k] = cx{shifts[2 * k], shifts[2 * k + 1]};
+    }
+    CLG_TRY(check_same_lattice(b, gauge));
+    (void)hipSetDevice(ctx->device);
+    return run_multishift(ctx, p, kind_solver, prec, X, sh, b, gauge, param, op, rep);
+}
+
+int clg_rational_apply(clg_context ctx, const clg_solver_params* p, int kind_solver, int prec, clg_field out,
+                       clg_field in, clg_field gauge, double param, int op, const double* coeffs, int degree,
+                       clg_solve_report* rep) {
+    CLG_CHECK_ARG(ctx && p && out && in && gauge && coeffs, "null argument");
+    CLG_CHECK_ARG(degree >= 1 && degree <= kMaxShift, "degree must be in [1, %d]", kMaxShift);
+    CLG_CHECK_ARG(out->kind == in->kind && out->prec == CLG_F64 && out->subset == CLG_FULL, "out must match in");
+    (void)hipSetDevice(ctx->device);
+    std::vector<clg_field_s*> X(degree);
+    std::vector<double> sh(2 * degree);
+    for (int k = 0; k < degree; ++k) {
+        CLG_TRY(scratch_field(ctx, in, in->kind, CLG_F64, CLG_FULL, 170 + k, &X[k]));
+        sh[2 * k] = coeffs[1 + degree + k];
+        sh[2 * k + 1] = 0.0;
+    }
+    std::vector<clg_field> xs(X.begin(), X.end());
+    const int st = clg_multishift_solve(ctx, p, kind_solver, prec, xs.data(), sh.data(), degree, in, gauge, param, op, rep);
+    if (st != CLG_OK && st != CLG_NOT_CONVERGED) return st;
+    // out = c in + sum_k a_k x_k, accumulated in FP64 (CField.cpp:140-146)
+    double* ws = nullptr;
+    CLG_TRY(workspace(ctx, 2 * (size_t)degree, &ws));
+    std::vector<double> a(2 * degree);
+    for (int k = 0; k < degree; ++k) { a[2 * k] = coeffs[1 + k]; a[2 * k + 1] = 0.0; }
+    CLG_HIP(hipMemcpyAsync(ws, a.data(), sizeof(double) * 2 * degree, hipMemcpyHostToDevice, ctx->stream));
+    if (out != in) CLG_TRY(clg_field_copy(out, in));
+    CLG_TRY(blas_axpby(ctx, out, 0.0, 0.0, out, coeffs[0], 0.0));
+    CLG_TRY(blas_multiaxpy_dev(ctx, out, X.data(), degree, ws, 1.0, nullptr, nullptr, nullptr, 0.0, nullptr));
+    CLG_HIP(hipStreamSynchronize(ctx->stream));
+    return st;
+}
+
+}  // extern "C"

@@ -556,8 +556,19 @@ struct Solve1 {
 }  // namespace
 
 namespace clg {
-// GMRES mode 3 (no stop, fixed cycles) is used by the normal-equation rounds above.
+// full-lattice operator application for other solver families (multi-shift, forces)
+int linop_apply(clg_context ctx, int kind, int op, const clg_field_s* gauge, double param, clg_field_s* out,
+                const clg_field_s* in) {
+    LinOp A;
+    A.ctx = ctx;
+    A.kind = kind;
+    A.schur = false;
+    A.op = op;
+    A.gauge = gauge;
+    A.param = param;
+    return A.apply(out, in);
 }
+}  // namespace clg
 
 extern "C" {
 

@@ -189,6 +189,25 @@ int clg_solve(clg_context ctx, const clg_solver_params* params, clg_field x, clg
 /* defaults matching the reference YAML defaults (CSolverGMRES.cpp:35-99) */
 int clg_solver_params_default(clg_solver_params* p);
 
+/* ---- multi-shift solvers and the rational approximation (RHMC) ---- */
+typedef enum { CLG_MS_FOM = 0, CLG_MS_BICGSTAB = 1 } clg_multishift_kind;
+
+/* CMultiShiftSolver::Solve (CMultiShiftSolver.h:38-47): xs[k] = (A + c_k)^-1 b for all k, with
+ * A = op of the params.kind fermion and c_k = shifts[2k] + i shifts[2k+1].  CLG_MS_FOM follows
+ * CMultiShiftFOM.cpp:97-270 (params max_dim, restart, accuracy, absolute_accuracy), CLG_MS_BICGSTAB
+ * follows CMultiShiftBiCGStab.cpp:73-270 (max_step, deviation_step, accuracy, absolute_accuracy).
+ * prec CLG_F32 runs the whole solve on FP32 copies (solutions written back as FP64).  The xs must
+ * not alias b.  Returns CLG_NOT_CONVERGED with the iterates written when the budget runs out. */
+int clg_multishift_solve(clg_context ctx, const clg_solver_params* params, int kind, int prec, clg_field* xs,
+                         const double* shifts, int nshift, clg_field b, clg_field gauge, double kappa_or_mass,
+                         int op, clg_solve_report* report);
+/* CFieldFermion::RationalApproximation (CField.cpp:114-148): out = c in + sum_k a_k (A + b_k)^-1 in,
+ * coeffs = [c, a_1..a_n, b_1..b_n] (CRationalApproximation.h:38-117), shift sums in FP64.
+ * out may alias in. */
+int clg_rational_apply(clg_context ctx, const clg_solver_params* params, int kind, int prec, clg_field out,
+                       clg_field in, clg_field gauge, double kappa_or_mass, int op, const double* coeffs, int degree,
+                       clg_solve_report* report);
+
 /* ============================== measurement ============================== */
 /* Times `reps` back-to-back launches of one Wilson operator on the context stream with HIP
  * events; writes the average milliseconds per launch.  variant: 0 full-lattice D,
