@@ -34,7 +34,8 @@ EXPORTED = [
     "clg_gauge_unit", "clg_field_gaussian", "clg_field_point_source", "clg_wilson_apply",
     "clg_wilson_hop", "clg_wilson_schur", "clg_ks_apply", "clg_axpy", "clg_scale", "clg_dot",
     "clg_norm2", "clg_solve", "clg_solver_params_default", "clg_time_wilson",
-    "clg_multishift_solve", "clg_rational_apply",
+    "clg_multishift_solve", "clg_rational_apply", "clg_wilson_force_kernel", "clg_ks_force_kernel",
+    "clg_wilson_force", "clg_ks_force", "clg_wilson_pf_energy", "clg_ks_pf_energy",
 ]
 
 
@@ -104,6 +105,16 @@ def lib():
                                                C.c_int, vp, vp, C.c_double, C.c_int, C.POINTER(SolveReport)]),
             "clg_rational_apply": (C.c_int, [vp, C.POINTER(SolverParams), C.c_int, C.c_int, vp, vp, vp, C.c_double,
                                              C.c_int, dp, C.c_int, C.POINTER(SolveReport)]),
+            "clg_wilson_force_kernel": (C.c_int, [vp, vp, vp, vp, vp, C.c_double]),
+            "clg_ks_force_kernel": (C.c_int, [vp, vp, vp, C.POINTER(vp), C.POINTER(vp), dp, C.c_int]),
+            "clg_wilson_force": (C.c_int, [vp, C.POINTER(SolverParams), vp, vp, vp, C.c_double, vp, vp,
+                                           C.POINTER(SolveReport)]),
+            "clg_ks_force": (C.c_int, [vp, C.POINTER(SolverParams), C.c_int, C.c_int, vp, vp, vp, C.c_double, dp,
+                                       C.c_int, C.POINTER(SolveReport)]),
+            "clg_wilson_pf_energy": (C.c_int, [vp, C.POINTER(SolverParams), vp, vp, C.c_double, dp,
+                                               C.POINTER(SolveReport)]),
+            "clg_ks_pf_energy": (C.c_int, [vp, C.POINTER(SolverParams), C.c_int, C.c_int, vp, vp, C.c_double, dp,
+                                           C.c_int, dp, C.POINTER(SolveReport)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -303,7 +314,7 @@ def time_wilson(ctx, out, inp, gauge, kappa, variant, reps):
 def multishift_solve(ctx, params, xs, shifts, b, gauge, kappa_or_mass, op, kind=MS_FOM, prec=F64):
     """xs[k] = (A + shifts[k])^-1 b for all k (A = op of the params.kind fermion)."""
     sh = np.ascontiguousarray([[complex(s).real, complex(s).imag] for s in shifts], dtype=np.float64).ravel()
-    arr = (C.c_void_p * len(xs))(*[x.h.value if isinstance(x.h, C.c_void_p) else x.h for x in xs])
+    arr = (C.c_void_p * len(xs))(*[x.h.value for x in xs])
     rep = SolveReport()
     st = lib().clg_multishift_solve(ctx.h, C.byref(params), kind, prec, arr,
                                     sh.ctypes.data_as(C.POINTER(C.c_double)), len(xs), b.h, gauge.h,
@@ -323,3 +334,52 @@ def rational_apply(ctx, params, out, inp, gauge, kappa_or_mass, op, coeffs, kind
                                   cf.ctypes.data_as(C.POINTER(C.c_double)), degree, C.byref(rep))
     check(st, allow=(OK, NOT_CONVERGED))
     return st, rep
+
+
+def _handles(fields):
+    return (C.c_void_p * len(fields))(*[f.h.value for f in fields])
+
+
+def wilson_force_kernel(ctx, force, gauge, inv_dddag, inv_d, kappa):
+    check(lib().clg_wilson_force_kernel(ctx.h, force.h, gauge.h, inv_dddag.h, inv_d.h, kappa))
+
+
+def ks_force_kernel(ctx, force, gauge, phis, dphis, numerators):
+    num = np.ascontiguousarray(numerators, dtype=np.float64)
+    check(lib().clg_ks_force_kernel(ctx.h, force.h, gauge.h, _handles(phis), _handles(dphis),
+                                    num.ctypes.data_as(C.POINTER(C.c_double)), len(phis)))
+
+
+def wilson_force(ctx, params, force, phi, gauge, kappa, start=None, x_out=None):
+    rep = SolveReport()
+    st = lib().clg_wilson_force(ctx.h, C.byref(params), force.h, phi.h, gauge.h, kappa,
+                                start.h if start is not None else None, x_out.h if x_out is not None else None,
+                                C.byref(rep))
+    check(st, allow=(OK, NOT_CONVERGED))
+    return st, rep
+
+
+def ks_force(ctx, params, force, phi, gauge, two_am, coeffs, kind=MS_FOM, prec=F64):
+    cf = np.ascontiguousarray(coeffs, dtype=np.float64)
+    rep = SolveReport()
+    st = lib().clg_ks_force(ctx.h, C.byref(params), kind, prec, force.h, phi.h, gauge.h, two_am,
+                            cf.ctypes.data_as(C.POINTER(C.c_double)), (cf.size - 1) // 2, C.byref(rep))
+    check(st, allow=(OK, NOT_CONVERGED))
+    return st, rep
+
+
+def wilson_pf_energy(ctx, params, phi, gauge, kappa):
+    e, rep = C.c_double(), SolveReport()
+    st = lib().clg_wilson_pf_energy(ctx.h, C.byref(params), phi.h, gauge.h, kappa, C.byref(e), C.byref(rep))
+    check(st, allow=(OK, NOT_CONVERGED))
+    return e.value
+
+
+def ks_pf_energy(ctx, params, phi, gauge, two_am, coeffs, kind=MS_FOM, prec=F64):
+    cf = np.ascontiguousarray(coeffs, dtype=np.float64)
+    e, rep = C.c_double(), SolveReport()
+    st = lib().clg_ks_pf_energy(ctx.h, C.byref(params), kind, prec, phi.h, gauge.h, two_am,
+                                cf.ctypes.data_as(C.POINTER(C.c_double)), (cf.size - 1) // 2, C.byref(e),
+                                C.byref(rep))
+    check(st, allow=(OK, NOT_CONVERGED))
+    return e.value

@@ -208,6 +208,33 @@ int clg_rational_apply(clg_context ctx, const clg_solver_params* params, int kin
                        clg_field in, clg_field gauge, double kappa_or_mass, int op, const double* coeffs, int degree,
                        clg_solve_report* report);
 
+/* ============================== HMC forces and energies ============================== */
+/* Forces live in FP64 CLG_GAUGE-shaped fields (traceless anti-Hermitian per link) and are
+ * accumulated (F += ...), as the reference's force kernels add into pForce. */
+/* _kernelDWilsonForceSU3 (CFieldFermionWilsonSquareSU3.cu:170-231): inv_dddag = (DD^dag)^-1 phi,
+ * inv_d = D^dag inv_dddag */
+int clg_wilson_force_kernel(clg_context ctx, clg_field force, clg_field gauge, clg_field inv_dddag, clg_field inv_d,
+                            double kappa);
+/* _kernelDFermionKSForce (CFieldFermionKSSU3.cu:216-264) over npole (phi_i, D0 phi_i) pairs */
+int clg_ks_force_kernel(clg_context ctx, clg_field force, clg_field gauge, const clg_field* phis, const clg_field* dphis,
+                        const double* numerators, int npole);
+/* CFieldFermionWilsonSquareSU3::CalculateForceS (CFieldFermionWilsonSquareSU3.cu:1016-1100); start
+ * (nullable) warm-starts the DD^dag solve as the reference's CFieldCache does; x_out (nullable)
+ * receives (DD^dag)^-1 phi */
+int clg_wilson_force(clg_context ctx, const clg_solver_params* params, clg_field force, clg_field phi, clg_field gauge,
+                     double kappa, clg_field start, clg_field x_out, clg_solve_report* report);
+/* CFieldFermionKSSU3::CalculateForceS (CFieldFermionKSSU3.cu:412-497), MD coefficients
+ * [c, a_1..a_n, b_1..b_n] */
+int clg_ks_force(clg_context ctx, const clg_solver_params* params, int ms_kind, int prec, clg_field force, clg_field phi,
+                 clg_field gauge, double two_am, const double* coeffs, int degree, clg_solve_report* report);
+/* CActionFermionWilsonNf2::Energy (CActionFermionWilsonNf2.cpp:53-68): |D^-1 phi|^2 */
+int clg_wilson_pf_energy(clg_context ctx, const clg_solver_params* params, clg_field phi, clg_field gauge, double kappa,
+                         double* energy, clg_solve_report* report);
+/* CActionFermionKS::Energy (CActionFermionKS.cpp:52-71): Re phi^dag r_MD(DD^dag) phi */
+int clg_ks_pf_energy(clg_context ctx, const clg_solver_params* params, int ms_kind, int prec, clg_field phi,
+                     clg_field gauge, double two_am, const double* coeffs, int degree, double* energy,
+                     clg_solve_report* report);
+
 /* ============================== measurement ============================== */
 /* Times `reps` back-to-back launches of one Wilson operator on the context stream with HIP
  * events; writes the average milliseconds per launch.  variant: 0 full-lattice D,
