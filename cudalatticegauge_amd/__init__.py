@@ -25,6 +25,9 @@ SOLVER_GMRES, SOLVER_BICGSTAB = 0, 1
 MS_FOM, MS_BICGSTAB = 0, 1
 STOP_REFERENCE, STOP_TRUE_RESIDUAL = 0, 1
 OK, NOT_CONVERGED = 0, 1
+ACTION_GAUGE_PLAQUETTE, ACTION_FERMION_WILSON_NF2, ACTION_FERMION_KS = 0, 1, 2
+INT_LEAPFROG, INT_NESTED_FORCE_GRADIENT = 0, 1
+HMC_MAX_ACTIONS = 8
 
 EXPORTED = [
     "clg_init", "clg_finalize", "clg_synchronize", "clg_status_string", "clg_last_error",
@@ -36,6 +39,9 @@ EXPORTED = [
     "clg_norm2", "clg_solve", "clg_solver_params_default", "clg_time_wilson",
     "clg_multishift_solve", "clg_rational_apply", "clg_wilson_force_kernel", "clg_ks_force_kernel",
     "clg_wilson_force", "clg_ks_force", "clg_wilson_pf_energy", "clg_ks_pf_energy",
+    "clg_gauge_momentum_random", "clg_gauge_kinetic_energy", "clg_gauge_axpy", "clg_gauge_exp_mult",
+    "clg_gauge_normalize", "clg_gauge_plaquette_energy", "clg_gauge_force", "clg_wilson_prepare_hmc",
+    "clg_ks_prepare_hmc", "clg_hmc_params_default", "clg_hmc_evolve", "clg_hmc_trajectory",
 ]
 
 
@@ -55,6 +61,26 @@ class SolveReport(C.Structure):
     _fields_ = [("converged", C.c_int), ("cycles", C.c_int), ("outer", C.c_int),
                 ("operator_applies", C.c_int), ("true_residual", C.c_double),
                 ("last_deviation", C.c_double), ("seconds", C.c_double)]
+
+
+class Action(C.Structure):
+    _fields_ = [("kind", C.c_int), ("beta", C.c_double), ("kappa_or_mass", C.c_double), ("phi", C.c_void_p),
+                ("mc", C.POINTER(C.c_double)), ("mc_degree", C.c_int), ("md", C.POINTER(C.c_double)),
+                ("md_degree", C.c_int), ("solver", SolverParams), ("ms_kind", C.c_int), ("ms_prec", C.c_int)]
+
+
+class HmcParams(C.Structure):
+    _fields_ = [("integrator", C.c_int), ("step_length", C.c_double), ("steps", C.c_int),
+                ("nested_steps", C.c_int), ("inner_leapfrog", C.c_int), ("exp_precision", C.c_int),
+                ("metropolis", C.c_int)]
+
+
+class HmcReport(C.Structure):
+    _fields_ = [("h_before", C.c_double), ("h_after", C.c_double), ("h_diff", C.c_double),
+                ("kinetic_before", C.c_double), ("kinetic_after", C.c_double),
+                ("action_before", C.c_double * HMC_MAX_ACTIONS), ("action_after", C.c_double * HMC_MAX_ACTIONS),
+                ("accepted", C.c_int), ("force_evaluations", C.c_int), ("solver_iterations", C.c_int),
+                ("unconverged_solves", C.c_int), ("seconds", C.c_double)]
 
 
 _lib = None
@@ -115,6 +141,21 @@ def lib():
                                                C.POINTER(SolveReport)]),
             "clg_ks_pf_energy": (C.c_int, [vp, C.POINTER(SolverParams), C.c_int, C.c_int, vp, vp, C.c_double, dp,
                                            C.c_int, dp, C.POINTER(SolveReport)]),
+            "clg_gauge_momentum_random": (C.c_int, [vp, C.c_uint, C.c_uint]),
+            "clg_gauge_kinetic_energy": (C.c_int, [vp, vp, dp]),
+            "clg_gauge_axpy": (C.c_int, [vp, vp, C.c_double, vp]),
+            "clg_gauge_exp_mult": (C.c_int, [vp, vp, vp, C.c_double, C.c_int]),
+            "clg_gauge_normalize": (C.c_int, [vp, vp]),
+            "clg_gauge_plaquette_energy": (C.c_int, [vp, vp, C.c_double, dp]),
+            "clg_gauge_force": (C.c_int, [vp, vp, vp, C.c_double]),
+            "clg_wilson_prepare_hmc": (C.c_int, [vp, vp, vp, C.c_double, C.c_uint]),
+            "clg_ks_prepare_hmc": (C.c_int, [vp, C.POINTER(SolverParams), C.c_int, C.c_int, vp, vp, C.c_double, dp,
+                                             C.c_int, C.c_uint, C.POINTER(SolveReport)]),
+            "clg_hmc_params_default": (C.c_int, [C.POINTER(HmcParams)]),
+            "clg_hmc_evolve": (C.c_int, [vp, vp, vp, C.POINTER(HmcParams), C.POINTER(Action), C.c_int,
+                                         C.POINTER(HmcReport)]),
+            "clg_hmc_trajectory": (C.c_int, [vp, vp, C.POINTER(HmcParams), C.POINTER(Action), C.c_int, C.c_ulonglong,
+                                             C.POINTER(HmcReport)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -383,3 +424,98 @@ def ks_pf_energy(ctx, params, phi, gauge, two_am, coeffs, kind=MS_FOM, prec=F64)
                                 C.byref(rep))
     check(st, allow=(OK, NOT_CONVERGED))
     return e.value
+
+
+# ---------------------------------------------------------------- HMC
+def momentum_random(P, seed, counter=0):
+    check(lib().clg_gauge_momentum_random(P.h, seed, counter))
+
+
+def kinetic_energy(ctx, P):
+    e = C.c_double()
+    check(lib().clg_gauge_kinetic_energy(ctx.h, P.h, C.byref(e)))
+    return e.value
+
+
+def gauge_axpy(ctx, y, a, x):
+    check(lib().clg_gauge_axpy(ctx.h, y.h, a, x.h))
+
+
+def exp_mult(ctx, P, gauge, a, exp_precision=0):
+    check(lib().clg_gauge_exp_mult(ctx.h, P.h, gauge.h, a, exp_precision))
+
+
+def gauge_normalize(ctx, gauge):
+    check(lib().clg_gauge_normalize(ctx.h, gauge.h))
+
+
+def plaquette_energy(ctx, gauge, beta):
+    e = C.c_double()
+    check(lib().clg_gauge_plaquette_energy(ctx.h, gauge.h, beta, C.byref(e)))
+    return e.value
+
+
+def gauge_force(ctx, force, gauge, beta):
+    check(lib().clg_gauge_force(ctx.h, force.h, gauge.h, beta))
+
+
+def wilson_prepare_hmc(ctx, phi, gauge, kappa, seed):
+    check(lib().clg_wilson_prepare_hmc(ctx.h, phi.h, gauge.h, kappa, seed))
+
+
+def ks_prepare_hmc(ctx, params, phi, gauge, two_am, mc, seed, kind=MS_FOM, prec=F64):
+    cf = np.ascontiguousarray(mc, dtype=np.float64)
+    rep = SolveReport()
+    st = lib().clg_ks_prepare_hmc(ctx.h, C.byref(params), kind, prec, phi.h, gauge.h, two_am,
+                                  cf.ctypes.data_as(C.POINTER(C.c_double)), (cf.size - 1) // 2, seed, C.byref(rep))
+    check(st, allow=(OK, NOT_CONVERGED))
+    return st, rep
+
+
+def hmc_params(**kw):
+    p = HmcParams()
+    check(lib().clg_hmc_params_default(C.byref(p)))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise CLGError(f"unknown HMC parameter {k}")
+        setattr(p, k, v)
+    return p
+
+
+class _Actions:
+    """ctypes array of clg_action built from dicts; keeps the coefficient arrays alive."""
+
+    def __init__(self, actions):
+        if not 1 <= len(actions) <= HMC_MAX_ACTIONS:
+            raise CLGError(f"need 1..{HMC_MAX_ACTIONS} actions")
+        self.keep = []
+        self.arr = (Action * len(actions))()
+        for i, a in enumerate(actions):
+            t = self.arr[i]
+            t.kind = a["kind"]
+            t.beta = a.get("beta", 0.0)
+            t.kappa_or_mass = a.get("kappa_or_mass", 0.0)
+            t.phi = a["phi"].h.value if a.get("phi") is not None else None
+            for key in ("mc", "md"):
+                if a.get(key) is not None:
+                    cf = np.ascontiguousarray(a[key], dtype=np.float64)
+                    self.keep.append(cf)
+                    setattr(t, key, cf.ctypes.data_as(C.POINTER(C.c_double)))
+                    setattr(t, key + "_degree", (cf.size - 1) // 2)
+            t.solver = a.get("solver", solver_params())
+            t.ms_kind = a.get("ms_kind", MS_FOM)
+            t.ms_prec = a.get("ms_prec", F64)
+
+
+def hmc_evolve(ctx, gauge, momentum, params, actions):
+    acts = _Actions(actions)
+    rep = HmcReport()
+    check(lib().clg_hmc_evolve(ctx.h, gauge.h, momentum.h, C.byref(params), acts.arr, len(actions), C.byref(rep)))
+    return rep
+
+
+def hmc_trajectory(ctx, gauge, params, actions, seed):
+    acts = _Actions(actions)
+    rep = HmcReport()
+    check(lib().clg_hmc_trajectory(ctx.h, gauge.h, C.byref(params), acts.arr, len(actions), seed, C.byref(rep)))
+    return rep

@@ -16,73 +16,8 @@ using namespace clgd;
 
 namespace {
 
-const int kRedBlock = 256;
 const int kMaxRedGrid = 2048;
 const int kMaxK = 16;  // vectors per multi-dot / multi-axpy launch
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-
-// Reduce NR per-lane values over the grid; the last workgroup to finish writes out[0..NR)
-// and returns true on its thread 0.
-template <int NR>
-__device__ bool reduce_finish(double (&v)[NR], double* partial, unsigned* counter, double* out) {
-    __shared__ double sm[kRedBlock / 64][NR];
-    __shared__ int is_last;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int nw = blockDim.x >> 6;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) v[r] = wave_sum(v[r]);
-    if (lane == 0) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) sm[wv][r] = v[r];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int r = 0; r < NR; ++r) {
-            double s = 0;
-            for (int w = 0; w < nw; ++w) s += sm[w][r];
-            partial[(size_t)blockIdx.x * NR + r] = s;
-        }
-        __threadfence();
-        const unsigned t = atomicAdd(counter, 1u);
-        is_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    // acquire: the fence invalidates this CU's vector L1, so the plain loads below see every
-    // block's partials (each published by a release fence before its counter increment)
-    __threadfence();
-    double acc[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) acc[r] = 0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) acc[r] += partial[(size_t)b * NR + r];
-    }
-#pragma unroll
-    for (int r = 0; r < NR; ++r) acc[r] = wave_sum(acc[r]);
-    __syncthreads();
-    if (lane == 0) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) sm[wv][r] = acc[r];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int r = 0; r < NR; ++r) {
-            double s = 0;
-            for (int w = 0; w < nw; ++w) s += sm[w][r];
-            out[r] = s;
-        }
-        *counter = 0u;
-        return true;
-    }
-    return false;
-}
 
 // ---- 16-byte packets: NC complex numbers per lane and load, values handled as doubles
 enum { PK_D = 0, PK_F4 = 1, PK_F2 = 2 };

@@ -34,34 +34,6 @@ namespace {
 
 typedef C2<double> cx;
 
-// traceless anti-Hermitian part (SU3::Ta, SU3.h:953-984)
-__device__ __forceinline__ void su3_ta(cx* m) {
-    const cx n1 = csub(m[1], cconj(m[3]));
-    const cx n2 = csub(m[2], cconj(m[6]));
-    const cx n5 = csub(m[5], cconj(m[7]));
-    m[1] = cscale(n1, 0.5);
-    m[3] = cx{-m[1].x, m[1].y};
-    m[2] = cscale(n2, 0.5);
-    m[6] = cx{-m[2].x, m[2].y};
-    m[5] = cscale(n5, 0.5);
-    m[7] = cx{-m[5].x, m[5].y};
-    const double tr = (m[0].y + m[4].y + m[8].y) / 3.0;
-    m[0] = cx{0.0, m[0].y - tr};
-    m[4] = cx{0.0, m[4].y - tr};
-    m[8] = cx{0.0, m[8].y - tr};
-}
-__device__ __forceinline__ void mat_mul(const cx* a, const cx* b, cx* r) {
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            cx acc = cmul(a[3 * i], b[j]);
-            cfma(acc, a[3 * i + 1], b[3 + j]);
-            cfma(acc, a[3 * i + 2], b[6 + j]);
-            r[3 * i + j] = acc;
-        }
-}
-
 // gamma_mu rows (chiral set, GammaMatrix.h:447-475): (g psi)[r] = i^z[r] psi[idx[r]]
 __constant__ int kGIdx[4][4] = {{3, 2, 1, 0}, {3, 2, 1, 0}, {2, 3, 0, 1}, {2, 3, 0, 1}};
 __constant__ int kGZ4[4][4] = {{3, 3, 1, 1}, {2, 0, 0, 2}, {3, 1, 1, 3}, {0, 0, 0, 0}};
@@ -125,7 +97,7 @@ __global__ void k_wilson_force(Geo g, const char* X, const char* Y, const char* 
         cx u[9], f[9];
         const unsigned el = (unsigned)((p * 4 + mu) * 9 * Vh + h);
         LinkIO<CLG_F64, 18>::load(U, Vh, el, u);
-        mat_mul(u, mid, f);
+        su3_mm(u, mid, f);
         su3_ta(f);
         add_link(F, Vh, el, f, n.sign < 0 ? -kappa : kappa);
     }
@@ -266,8 +238,8 @@ int clg_ks_force(clg_context ctx, const clg_solver_params* params, int ms_kind, 
     std::vector<clg_field_s*> P(degree), Dp(degree);
     std::vector<double> sh(2 * degree), a(degree);
     for (int k = 0; k < degree; ++k) {
-        CLG_TRY(scratch_field(ctx, phi, CLG_KS, CLG_F64, CLG_FULL, 210 + k, &P[k]));
-        CLG_TRY(scratch_field(ctx, phi, CLG_KS, CLG_F64, CLG_FULL, 250 + k, &Dp[k]));
+        CLG_TRY(scratch_field(ctx, phi, CLG_KS, CLG_F64, CLG_FULL, 6000 + k, &P[k]));
+        CLG_TRY(scratch_field(ctx, phi, CLG_KS, CLG_F64, CLG_FULL, 7000 + k, &Dp[k]));
         for (int i = 0; i < 4; ++i) { P[k]->bc[i] = phi->bc[i]; Dp[k]->bc[i] = phi->bc[i]; }
         a[k] = coeffs[1 + k];
         sh[2 * k] = coeffs[1 + degree + k];

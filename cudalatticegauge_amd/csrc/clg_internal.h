@@ -99,6 +99,7 @@ struct clg_field_s {
     clg_field_s* parent = nullptr;  // views
     // size of one parity block in bytes (fermions) or one (parity,mu) block (gauge)
     size_t block_bytes = 0;
+    int scratch_slot = -1;  // context-owned scratch fields: the slot id they were created for
 };
 
 namespace clg {
@@ -544,4 +545,152 @@ __device__ __forceinline__ void su3_mdv(const C2<R>* u, const C2<R>* x, C2<R>* y
     }
 }
 
+// ------------------------------------------------------------------ SU(3) helpers
+// r = a b
+template <typename R>
+__device__ __forceinline__ void su3_mm(const C2<R>* a, const C2<R>* b, C2<R>* r) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            C2<R> acc = cmul(a[3 * i], b[j]);
+            cfma(acc, a[3 * i + 1], b[3 + j]);
+            cfma(acc, a[3 * i + 2], b[6 + j]);
+            r[3 * i + j] = acc;
+        }
+}
+// traceless anti-Hermitian part (SU3::Ta, SU3.h:953-984)
+__device__ __forceinline__ void su3_ta(cd* m) {
+    const cd n1 = csub(m[1], cconj(m[3]));
+    const cd n2 = csub(m[2], cconj(m[6]));
+    const cd n5 = csub(m[5], cconj(m[7]));
+    m[1] = cscale(n1, 0.5);
+    m[3] = cd{-m[1].x, m[1].y};
+    m[2] = cscale(n2, 0.5);
+    m[6] = cd{-m[2].x, m[2].y};
+    m[5] = cscale(n5, 0.5);
+    m[7] = cd{-m[5].x, m[5].y};
+    const double tr = (m[0].y + m[4].y + m[8].y) / 3.0;
+    m[0] = cd{0.0, m[0].y - tr};
+    m[4] = cd{0.0, m[4].y - tr};
+    m[8] = cd{0.0, m[8].y - tr};
+}
+// re-unitarisation as SU3::Norm does it (SU3.h:1058-1088): Gram-Schmidt on rows 0 and 1,
+// row 2 = conj(r0 x r1)
+__device__ __forceinline__ void su3_reunit(cd* m) {
+    const double d1 = 1.0 / sqrt(m[0].x * m[0].x + m[0].y * m[0].y + m[1].x * m[1].x + m[1].y * m[1].y +
+                                 m[2].x * m[2].x + m[2].y * m[2].y);
+    m[0] = cscale(m[0], d1); m[1] = cscale(m[1], d1); m[2] = cscale(m[2], d1);
+    cd sp = cadd(cadd(cmul(m[0], cconj(m[3])), cmul(m[1], cconj(m[4]))), cmul(m[2], cconj(m[5])));
+    sp = cconj(sp);
+    m[3] = csub(m[3], cmul(sp, m[0])); m[4] = csub(m[4], cmul(sp, m[1])); m[5] = csub(m[5], cmul(sp, m[2]));
+    const double d2 = 1.0 / sqrt(m[3].x * m[3].x + m[3].y * m[3].y + m[4].x * m[4].x + m[4].y * m[4].y +
+                                 m[5].x * m[5].x + m[5].y * m[5].y);
+    m[3] = cscale(m[3], d2); m[4] = cscale(m[4], d2); m[5] = cscale(m[5], d2);
+    rebuild_row2(m);
+}
+
+// ------------------------------------------------------------------ random numbers
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), counter = (index, draw, stream, c3),
+// key = (seed, 0x434C4721); two uniforms of 53 bits -> one Box-Muller pair of N(0,1)
+__device__ __forceinline__ void philox(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0,
+                                       unsigned k1, unsigned out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const unsigned hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        const unsigned hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        const unsigned n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+__device__ __forceinline__ double u53(unsigned a, unsigned b) {
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6) + 0.5) * (1.0 / 9007199254740992.0);
+}
+__device__ __forceinline__ void gauss2(unsigned idx, unsigned draw, unsigned stream, unsigned seed, double g[2],
+                                       unsigned c3 = 0u) {
+    unsigned w[4];
+    philox(idx, draw, stream, c3, seed, 0x434C4721u, w);
+    const double u1 = u53(w[0], w[1]), u2 = u53(w[2], w[3]);
+    const double r = sqrt(-2.0 * log(u1));
+    double s, c;
+    sincos(6.283185307179586476925286766559 * u2, &s, &c);
+    g[0] = r * c;
+    g[1] = r * s;
+}
+enum { STREAM_GAUGE = 0, STREAM_WILSON = 1, STREAM_KS = 2, STREAM_MOMENTUM = 3 };
+
+// ------------------------------------------------------------------ grid reductions
+constexpr int kRedBlock = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Reduce NR per-lane FP64 values over the grid (blockDim.x <= kRedBlock, a multiple of 64);
+// the last workgroup to finish writes out[0..NR), resets the counter and returns true on its
+// thread 0.  partial holds gridDim.x * NR doubles.
+template <int NR>
+__device__ bool reduce_finish(double (&v)[NR], double* partial, unsigned* counter, double* out) {
+    __shared__ double sm[kRedBlock / 64][NR];
+    __shared__ int is_last;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) v[r] = wave_sum(v[r]);
+    if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) sm[wv][r] = v[r];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int r = 0; r < NR; ++r) {
+            double s = 0;
+            for (int w = 0; w < nw; ++w) s += sm[w][r];
+            partial[(size_t)blockIdx.x * NR + r] = s;
+        }
+        __threadfence();
+        const unsigned t = atomicAdd(counter, 1u);
+        is_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    // acquire: the fence invalidates this CU's vector L1, so the plain loads below see every
+    // block's partials (each published by a release fence before its counter increment)
+    __threadfence();
+    double acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = 0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[r] += partial[(size_t)b * NR + r];
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = wave_sum(acc[r]);
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) sm[wv][r] = acc[r];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int r = 0; r < NR; ++r) {
+            double s = 0;
+            for (int w = 0; w < nw; ++w) s += sm[w][r];
+            out[r] = s;
+        }
+        *counter = 0u;
+        return true;
+    }
+    return false;
+}
+
 }  // namespace clgd
+

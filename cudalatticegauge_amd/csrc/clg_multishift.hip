@@ -254,7 +254,7 @@ int msfom(clg_context ctx, OpRef& A, std::vector<clg_field_s*>& X, const std::ve
     CLG_CHECK_ARG(m >= 2 && m <= 63, "MS-FOM MaxDim must be in [2, 63]");
     // V[0..m-1] basis, V[m] = w, vnext = v_m seeding the next cycle
     std::vector<clg_field_s*> V(m + 1);
-    for (int k = 0; k <= m; ++k) CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 60 + k, &V[k]));
+    for (int k = 0; k <= m; ++k) CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 2000 + k, &V[k]));
     clg_field_s* vnext = nullptr;
     CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 59, &vnext));
     clg_field_s* w = V[m];
@@ -339,7 +339,7 @@ int msbicgstab(clg_context ctx, OpRef& A, std::vector<clg_field_s*>& X, const st
                double& maxerr) {
     const int ns = (int)cn.size();
     std::vector<clg_field_s*> S(ns);
-    for (int k = 0; k < ns; ++k) CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 130 + k, &S[k]));
+    for (int k = 0; k < ns; ++k) CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 3000 + k, &S[k]));
     clg_field_s *R, *W, *W0, *Sv, *SA, *WA;
     CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 120, &R));
     CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 121, &W));
@@ -505,7 +505,7 @@ int run_multishift(clg_context ctx, const clg_solver_params* p, int kind_solver,
         CLG_TRY(scratch_field(ctx, gauge, CLG_GAUGE, CLG_F32, CLG_FULL, 0, &g32));
         CLG_TRY(gauge_convert(ctx, g32, gauge));
         for (size_t k = 0; k < X.size(); ++k)
-            CLG_TRY(scratch_field(ctx, b, b->kind, CLG_F32, b->subset, 160 + (int)k, &Xw[k]));
+            CLG_TRY(scratch_field(ctx, b, b->kind, CLG_F32, b->subset, 4000 + (int)k, &Xw[k]));
         bw = b32;
         gw = g32;
     }
@@ -583,7 +583,7 @@ int clg_rational_apply(clg_context ctx, const clg_solver_params* p, int kind_sol
     std::vector<clg_field_s*> X(degree);
     std::vector<double> sh(2 * degree);
     for (int k = 0; k < degree; ++k) {
-        CLG_TRY(scratch_field(ctx, in, in->kind, CLG_F64, CLG_FULL, 170 + k, &X[k]));
+        CLG_TRY(scratch_field(ctx, in, in->kind, CLG_F64, CLG_FULL, 5000 + k, &X[k]));
         sh[2 * k] = coeffs[1 + degree + k];
         sh[2 * k + 1] = 0.0;
     }

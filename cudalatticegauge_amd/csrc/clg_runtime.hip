@@ -49,61 +49,6 @@ using namespace clgd;
 // ============================================================== device kernels
 namespace {
 
-// ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), counter = (index, draw, stream, 0),
-//      key = (seed, 0x434C4721); two uniforms of 53 bits -> Box-Muller pair
-__device__ __forceinline__ void philox(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0,
-                                       unsigned k1, unsigned out[4]) {
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        const unsigned hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
-        const unsigned hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
-        const unsigned n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
-        c0 = n0;
-        c1 = lo1;
-        c2 = n2;
-        c3 = lo0;
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
-}
-__device__ __forceinline__ double u53(unsigned a, unsigned b) {
-    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6) + 0.5) * (1.0 / 9007199254740992.0);
-}
-__device__ __forceinline__ void gauss2(unsigned idx, unsigned draw, unsigned stream, unsigned seed, double g[2]) {
-    unsigned w[4];
-    philox(idx, draw, stream, 0u, seed, 0x434C4721u, w);
-    const double u1 = u53(w[0], w[1]), u2 = u53(w[2], w[3]);
-    const double r = sqrt(-2.0 * log(u1));
-    double s, c;
-    sincos(6.283185307179586476925286766559 * u2, &s, &c);
-    g[0] = r * c;
-    g[1] = r * s;
-}
-enum { STREAM_GAUGE = 0, STREAM_WILSON = 1, STREAM_KS = 2 };
-
-__device__ void mat_mul(const cd* a, const cd* b, cd* r) {
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) {
-            cd acc = cmul(a[3 * i], b[j]);
-            acc = cadd(acc, cmul(a[3 * i + 1], b[3 + j]));
-            acc = cadd(acc, cmul(a[3 * i + 2], b[6 + j]));
-            r[3 * i + j] = acc;
-        }
-}
-// SU3::Norm (SU3.h:1058-1088)
-__device__ void su3_norm(cd* m) {
-    const double d1 = 1.0 / sqrt(m[0].x * m[0].x + m[0].y * m[0].y + m[1].x * m[1].x + m[1].y * m[1].y +
-                                 m[2].x * m[2].x + m[2].y * m[2].y);
-    m[0] = cscale(m[0], d1); m[1] = cscale(m[1], d1); m[2] = cscale(m[2], d1);
-    cd sp = cadd(cadd(cmul(m[0], cconj(m[3])), cmul(m[1], cconj(m[4]))), cmul(m[2], cconj(m[5])));
-    sp = cconj(sp);
-    m[3] = csub(m[3], cmul(sp, m[0])); m[4] = csub(m[4], cmul(sp, m[1])); m[5] = csub(m[5], cmul(sp, m[2]));
-    const double d2 = 1.0 / sqrt(m[3].x * m[3].x + m[3].y * m[3].y + m[4].x * m[4].x + m[4].y * m[4].y +
-                                 m[5].x * m[5].x + m[5].y * m[5].y);
-    m[3] = cscale(m[3], d2); m[4] = cscale(m[4], d2); m[5] = cscale(m[5], d2);
-    rebuild_row2(m);
-}
 // reunitarise(exp(i eps H)) with Taylor scaling-and-squaring (same recipe as the oracle)
 __device__ void random_link(unsigned link, double eps, unsigned seed, cd* out) {
     double g[10];
@@ -136,14 +81,14 @@ __device__ void random_link(unsigned link, double eps, unsigned seed, cd* out) {
     T[0] = T[4] = T[8] = cd{1.0, 0.0};
     for (int i = 0; i < 9; ++i) S[i] = T[i];
     for (int k = 1; k <= 20; ++k) {
-        mat_mul(T, A, tmp);
+        su3_mm(T, A, tmp);
         for (int i = 0; i < 9; ++i) { T[i] = cscale(tmp[i], 1.0 / k); S[i] = cadd(S[i], T[i]); }
     }
     for (int k = 0; k < s; ++k) {
-        mat_mul(S, S, tmp);
+        su3_mm(S, S, tmp);
         for (int i = 0; i < 9; ++i) S[i] = tmp[i];
     }
-    su3_norm(S);
+    su3_reunit(S);
     for (int i = 0; i < 9; ++i) out[i] = S[i];
 }
 
@@ -381,23 +326,20 @@ int gauge_convert(clg_context ctx, clg_field_s* dst, const clg_field_s* src) {
 
 int scratch_field(clg_context ctx, const clg_field_s* like, int kind, int prec, int subset, int slot,
                   clg_field_s** out) {
-    int seen = 0;
+    // one context-owned field per (kind, prec, subset, lattice, slot id)
     for (clg_field_s* f : ctx->scratch) {
-        if (f->kind == kind && f->prec == prec && f->subset == subset && f->dims[0] == like->dims[0] &&
-            f->dims[1] == like->dims[1] && f->dims[2] == like->dims[2] && f->dims[3] == like->dims[3]) {
-            if (seen == slot) {
-                for (int i = 0; i < 4; ++i) f->bc[i] = like->bc[i];
-                *out = f;
-                return CLG_OK;
-            }
-            ++seen;
+        if (f->scratch_slot == slot && f->kind == kind && f->prec == prec && f->subset == subset &&
+            f->dims[0] == like->dims[0] && f->dims[1] == like->dims[1] && f->dims[2] == like->dims[2] &&
+            f->dims[3] == like->dims[3]) {
+            for (int i = 0; i < 4; ++i) f->bc[i] = like->bc[i];
+            *out = f;
+            return CLG_OK;
         }
     }
     clg_field_s* f = nullptr;
-    for (int k = seen; k <= slot; ++k) {
-        CLG_TRY(clg_field_create(ctx, kind, prec, subset, like->dims, &f));
-        ctx->scratch.push_back(f);
-    }
+    CLG_TRY(clg_field_create(ctx, kind, prec, subset, like->dims, &f));
+    f->scratch_slot = slot;
+    ctx->scratch.push_back(f);
     for (int i = 0; i < 4; ++i) f->bc[i] = like->bc[i];
     *out = f;
     return CLG_OK;

@@ -225,7 +225,7 @@ int gmres(clg_context ctx, LinOp& A, clg_field_s* x, const clg_field_s* b, int m
     CLG_CHECK_ARG(m >= 1 && m <= 64, "MaxDim must be in [1, 64]");
     std::vector<clg_field_s*> V(m + 1);
     for (int k = 0; k <= m; ++k)
-        CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 10 + k, &V[k]));
+        CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 1000 + k, &V[k]));
     clg_field_s* w = V[m];
     const int ld = m + 1;
     // device workspace: H, c2, g, y, scalars

@@ -235,6 +235,97 @@ int clg_ks_pf_energy(clg_context ctx, const clg_solver_params* params, int ms_ki
                      clg_field gauge, double two_am, const double* coeffs, int degree, double* energy,
                      clg_solve_report* report);
 
+/* ============================== HMC ============================== */
+/* Gauge, momentum and force fields are FP64 18-real CLG_GAUGE fields.  Momenta are traceless
+ * anti-Hermitian; the kinetic energy is sum ReTr P^dag P; P += eps F; U <- exp(eps P) U. */
+#define CLG_HMC_MAX_ACTIONS 8
+
+/* CFieldGaugeSU3::MakeRandomGenerator (CFieldGaugeSU3.cu:32-82 with EFIT_RandomGenerator,
+ * SU3.h:193-226): 8 Gaussian generator weights of variance 1/4 per link, drawn from a Philox
+ * stream keyed by (seed, counter, link) */
+int clg_gauge_momentum_random(clg_field momentum, unsigned seed, unsigned counter);
+/* CFieldGaugeSU3::CalculateKinematicEnergy (CFieldGaugeSU3.cu:492-516): sum ReTr P^dag P, FP64 */
+int clg_gauge_kinetic_energy(clg_context ctx, clg_field momentum, double* out);
+/* CFieldGaugeSU3::Axpy with a real coefficient (AddForce, CIntegrator.h:101-128): y += a x */
+int clg_gauge_axpy(clg_context ctx, clg_field y, double a, clg_field x);
+/* CFieldGaugeSU3::ExpMult (CFieldGaugeSU3.cu:452-490, 1457-1477): U <- exp(a P) U per link;
+ * exp_precision 0 = SU3::QuickExp (SU3.h:1361-1640), n > 0 = order-n Taylor (SU3.h:1286-1318),
+ * as the YAML key ExponentialPrecision selects (CLGLibManager.cpp:282-289) */
+int clg_gauge_exp_mult(clg_context ctx, clg_field momentum, clg_field gauge, double a, int exp_precision);
+/* CFieldGaugeSU3::ElementNormalize (CFieldGaugeSU3.cu:518-528, SU3::Norm) */
+int clg_gauge_normalize(clg_context ctx, clg_field gauge);
+/* CActionGaugePlaquette::Energy (CActionGaugePlaquette.cpp:14-107, CFieldGaugeSU3.cu:308-371):
+ * beta/3 sum_{x, mu<nu} (3 - ReTr U_mu nu(x)), FP64 */
+int clg_gauge_plaquette_energy(clg_context ctx, clg_field gauge, double beta, double* out);
+/* CActionGaugePlaquette::CalculateForce (CFieldGaugeSU3.cu:194-257): F += -(beta/6) Ta[U staple] */
+int clg_gauge_force(clg_context ctx, clg_field force, clg_field gauge, double beta);
+/* CFieldFermionWilsonSquareSU3::PrepareForHMCS (CFieldFermionWilsonSquareSU3.cu:867-906):
+ * phi = D eta, eta Gaussian with E|eta_i|^2 = 1 */
+int clg_wilson_prepare_hmc(clg_context ctx, clg_field phi, clg_field gauge, double kappa, unsigned seed);
+/* CFieldFermionKSSU3::PrepareForHMC (CFieldFermionKSSU3.cu:391-407): phi = r_MC(DD^dag) eta */
+int clg_ks_prepare_hmc(clg_context ctx, const clg_solver_params* params, int ms_kind, int prec, clg_field phi,
+                       clg_field gauge, double two_am, const double* mc, int degree, unsigned seed,
+                       clg_solve_report* report);
+
+/* CAction subclasses of a trajectory */
+typedef enum {
+    CLG_ACTION_GAUGE_PLAQUETTE = 0,    /* CActionGaugePlaquette (Beta) */
+    CLG_ACTION_FERMION_WILSON_NF2 = 1, /* CActionFermionWilsonNf2 (Hopping) */
+    CLG_ACTION_FERMION_KS = 2          /* CActionFermionKS (Mass = 2am, MC, MD) */
+} clg_action_kind;
+typedef struct {
+    int kind;                 /* clg_action_kind */
+    double beta;              /* plaquette Beta */
+    double kappa_or_mass;     /* Wilson Hopping or staggered Mass (= 2am) */
+    clg_field phi;            /* pseudofermion: FULL FP64 field of the fermion kind (its bc apply) */
+    const double* mc;         /* KS: MC rational [c, a_1..a_n, b_1..b_n] (heat-bath) */
+    int mc_degree;
+    const double* md;         /* KS: MD rational (force and energy) */
+    int md_degree;
+    clg_solver_params solver; /* Wilson: the DD^dag / D solves; KS: the multi-shift solve */
+    int ms_kind;              /* KS: clg_multishift_kind */
+    int ms_prec;              /* KS: CLG_F64, or CLG_F32 inner multi-shift with FP64 shift sums */
+} clg_action;
+
+/* IntegratorType */
+typedef enum {
+    CLG_INT_LEAPFROG = 0,              /* CIntegratorLeapFrog (CIntegratorLeapFrog.h) */
+    CLG_INT_NESTED_FORCE_GRADIENT = 1  /* CIntegratorNestedForceGradient (CIntegratorNestedForceGradient.cpp:18-157) */
+} clg_integrator_kind;
+typedef struct {
+    int integrator;      /* clg_integrator_kind */
+    double step_length;  /* IntegratorStepLength: trajectory length (CIntegrator.cpp:63-66) */
+    int steps;           /* IntegratorStep */
+    int nested_steps;    /* NestedStep (CIntegrator.cpp:281-297) */
+    int inner_leapfrog;  /* InnerLeapfrog: nested leapfrog inside the force-gradient steps */
+    int exp_precision;   /* ExponentialPrecision: 0 quick exponential, n > 0 Taylor order n */
+    int metropolis;      /* Metropolis */
+} clg_hmc_params;
+typedef struct {
+    double h_before, h_after; /* kinetic + every action, FP64 (CIntegrator::GetEnergy) */
+    double h_diff;            /* h_before - h_after (CHMC.cpp:118-124) */
+    double kinetic_before, kinetic_after;
+    double action_before[CLG_HMC_MAX_ACTIONS], action_after[CLG_HMC_MAX_ACTIONS];
+    int accepted;
+    int force_evaluations;    /* force passes (gauge or fermion) */
+    int solver_iterations;    /* solver cycles summed over every solve of the trajectory */
+    int unconverged_solves;
+    double seconds;
+} clg_hmc_report;
+
+/* defaults: nested force gradient, tau 1, 8 steps, 3 nested, quick exponential, Metropolis on */
+int clg_hmc_params_default(clg_hmc_params* p);
+/* CIntegrator::Evaluate between two GetEnergy calls (CHMC.cpp:100-108) with the caller's momentum
+ * and pseudofermions (no heat-bath); gauge and momentum are updated in place and the gauge field
+ * is re-unitarised at the end (FinishEvaluate, CIntegrator.cpp:244-254). */
+int clg_hmc_evolve(clg_context ctx, clg_field gauge, clg_field momentum, const clg_hmc_params* params,
+                   const clg_action* actions, int nactions, clg_hmc_report* report);
+/* CHMC::Update for one trajectory (CHMC.cpp:76-219): pseudofermion heat-bath of every fermion
+ * action, momentum noise, H before, integrator, H after, Metropolis accept on exp(h_diff); a
+ * rejected trajectory restores the gauge field.  All randomness derives from seed. */
+int clg_hmc_trajectory(clg_context ctx, clg_field gauge, const clg_hmc_params* params, clg_action* actions,
+                       int nactions, unsigned long long seed, clg_hmc_report* report);
+
 /* ============================== measurement ============================== */
 /* Times `reps` back-to-back launches of one Wilson operator on the context stream with HIP
  * events; writes the average milliseconds per launch.  variant: 0 full-lattice D,

@@ -922,6 +922,301 @@ int orc_ks_force_kernel(const int dims[4], const int bc[4], const double* U, con
     return 0;
 }
 
+
+// ---------------- gauge action (CActionGaugePlaquette.cpp:14-107, CFieldGaugeSU3.cu:194-371) ----------------
+// sum_x sum_{mu<nu} (3 - ReTr U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag), periodic links
+int orc_plaquette_sum(const int dims[4], const double* U, double* out) {
+    if (!dims_ok(dims)) return -1;
+    Lat lat(dims);
+    const int per[4] = {1, 1, 1, 1};
+    Moves m = build_moves(lat, per);
+    long double acc = 0;
+    for (long s = 0; s < lat.V; ++s)
+        for (int mu = 0; mu < 4; ++mu)
+            for (int nu = mu + 1; nu < 4; ++nu) {
+                const long xm = m.fwd[s * 4 + mu], xn = m.fwd[s * 4 + nu];
+                Mat<double> P = mmul(load_mat(U + (s * 4 + mu) * 18), load_mat(U + (xm * 4 + nu) * 18));
+                P = mmul(P, dag(load_mat(U + (xn * 4 + mu) * 18)));
+                P = mmul(P, dag(load_mat(U + (s * 4 + nu) * 18)));
+                acc += 3.0 - (P.m[0].re + P.m[4].re + P.m[8].re);
+            }
+    *out = (double)acc;
+    return 0;
+}
+
+// F_mu(x) += -(beta/6) Ta[U_mu(x) C_mu(x)], C = sum over the 6 staples that close the plaquettes
+// through the link (_kernelStapleAtSiteSU3CacheIndex, CFieldGaugeSU3.cu:194-257, betaOverN * -1/2)
+int orc_gauge_force(const int dims[4], const double* U, double beta, double* Fo) {
+    if (!dims_ok(dims)) return -1;
+    Lat lat(dims);
+    const int per[4] = {1, 1, 1, 1};
+    Moves m = build_moves(lat, per);
+    for (long s = 0; s < lat.V; ++s)
+        for (int mu = 0; mu < 4; ++mu) {
+            Mat<double> C;
+            for (int k = 0; k < 9; ++k) C.m[k] = mk(0.0, 0.0);
+            const long xpm = m.fwd[s * 4 + mu];
+            for (int nu = 0; nu < 4; ++nu) {
+                if (nu == mu) continue;
+                const long xpn = m.fwd[s * 4 + nu], xmn = m.bwd[s * 4 + nu];
+                const long xpm_mn = m.bwd[xpm * 4 + nu];
+                // upper: U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
+                Mat<double> up = mmul(load_mat(U + (xpm * 4 + nu) * 18), dag(load_mat(U + (xpn * 4 + mu) * 18)));
+                up = mmul(up, dag(load_mat(U + (s * 4 + nu) * 18)));
+                // lower: U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu)
+                Mat<double> lo = mmul(dag(load_mat(U + (xpm_mn * 4 + nu) * 18)), dag(load_mat(U + (xmn * 4 + mu) * 18)));
+                lo = mmul(lo, load_mat(U + (xmn * 4 + nu) * 18));
+                for (int k = 0; k < 9; ++k) C.m[k] = C.m[k] + up.m[k] + lo.m[k];
+            }
+            Mat<double> f = mmul(load_mat(U + (s * 4 + mu) * 18), C);
+            su3_ta(f);
+            for (int q = 0; q < 9; ++q) {
+                Fo[(s * 4 + mu) * 18 + 2 * q] += -beta / 6.0 * f.m[q].re;
+                Fo[(s * 4 + mu) * 18 + 2 * q + 1] += -beta / 6.0 * f.m[q].im;
+            }
+        }
+    return 0;
+}
+
+
+// ---------------- HMC support: exponentials, kinetic energy, integrators ----------------
+namespace {
+
+// SU(2) Cayley map embedded at rows/cols (i, j): for X = [[i y, z], [-z*, -i y]],
+// (1 + X)(1 - X)^-1 = ((1 - l) + 2X) / (1 + l), l = y^2 + |z|^2
+Mat<double> cayley_embed(int i, int j, double y, cx<double> z) {
+    Mat<double> r;
+    for (int k = 0; k < 9; ++k) r.m[k] = mk(0.0, 0.0);
+    r.m[0] = r.m[4] = r.m[8] = mk(1.0, 0.0);
+    const double l = y * y + abs2(z), d = 1.0 / (1.0 + l);
+    const cx<double> c = mk(d * (1.0 - l), 2.0 * y * d);
+    const cx<double> o = scale(z, 2.0 * d);
+    r.m[3 * i + i] = c;
+    r.m[3 * j + j] = conj(c);
+    r.m[3 * i + j] = o;
+    r.m[3 * j + i] = mk(-o.re, o.im);
+    return r;
+}
+
+// SU3::QuickExp (SU3.h:1361-1640), for traceless anti-Hermitian M: the symmetric product
+// C01 C02 C12 C02 C01 of SU(2) Cayley maps; C01 and C02 carry a quarter, C12 half of the
+// off-diagonal entries, and the diagonal is split so the product reproduces exp(a M) to O(a^3)
+Mat<double> quick_exp(const Mat<double>& M, double a) {
+    const double y1 = a * (M.m[0].im - M.m[4].im) / 12.0;
+    const double y2 = a * (M.m[0].im - M.m[8].im) / 12.0;
+    const double y3 = a * (M.m[4].im - M.m[8].im) / 6.0;
+    const Mat<double> c01 = cayley_embed(0, 1, y1, scale(M.m[1], a / 4.0));
+    const Mat<double> c02 = cayley_embed(0, 2, y2, scale(M.m[2], a / 4.0));
+    const Mat<double> c12 = cayley_embed(1, 2, y3, scale(M.m[5], a / 2.0));
+    return mmul(c01, mmul(c02, mmul(c12, mmul(c02, c01))));
+}
+
+// SU3::ExpReal (SU3.h:1286-1318): Horner form of the order-n Taylor polynomial of exp(a M)
+Mat<double> taylor_exp(const Mat<double>& M, double a, int n) {
+    Mat<double> t;
+    for (int k = 0; k < 9; ++k) t.m[k] = mk(0.0, 0.0);
+    t.m[0] = t.m[4] = t.m[8] = mk(1.0, 0.0);
+    for (int i = 0; i < n; ++i) {
+        Mat<double> s;
+        for (int k = 0; k < 9; ++k) s.m[k] = scale(M.m[k], a / (double)(n - i));
+        t = mmul(s, t);
+        t.m[0].re += 1.0; t.m[4].re += 1.0; t.m[8].re += 1.0;
+    }
+    return t;
+}
+
+Mat<double> su3_exp(const Mat<double>& M, double a, int prec) {
+    return prec == 0 ? quick_exp(M, a) : taylor_exp(M, a, prec);
+}
+
+struct Hmc {
+    const int* dims;
+    long V;
+    double* U;
+    double* P;
+    const orc_action* acts;
+    int nact;
+    int exp_prec;
+    std::vector<double> F, Ubak;
+
+    // U <- exp(a M) U for every link (CFieldGaugeSU3::ExpMult, CFieldGaugeSU3.cu:452-490,1457-1477)
+    void exp_mult(const double* M, double a, double* W) {
+        for (long l = 0; l < V * 4; ++l) {
+            const Mat<double> e = su3_exp(load_mat(M + 18 * l), a, exp_prec);
+            store_mat(W + 18 * l, mmul(e, load_mat(W + 18 * l)));
+        }
+    }
+    void fermion_force(const orc_action& A, double* Fo) {
+        if (A.kind == 1) {  // Wilson Nf=2, CFieldFermionWilsonSquareSU3.cu:1016-1100
+            const long n = V * 24;
+            std::vector<double> X(n), Y(n);
+            orc_operator op{0, ORC_DDDAG, {dims[0], dims[1], dims[2], dims[3]},
+                            {A.bc[0], A.bc[1], A.bc[2], A.bc[3]}, A.kappa_or_mass};
+            std::vector<double> zero(n, 0.0);
+            int cyc; double dev;
+            orc_gmres(&op, U, A.phi, X.data(), zero.data(), 30, 400, A.accuracy, 1.0, 1, &cyc, &dev);
+            orc_wilson_op(dims, A.bc, A.kappa_or_mass, U, X.data(), Y.data(), ORC_DDAG, 0, 1.0, 0.0);
+            orc_wilson_force_kernel(dims, A.bc, A.kappa_or_mass, U, Y.data(), X.data(), Fo);
+        } else if (A.kind == 2) {  // KS rational, CFieldFermionKSSU3.cu:412-497
+            const int np = A.md_degree;
+            const long n = V * 6;
+            std::vector<double> xs((size_t)np * n), dx((size_t)np * n), sh(2 * np);
+            for (int k = 0; k < np; ++k) { sh[2 * k] = A.md[1 + np + k]; sh[2 * k + 1] = 0.0; }
+            orc_operator op{1, ORC_DDDAG, {dims[0], dims[1], dims[2], dims[3]},
+                            {A.bc[0], A.bc[1], A.bc[2], A.bc[3]}, A.kappa_or_mass};
+            double b2[2];
+            orc_dot(A.phi, A.phi, V * 3, b2);
+            int cyc; double err;
+            orc_msfom(&op, U, A.phi, xs.data(), sh.data(), np, 40, 400, A.accuracy, std::sqrt(b2[0]), &cyc, &err);
+            for (int k = 0; k < np; ++k)
+                orc_ks_op(dims, A.bc, 0.0, U, xs.data() + (size_t)k * n, dx.data() + (size_t)k * n, ORC_D, 0, 1.0, 0.0);
+            orc_ks_force_kernel(dims, A.bc, U, xs.data(), dx.data(), A.md + 1, np, Fo);
+        }
+    }
+    // UpdatePF / UpdatePG (CIntegrator.cpp:305-357): P += step * F(fermion or gauge actions)
+    void update_p(double step, bool fermion) {
+        std::fill(F.begin(), F.end(), 0.0);
+        force_into(F.data(), fermion);
+        for (long i = 0; i < V * 72; ++i) P[i] += step * F[i];
+    }
+    void force_into(double* Fo, bool fermion) {
+        for (int i = 0; i < nact; ++i) {
+            if (fermion && acts[i].kind != 0) fermion_force(acts[i], Fo);
+            if (!fermion && acts[i].kind == 0) orc_gauge_force(dims, U, acts[i].beta, Fo);
+        }
+    }
+    // force-gradient step: P += step * F(exp(fg F) U), U restored afterwards
+    void fg_update(double fg, double step, bool fermion) {
+        std::fill(F.begin(), F.end(), 0.0);
+        force_into(F.data(), fermion);
+        std::copy(U, U + V * 72, Ubak.begin());
+        exp_mult(F.data(), fg, U);
+        update_p(step, fermion);
+        std::copy(Ubak.begin(), Ubak.end(), U);
+    }
+    void update_u(double step) { exp_mult(P, step, U); }
+
+    // CIntegratorNestedForceGradient::NestedEvaluate (CIntegratorNestedForceGradient.cpp:110-157)
+    void nested_fg(double h, int nsteps) {
+        update_p(h / 6.0, false);
+        for (int s = 1; s <= nsteps; ++s) {
+            update_u(0.5 * h);
+            fg_update(h * h / 24.0, 2.0 * h / 3.0, false);
+            update_u(0.5 * h);
+            update_p(s < nsteps ? h / 3.0 : h / 6.0, false);
+        }
+    }
+    // CNestedIntegrator::NestedEvaluateLeapfrog (CIntegrator.cpp:359-380)
+    void nested_leapfrog(double h, int nsteps) {
+        update_p(0.5 * h, false);
+        for (int s = 1; s <= nsteps; ++s) {
+            update_u(h);
+            update_p(s < nsteps ? h : 0.5 * h, false);
+        }
+    }
+    double energy() {
+        double e = 0.0;
+        for (long i = 0; i < V * 72; ++i) e += P[i] * P[i];
+        for (int i = 0; i < nact; ++i) {
+            const orc_action& A = acts[i];
+            if (A.kind == 0) {
+                double ps;
+                orc_plaquette_sum(dims, U, &ps);
+                e += A.beta / 3.0 * ps;
+            } else if (A.kind == 1) {  // |D^-1 phi|^2, CActionFermionWilsonNf2.cpp:53-68
+                const long n = V * 24;
+                std::vector<double> X(n), zero(n, 0.0);
+                orc_operator op{0, ORC_D, {dims[0], dims[1], dims[2], dims[3]},
+                                {A.bc[0], A.bc[1], A.bc[2], A.bc[3]}, A.kappa_or_mass};
+                int cyc; double dev;
+                orc_gmres(&op, U, A.phi, X.data(), zero.data(), 30, 400, A.accuracy, 1.0, 1, &cyc, &dev);
+                double d[2];
+                orc_dot(X.data(), X.data(), n / 2, d);
+                e += d[0];
+            } else {  // Re phi^dag r_MD(DD^dag) phi, CActionFermionKS.cpp:52-71
+                const int np = A.md_degree;
+                const long n = V * 6;
+                std::vector<double> xs((size_t)np * n), sh(2 * np);
+                for (int k = 0; k < np; ++k) { sh[2 * k] = A.md[1 + np + k]; sh[2 * k + 1] = 0.0; }
+                orc_operator op{1, ORC_DDDAG, {dims[0], dims[1], dims[2], dims[3]},
+                                {A.bc[0], A.bc[1], A.bc[2], A.bc[3]}, A.kappa_or_mass};
+                double b2[2];
+                orc_dot(A.phi, A.phi, V * 3, b2);
+                int cyc; double err;
+                orc_msfom(&op, U, A.phi, xs.data(), sh.data(), np, 40, 400, A.accuracy, std::sqrt(b2[0]), &cyc, &err);
+                double acc = A.md[0] * b2[0];
+                for (int k = 0; k < np; ++k) {
+                    double d[2];
+                    orc_dot(A.phi, xs.data() + (size_t)k * n, V * 3, d);
+                    acc += A.md[1 + k] * d[0];
+                }
+                e += acc;
+            }
+        }
+        return e;
+    }
+};
+
+}  // namespace
+
+int orc_su3_exp(const double* M, double a, int prec, double* out) {
+    if (!M || !out || prec < 0) return -1;
+    store_mat(out, su3_exp(load_mat(M), a, prec));
+    return 0;
+}
+
+int orc_hmc_evolve(const int dims[4], double* U, double* P, const orc_action* acts, int nact, int integrator,
+                   double tau, int steps, int nested, int exp_prec, double* h_before, double* h_after) {
+    if (!dims_ok(dims) || !U || !P || nact < 1 || steps < 1 || nested < 1 || exp_prec < 0) return -1;
+    Hmc h;
+    h.dims = dims;
+    h.V = Lat(dims).V;
+    h.U = U;
+    h.P = P;
+    h.acts = acts;
+    h.nact = nact;
+    h.exp_prec = exp_prec;
+    h.F.assign(h.V * 72, 0.0);
+    h.Ubak.assign(h.V * 72, 0.0);
+    if (h_before) *h_before = h.energy();
+    const double eps = tau / steps;
+    if (integrator == ORC_INT_LEAPFROG) {
+        // CIntegratorLeapFrog::Evaluate (CIntegratorLeapFrog.h): all actions in one force
+        auto update_all = [&](double st) {
+            std::fill(h.F.begin(), h.F.end(), 0.0);
+            h.force_into(h.F.data(), false);
+            h.force_into(h.F.data(), true);
+            for (long i = 0; i < h.V * 72; ++i) P[i] += st * h.F[i];
+        };
+        update_all(0.5 * eps);
+        for (int s = 1; s <= steps; ++s) {
+            h.update_u(eps);
+            update_all(s < steps ? eps : 0.5 * eps);
+        }
+    } else if (integrator == ORC_INT_NESTED_FORCE_GRADIENT || integrator == ORC_INT_NESTED_FG_INNER_LEAPFROG) {
+        // CIntegratorNestedForceGradient::Evaluate (CIntegratorNestedForceGradient.cpp:26-108);
+        // the nested step is half the outer one over `nested` sub-steps (its Initial, :18-23)
+        const double hn = 0.5 * eps / nested;
+        auto inner = [&]() {
+            if (integrator == ORC_INT_NESTED_FORCE_GRADIENT) h.nested_fg(hn, nested);
+            else h.nested_leapfrog(hn, nested);
+        };
+        h.update_p(eps / 6.0, true);
+        for (int s = 1; s <= steps; ++s) {
+            inner();
+            h.fg_update(eps * eps / 24.0, 2.0 * eps / 3.0, true);
+            inner();
+            h.update_p(s < steps ? eps / 3.0 : eps / 6.0, true);
+        }
+    } else {
+        return -1;
+    }
+    // FinishEvaluate (CIntegrator.cpp:244-254): re-unitarise every link
+    orc_su3_norm(U, h.V * 4);
+    if (h_after) *h_after = h.energy();
+    return 0;
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

@@ -104,6 +104,34 @@ int orc_ks_force_kernel(const int dims[4], const int bc[4], const double* U,
                         const double* phis, const double* dphis, const double* numer,
                         int npole, double* F);
 
+/* ---------------- gauge action ---------------- */
+/* sum over sites and mu<nu of (3 - ReTr plaquette), periodic gauge links; S_g = beta/3 * this */
+int orc_plaquette_sum(const int dims[4], const double* U, double* out);
+/* plaquette-action force F += -(beta/6) Ta[U C] (CFieldGaugeSU3.cu:194-257) */
+int orc_gauge_force(const int dims[4], const double* U, double beta, double* F);
+
+/* ---------------- HMC ---------------- */
+/* out = exp(a M) for one traceless anti-Hermitian 3x3 M: prec 0 = SU3::QuickExp
+ * (SU3.h:1361-1640), prec n > 0 = order-n Taylor polynomial SU3::ExpReal (SU3.h:1286-1318) */
+int orc_su3_exp(const double* M, double a, int prec, double* out);
+
+/* one action of the trajectory (CAction subclasses) */
+typedef struct {
+    int kind;              /* 0 CActionGaugePlaquette, 1 CActionFermionWilsonNf2, 2 CActionFermionKS */
+    double beta;           /* plaquette action */
+    double kappa_or_mass;  /* Wilson hopping / KS Mass (= 2am) */
+    int bc[4];             /* fermion boundary signs */
+    const double* phi;     /* pseudofermion (AoS) */
+    const double* md;      /* KS MD rational [c, a_1..a_n, b_1..b_n] */
+    int md_degree;
+    double accuracy;       /* oracle solve accuracy (relative) */
+} orc_action;
+enum { ORC_INT_LEAPFROG = 0, ORC_INT_NESTED_FORCE_GRADIENT = 1, ORC_INT_NESTED_FG_INNER_LEAPFROG = 2 };
+/* CIntegrator::Evaluate with the given momentum P and pseudofermions (no heat-bath); U and P
+ * are updated in place, U re-unitarised at the end; h_before/h_after = kinetic + actions */
+int orc_hmc_evolve(const int dims[4], double* U, double* P, const orc_action* acts, int nact, int integrator,
+                   double tau, int steps, int nested, int exp_prec, double* h_before, double* h_after);
+
 /* ---------------- support ---------------- */
 int orc_num_threads(void);
 /* wall-clock seconds of `reps` Wilson D applications, FP64 (prec 0) or FP32 (prec 1) */

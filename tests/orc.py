@@ -244,3 +244,77 @@ def from_complex(z):
     out[0::2] = z.real
     out[1::2] = z.imag
     return out
+
+
+def plaquette_sum(dims, U):
+    out = C.c_double()
+    assert lib().orc_plaquette_sum(_i4(dims), _dp(U), C.byref(out)) == 0
+    return out.value
+
+
+def gauge_force(dims, U, beta, F=None):
+    F = np.zeros(volume(dims) * 72) if F is None else F.copy()
+    assert lib().orc_gauge_force(_i4(dims), _dp(U), C.c_double(beta), _dp(F)) == 0
+    return F
+
+
+def su3_exp(M, a, prec):
+    M = np.ascontiguousarray(M, np.float64)
+    out = np.zeros(18)
+    assert lib().orc_su3_exp(_dp(M), C.c_double(a), prec, _dp(out)) == 0
+    return out
+
+
+class Action(C.Structure):
+    _fields_ = [("kind", C.c_int), ("beta", C.c_double), ("kappa_or_mass", C.c_double),
+                ("bc", C.c_int * 4), ("phi", C.POINTER(C.c_double)), ("md", C.POINTER(C.c_double)),
+                ("md_degree", C.c_int), ("accuracy", C.c_double)]
+
+
+INT_LEAPFROG, INT_NESTED_FG, INT_NESTED_FG_INNER_LEAPFROG = 0, 1, 2
+
+
+def hmc_evolve(dims, U, P, actions, integrator=INT_NESTED_FG, tau=1.0, steps=2, nested=2, exp_prec=0):
+    """actions: list of dicts kind/beta/kappa_or_mass/bc/phi/md/accuracy.  Returns U, P, H0, H1."""
+    U = np.ascontiguousarray(U, np.float64).copy()
+    P = np.ascontiguousarray(P, np.float64).copy()
+    keep = []
+    arr = (Action * len(actions))()
+    for i, a in enumerate(actions):
+        arr[i].kind = a["kind"]
+        arr[i].beta = a.get("beta", 0.0)
+        arr[i].kappa_or_mass = a.get("kappa_or_mass", 0.0)
+        for d in range(4):
+            arr[i].bc[d] = a.get("bc", BC_FERMION)[d]
+        if "phi" in a:
+            phi = np.ascontiguousarray(a["phi"], np.float64)
+            keep.append(phi)
+            arr[i].phi = _dp(phi)
+        if "md" in a:
+            md = np.ascontiguousarray(a["md"], np.float64)
+            keep.append(md)
+            arr[i].md = _dp(md)
+            arr[i].md_degree = (len(md) - 1) // 2
+        arr[i].accuracy = a.get("accuracy", 1e-13)
+    h0, h1 = C.c_double(), C.c_double()
+    rc = lib().orc_hmc_evolve(_i4(dims), _dp(U), _dp(P), arr, len(actions), integrator,
+                              C.c_double(tau), steps, nested, exp_prec, C.byref(h0), C.byref(h1))
+    assert rc == 0
+    return U, P, h0.value, h1.value
+
+
+def random_momentum(rng, nlinks, sigma=0.5):
+    """traceless anti-Hermitian links in the generator layout of the reference (8 real N(0, sigma^2))"""
+    r = rng.normal(0.0, sigma, (nlinks, 8))
+    i3, i32 = 1 / np.sqrt(3.0), 2 / np.sqrt(3.0)
+    M = np.zeros((nlinks, 9), complex)
+    M[:, 0] = 1j * (r[:, 7] * i3 + r[:, 2])
+    M[:, 1] = r[:, 1] + 1j * r[:, 0]
+    M[:, 2] = r[:, 4] + 1j * r[:, 3]
+    M[:, 3] = -r[:, 1] + 1j * r[:, 0]
+    M[:, 4] = 1j * (r[:, 7] * i3 - r[:, 2])
+    M[:, 5] = r[:, 6] + 1j * r[:, 5]
+    M[:, 6] = -r[:, 4] + 1j * r[:, 3]
+    M[:, 7] = -r[:, 6] + 1j * r[:, 5]
+    M[:, 8] = -1j * r[:, 7] * i32
+    return from_complex(M.ravel())

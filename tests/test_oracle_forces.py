@@ -95,3 +95,21 @@ def test_ks_force_finite_difference():
     fd = richardson(lambda u: ks_action(u, phi, two_am, coeffs), U, T)
     pred = predicted(F, T)
     assert abs(fd - pred) < 1e-8 * abs(pred), (fd, pred)
+
+
+def test_gauge_force_finite_difference():
+    rng = np.random.default_rng(13)
+    beta = 5.5
+    U = orc.gauge_random(DIMS, 0.4, 1)
+    F = orc.gauge_force(DIMS, U, beta)
+    T = random_algebra(rng, orc.volume(DIMS) * 4)
+    fd = richardson(lambda u: beta / 3.0 * orc.plaquette_sum(DIMS, u), U, T)
+    pred = predicted(F, T)
+    assert abs(fd - pred) < 1e-8 * abs(pred), (fd, pred)
+
+
+def test_plaquette_of_unit_and_random_fields():
+    assert orc.plaquette_sum(DIMS, orc.gauge_unit(DIMS)) == 0.0
+    U = orc.gauge_random(DIMS, 0.4, 1)
+    s = orc.plaquette_sum(DIMS, U)
+    assert 0 < s < 6 * orc.volume(DIMS) * 4.5
