@@ -10,7 +10,10 @@ _kernelDFermionWilsonSquareSU3) in FP64 to one right-hand side already resident 
 spinors, 8 links, the site spinor in and out; no cache reuse counted) times the volume,
 divided by the measured time per step.  The same line carries FP32 / 12-real-link Dslash
 rates, the even-odd Schur operator, GMRES(12) solve times to |r|/|b| < 1e-10 in plain FP64
-and in FP32/FP16-inner defect correction, and the CPU oracle's Dslash on the host cores.
+and in FP32/FP16-inner defect correction, the CPU oracle's Dslash on the host cores, and one
+staggered RHMC trajectory at 24^3x8 (2+1 flavours, 12-pole rational functions on log-spaced
+shifts in [1e-4, 10], nested force gradient 8 outer x 6 nested steps, beta 5.5) run once with
+FP32 multi-shift force solves and once in plain FP64, with Delta H of both.
 """
 import argparse
 import json
@@ -42,6 +45,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-hmc", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="only run the FP64 D steps plus 5 in-place scalings of one FP64 Wilson field "
                          "(a read+write pass of known size used to calibrate rocprofv3 byte counters)")
@@ -149,6 +153,10 @@ def main():
         extra["solve_speedup_mixed_fp16link_vs_fp64"] = round(
             results["fp64"].seconds / results["mixed_fp16link"].seconds, 3)
 
+    # ---------------- staggered RHMC trajectory (2+1 flavours)
+    if not args.no_hmc:
+        extra.update(rhmc_trajectory(ctx, clg, np))
+
     # ---------------- CPU baseline: the oracle's Wilson D on the host cores (bounded sample)
     cpu = None
     if not args.no_cpu:
@@ -194,6 +202,58 @@ def main():
     }
     print(json.dumps(line), flush=True)
     ctx.close()
+
+
+HMC_DIMS = (24, 24, 24, 8)
+
+
+def fit_rational(power, shifts, lo, hi):
+    """[c, a_1..a_n, b_1..b_n] with fixed shifts b: least-squares fit of x^power on [lo, hi] in
+    relative error (synthetic RHMC coefficients)."""
+    import numpy as np
+    x = np.logspace(np.log10(lo), np.log10(hi), 600)
+    A = np.concatenate([np.ones((x.size, 1)), 1.0 / (x[:, None] + shifts[None, :])], axis=1)
+    w = x ** (-power)
+    coef, *_ = np.linalg.lstsq(A * w[:, None], np.ones_like(x), rcond=None)
+    err = np.abs((A @ coef) * w - 1).max()
+    return np.concatenate([coef, shifts]), float(err)
+
+
+def rhmc_trajectory(ctx, clg, np):
+    shifts = np.logspace(-4, 1, 12)
+    flavours = []
+    for mass, nf in ((0.01, 2), (0.05, 1)):
+        md, err_md = fit_rational(-nf / 4.0, shifts, mass * mass, 20.0)
+        mc, err_mc = fit_rational(nf / 8.0, shifts, mass * mass, 20.0)
+        flavours.append((mass, md, mc, max(err_md, err_mc)))
+    out = {}
+    for tag, prec in (("fp32", clg.F32), ("fp64", clg.F64)):
+        g = clg.Field(ctx, clg.GAUGE, clg.F64, HMC_DIMS)
+        g.gauge_random(0.4, 1)
+        acts = [dict(kind=clg.ACTION_GAUGE_PLAQUETTE, beta=5.5)]
+        phis = []
+        for mass, md, mc, _ in flavours:
+            phi = clg.Field(ctx, clg.KS, clg.F64, HMC_DIMS, bc=(1, 1, 1, -1))
+            phis.append(phi)
+            sp = clg.solver_params(kind=clg.KS, max_dim=40, restart=200, absolute_accuracy=1,
+                                   accuracy=1e-5 if prec == clg.F32 else 1e-8)
+            acts.append(dict(kind=clg.ACTION_FERMION_KS, kappa_or_mass=mass, phi=phi, md=md, mc=mc, solver=sp,
+                             ms_kind=clg.MS_FOM, ms_prec=prec, energy_accuracy=1e-9))
+        hp = clg.hmc_params(integrator=clg.INT_NESTED_FORCE_GRADIENT, step_length=1.0, steps=8, nested_steps=6,
+                            exp_precision=8, metropolis=1)
+        rep = clg.hmc_trajectory(ctx, g, hp, acts, 1)
+        log(f"RHMC {tag}: dH {rep.h_diff:.6f} (H {rep.h_before:.4f} -> {rep.h_after:.4f}), "
+            f"{rep.seconds:.2f} s, {rep.force_evaluations} force passes, {rep.solver_iterations} solver cycles, "
+            f"unconverged {rep.unconverged_solves}, accepted {rep.accepted}")
+        out[f"rhmc_{tag}_dH"] = float(f"{rep.h_diff:.9g}")
+        out[f"rhmc_{tag}_s"] = round(rep.seconds, 3)
+        out[f"rhmc_{tag}_solver_cycles"] = rep.solver_iterations
+        out[f"rhmc_{tag}_unconverged"] = rep.unconverged_solves
+        del g, phis
+    out["rhmc_dH_abs_diff_fp32_vs_fp64"] = float(f"{abs(out['rhmc_fp32_dH'] - out['rhmc_fp64_dH']):.3g}")
+    out["rhmc_rational_fit_max_rel_err"] = float(f"{max(f[3] for f in flavours):.2g}")
+    out["rhmc_config"] = "24^3x8 KS 2+1 (Mass 0.01, 0.05), 12 poles, NFG 8x6, beta 5.5, eps 0.4"
+    return out
 
 
 if __name__ == "__main__":

@@ -66,7 +66,8 @@ class SolveReport(C.Structure):
 class Action(C.Structure):
     _fields_ = [("kind", C.c_int), ("beta", C.c_double), ("kappa_or_mass", C.c_double), ("phi", C.c_void_p),
                 ("mc", C.POINTER(C.c_double)), ("mc_degree", C.c_int), ("md", C.POINTER(C.c_double)),
-                ("md_degree", C.c_int), ("solver", SolverParams), ("ms_kind", C.c_int), ("ms_prec", C.c_int)]
+                ("md_degree", C.c_int), ("solver", SolverParams), ("energy_accuracy", C.c_double),
+                ("ms_kind", C.c_int), ("ms_prec", C.c_int)]
 
 
 class HmcParams(C.Structure):
@@ -503,6 +504,7 @@ class _Actions:
                     setattr(t, key, cf.ctypes.data_as(C.POINTER(C.c_double)))
                     setattr(t, key + "_degree", (cf.size - 1) // 2)
             t.solver = a.get("solver", solver_params())
+            t.energy_accuracy = a.get("energy_accuracy", 0.0)
             t.ms_kind = a.get("ms_kind", MS_FOM)
             t.ms_prec = a.get("ms_prec", F64)
 

@@ -493,14 +493,17 @@ struct Integrator {
             double e = 0.0;
             clg_solve_report r{};
             int st = CLG_OK;
+            clg_solver_params ep = a.solver;
+            if (a.energy_accuracy > 0) ep.accuracy = a.energy_accuracy;
             if (a.kind == CLG_ACTION_GAUGE_PLAQUETTE) {
                 CLG_TRY(plaquette_dev(ctx, U, dv));
                 CLG_TRY(fetch_doubles(ctx, dv, 1, &e));
                 e *= a.beta / 3.0;
             } else if (a.kind == CLG_ACTION_FERMION_WILSON_NF2) {
-                st = clg_wilson_pf_energy(ctx, &a.solver, a.phi, U, a.kappa_or_mass, &e, &r);
+                st = clg_wilson_pf_energy(ctx, &ep, a.phi, U, a.kappa_or_mass, &e, &r);
             } else {
-                st = clg_ks_pf_energy(ctx, &a.solver, a.ms_kind, a.ms_prec, a.phi, U, a.kappa_or_mass, a.md,
+                // energies enter H_before - H_after, so their shift solves stay FP64 whatever ms_prec
+                st = clg_ks_pf_energy(ctx, &ep, a.ms_kind, CLG_F64, a.phi, U, a.kappa_or_mass, a.md,
                                       a.md_degree, &e, &r);
             }
             if (st != CLG_OK && st != CLG_NOT_CONVERGED) return st;

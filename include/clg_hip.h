@@ -283,8 +283,10 @@ typedef struct {
     const double* md;         /* KS: MD rational (force and energy) */
     int md_degree;
     clg_solver_params solver; /* Wilson: the DD^dag / D solves; KS: the multi-shift solve */
+    double energy_accuracy;   /* accuracy of the energy solves; 0 = solver.accuracy */
     int ms_kind;              /* KS: clg_multishift_kind */
-    int ms_prec;              /* KS: CLG_F64, or CLG_F32 inner multi-shift with FP64 shift sums */
+    int ms_prec;              /* KS: CLG_F64, or CLG_F32 multi-shift (FP64 shift sums) in the force and
+                                 heat-bath solves; the energy solves are always FP64 */
 } clg_action;
 
 /* IntegratorType */
