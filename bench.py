@@ -156,6 +156,7 @@ def main():
     # ---------------- staggered RHMC trajectory (2+1 flavours)
     if not args.no_hmc:
         extra.update(rhmc_trajectory(ctx, clg, np))
+        extra.update(wilson_hmc_trajectory(ctx, clg))
 
     # ---------------- CPU baseline: the oracle's Wilson D on the host cores (bounded sample)
     cpu = None
@@ -253,6 +254,35 @@ def rhmc_trajectory(ctx, clg, np):
     out["rhmc_dH_abs_diff_fp32_vs_fp64"] = float(f"{abs(out['rhmc_fp32_dH'] - out['rhmc_fp64_dH']):.3g}")
     out["rhmc_rational_fit_max_rel_err"] = float(f"{max(f[3] for f in flavours):.2g}")
     out["rhmc_config"] = "24^3x8 KS 2+1 (Mass 0.01, 0.05), 12 poles, NFG 8x6, beta 5.5, eps 0.4"
+    return out
+
+
+def wilson_hmc_trajectory(ctx, clg):
+    """One Wilson Nf=2 HMC trajectory at 32^3x16 with the reference's published set-up (plaquette
+    beta 2.5, kappa 0.1355, nested force gradient 8 outer x 6 nested steps, exponential order 8),
+    on the synthetic eps 0.5 gauge field: even-odd GMRES(12) force solves to |r|/|b| < 1e-8, in
+    plain FP64 and with FP32 inner defect correction; energies to 1e-10."""
+    out = {}
+    for tag, inner, inner_acc in (("fp64", clg.F64, 0.0), ("mixed_fp32", clg.F32, 1e-4)):
+        g = clg.Field(ctx, clg.GAUGE, clg.F64, DIMS)
+        g.gauge_random(0.5, 1)
+        phi = clg.Field(ctx, clg.WILSON, clg.F64, DIMS, bc=(1, 1, 1, -1))
+        sp = clg.solver_params(max_dim=12, restart=500, accuracy=1e-8, stop_rule=clg.STOP_TRUE_RESIDUAL, even_odd=1,
+                               inner_prec=inner, inner_accuracy=inner_acc, max_outer=60)
+        acts = [dict(kind=clg.ACTION_GAUGE_PLAQUETTE, beta=2.5),
+                dict(kind=clg.ACTION_FERMION_WILSON_NF2, kappa_or_mass=KAPPA, phi=phi, solver=sp,
+                     energy_accuracy=1e-10)]
+        hp = clg.hmc_params(integrator=clg.INT_NESTED_FORCE_GRADIENT, step_length=1.0, steps=8, nested_steps=6,
+                            exp_precision=8, metropolis=1)
+        rep = clg.hmc_trajectory(ctx, g, hp, acts, 1)
+        log(f"Wilson HMC {tag}: dH {rep.h_diff:.6f} (H {rep.h_before:.3f} -> {rep.h_after:.3f}), {rep.seconds:.2f} s, "
+            f"{rep.solver_iterations} solver cycles, unconverged {rep.unconverged_solves}, accepted {rep.accepted}")
+        out[f"wilson_hmc_{tag}_s"] = round(rep.seconds, 3)
+        out[f"wilson_hmc_{tag}_dH"] = float(f"{rep.h_diff:.9g}")
+        out[f"wilson_hmc_{tag}_solver_cycles"] = rep.solver_iterations
+        out[f"wilson_hmc_{tag}_unconverged"] = rep.unconverged_solves
+        del g, phi
+    out["wilson_hmc_config"] = "32^3x16 Wilson Nf=2 kappa 0.1355, beta 2.5, NFG 8x6, exp order 8, eps 0.5 gauge"
     return out
 
 

@@ -219,6 +219,17 @@ bool tuning_enabled() {
     const char* e = std::getenv("CLG_TUNE");
     return !(e && e[0] == '0');
 }
+// CLG_TUNE_FORCE="pin,block,xcd" fixes the launch configuration of every hop launch (used to
+// profile one configuration under counters, where event timings are not representative)
+bool forced_cfg(HopCfg& c) {
+    const char* e = std::getenv("CLG_TUNE_FORCE");
+    if (!e) return false;
+    int pm = 0, blk = 0, xcd = 0;
+    if (std::sscanf(e, "%d,%d,%d", &pm, &blk, &xcd) != 3) return false;
+    if ((pm != 1 && pm != 2) || (blk != 64 && blk != 128 && blk != 256) || (xcd != 0 && xcd != 8)) return false;
+    c = HopCfg{pm, blk, xcd};
+    return true;
+}
 
 int launch_hop(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool hasx, hipStream_t st) {
     const HopCfg dflt{1, 256, 8};
@@ -227,6 +238,7 @@ int launch_hop(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool 
     k.prec = prec; k.gprec = gprec; k.recon = recon; k.dag = dag; k.hasx = hasx; k.nblk = a.nblk;
     k.L0 = a.g.L[0]; k.L1 = a.g.L[1]; k.L2 = a.g.L[2]; k.L3 = a.g.L[3];
     HopCfg cfg = dflt;
+    if (forced_cfg(cfg)) return launch_cfg(a, prec, gprec, recon, dag, hasx, st, cfg);
     bool need_tune = false;
     {
         std::lock_guard<std::mutex> lk(tune_mutex());
