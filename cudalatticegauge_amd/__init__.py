@@ -42,6 +42,7 @@ EXPORTED = [
     "clg_gauge_momentum_random", "clg_gauge_kinetic_energy", "clg_gauge_axpy", "clg_gauge_exp_mult",
     "clg_gauge_normalize", "clg_gauge_plaquette_energy", "clg_gauge_force", "clg_wilson_prepare_hmc",
     "clg_ks_prepare_hmc", "clg_hmc_params_default", "clg_hmc_evolve", "clg_hmc_trajectory",
+    "clg_wilson_hop_batch", "clg_solve_batch",
 ]
 
 
@@ -157,6 +158,10 @@ def lib():
                                          C.POINTER(HmcReport)]),
             "clg_hmc_trajectory": (C.c_int, [vp, vp, C.POINTER(HmcParams), C.POINTER(Action), C.c_int, C.c_ulonglong,
                                              C.POINTER(HmcReport)]),
+            "clg_wilson_hop_batch": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.c_int, dp,
+                                               C.POINTER(vp), C.c_double, C.c_int]),
+            "clg_solve_batch": (C.c_int, [vp, C.POINTER(SolverParams), C.c_int, C.POINTER(vp), C.POINTER(vp),
+                                          C.POINTER(vp), C.c_double, C.c_int, C.POINTER(SolveReport)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -521,3 +526,21 @@ def hmc_trajectory(ctx, gauge, params, actions, seed):
     rep = HmcReport()
     check(lib().clg_hmc_trajectory(ctx.h, gauge.h, C.byref(params), acts.arr, len(actions), seed, C.byref(rep)))
     return rep
+
+
+# ---------------------------------------------------------------- batches
+def wilson_hop_batch(ctx, outs, ins, gauges, a, dagger=False, xs=None, b=0.0, tile=0):
+    n = len(outs)
+    av = np.ascontiguousarray(a if np.ndim(a) else [a] * n, dtype=np.float64)
+    check(lib().clg_wilson_hop_batch(ctx.h, n, _handles(outs), _handles(ins), _handles(gauges), int(dagger),
+                                     av.ctypes.data_as(C.POINTER(C.c_double)),
+                                     _handles(xs) if xs is not None else None, b, tile))
+
+
+def solve_batch(ctx, params, xs, bs, gauges, kappa, op=EFO_F_InverseD):
+    n = len(xs)
+    reps = (SolveReport * n)()
+    st = lib().clg_solve_batch(ctx.h, C.byref(params), n, _handles(xs), _handles(bs), _handles(gauges), kappa, op,
+                               reps)
+    check(st, allow=(OK, NOT_CONVERGED))
+    return st, list(reps)

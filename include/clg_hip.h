@@ -328,6 +328,24 @@ int clg_hmc_evolve(clg_context ctx, clg_field gauge, clg_field momentum, const c
 int clg_hmc_trajectory(clg_context ctx, clg_field gauge, const clg_hmc_params* params, clg_action* actions,
                        int nactions, unsigned long long seed, clg_hmc_report* report);
 
+/* ============================== batched operators and solves ============================== */
+/* Batched Wilson hop: out_i = a_i H in_i + b x_i for n <= 16 slots in one launch (slots may live on
+ * lattices of different extents; x may be NULL).  Same parity/layout rules as clg_wilson_hop for
+ * half fields; FULL fields give the hop over both parities.  FP64 spinors with FP64 links, or FP32
+ * spinors with FP32 18-real or FP16 12-real links.  tile = right-hand sides per thread sharing one
+ * link load (1, 2, 4 (FP32 only)); 0 lets the autotuner choose.  The reference applies D to one
+ * field per launch (CFieldFermionWilsonSquareSU3.cu:909-1014); this is its batched form. */
+int clg_wilson_hop_batch(clg_context ctx, int n, clg_field* out, clg_field* in, clg_field* gauge, int dagger,
+                         const double* a, clg_field* x, double b, int tile);
+/* Batched solve of n <= 16 Wilson systems D x_i = b_i (op EFO_F_InverseD) or D^dag x_i = b_i
+ * (EFO_F_InverseDdagger), each on its own FP64 gauge field (lattices may differ), one kappa.  Even-odd
+ * Schur GMRES(MaxDim <= 16) per system in lock step, inner precision params.inner_prec inside FP64
+ * defect correction (CLG_F64 = plain), true-residual stop rule per system.  Replaces the reference's
+ * per-source loop of CSLASolver::Solve over GetSourcesAtSiteFromPool
+ * (CFieldFermionWilsonSquareSU3.cu:1222-1255).  reports (nullable) holds n entries. */
+int clg_solve_batch(clg_context ctx, const clg_solver_params* params, int n, clg_field* x, clg_field* b,
+                    clg_field* gauge, double kappa, int op, clg_solve_report* reports);
+
 /* ============================== measurement ============================== */
 /* Times `reps` back-to-back launches of one Wilson operator on the context stream with HIP
  * events; writes the average milliseconds per launch.  variant: 0 full-lattice D,
