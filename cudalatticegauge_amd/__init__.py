@@ -26,6 +26,8 @@ MS_FOM, MS_BICGSTAB = 0, 1
 STOP_REFERENCE, STOP_TRUE_RESIDUAL = 0, 1
 OK, NOT_CONVERGED = 0, 1
 ACTION_GAUGE_PLAQUETTE, ACTION_FERMION_WILSON_NF2, ACTION_FERMION_KS = 0, 1, 2
+DEFLATE_REV, DEFLATE_HEV, DEFLATE_SVD = 0, 1, 2
+PHASE_START, PHASE_IN, PHASE_END, PHASE_ONCE = 0, 1, 2, 3
 INT_LEAPFROG, INT_NESTED_FORCE_GRADIENT = 0, 1
 HMC_MAX_ACTIONS = 8
 
@@ -42,7 +44,8 @@ EXPORTED = [
     "clg_gauge_momentum_random", "clg_gauge_kinetic_energy", "clg_gauge_axpy", "clg_gauge_exp_mult",
     "clg_gauge_normalize", "clg_gauge_plaquette_energy", "clg_gauge_force", "clg_wilson_prepare_hmc",
     "clg_ks_prepare_hmc", "clg_hmc_params_default", "clg_hmc_evolve", "clg_hmc_trajectory",
-    "clg_wilson_hop_batch", "clg_solve_batch",
+    "clg_wilson_hop_batch", "clg_solve_batch", "clg_gcrodr_params_default", "clg_gcrodr_create",
+    "clg_gcrodr_destroy", "clg_gcrodr_reset", "clg_gcrodr_solve",
 ]
 
 
@@ -62,6 +65,12 @@ class SolveReport(C.Structure):
     _fields_ = [("converged", C.c_int), ("cycles", C.c_int), ("outer", C.c_int),
                 ("operator_applies", C.c_int), ("true_residual", C.c_double),
                 ("last_deviation", C.c_double), ("seconds", C.c_double)]
+
+
+class GcrodrParams(C.Structure):
+    _fields_ = [("m_dim", C.c_int), ("k_dim", C.c_int), ("restart", C.c_int), ("recalculate_r", C.c_int),
+                ("accuracy", C.c_double), ("absolute_accuracy", C.c_int), ("deflate", C.c_int), ("kind", C.c_int),
+                ("even_odd", C.c_int)]
 
 
 class Action(C.Structure):
@@ -162,6 +171,11 @@ def lib():
                                                C.POINTER(vp), C.c_double, C.c_int]),
             "clg_solve_batch": (C.c_int, [vp, C.POINTER(SolverParams), C.c_int, C.POINTER(vp), C.POINTER(vp),
                                           C.POINTER(vp), C.c_double, C.c_int, C.POINTER(SolveReport)]),
+            "clg_gcrodr_params_default": (C.c_int, [C.POINTER(GcrodrParams)]),
+            "clg_gcrodr_create": (C.c_int, [vp, C.POINTER(GcrodrParams), C.POINTER(vp)]),
+            "clg_gcrodr_destroy": (C.c_int, [vp]),
+            "clg_gcrodr_reset": (C.c_int, [vp]),
+            "clg_gcrodr_solve": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, vp, C.POINTER(SolveReport)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -544,3 +558,41 @@ def solve_batch(ctx, params, xs, bs, gauges, kappa, op=EFO_F_InverseD):
                                reps)
     check(st, allow=(OK, NOT_CONVERGED))
     return st, list(reps)
+
+
+# ---------------------------------------------------------------- GCRO-DR
+class Gcrodr:
+    """CSLASolverGCRODR with its recycled subspace kept between solves."""
+
+    def __init__(self, ctx, **kw):
+        p = GcrodrParams()
+        check(lib().clg_gcrodr_params_default(C.byref(p)))
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise CLGError(f"unknown GCRODR parameter {k}")
+            setattr(p, k, v)
+        self.ctx = ctx
+        self.params = p
+        self.h = C.c_void_p()
+        check(lib().clg_gcrodr_create(ctx.h, C.byref(p), C.byref(self.h)))
+
+    def solve(self, x, b, gauge, kappa_or_mass, op=EFO_F_D, phase=PHASE_ONCE, start=None):
+        rep = SolveReport()
+        st = lib().clg_gcrodr_solve(self.h, x.h, b.h, gauge.h, kappa_or_mass, op, phase,
+                                    start.h if start is not None else None, C.byref(rep))
+        check(st, allow=(OK, NOT_CONVERGED))
+        return st, rep
+
+    def reset(self):
+        check(lib().clg_gcrodr_reset(self.h))
+
+    def close(self):
+        if self.h:
+            lib().clg_gcrodr_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

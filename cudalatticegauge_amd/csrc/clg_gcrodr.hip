@@ -1,0 +1,632 @@
+// clg_gcrodr.hip -- GCRO-DR: restarted GMRES with deflation by a recycled subspace (Parks, de Sturler,
+// Mackey, Johnson, Maiti 2006), as CSLASolverGCRODR runs it (CSolverGCRODR.cpp:196-398).
+//
+// State: k recycled vectors U (kept between solves, as the reference keeps m_lstU) and, within a
+// solve, C = A U with C^H C = I, the Krylov vectors V and the (m+1) x m matrix G = [[D, B], [0, H]].
+// Per solve:
+//   * with a recycled U and phase In/End trajectory: C = A U, thin QR, U <- U R^-1 (GenerateCUFirstTime);
+//     otherwise one GMRES(m) cycle, harmonic Ritz vectors P_k, U = V_m P_k R^-1, C = V_{m+1} Q with
+//     H P_k = Q R (FirstTimeGMERESSolve / FindPk1 / GenerateCU);
+//   * cycles: x += U C^H r, r -= C C^H r, Arnoldi on (1 - C C^H) A from r/|r|, least squares in
+//     G, new P_k from the generalized eigenproblem of the cycle (FindPk2), U and C regenerated.
+// The deflation space (DeflateSpace) is REV (Ritz), HEV (harmonic) or SVD (default) as in FindPk1 /
+// FindPk2.  Stop rule: |r| < Accuracy * (AbsoluteAccuracy ? 1 : |b|^2), checked after the first
+// stage and after every cycle, as the reference checks it.
+// The MI355X form: the Arnoldi step is classical Gram-Schmidt with a conditional second pass (one
+// multi-dot and one multi-axpy launch over C and V together), the G column stays on the device,
+// and the tall-skinny products U <- [U V] P R^-1, C <- [C V] Q run as one in-place kernel each
+// (the reference's _kernelMatrixMultiply plus FieldSolveY's k^2 axpys, CFieldFermionWilsonSquareSU3.cu:1525-1606).
+// The small dense problems (least squares, QR, eigenproblems; m <= 31) run in FP64 on the host.
+#include "clg_internal.h"
+#include "clg_smallla.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+using namespace clg;
+using namespace clgd;
+using clg::sla::cx;
+using clg::sla::Mat;
+
+namespace clg {
+int blas_axpby(clg_context, clg_field_s*, double, double, const clg_field_s*, double, double);
+int blas_norm2_dev(clg_context, const clg_field_s*, double*);
+int blas_dot_dev(clg_context, const clg_field_s*, const clg_field_s*, double*);
+int blas_multidot_dev(clg_context, clg_field_s* const*, int, const clg_field_s*, double*, double*, const int*);
+int blas_multiaxpy_dev(clg_context, clg_field_s*, clg_field_s* const*, int, const double*, double, double*, const int*,
+                       const double*, double, int*);
+int fetch_doubles(clg_context, const double*, int, double*);
+int workspace(clg_context, size_t, double**);
+int linop_apply(clg_context, int, int, const clg_field_s*, double, clg_field_s*, const clg_field_s*);
+int wilson_hop_half(clg_context, clg_field_s*, const clg_field_s*, const clg_field_s*, bool, double, double,
+                    const clg_field_s*, double, double);
+int wilson_schur_once(clg_context, clg_field_s*, const clg_field_s*, const clg_field_s*, double, bool, clg_field_s*);
+}  // namespace clg
+
+struct clg_gcrodr_s {
+    clg_context ctx = nullptr;
+    clg_gcrodr_params p{};
+    std::vector<clg_field_s*> U;  // recycled vectors (owned)
+    int dims[4] = {0, 0, 0, 0};
+    int subset = -1, kind = -1;
+    bool have_u = false;
+};
+
+namespace {
+
+constexpr int kMaxDimDR = 31;  // MDim limit of the reference (_kMaxDimDR, CSolverGCRODR.h)
+
+// out_i = sum_{j<X} in_j M[j*Y + i] (i < Y) for FP64 complex vectors of n elements; out may be the
+// first Y inputs (each thread reads all inputs of its element before writing)
+struct VmmArgs {
+    const double2* in[kMaxDimDR + 1];
+    double2* out[kMaxDimDR];
+    int X, Y;
+    long n;
+};
+__global__ __launch_bounds__(256) void k_vmm(VmmArgs a, const double* __restrict__ M) {
+    __shared__ double2 sm[(kMaxDimDR + 1) * kMaxDimDR];
+    const int tot = a.X * a.Y;
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) sm[i] = make_double2(M[2 * i], M[2 * i + 1]);
+    __syncthreads();
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < a.n; e += (long)gridDim.x * blockDim.x) {
+        double2 acc[kMaxDimDR];
+#pragma unroll
+        for (int i = 0; i < kMaxDimDR; ++i) acc[i] = make_double2(0.0, 0.0);
+        for (int j = 0; j < a.X; ++j) {
+            const double2 v = a.in[j][e];
+            const double2* row = sm + j * a.Y;
+#pragma unroll
+            for (int i = 0; i < kMaxDimDR; ++i) {
+                if (i < a.Y) {
+                    const double2 m = row[i];
+                    acc[i].x = fma(v.x, m.x, fma(-v.y, m.y, acc[i].x));
+                    acc[i].y = fma(v.x, m.y, fma(v.y, m.x, acc[i].y));
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kMaxDimDR; ++i)
+            if (i < a.Y) a.out[i][e] = acc[i];
+    }
+}
+
+// column j of the device G (column-major, ld rows): rows 0..nr-1 = c1 (+ c2 when the second pass
+// ran), row j+1 = |w|
+__global__ void k_gcol(double* G, int ld, int j, int nr, const double* c1, const double* c2, const int* flag,
+                       const double* wn2) {
+    const int i = threadIdx.x;
+    if (i < nr) {
+        double re = c1[2 * i], im = c1[2 * i + 1];
+        if (*flag) {
+            re += c2[2 * i];
+            im += c2[2 * i + 1];
+        }
+        G[2 * (j * ld + i)] = re;
+        G[2 * (j * ld + i) + 1] = im;
+    }
+    if (i == 0) {
+        G[2 * (j * ld + j + 1)] = sqrt(wn2[0]);
+        G[2 * (j * ld + j + 1) + 1] = 0.0;
+    }
+}
+
+struct Gcro {
+    clg_gcrodr_s* S;
+    clg_context ctx;
+    const clg_gcrodr_params* p;
+    int m, k;
+    const clg_field_s* gauge;
+    double param;
+    int op;        // operator A (EFO_F_D .. EFO_F_DDdagger), or the Schur D / D^dag with schur
+    bool schur;
+    clg_field_s* todd = nullptr;
+    std::vector<clg_field_s*> C, V;  // C[0..k), V[0..m-k]
+    clg_field_s *X = nullptr, *Wr = nullptr, *tmp = nullptr;
+    double* dws = nullptr;  // device scratch: G, coefficient vectors, scalars
+    size_t offG = 0, offC1 = 0, offC2 = 0, offS = 0, offM = 0;
+    int applies = 0, cycles = 0;
+    double dev = 0;
+
+    clg_field_s* W(int j) { return j < k ? C[j] : V[j - k]; }           // GetW
+    clg_field_s* Vv(int j) { return j < k ? S->U[j] : V[j - k]; }       // GetV
+
+    int apply(clg_field_s* out, const clg_field_s* in) {
+        ++applies;
+        if (schur) return wilson_schur_once(ctx, out, in, gauge, param, op == EFO_F_Ddagger, todd);
+        return linop_apply(ctx, p->kind, op, gauge, param, out, in);
+    }
+    int norm(const clg_field_s* f, double& out) {
+        CLG_TRY(blas_norm2_dev(ctx, f, dws + offS));
+        double v;
+        CLG_TRY(fetch_doubles(ctx, dws + offS, 1, &v));
+        out = std::sqrt(v);
+        return CLG_OK;
+    }
+    int scale(clg_field_s* f, double a) { return blas_axpby(ctx, f, a, 0.0, f, 0.0, 0.0); }
+    int dot(const clg_field_s* a, const clg_field_s* b, cx& out) {
+        CLG_TRY(blas_dot_dev(ctx, a, b, dws + offS));
+        double z[2];
+        CLG_TRY(fetch_doubles(ctx, dws + offS, 2, z));
+        out = cx(z[0], z[1]);
+        return CLG_OK;
+    }
+    int upload(const std::vector<cx>& v, double* dst) {
+        CLG_HIP(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(cx), hipMemcpyHostToDevice, ctx->stream));
+        CLG_HIP(hipStreamSynchronize(ctx->stream));  // the host vector is a temporary
+        return CLG_OK;
+    }
+    // y += s * sum_j c_j vecs_j with host coefficients
+    int axpys(clg_field_s* y, const std::vector<clg_field_s*>& vecs, const std::vector<cx>& c, double s) {
+        if (vecs.empty()) return CLG_OK;
+        CLG_TRY(upload(c, dws + offC1));
+        return blas_multiaxpy_dev(ctx, y, vecs.data(), (int)vecs.size(), dws + offC1, s, nullptr, nullptr, nullptr, 0.0,
+                                  nullptr);
+    }
+    // out_i = sum_j in_j M(j, i), in place allowed when out are the first inputs
+    int vmm(const std::vector<clg_field_s*>& in, const std::vector<clg_field_s*>& out, const Mat& M) {
+        VmmArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.X = (int)in.size();
+        a.Y = (int)out.size();
+        CLG_CHECK_ARG(a.X <= kMaxDimDR + 1 && a.Y <= kMaxDimDR && M.r == a.X && M.c == a.Y, "bad VMM shape");
+        a.n = (long)(in[0]->bytes / sizeof(double2));
+        for (int j = 0; j < a.X; ++j) a.in[j] = (const double2*)in[j]->data;
+        for (int i = 0; i < a.Y; ++i) a.out[i] = (double2*)out[i]->data;
+        std::vector<cx> mv(M.a.begin(), M.a.end());
+        CLG_TRY(upload(mv, dws + offM));
+        const unsigned grid = (unsigned)std::max<long>(1, std::min<long>((a.n + 255) / 256, (long)ctx->num_cu * 8));
+        k_vmm<<<grid, 256, 0, ctx->stream>>>(a, dws + offM);
+        CLG_HIP(hipGetLastError());
+        return CLG_OK;
+    }
+    // w <- w - sum_j W_j (W_j^H w) over W_0..W_{nw-1} with a conditional second pass; column j of G
+    int orthogonalise(clg_field_s* w, int nw, int j) {
+        std::vector<clg_field_s*> list(nw);
+        for (int i = 0; i < nw; ++i) list[i] = W(i);
+        double* c1 = dws + offC1;
+        double* c2 = dws + offC2;
+        double* sc = dws + offS;  // sc[1] |w'|^2, sc[4] |w|^2
+        int* flag = reinterpret_cast<int*>(sc + 8);
+        CLG_TRY(blas_multidot_dev(ctx, list.data(), nw, w, c1, sc + 4, nullptr));
+        CLG_TRY(blas_multiaxpy_dev(ctx, w, list.data(), nw, c1, -1.0, sc + 1, nullptr, sc + 4, 0.5, flag));
+        CLG_TRY(blas_multidot_dev(ctx, list.data(), nw, w, c2, nullptr, flag));
+        CLG_TRY(blas_multiaxpy_dev(ctx, w, list.data(), nw, c2, -1.0, sc + 1, flag, nullptr, 0.0, nullptr));
+        k_gcol<<<1, 64, 0, ctx->stream>>>(dws + offG, m + 1, j, nw, c1, c2, flag, sc + 1);
+        CLG_HIP(hipGetLastError());
+        return CLG_OK;
+    }
+    int fetch_G(Mat& G) {
+        std::vector<double> g(2 * (size_t)(m + 1) * m);
+        CLG_TRY(fetch_doubles(ctx, dws + offG, (int)g.size(), g.data()));
+        G = Mat(m + 1, m);
+        for (int j = 0; j < m; ++j)
+            for (int i = 0; i <= m; ++i) G(i, j) = cx(g[2 * (j * (m + 1) + i)], g[2 * (j * (m + 1) + i) + 1]);
+        return CLG_OK;
+    }
+    int zero_G() {
+        std::vector<cx> z((size_t)(m + 1) * m, cx(0.0, 0.0));
+        return upload(z, dws + offG);
+    }
+    // r = b - A x
+    int residual(clg_field_s* r, const clg_field_s* x, const clg_field_s* b) {
+        CLG_TRY(apply(r, x));
+        return blas_axpby(ctx, r, 1.0, 0.0, b, -1.0, 0.0);
+    }
+    // Gram matrix entries  list_i^H w_j  (rows i, columns j)
+    int gram(const std::vector<clg_field_s*>& rows, const std::vector<clg_field_s*>& cols, Mat& out) {
+        out = Mat((int)rows.size(), (int)cols.size());
+        std::vector<clg_field_s*> r = rows;
+        for (size_t j = 0; j < cols.size(); ++j) {
+            CLG_TRY(blas_multidot_dev(ctx, r.data(), (int)r.size(), cols[j], dws + offC1, nullptr, nullptr));
+            std::vector<double> v(2 * r.size());
+            CLG_TRY(fetch_doubles(ctx, dws + offC1, (int)v.size(), v.data()));
+            for (size_t i = 0; i < r.size(); ++i) out((int)i, (int)j) = cx(v[2 * i], v[2 * i + 1]);
+        }
+        return CLG_OK;
+    }
+
+    // ---- deflation vectors (FindPk1 after the first GMRES cycle, FindPk2 after GCRO-DR cycles)
+    int find_pk1(const Mat& H, Mat& P) {
+        Mat Hm(m, m);
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < m; ++j) Hm(i, j) = H(i, j);
+        bool ok = false;
+        if (p->deflate == CLG_DEFLATE_REV) {
+            ok = sla::select_eigvecs(Hm, nullptr, false, k, true, P);
+        } else if (p->deflate == CLG_DEFLATE_HEV) {
+            const Mat B = sla::mul(sla::adj(H), H);
+            ok = sla::select_eigvecs(Hm, &B, false, k, false, P);
+        } else {
+            ok = sla::select_eigvecs(sla::mul(sla::adj(H), H), nullptr, true, k, true, P);
+        }
+        CLG_CHECK_ARG(ok, "GCRODR: deflation eigenproblem failed");
+        return CLG_OK;
+    }
+    int find_pk2(const Mat& G, Mat& P) {
+        std::vector<clg_field_s*> Us(S->U.begin(), S->U.end());
+        bool ok = false;
+        if (p->deflate == CLG_DEFLATE_SVD) {
+            Mat UU;
+            CLG_TRY(gram(Us, Us, UU));
+            Mat B = Mat::eye(m);
+            for (int i = 0; i < k; ++i)
+                for (int j = 0; j < k; ++j)
+                    if (i != j) B(i, j) = UU(i, j);
+            ok = sla::select_eigvecs(sla::mul(sla::adj(G), G), &B, true, k, true, P);
+        } else if (p->deflate == CLG_DEFLATE_HEV) {
+            // ALeft ((m+1) x m): rows of C and V against the columns U, identity elsewhere
+            std::vector<clg_field_s*> rows;
+            for (int i = 0; i <= m; ++i) rows.push_back(W(i));
+            Mat WU;
+            CLG_TRY(gram(rows, Us, WU));
+            Mat AL(m + 1, m);
+            for (int i = 0; i <= m; ++i)
+                for (int j = 0; j < m; ++j) AL(i, j) = j < k ? WU(i, j) : (i == j ? cx(1.0, 0.0) : cx(0.0, 0.0));
+            const Mat A = sla::mul(sla::adj(G), AL), B = sla::mul(sla::adj(G), G);
+            ok = sla::select_eigvecs(A, &B, false, k, false, P);
+        } else {
+            // REV: ALeft (m x (m+1)) = [U^H C, U^H V; 0 I]; B = Gram of [U, V_0..V_{m-k-1}]
+            std::vector<clg_field_s*> cols;
+            for (int j = 0; j <= m; ++j) cols.push_back(W(j));
+            Mat UW, UU;
+            CLG_TRY(gram(Us, cols, UW));
+            CLG_TRY(gram(Us, Us, UU));
+            Mat AL(m, m + 1);
+            for (int i = 0; i < m; ++i)
+                for (int j = 0; j <= m; ++j) AL(i, j) = i < k ? UW(i, j) : (i == j ? cx(1.0, 0.0) : cx(0.0, 0.0));
+            Mat B(m, m);
+            for (int i = 0; i < m; ++i)
+                for (int j = 0; j < m; ++j) {
+                    if (i < k && j < k) B(i, j) = i == j ? cx(1.0, 0.0) : UU(i, j);
+                    else if (i < k) B(i, j) = AL(i, j);
+                    else if (j < k) B(i, j) = std::conj(AL(j, i));
+                    else B(i, j) = i == j ? cx(1.0, 0.0) : cx(0.0, 0.0);
+                }
+            ok = sla::select_eigvecs(sla::mul(AL, G), &B, false, k, true, P);
+        }
+        CLG_CHECK_ARG(ok, "GCRODR: deflation eigenproblem failed");
+        return CLG_OK;
+    }
+    // GenerateCU (CSolverGCRODR.cpp:541-560): G P = Q R; U <- [first m of the basis] P R^-1;
+    // C <- [C, V] Q when update_c
+    int generate_cu(const Mat& G, const Mat& P, bool update_c, bool just_after_gmres) {
+        Mat Q, R;
+        sla::thin_qr(sla::mul(G, P), Q, R);
+        const Mat PR = sla::mul(P, sla::upper_inverse(R));
+        std::vector<clg_field_s*> in, outU(S->U.begin(), S->U.end());
+        for (int j = 0; j < m; ++j) in.push_back(just_after_gmres ? W(j) : Vv(j));
+        CLG_TRY(vmm(in, outU, PR));
+        if (update_c) {
+            std::vector<clg_field_s*> inC, outC(C.begin(), C.end());
+            for (int j = 0; j <= m; ++j) inC.push_back(W(j));
+            CLG_TRY(vmm(inC, outC, Q));
+        }
+        S->have_u = true;
+        return CLG_OK;
+    }
+    // GenerateCUFirstTime: C = A U orthonormalised (thin QR by Gram-Schmidt), U <- U R^-1, r = b - A x
+    int generate_cu_first(clg_field_s* x, clg_field_s* r, const clg_field_s* b) {
+        for (int i = 0; i < k; ++i) CLG_TRY(apply(C[i], S->U[i]));
+        Mat R(k, k);
+        for (int i = 0; i < k; ++i) {
+            for (int pass = 0; pass < 2; ++pass) {
+                if (i == 0) break;
+                std::vector<clg_field_s*> prev(C.begin(), C.begin() + i);
+                CLG_TRY(blas_multidot_dev(ctx, prev.data(), i, C[i], dws + offC1, nullptr, nullptr));
+                std::vector<double> v(2 * i);
+                CLG_TRY(fetch_doubles(ctx, dws + offC1, 2 * i, v.data()));
+                for (int j = 0; j < i; ++j) R(j, i) += cx(v[2 * j], v[2 * j + 1]);
+                CLG_TRY(blas_multiaxpy_dev(ctx, C[i], prev.data(), i, dws + offC1, -1.0, nullptr, nullptr, nullptr, 0.0,
+                                           nullptr));
+            }
+            double n;
+            CLG_TRY(norm(C[i], n));
+            R(i, i) = n;
+            CLG_TRY(scale(C[i], 1.0 / n));
+        }
+        std::vector<clg_field_s*> Us(S->U.begin(), S->U.end());
+        CLG_TRY(vmm(Us, Us, sla::upper_inverse(R)));
+        CLG_TRY(residual(r, x, b));
+        return norm(r, dev);
+    }
+
+    // the solve on the system vectors (full lattice or even sites)
+    int run(clg_field_s* xout, const clg_field_s* b, const clg_field_s* start, int phase, double blen, bool& conv) {
+        const double tol = p->accuracy * blen;
+        CLG_TRY(clg_field_copy(X, const_cast<clg_field_s*>(start ? start : b)));
+        const bool reuse = S->have_u && (phase == CLG_PHASE_IN_TRAJECTORY || phase == CLG_PHASE_END_TRAJECTORY);
+        clg_field_s* r = V[0];  // the reference keeps r in the first Krylov vector
+        if (reuse) {
+            CLG_TRY(generate_cu_first(X, r, b));
+        } else {
+            // one GMRES(m) cycle with its basis in W (FirstTimeGMERESSolve)
+            CLG_TRY(zero_G());
+            clg_field_s* w0 = W(0);
+            CLG_TRY(residual(w0, X, b));
+            double beta;
+            CLG_TRY(norm(w0, beta));
+            CLG_TRY(scale(w0, 1.0 / beta));
+            for (int j = 0; j < m; ++j) {
+                CLG_TRY(apply(W(j + 1), W(j)));
+                CLG_TRY(orthogonalise(W(j + 1), j + 1, j));
+                double wn;
+                CLG_TRY(fetch_doubles(ctx, dws + offS + 1, 1, &wn));
+                CLG_TRY(scale(W(j + 1), 1.0 / std::sqrt(wn)));
+            }
+            ++cycles;
+            Mat H;
+            CLG_TRY(fetch_G(H));
+            std::vector<cx> rhs(m + 1, cx(0.0, 0.0)), y;
+            rhs[0] = beta;
+            sla::hessenberg_lsq(H, rhs, y);
+            std::vector<clg_field_s*> basis;
+            for (int j = 0; j < m; ++j) basis.push_back(W(j));
+            CLG_TRY(axpys(X, basis, y, 1.0));
+            if (p->recalculate_r > 1) {
+                // r = -sum_j (H y - beta e_0)_j W_j
+                std::vector<cx> hy(m + 1, cx(0.0, 0.0));
+                for (int i = 0; i <= m; ++i)
+                    for (int j = 0; j < m; ++j) hy[i] += H(i, j) * y[j];
+                hy[0] -= beta;
+                std::vector<clg_field_s*> full;
+                for (int j = 0; j <= m; ++j) full.push_back(W(j));
+                CLG_TRY(clg_field_zero(Wr));
+                CLG_TRY(axpys(Wr, full, hy, -1.0));
+            } else {
+                CLG_TRY(residual(Wr, X, b));
+            }
+            CLG_TRY(norm(Wr, dev));
+            Mat P;
+            CLG_TRY(find_pk1(H, P));
+            CLG_TRY(generate_cu(H, P, dev >= tol, true));
+            CLG_TRY(clg_field_copy(r, Wr));
+        }
+        if (dev < tol) {
+            conv = true;
+            return clg_field_copy(xout, X);
+        }
+        for (int it = 0; it < p->restart; ++it) {
+            // OrthognalXR: x += U C^H r, r -= C C^H r (coefficients from the unmodified r)
+            {
+                std::vector<clg_field_s*> Cs(C.begin(), C.end()), Us(S->U.begin(), S->U.end());
+                CLG_TRY(blas_multidot_dev(ctx, Cs.data(), k, r, dws + offC1, nullptr, nullptr));
+                CLG_TRY(blas_multiaxpy_dev(ctx, X, Us.data(), k, dws + offC1, 1.0, nullptr, nullptr, nullptr, 0.0,
+                                           nullptr));
+                CLG_TRY(blas_multiaxpy_dev(ctx, r, Cs.data(), k, dws + offC1, -1.0, nullptr, nullptr, nullptr, 0.0,
+                                           nullptr));
+            }
+            // NormUkAndSetD: U_i normalised, D = diag(1/|U_i|) in G
+            CLG_TRY(zero_G());
+            std::vector<double> dinv(k);
+            for (int i = 0; i < k; ++i) {
+                double n;
+                CLG_TRY(norm(S->U[i], n));
+                dinv[i] = 1.0 / n;
+                CLG_TRY(scale(S->U[i], dinv[i]));
+            }
+            CLG_TRY(norm(r, dev));
+            CLG_TRY(scale(r, 1.0 / dev));
+            for (int j = k; j < m; ++j) {
+                CLG_TRY(apply(W(j + 1), W(j)));
+                CLG_TRY(orthogonalise(W(j + 1), j + 1, j));
+                double wn;
+                CLG_TRY(fetch_doubles(ctx, dws + offS + 1, 1, &wn));
+                CLG_TRY(scale(W(j + 1), 1.0 / std::sqrt(wn)));
+            }
+            ++cycles;
+            Mat G;
+            CLG_TRY(fetch_G(G));
+            for (int i = 0; i < k; ++i) {
+                for (int j = 0; j < m; ++j)
+                    if (j < k) G(i, j) = cx(0.0, 0.0);
+                G(i, i) = dinv[i];
+            }
+            // Wr = dev * V_0 (the residual before this cycle's correction)
+            CLG_TRY(blas_axpby(ctx, Wr, dev, 0.0, r, 0.0, 0.0));
+            std::vector<cx> rhs(m + 1, cx(0.0, 0.0)), y;
+            rhs[k] = dev;
+            sla::hessenberg_lsq(G, rhs, y);
+            std::vector<clg_field_s*> basis;
+            for (int j = 0; j < m; ++j) basis.push_back(Vv(j));
+            CLG_TRY(axpys(X, basis, y, 1.0));
+            if (p->recalculate_r != 0 && it != 0 && it % p->recalculate_r == 0) {
+                CLG_TRY(residual(Wr, X, b));
+            } else {
+                std::vector<cx> gy(m + 1, cx(0.0, 0.0));
+                for (int i = 0; i <= m; ++i)
+                    for (int j = 0; j < m; ++j) gy[i] += G(i, j) * y[j];
+                std::vector<clg_field_s*> full;
+                for (int j = 0; j <= m; ++j) full.push_back(W(j));
+                CLG_TRY(axpys(Wr, full, gy, -1.0));
+            }
+            CLG_TRY(norm(Wr, dev));
+            Mat P;
+            CLG_TRY(find_pk2(G, P));
+            if (dev >= tol || phase != CLG_PHASE_END_TRAJECTORY) CLG_TRY(generate_cu(G, P, dev >= tol, false));
+            if (dev < tol) {
+                conv = true;
+                return clg_field_copy(xout, X);
+            }
+            CLG_TRY(clg_field_copy(r, Wr));
+        }
+        conv = false;
+        return clg_field_copy(xout, X);
+    }
+};
+
+int check_params(const clg_gcrodr_params* p) {
+    CLG_CHECK_ARG(p, "null params");
+    CLG_CHECK_ARG(p->m_dim >= 5 && p->m_dim <= kMaxDimDR, "MDim must be in [5, %d]", kMaxDimDR);
+    CLG_CHECK_ARG(p->k_dim >= 2 && p->k_dim <= p->m_dim - 2, "KDim must be in [2, MDim - 2]");
+    CLG_CHECK_ARG(p->restart >= 0 && p->recalculate_r >= 0, "Restart / RecalculateR must be >= 0");
+    CLG_CHECK_ARG(p->accuracy > 0, "Accuracy must be positive");
+    CLG_CHECK_ARG(p->deflate >= CLG_DEFLATE_REV && p->deflate <= CLG_DEFLATE_SVD, "unknown DeflateSpace");
+    CLG_CHECK_ARG(p->kind == CLG_WILSON || p->kind == CLG_KS, "kind must be CLG_WILSON or CLG_KS");
+    CLG_CHECK_ARG(p->even_odd == 0 || p->kind == CLG_WILSON, "even_odd applies to Wilson fermions");
+    return CLG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int clg_gcrodr_params_default(clg_gcrodr_params* p) {
+    CLG_CHECK_ARG(p, "null params");
+    std::memset(p, 0, sizeof(*p));
+    p->m_dim = 10;  // CSolverGCRODR.cpp:69-78 defaults
+    p->k_dim = 8;
+    p->restart = 100;
+    p->recalculate_r = 0;
+    p->accuracy = 1e-8;
+    p->absolute_accuracy = 0;
+    p->deflate = CLG_DEFLATE_SVD;
+    p->kind = CLG_WILSON;
+    p->even_odd = 0;
+    return CLG_OK;
+}
+
+int clg_gcrodr_create(clg_context ctx, const clg_gcrodr_params* params, clg_gcrodr* out) {
+    CLG_CHECK_ARG(ctx && out, "null argument");
+    CLG_TRY(check_params(params));
+    clg_gcrodr_s* s = new clg_gcrodr_s;
+    s->ctx = ctx;
+    s->p = *params;
+    *out = s;
+    return CLG_OK;
+}
+
+int clg_gcrodr_destroy(clg_gcrodr s) {
+    if (!s) return CLG_OK;
+    for (clg_field_s* f : s->U) clg_field_destroy(f);
+    delete s;
+    return CLG_OK;
+}
+
+int clg_gcrodr_reset(clg_gcrodr s) {
+    CLG_CHECK_ARG(s, "null solver");
+    s->have_u = false;
+    return CLG_OK;
+}
+
+int clg_gcrodr_solve(clg_gcrodr s, clg_field x, clg_field b, clg_field gauge, double kappa_or_mass, int op, int phase,
+                     clg_field start, clg_solve_report* rep) {
+    CLG_CHECK_ARG(s && x && b && gauge, "null argument");
+    clg_context ctx = s->ctx;
+    const clg_gcrodr_params* p = &s->p;
+    CLG_CHECK_ARG(op >= EFO_F_D && op <= EFO_F_DDdagger, "op is the operator A (EFO_F_D .. EFO_F_DDdagger)");
+    CLG_CHECK_ARG(!p->even_odd || op == EFO_F_D || op == EFO_F_Ddagger, "even-odd GCRODR takes D or D^dagger");
+    CLG_CHECK_ARG(phase >= CLG_PHASE_START_TRAJECTORY && phase <= CLG_PHASE_ONCE, "unknown solver phase");
+    CLG_CHECK_ARG(x->kind == p->kind && b->kind == p->kind, "field kind does not match params.kind");
+    CLG_CHECK_ARG(x->prec == CLG_F64 && b->prec == CLG_F64 && x->subset == CLG_FULL && b->subset == CLG_FULL,
+                  "GCRODR takes FULL FP64 fields");
+    CLG_TRY(check_same_lattice(x, b));
+    CLG_TRY(check_same_lattice(x, gauge));
+    CLG_CHECK_ARG(gauge->kind == CLG_GAUGE && gauge->prec == CLG_F64, "GCRODR takes FP64 18-real links");
+    if (start) {
+        CLG_CHECK_ARG(start->kind == p->kind && start->prec == CLG_F64 && start->subset == CLG_FULL, "bad start field");
+        CLG_TRY(check_same_lattice(start, b));
+    }
+    (void)hipSetDevice(ctx->device);
+    const auto t0 = std::chrono::steady_clock::now();
+    const int sub = p->even_odd ? CLG_EVEN : CLG_FULL;
+    // (re)allocate the recycled space for this lattice
+    bool same = s->subset == sub && s->kind == p->kind;
+    for (int i = 0; i < 4; ++i) same = same && s->dims[i] == b->dims[i];
+    if (!same || (int)s->U.size() != p->k_dim) {
+        for (clg_field_s* f : s->U) clg_field_destroy(f);
+        s->U.clear();
+        for (int i = 0; i < p->k_dim; ++i) {
+            clg_field_s* f = nullptr;
+            CLG_TRY(clg_field_create(ctx, p->kind, CLG_F64, sub, b->dims, &f));
+            s->U.push_back(f);
+        }
+        for (int i = 0; i < 4; ++i) s->dims[i] = b->dims[i];
+        s->subset = sub;
+        s->kind = p->kind;
+        s->have_u = false;
+    }
+    for (clg_field_s* f : s->U)
+        for (int i = 0; i < 4; ++i) f->bc[i] = b->bc[i];
+    Gcro g;
+    g.S = s;
+    g.ctx = ctx;
+    g.p = p;
+    g.m = p->m_dim;
+    g.k = p->k_dim;
+    g.gauge = gauge;
+    g.param = kappa_or_mass;
+    g.schur = p->even_odd != 0;
+    g.op = op;
+    g.C.resize(g.k);
+    g.V.resize(g.m - g.k + 1);
+    for (int i = 0; i < g.k; ++i) CLG_TRY(scratch_field(ctx, b, p->kind, CLG_F64, sub, 30000 + i, &g.C[i]));
+    for (int i = 0; i <= g.m - g.k; ++i) CLG_TRY(scratch_field(ctx, b, p->kind, CLG_F64, sub, 30100 + i, &g.V[i]));
+    CLG_TRY(scratch_field(ctx, b, p->kind, CLG_F64, sub, 30200, &g.X));
+    CLG_TRY(scratch_field(ctx, b, p->kind, CLG_F64, sub, 30201, &g.Wr));
+    if (g.schur) CLG_TRY(scratch_field(ctx, b, CLG_WILSON, CLG_F64, CLG_ODD, 30202, &g.todd));
+    const int m = g.m;
+    g.offG = 0;
+    g.offC1 = g.offG + 2 * (size_t)(m + 1) * m;
+    g.offC2 = g.offC1 + 2 * (size_t)(m + 2) * (m + 2);
+    g.offS = g.offC2 + 2 * (size_t)(m + 2);
+    g.offM = g.offS + 16;
+    CLG_TRY(workspace(ctx, g.offM + 2 * (size_t)(kMaxDimDR + 1) * kMaxDimDR + 16, &g.dws));
+    // |b|^2 of the full system for the reference stop rule
+    double bn2;
+    CLG_TRY(blas_norm2_dev(ctx, b, g.dws + g.offS));
+    CLG_TRY(fetch_doubles(ctx, g.dws + g.offS, 1, &bn2));
+    const double blen = p->absolute_accuracy ? 1.0 : bn2;
+    bool conv = false;
+    if (!g.schur) {
+        CLG_TRY(g.run(x, b, start, phase, blen, conv));
+    } else {
+        // bhat_e = b_e + kappa H_eo b_o ; Mhat x_e = bhat_e ; x_o = b_o + kappa H_oe x_e
+        const bool dag = op == EFO_F_Ddagger;
+        clg_field_s *be, *bo, *xe, *xo, *se = nullptr;
+        CLG_TRY(clg_field_parity_view(b, CLG_EVEN, &be));
+        CLG_TRY(clg_field_parity_view(b, CLG_ODD, &bo));
+        CLG_TRY(clg_field_parity_view(x, CLG_EVEN, &xe));
+        CLG_TRY(clg_field_parity_view(x, CLG_ODD, &xo));
+        if (start) CLG_TRY(clg_field_parity_view(start, CLG_EVEN, &se));
+        struct Guard {
+            clg_field_s* v[5];
+            ~Guard() {
+                for (auto f : v)
+                    if (f) clg_field_destroy(f);
+            }
+        } guard{{be, bo, xe, xo, se}};
+        clg_field_s* bhat = nullptr;
+        CLG_TRY(scratch_field(ctx, b, CLG_WILSON, CLG_F64, CLG_EVEN, 30203, &bhat));
+        CLG_TRY(wilson_hop_half(ctx, bhat, bo, gauge, dag, kappa_or_mass, 0.0, be, 1.0, 0.0));
+        CLG_TRY(g.run(xe, bhat, se, phase, blen, conv));
+        CLG_TRY(wilson_hop_half(ctx, xo, xe, gauge, dag, kappa_or_mass, 0.0, bo, 1.0, 0.0));
+    }
+    // true residual of the requested full system
+    double r2 = 0;
+    {
+        clg_field_s* t = nullptr;
+        CLG_TRY(scratch_field(ctx, b, p->kind, CLG_F64, CLG_FULL, 30204, &t));
+        const int fop = g.schur ? op : op;
+        CLG_TRY(linop_apply(ctx, p->kind, fop, gauge, kappa_or_mass, t, x));
+        CLG_TRY(blas_axpby(ctx, t, 1.0, 0.0, b, -1.0, 0.0));
+        CLG_TRY(blas_norm2_dev(ctx, t, g.dws + g.offS));
+        CLG_TRY(fetch_doubles(ctx, g.dws + g.offS, 1, &r2));
+    }
+    CLG_HIP(hipStreamSynchronize(ctx->stream));
+    if (rep) {
+        std::memset(rep, 0, sizeof(*rep));
+        rep->converged = conv ? 1 : 0;
+        rep->cycles = g.cycles;
+        rep->outer = 1;
+        rep->operator_applies = g.applies;
+        rep->true_residual = bn2 > 0 ? std::sqrt(r2 / bn2) : std::sqrt(r2);
+        rep->last_deviation = g.dev;
+        rep->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return conv ? CLG_OK : CLG_NOT_CONVERGED;
+}
+
+}  // extern "C"

@@ -189,6 +189,40 @@ int clg_solve(clg_context ctx, const clg_solver_params* params, clg_field x, clg
 /* defaults matching the reference YAML defaults (CSolverGMRES.cpp:35-99) */
 int clg_solver_params_default(clg_solver_params* p);
 
+/* ---- GCRO-DR with a recycled subspace (CSLASolverGCRODR, CSolverGCRODR.h:24-140, .cpp:67-700) ---- */
+typedef struct clg_gcrodr_s* clg_gcrodr;
+/* DeflateSpace (EEigenDeflationType, CSolverGCRODR.h:16-20) */
+typedef enum { CLG_DEFLATE_REV = 0, CLG_DEFLATE_HEV = 1, CLG_DEFLATE_SVD = 2 } clg_deflation;
+/* ESolverPhase (CCommonData.h:168-172); In/End trajectory reuse the recycled subspace */
+typedef enum {
+    CLG_PHASE_START_TRAJECTORY = 0,
+    CLG_PHASE_IN_TRAJECTORY = 1,
+    CLG_PHASE_END_TRAJECTORY = 2,
+    CLG_PHASE_ONCE = 3
+} clg_solver_phase;
+typedef struct {
+    int m_dim;          /* MDim, 5..31 */
+    int k_dim;          /* KDim, 2..MDim-2 (recycled vectors) */
+    int restart;        /* Restart: GCRO-DR cycles after the first stage */
+    int recalculate_r;  /* RecalculateR: recompute b - A x every n cycles (0: recurrence only) */
+    double accuracy;    /* Accuracy: stop when |r| < accuracy * (absolute ? 1 : |b|^2) */
+    int absolute_accuracy;
+    int deflate;        /* clg_deflation */
+    int kind;           /* CLG_WILSON or CLG_KS */
+    int even_odd;       /* Wilson D / D^dag only: run on the even-site Schur complement */
+} clg_gcrodr_params;
+/* defaults of CSolverGCRODR.cpp:67-118 (MDim 10, KDim MDim-2, SVD deflation) */
+int clg_gcrodr_params_default(clg_gcrodr_params* p);
+int clg_gcrodr_create(clg_context ctx, const clg_gcrodr_params* params, clg_gcrodr* out);
+int clg_gcrodr_destroy(clg_gcrodr solver);
+/* forget the recycled subspace */
+int clg_gcrodr_reset(clg_gcrodr solver);
+/* CSLASolverGCRODR::Solve: x = A^-1 b with A = op (EFO_F_D .. EFO_F_DDdagger) of the params.kind
+ * fermion on FULL FP64 fields; phase as ESolverPhase; start may be NULL (x0 = b).  Returns
+ * CLG_NOT_CONVERGED with the last iterate after Restart cycles. */
+int clg_gcrodr_solve(clg_gcrodr solver, clg_field x, clg_field b, clg_field gauge, double kappa_or_mass, int op,
+                     int phase, clg_field start, clg_solve_report* report);
+
 /* ---- multi-shift solvers and the rational approximation (RHMC) ---- */
 typedef enum { CLG_MS_FOM = 0, CLG_MS_BICGSTAB = 1 } clg_multishift_kind;
 

@@ -1217,6 +1217,352 @@ int orc_hmc_evolve(const int dims[4], double* U, double* P, const orc_action* ac
     return 0;
 }
 
+
+// ---------------- GCRO-DR, SVD deflation (CSolverGCRODR.cpp:196-700, CLinearAlgebraHelper.h) ----------------
+namespace {
+
+typedef std::vector<std::vector<cx<double>>> DM;  // dense row-major complex matrix
+DM dm(int r, int c) { return DM(r, std::vector<cx<double>>(c, mk(0.0, 0.0))); }
+DM dmul(const DM& A, const DM& B) {
+    DM C = dm((int)A.size(), (int)B[0].size());
+    for (size_t i = 0; i < A.size(); ++i)
+        for (size_t k = 0; k < B.size(); ++k)
+            for (size_t j = 0; j < B[0].size(); ++j) C[i][j] = C[i][j] + A[i][k] * B[k][j];
+    return C;
+}
+DM dadj(const DM& A) {
+    DM B = dm((int)A[0].size(), (int)A.size());
+    for (size_t i = 0; i < A.size(); ++i)
+        for (size_t j = 0; j < A[0].size(); ++j) B[j][i] = conj(A[i][j]);
+    return B;
+}
+double cabs(cx<double> a) { return std::sqrt(a.re * a.re + a.im * a.im); }
+
+// RotateHenssenbergHost + SolveYHost: Givens least squares of an upper-Hessenberg (m+1) x m system
+std::vector<cx<double>> lsq(DM H, std::vector<cx<double>> g) {
+    const int m = (int)H[0].size();
+    for (int k = 0; k < m; ++k) {
+        const cx<double> a = H[k][k], b = H[k + 1][k];
+        const double den = std::sqrt(abs2(a) + abs2(b));
+        const cx<double> c = scale(a, 1.0 / den), s = scale(b, 1.0 / den);
+        for (int j = k; j < m; ++j) {
+            const cx<double> x = H[k][j], y = H[k + 1][j];
+            H[k][j] = conj(c) * x + conj(s) * y;
+            H[k + 1][j] = c * y - s * x;
+        }
+        const cx<double> x = g[k], y = g[k + 1];
+        g[k] = conj(c) * x + conj(s) * y;
+        g[k + 1] = c * y - s * x;
+    }
+    std::vector<cx<double>> y(m);
+    for (int k = m - 1; k >= 0; --k) {
+        cx<double> acc = g[k];
+        for (int j = k + 1; j < m; ++j) acc = acc - H[k][j] * y[j];
+        y[k] = cdiv(acc, H[k][k]);
+    }
+    return y;
+}
+// Householder-free thin QR by modified Gram-Schmidt (ThinQRFactorizationHost)
+void dqr(const DM& T, DM& Q, DM& R) {
+    const int n = (int)T.size(), k = (int)T[0].size();
+    Q = T;
+    R = dm(k, k);
+    for (int i = 0; i < k; ++i) {
+        double nn = 0;
+        for (int r = 0; r < n; ++r) nn += abs2(Q[r][i]);
+        nn = std::sqrt(nn);
+        R[i][i] = mk(nn, 0.0);
+        for (int r = 0; r < n; ++r) Q[r][i] = scale(Q[r][i], 1.0 / nn);
+        for (int j = i + 1; j < k; ++j) {
+            cx<double> d = mk(0.0, 0.0);
+            for (int r = 0; r < n; ++r) d = d + conj(Q[r][i]) * Q[r][j];
+            R[i][j] = d;
+            for (int r = 0; r < n; ++r) Q[r][j] = Q[r][j] - d * Q[r][i];
+        }
+    }
+}
+// Hermitian eigenproblem by Jacobi sweeps; eigenvectors as columns of V, eigenvalues ascending
+void djacobi(DM A, std::vector<double>& w, DM& V) {
+    const int n = (int)A.size();
+    V = dm(n, n);
+    for (int i = 0; i < n; ++i) V[i][i] = mk(1.0, 0.0);
+    for (int sweep = 0; sweep < 80; ++sweep) {
+        double off = 0, tot = 0;
+        for (int p = 0; p < n; ++p)
+            for (int q = 0; q < n; ++q) { tot += abs2(A[p][q]); if (p != q) off += abs2(A[p][q]); }
+        if (off <= 1e-32 * tot) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double b = cabs(A[p][q]);
+                if (b == 0.0) continue;
+                // unitary 2x2 rotation that zeroes A[p][q]: phase-align, then a real rotation
+                const cx<double> e = scale(A[p][q], 1.0 / b);
+                const double tau = (A[q][q].re - A[p][p].re) / (2.0 * b);
+                const double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+                const double c = 1.0 / std::sqrt(1.0 + t * t), sn = t * c;
+                // columns: new_p = c A_p - sn conj(e) A_q ; new_q = sn e... written as J = [[c, sn], [-sn e*, c e*]]
+                const cx<double> J00 = mk(c, 0.0), J01 = mk(sn, 0.0), J10 = scale(conj(e), -sn), J11 = scale(conj(e), c);
+                for (int r = 0; r < n; ++r) {
+                    const cx<double> ap = A[r][p], aq = A[r][q];
+                    A[r][p] = ap * J00 + aq * J10;
+                    A[r][q] = ap * J01 + aq * J11;
+                    const cx<double> vp = V[r][p], vq = V[r][q];
+                    V[r][p] = vp * J00 + vq * J10;
+                    V[r][q] = vp * J01 + vq * J11;
+                }
+                for (int r = 0; r < n; ++r) {
+                    const cx<double> ap = A[p][r], aq = A[q][r];
+                    A[p][r] = conj(J00) * ap + conj(J10) * aq;
+                    A[q][r] = conj(J01) * ap + conj(J11) * aq;
+                }
+            }
+    }
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::sort(idx.begin(), idx.end(), [&](int a, int b) { return A[a][a].re < A[b][b].re; });
+    DM Vs = dm(n, n);
+    w.resize(n);
+    for (int j = 0; j < n; ++j) {
+        w[j] = A[idx[j]][idx[j]].re;
+        for (int r = 0; r < n; ++r) Vs[r][j] = V[r][idx[j]];
+    }
+    V = Vs;
+}
+// k eigenvectors of A p = lam B p (A Hermitian, B Hermitian positive definite) with the smallest lam
+DM dgen_small(const DM& A, const DM& B, int k) {
+    const int n = (int)A.size();
+    DM L = dm(n, n);
+    for (int j = 0; j < n; ++j) {
+        double d = B[j][j].re;
+        for (int q = 0; q < j; ++q) d -= abs2(L[j][q]);
+        L[j][j] = mk(std::sqrt(d), 0.0);
+        for (int i = j + 1; i < n; ++i) {
+            cx<double> s = B[i][j];
+            for (int q = 0; q < j; ++q) s = s - L[i][q] * conj(L[j][q]);
+            L[i][j] = scale(s, 1.0 / L[j][j].re);
+        }
+    }
+    // C = L^-1 A L^-H
+    auto lsolve = [&](DM M) {
+        for (size_t c = 0; c < M[0].size(); ++c)
+            for (int i = 0; i < n; ++i) {
+                cx<double> s = M[i][c];
+                for (int q = 0; q < i; ++q) s = s - L[i][q] * M[q][c];
+                M[i][c] = scale(s, 1.0 / L[i][i].re);
+            }
+        return M;
+    };
+    const DM Cm = lsolve(dadj(lsolve(dadj(A))));
+    DM Ch = dm(n, n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) Ch[i][j] = scale(Cm[i][j] + conj(Cm[j][i]), 0.5);
+    std::vector<double> w;
+    DM Y;
+    djacobi(Ch, w, Y);
+    DM P = dm(n, k);
+    for (int c = 0; c < k; ++c)
+        for (int i = n - 1; i >= 0; --i) {  // p = L^-H y
+            cx<double> s = Y[i][c];
+            for (int q = i + 1; q < n; ++q) s = s - conj(L[q][i]) * P[q][c];
+            P[i][c] = scale(s, 1.0 / L[i][i].re);
+        }
+    return P;
+}
+DM dupper_inv(const DM& R) {
+    const int k = (int)R.size();
+    DM X = dm(k, k);
+    for (int j = 0; j < k; ++j) {
+        X[j][j] = cdiv(mk(1.0, 0.0), R[j][j]);
+        for (int i = j - 1; i >= 0; --i) {
+            cx<double> acc = mk(0.0, 0.0);
+            for (int l = i + 1; l <= j; ++l) acc = acc + R[i][l] * X[l][j];
+            X[i][j] = cdiv(mk(-acc.re, -acc.im), R[i][i]);
+        }
+    }
+    return X;
+}
+// out_i = sum_j in_j M[j][i]
+std::vector<F> vmul(const std::vector<const F*>& in, const DM& M) {
+    std::vector<F> out(M[0].size(), F(in[0]->size(), 0.0));
+    for (size_t i = 0; i < M[0].size(); ++i)
+        for (size_t j = 0; j < in.size(); ++j) axpy(out[i], M[j][i], *in[j]);
+    return out;
+}
+
+}  // namespace
+
+int orc_gcrodr(const orc_operator* A, const double* Ug, const double* b, double* x, double* urec, int* have_u,
+               int m, int k, int restart, int recalc, double accuracy, int absolute, int phase, int* cycles_out,
+               double* dev_out) {
+    if (!A || !dims_ok(A->dims) || m < 5 || m > 31 || k < 2 || k > m - 2 || !urec || !have_u) return -1;
+    Op op(A, Ug);
+    const long n2 = op.n * 2;
+    F B(b, b + n2);
+    const double bl = absolute ? 1.0 : dotc(B, B).re;
+    const double tol = accuracy * bl;
+    std::vector<F> U(k, F(n2)), Cv(k, F(n2)), V(m - k + 1, F(n2));
+    for (int i = 0; i < k; ++i) std::copy(urec + i * n2, urec + (i + 1) * n2, U[i].begin());
+    auto W = [&](int j) -> F& { return j < k ? Cv[j] : V[j - k]; };
+    auto Vv = [&](int j) -> F& { return j < k ? U[j] : V[j - k]; };
+    auto applyA = [&](const F& in, F& out) { op.apply(in.data(), out.data()); };
+    F X = B, R(n2), Wr(n2), T(n2);
+    auto resid = [&](F& r) {  // r = b - A X
+        applyA(X, r);
+        for (long i = 0; i < n2; ++i) r[i] = B[i] - r[i];
+    };
+    auto nrm = [&](const F& v) { return std::sqrt(dotc(v, v).re); };
+    int cycles = 0;
+    double dev = 0;
+    // GenerateCU: G P = Q R, U = [basis] P R^-1, C = [C V] Q
+    auto gen_cu = [&](const DM& G, const DM& P, bool upd, bool after_gmres) {
+        DM Q, Rr;
+        dqr(dmul(G, P), Q, Rr);
+        const DM PR = dmul(P, dupper_inv(Rr));
+        std::vector<const F*> in;
+        for (int j = 0; j < m; ++j) in.push_back(after_gmres ? &W(j) : &Vv(j));
+        std::vector<F> nu = vmul(in, PR);
+        if (upd) {
+            std::vector<const F*> inc;
+            for (int j = 0; j <= m; ++j) inc.push_back(&W(j));
+            std::vector<F> nc = vmul(inc, Q);
+            for (int i = 0; i < k; ++i) Cv[i] = nc[i];
+        }
+        for (int i = 0; i < k; ++i) U[i] = nu[i];
+        *have_u = 1;
+    };
+    const bool reuse = *have_u && (phase == 1 || phase == 2);
+    F& r = V[0];
+    if (reuse) {
+        // GenerateCUFirstTime: C = A U, MGS QR, U = U R^-1, r = b - A x
+        DM Rr = dm(k, k);
+        for (int i = 0; i < k; ++i) applyA(U[i], Cv[i]);
+        for (int i = 0; i < k; ++i) {
+            const double l = nrm(Cv[i]);
+            Rr[i][i] = mk(l, 0.0);
+            scal(Cv[i], mk(1.0 / l, 0.0));
+            for (int j = i + 1; j < k; ++j) {
+                Rr[i][j] = dotc(Cv[i], Cv[j]);
+                axpy(Cv[j], mk(-Rr[i][j].re, -Rr[i][j].im), Cv[i]);
+            }
+        }
+        std::vector<const F*> in;
+        for (int i = 0; i < k; ++i) in.push_back(&U[i]);
+        std::vector<F> nu = vmul(in, dupper_inv(Rr));
+        for (int i = 0; i < k; ++i) U[i] = nu[i];
+        resid(r);
+        dev = nrm(r);
+    } else {
+        // FirstTimeGMERESSolve: one GMRES(m) cycle (MGS), basis in W
+        DM H = dm(m + 1, m);
+        F& v0 = W(0);
+        resid(v0);
+        const double beta = nrm(v0);
+        scal(v0, mk(1.0 / beta, 0.0));
+        for (int j = 0; j < m; ++j) {
+            applyA(W(j), W(j + 1));
+            for (int q = 0; q <= j; ++q) {
+                const cx<double> d = dotc(W(q), W(j + 1));
+                H[q][j] = d;
+                axpy(W(j + 1), mk(-d.re, -d.im), W(q));
+            }
+            const double wn = nrm(W(j + 1));
+            H[j + 1][j] = mk(wn, 0.0);
+            scal(W(j + 1), mk(1.0 / wn, 0.0));
+        }
+        ++cycles;
+        std::vector<cx<double>> g(m + 1, mk(0.0, 0.0));
+        g[0] = mk(beta, 0.0);
+        const std::vector<cx<double>> y = lsq(H, g);
+        for (int j = 0; j < m; ++j) axpy(X, y[j], W(j));
+        if (recalc > 1) {
+            std::fill(Wr.begin(), Wr.end(), 0.0);
+            for (int i = 0; i <= m; ++i) {
+                cx<double> hy = mk(0.0, 0.0);
+                for (int j = 0; j < m; ++j) hy = hy + H[i][j] * y[j];
+                if (i == 0) hy.re -= beta;
+                axpy(Wr, mk(-hy.re, -hy.im), W(i));
+            }
+        } else {
+            resid(Wr);
+        }
+        dev = nrm(Wr);
+        // FindPk1, SVD: smallest right singular vectors of H (eigenvectors of H^H H)
+        DM I = dm(m, m);
+        for (int i = 0; i < m; ++i) I[i][i] = mk(1.0, 0.0);
+        const DM P = dgen_small(dmul(dadj(H), H), I, k);
+        gen_cu(H, P, dev >= tol, true);
+        r = Wr;
+    }
+    int status = 1;
+    if (dev < tol) status = 0;
+    for (int it = 0; it < restart && status != 0; ++it) {
+        // OrthognalXR with coefficients from the unmodified r
+        T = r;
+        for (int i = 0; i < k; ++i) {
+            const cx<double> c = dotc(Cv[i], T);
+            axpy(X, c, U[i]);
+            axpy(r, mk(-c.re, -c.im), Cv[i]);
+        }
+        DM G = dm(m + 1, m);
+        for (int i = 0; i < k; ++i) {
+            const double l = 1.0 / nrm(U[i]);
+            G[i][i] = mk(l, 0.0);
+            scal(U[i], mk(l, 0.0));
+        }
+        dev = nrm(r);
+        scal(r, mk(1.0 / dev, 0.0));
+        for (int j = k; j < m; ++j) {
+            applyA(W(j), T);
+            W(j + 1) = T;
+            for (int q = 0; q < k; ++q) {  // CGS against C (coefficients from A v_j)
+                const cx<double> d = dotc(Cv[q], T);
+                G[q][j] = d;
+                axpy(W(j + 1), mk(-d.re, -d.im), Cv[q]);
+            }
+            for (int q = k; q <= j; ++q) {  // MGS against V
+                const cx<double> d = dotc(W(q), W(j + 1));
+                G[q][j] = d;
+                axpy(W(j + 1), mk(-d.re, -d.im), W(q));
+            }
+            const double wn = nrm(W(j + 1));
+            G[j + 1][j] = mk(wn, 0.0);
+            scal(W(j + 1), mk(1.0 / wn, 0.0));
+        }
+        ++cycles;
+        Wr = r;
+        scal(Wr, mk(dev, 0.0));
+        std::vector<cx<double>> g(m + 1, mk(0.0, 0.0));
+        g[k] = mk(dev, 0.0);
+        const std::vector<cx<double>> y = lsq(G, g);
+        for (int j = 0; j < m; ++j) axpy(X, y[j], Vv(j));
+        if (recalc != 0 && it != 0 && it % recalc == 0) {
+            resid(Wr);
+        } else {
+            for (int i = 0; i <= m; ++i) {
+                cx<double> gy = mk(0.0, 0.0);
+                for (int j = 0; j < m; ++j) gy = gy + G[i][j] * y[j];
+                axpy(Wr, mk(-gy.re, -gy.im), W(i));
+            }
+        }
+        dev = nrm(Wr);
+        // FindPk2, SVD: G^H G p = lam diag(U^H U block, I) p, smallest lam
+        DM Bm = dm(m, m);
+        for (int i = 0; i < m; ++i) Bm[i][i] = mk(1.0, 0.0);
+        for (int i = 0; i < k; ++i)
+            for (int j = 0; j < k; ++j)
+                if (i != j) Bm[i][j] = dotc(U[i], U[j]);
+        const DM P = dgen_small(dmul(dadj(G), G), Bm, k);
+        if (dev >= tol || phase != 2) gen_cu(G, P, dev >= tol, false);
+        if (dev < tol) { status = 0; break; }
+        r = Wr;
+    }
+    std::copy(X.begin(), X.end(), x);
+    for (int i = 0; i < k; ++i) std::copy(U[i].begin(), U[i].end(), urec + i * n2);
+    if (cycles_out) *cycles_out = cycles;
+    if (dev_out) *dev_out = dev;
+    return status;
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

@@ -95,6 +95,13 @@ int orc_msfom(const orc_operator* A, const double* U, const double* b, double* x
               const double* shifts, int nshift, int max_dim, int restart,
               double accuracy, double blength, int* cycles_out, double* max_err_out);
 
+/* GCRO-DR with SVD deflation (CSolverGCRODR.cpp:196-700): x0 = b, urec holds the k recycled vectors
+ * (k * 2n doubles) between calls with *have_u set after the first; phase as ESolverPhase (1, 2 reuse);
+ * stop |r| < accuracy * (absolute ? 1 : |b|^2).  Returns 0 converged, 1 not converged. */
+int orc_gcrodr(const orc_operator* A, const double* U, const double* b, double* x, double* urec, int* have_u,
+               int m, int k, int restart, int recalc, double accuracy, int absolute, int phase, int* cycles_out,
+               double* dev_out);
+
 /* ---------------- forces ---------------- */
 /* Wilson force kernel (CFieldFermionWilsonSquareSU3.cu:170-231): F += ... */
 int orc_wilson_force_kernel(const int dims[4], const int bc[4], double kappa, const double* U,

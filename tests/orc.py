@@ -318,3 +318,17 @@ def random_momentum(rng, nlinks, sigma=0.5):
     M[:, 7] = -r[:, 6] + 1j * r[:, 5]
     M[:, 8] = -1j * r[:, 7] * i32
     return from_complex(M.ravel())
+
+
+def gcrodr(kind, op, dims, bc, m_or_kappa, U, b, m=10, k=8, restart=100, recalc=0, accuracy=1e-20,
+           absolute=0, phase=3, urec=None, have_u=0):
+    """returns x, status, cycles, deviation, urec, have_u (pass the last two back to recycle)"""
+    o = make_op(kind, op, dims, bc, m_or_kappa)
+    x = np.zeros_like(b)
+    urec = np.zeros(k * b.size) if urec is None else urec.copy()
+    hu = C.c_int(have_u)
+    cyc, dev = C.c_int(), C.c_double()
+    rc = lib().orc_gcrodr(C.byref(o), _dp(U), _dp(b), _dp(x), _dp(urec), C.byref(hu), m, k, restart, recalc,
+                          C.c_double(accuracy), absolute, phase, C.byref(cyc), C.byref(dev))
+    assert rc in (0, 1)
+    return x, rc, cyc.value, dev.value, urec, hu.value

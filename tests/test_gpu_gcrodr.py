@@ -1,0 +1,85 @@
+"""GCRO-DR on the GPU against the oracle's restatement (SVD deflation) and the other solvers."""
+import numpy as np
+import pytest
+
+import cudalatticegauge_amd as clg
+import orc
+
+pytestmark = pytest.mark.gpu
+
+DIMS = (4, 4, 4, 4)
+BC = orc.BC_FERMION
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = clg.Context(0)
+    yield c
+    c.close()
+
+
+def up(ctx, kind, data):
+    f = clg.Field(ctx, kind, clg.F64, DIMS, bc=BC)
+    f.upload(data)
+    return f
+
+
+def test_gcrodr_svd_parity_with_oracle(ctx):
+    kappa = 0.15
+    U = orc.gauge_random(DIMS, 0.5, 1)
+    b1, b2 = orc.wilson_gaussian(DIMS, 5), orc.wilson_gaussian(DIMS, 6)
+    acc = 1e-11 / np.linalg.norm(b1)
+    kw = dict(m=12, k=6, restart=300, accuracy=acc)
+    x1o, rc1, c1o, _, urec, hu = orc.gcrodr(0, orc.D, DIMS, BC, kappa, U, b1, phase=0, **kw)
+    x2o, rc2, c2o, _, _, _ = orc.gcrodr(0, orc.D, DIMS, BC, kappa, U, b2, phase=1, urec=urec, have_u=1, **kw)
+    g = up(ctx, clg.GAUGE, U)
+    fb1, fb2 = up(ctx, clg.WILSON, b1), up(ctx, clg.WILSON, b2)
+    x1, x2 = clg.Field(ctx, clg.WILSON, clg.F64, DIMS, bc=BC), clg.Field(ctx, clg.WILSON, clg.F64, DIMS, bc=BC)
+    s = clg.Gcrodr(ctx, m_dim=12, k_dim=6, restart=300, accuracy=acc, deflate=clg.DEFLATE_SVD)
+    st1, r1 = s.solve(x1, fb1, g, kappa, clg.EFO_F_D, phase=clg.PHASE_START)
+    st2, r2 = s.solve(x2, fb2, g, kappa, clg.EFO_F_D, phase=clg.PHASE_IN)
+    print("cycles gpu", r1.cycles, r2.cycles, "oracle", c1o, c2o, "res", r1.true_residual, r2.true_residual)
+    assert st1 == clg.OK and st2 == clg.OK
+    assert abs(r1.cycles - c1o) <= 2 and abs(r2.cycles - c2o) <= 2
+    for x, xo in ((x1, x1o), (x2, x2o)):
+        assert np.abs(x.download() - xo).max() < 1e-8 * np.abs(xo).max()
+    assert r1.true_residual < 2e-11 and r2.true_residual < 2e-11
+    s.close()
+
+
+@pytest.mark.parametrize("deflate", [clg.DEFLATE_REV, clg.DEFLATE_HEV, clg.DEFLATE_SVD])
+@pytest.mark.parametrize("even_odd", [0, 1])
+def test_gcrodr_deflations_converge(ctx, deflate, even_odd):
+    kappa = 0.15
+    U = orc.gauge_random(DIMS, 0.5, 1)
+    g = up(ctx, clg.GAUGE, U)
+    s = clg.Gcrodr(ctx, m_dim=14, k_dim=6, restart=300, accuracy=1e-22, deflate=deflate, even_odd=even_odd)
+    for i, phase in enumerate((clg.PHASE_START, clg.PHASE_IN, clg.PHASE_END)):
+        b = orc.wilson_gaussian(DIMS, 20 + i)
+        fb = up(ctx, clg.WILSON, b)
+        x = clg.Field(ctx, clg.WILSON, clg.F64, DIMS, bc=BC)
+        st, rep = s.solve(x, fb, g, kappa, clg.EFO_F_D, phase=phase)
+        assert st == clg.OK, (deflate, even_odd, i)
+        Dx = orc.wilson_op(DIMS, BC, kappa, U, x.download(), orc.D)
+        assert np.linalg.norm(Dx - b) < 1e-9 * np.linalg.norm(b)
+    s.close()
+
+
+def test_gcrodr_staggered_dddagger(ctx):
+    U = orc.gauge_random(DIMS, 0.5, 1)
+    chi = orc.ks_gaussian(DIMS, 3)
+    g, fb = up(ctx, clg.GAUGE, U), up(ctx, clg.KS, chi)
+    x = clg.Field(ctx, clg.KS, clg.F64, DIMS, bc=BC)
+    s = clg.Gcrodr(ctx, m_dim=12, k_dim=4, restart=300, accuracy=1e-22, kind=clg.KS)
+    st, rep = s.solve(x, fb, g, 0.2, clg.EFO_F_DDdagger, phase=clg.PHASE_ONCE)
+    assert st == clg.OK
+    Ax = orc.ks_op(DIMS, BC, 0.2, U, x.download(), orc.DDDAG)
+    assert np.linalg.norm(Ax - chi) < 1e-9 * np.linalg.norm(chi)
+    s.close()
+
+
+def test_gcrodr_bad_params(ctx):
+    with pytest.raises(clg.CLGError):
+        clg.Gcrodr(ctx, m_dim=4)
+    with pytest.raises(clg.CLGError):
+        clg.Gcrodr(ctx, m_dim=10, k_dim=9)

@@ -73,3 +73,42 @@ def test_bicgstab_solution():
     assert rc == 0
     r = b - orc.wilson_op(dims, orc.BC_FERMION, 0.1355, U, x)
     assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-7
+
+
+def _dense_wilson(dims, kappa, U):
+    n = orc.volume(dims) * 24
+    A = np.zeros((n // 2, n // 2), complex)
+    for j in range(n // 2):
+        e = np.zeros(n)
+        e[2 * j] = 1.0
+        A[:, j] = orc.as_complex(orc.wilson_op(dims, orc.BC_FERMION, kappa, U, e, orc.D))
+    return A
+
+
+def test_gcrodr_matches_dense_solve():
+    dims = (2, 2, 2, 4)
+    U = orc.gauge_random(dims, 0.5, 3)
+    b = orc.wilson_gaussian(dims, 4)
+    A = _dense_wilson(dims, 0.12, U)
+    want = np.linalg.solve(A, orc.as_complex(b))
+    x, rc, cyc, dev, _, _ = orc.gcrodr(0, orc.D, dims, orc.BC_FERMION, 0.12, U, b, m=10, k=4, restart=200,
+                                       accuracy=1e-24)
+    assert rc == 0
+    assert np.abs(orc.as_complex(x) - want).max() < 1e-9 * np.abs(want).max()
+    assert dev < 1e-24 * np.dot(b, b)
+
+
+def test_gcrodr_recycled_solve_agrees_with_fresh_solve():
+    dims = (4, 4, 4, 4)
+    U = orc.gauge_random(dims, 0.5, 1)
+    kappa = 0.15
+    b1, b2 = orc.wilson_gaussian(dims, 5), orc.wilson_gaussian(dims, 6)
+    kw = dict(m=12, k=6, restart=300, accuracy=1e-22)
+    x1, rc1, c1, _, urec, hu = orc.gcrodr(0, orc.D, dims, orc.BC_FERMION, kappa, U, b1, phase=0, **kw)
+    assert rc1 == 0 and hu == 1
+    x2, rc2, c2, _, _, _ = orc.gcrodr(0, orc.D, dims, orc.BC_FERMION, kappa, U, b2, phase=1, urec=urec, have_u=1, **kw)
+    x3, rc3, c3, _, _, _ = orc.gcrodr(0, orc.D, dims, orc.BC_FERMION, kappa, U, b2, phase=3, **kw)
+    assert rc2 == 0 and rc3 == 0
+    r = orc.wilson_op(dims, orc.BC_FERMION, kappa, U, x2, orc.D) - b2
+    assert np.linalg.norm(r) < 1e-10 * np.linalg.norm(b2)
+    assert np.abs(x2 - x3).max() < 1e-8 * np.abs(x3).max()
