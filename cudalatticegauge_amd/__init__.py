@@ -22,7 +22,7 @@ EFO_F_D, EFO_F_Ddagger, EFO_F_DD, EFO_F_DDdagger = 0, 1, 2, 3
 EFO_F_InverseD, EFO_F_InverseDdagger, EFO_F_InverseDD, EFO_F_InverseDDdagger = 4, 5, 6, 7
 EOCT_None, EOCT_Minus, EOCT_Real, EOCT_Complex = 0, 1, 2, 3
 SOLVER_GMRES, SOLVER_BICGSTAB = 0, 1
-MS_FOM, MS_BICGSTAB = 0, 1
+MS_FOM, MS_BICGSTAB, MS_GMRES, MS_NESTED = 0, 1, 2, 3
 STOP_REFERENCE, STOP_TRUE_RESIDUAL = 0, 1
 OK, NOT_CONVERGED = 0, 1
 ACTION_GAUGE_PLAQUETTE, ACTION_FERMION_WILSON_NF2, ACTION_FERMION_KS = 0, 1, 2

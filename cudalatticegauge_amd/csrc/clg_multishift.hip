@@ -15,6 +15,7 @@
 // shifts are compacted out of the launches, the FOM small systems are solved on device, and
 // the shift sums accumulate in FP64 whatever the storage precision of the solutions.
 #include "clg_internal.h"
+#include "clg_smallla.h"
 
 #include <algorithm>
 #include <chrono>
@@ -333,6 +334,122 @@ int msfom(clg_context ctx, OpRef& A, std::vector<clg_field_s*>& X, const std::ve
     return CLG_NOT_CONVERGED;
 }
 
+// ------------------------------------------------------------------ MS-GMRES (CMultiShiftGMRES.cpp:112-330)
+// Restarted GMRES on the unshifted system; each shifted residual is kept collinear with the
+// unshifted one, r_n = beta_n r, by solving the square system [Hbar + c_n I~ | z] (y_n, beta_n') =
+// beta_n beta e_1 with z = beta e_1 - Hbar y (Frommer-Glaessner).  Stop at the end of a cycle when
+// every shifted residual |beta_n'| |z| < Accuracy * (AbsoluteAccuracy ? 1 : |b|^2).  (The reference
+// compares the collinearity factor |beta_n'| alone, which does not bound the residual and grows as
+// the unshifted system converges; its own test of this solver is disabled for that reason,
+// TestMultiShiftSolver.cpp:318.)
+int msgmres(clg_context ctx, OpRef& A, std::vector<clg_field_s*>& X, const std::vector<cx>& shifts,
+            const clg_field_s* b, int m, int restart, double accuracy, double blen, int& cycles, double& maxerr) {
+    using clg::sla::Mat;
+    typedef std::complex<double> zc;
+    const int ns = (int)shifts.size();
+    CLG_CHECK_ARG(m >= 2 && m <= 63, "MS-GMRES MaxDim must be in [2, 63]");
+    std::vector<clg_field_s*> V(m + 1);
+    for (int k = 0; k <= m; ++k) CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 2000 + k, &V[k]));
+    clg_field_s *xb = nullptr, *w = V[m];
+    CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 2100, &xb));
+    const int ld = m + 1;
+    const size_t oH = 0, oY = oH + 2 * (size_t)ld * m, oS = oY + 2 * (size_t)m * std::max(ns, 1), oC = oS + 16,
+                 tot = oC + 2 * (size_t)(m + 1);
+    double* ws = nullptr;
+    CLG_TRY(workspace(ctx, tot, &ws));
+    double* H = ws + oH;
+    double* sc = ws + oS;
+    int* reorth = reinterpret_cast<int*>(ws + oS + 8);
+    CLG_TRY(clg_field_zero(xb));
+    for (auto x : X) CLG_TRY(clg_field_zero(x));
+    std::vector<zc> beta0(ns, zc(1.0, 0.0));
+    cycles = 0;
+    maxerr = 0;
+    for (int cyc = 0; cyc < restart; ++cyc) {
+        ++cycles;
+        if (cyc == 0) {
+            CLG_TRY(blas_axpby(ctx, V[0], 1.0, 0.0, b, 0.0, 0.0));
+        } else {
+            CLG_TRY(A.apply(ctx, V[0], xb));
+            CLG_TRY(blas_axpby(ctx, V[0], 1.0, 0.0, b, -1.0, 0.0));
+        }
+        CLG_TRY(blas_norm2_dev(ctx, V[0], sc));
+        double b2;
+        CLG_TRY(fetch_doubles(ctx, sc, 1, &b2));
+        const double beta = std::sqrt(b2);
+        CLG_TRY(blas_scale_inv_sqrt(ctx, V[0], V[0], sc));
+        for (int j = 0; j < m; ++j) {
+            CLG_TRY(A.apply(ctx, w, V[j]));
+            double* Hj = H + 2 * (size_t)j * ld;
+            CLG_TRY(blas_multidot_dev(ctx, V.data(), j + 1, w, Hj, sc + 4, nullptr));
+            CLG_TRY(blas_multiaxpy_dev(ctx, w, V.data(), j + 1, Hj, -1.0, sc + 1, nullptr, sc + 4, 0.5, reorth));
+            CLG_TRY(blas_multidot_dev(ctx, V.data(), j + 1, w, ws + oC, nullptr, reorth));
+            CLG_TRY(blas_multiaxpy_dev(ctx, w, V.data(), j + 1, ws + oC, -1.0, sc + 1, reorth, nullptr, 0.0, nullptr));
+            k_ms_hess<<<1, 64, 0, ctx->stream>>>(H, ld, j, ws + oC, sc + 1, reorth);
+            CLG_HIP(hipGetLastError());
+            if (j < m - 1) CLG_TRY(blas_scale_inv_sqrt(ctx, V[j + 1], w, sc + 1));
+        }
+        std::vector<double> hh(2 * (size_t)ld * m);
+        CLG_TRY(fetch_doubles(ctx, H, (int)hh.size(), hh.data()));
+        Mat Hb(m + 1, m);
+        for (int j = 0; j < m; ++j)
+            for (int i = 0; i <= m; ++i) Hb(i, j) = zc(hh[2 * (j * ld + i)], hh[2 * (j * ld + i) + 1]);
+        // unshifted least squares, x_base += V y, z = beta e_1 - Hbar y
+        std::vector<zc> rhs(m + 1, zc(0.0, 0.0)), y;
+        rhs[0] = beta;
+        clg::sla::hessenberg_lsq(Hb, rhs, y);
+        std::vector<zc> z(m + 1, zc(0.0, 0.0));
+        z[0] = beta;
+        for (int i = 0; i <= m; ++i)
+            for (int j = 0; j < m; ++j) z[i] -= Hb(i, j) * y[j];
+        std::vector<double> up(2 * (size_t)m * (ns + 1));
+        for (int j = 0; j < m; ++j) { up[2 * j] = y[j].real(); up[2 * j + 1] = y[j].imag(); }
+        // shifted systems
+        maxerr = 0;
+        for (int n = 0; n < ns; ++n) {
+            Mat M(m + 1, m + 1);
+            for (int i = 0; i <= m; ++i) {
+                for (int j = 0; j < m; ++j) M(i, j) = Hb(i, j) + (i == j ? zc(shifts[n].x, shifts[n].y) : zc(0.0, 0.0));
+                M(i, m) = z[i];
+            }
+            Mat Q, R;
+            clg::sla::thin_qr(M, Q, R);
+            std::vector<zc> g(m + 1, zc(0.0, 0.0)), yh(m + 1, zc(0.0, 0.0));
+            const zc g0 = beta0[n] * beta;
+            for (int i = 0; i <= m; ++i) g[i] = std::conj(Q(0, i)) * g0;  // Q^H (g0 e_1)
+            for (int i = m; i >= 0; --i) {
+                zc acc = g[i];
+                for (int l = i + 1; l <= m; ++l) acc -= R(i, l) * yh[l];
+                yh[i] = acc / R(i, i);
+            }
+            beta0[n] = yh[m];
+            double zn = 0;
+            for (int i = 0; i <= m; ++i) zn += std::norm(z[i]);
+            maxerr = std::max(maxerr, std::abs(yh[m]) * std::sqrt(zn));
+            for (int j = 0; j < m; ++j) {
+                up[2 * (size_t)m * (n + 1) + 2 * j] = yh[j].real();
+                up[2 * (size_t)m * (n + 1) + 2 * j + 1] = yh[j].imag();
+            }
+        }
+        CLG_HIP(hipMemcpyAsync(ws + oY, up.data(), sizeof(double) * up.size(), hipMemcpyHostToDevice, ctx->stream));
+        ShiftPtrs sp;
+        VPtrs vp;
+        sp.x[0] = xb->data;
+        sp.s[0] = nullptr;
+        for (int n = 0; n < ns; ++n) { sp.x[n + 1] = X[n]->data; sp.s[n + 1] = nullptr; }
+        for (int j = 0; j < m; ++j) vp.v[j] = V[j]->data;
+        const long np = n_packets(b);
+        const unsigned gg = ew_grid(ctx, np);
+        const size_t shm = sizeof(double) * 2 * (size_t)m * (ns + 1);
+        if (b->prec == CLG_F64) k_msfom_update<CLG_F64><<<gg, 256, shm, ctx->stream>>>(np, ns + 1, m, sp, vp, ws + oY);
+        else k_msfom_update<CLG_F32><<<gg, 256, shm, ctx->stream>>>(np, ns + 1, m, sp, vp, ws + oY);
+        CLG_HIP(hipGetLastError());
+        CLG_HIP(hipStreamSynchronize(ctx->stream));  // up is a host temporary
+        if (maxerr < accuracy * blen) return CLG_OK;
+    }
+    return CLG_NOT_CONVERGED;
+}
+
 // ------------------------------------------------------------------ shifted BiCGStab
 int msbicgstab(clg_context ctx, OpRef& A, std::vector<clg_field_s*>& X, const std::vector<cx>& cn,
                const clg_field_s* b, int step_count, int dev_check, double accuracy, double blen, int& iters,
@@ -488,7 +605,7 @@ int msbicgstab(clg_context ctx, OpRef& A, std::vector<clg_field_s*>& X, const st
     return done ? CLG_OK : CLG_NOT_CONVERGED;
 }
 
-enum { MS_FOM = 0, MS_BICGSTAB = 1 };
+enum { MS_FOM = 0, MS_BICGSTAB = 1, MS_GMRES = 2, MS_NESTED = 3 };
 
 int run_multishift(clg_context ctx, const clg_solver_params* p, int kind_solver, int prec, std::vector<clg_field_s*>& X,
                    const std::vector<cx>& shifts, const clg_field_s* b, const clg_field_s* gauge, double param, int op,
@@ -518,7 +635,11 @@ int run_multishift(clg_context ctx, const clg_solver_params* p, int kind_solver,
     int it = 0;
     double err = 0;
     int st;
-    if (kind_solver == MS_FOM) {
+    if (kind_solver == MS_GMRES) {
+        // CMultiShiftGMRES compares max |beta_n| with Accuracy * (AbsoluteAccuracy ? 1 : |b|^2)
+        st = msgmres(ctx, A, Xw, shifts, bw, p->max_dim, p->restart, p->accuracy, p->absolute_accuracy ? 1.0 : bn2, it,
+                     err);
+    } else if (kind_solver == MS_FOM) {
         // FOM compares |beta0| with Accuracy * (AbsoluteAccuracy ? 1 : |b|^2) (CMultiShiftFOM.cpp:119-124)
         st = msfom(ctx, A, Xw, shifts, bw, p->max_dim, p->restart, p->accuracy, p->absolute_accuracy ? 1.0 : bn2, it,
                    err);
@@ -553,7 +674,8 @@ int clg_multishift_solve(clg_context ctx, const clg_solver_params* p, int kind_s
                          clg_solve_report* rep) {
     CLG_CHECK_ARG(ctx && p && xs && shifts && b && gauge, "null argument");
     CLG_CHECK_ARG(nshift >= 1 && nshift <= kMaxShift, "nshift must be in [1, %d]", kMaxShift);
-    CLG_CHECK_ARG(kind_solver == MS_FOM || kind_solver == MS_BICGSTAB, "unknown multi-shift solver %d", kind_solver);
+    CLG_CHECK_ARG(kind_solver >= MS_FOM && kind_solver <= MS_NESTED, "unknown multi-shift solver %d", kind_solver);
+    CLG_CHECK_ARG(kind_solver != MS_GMRES || nshift < kMaxShift, "MS-GMRES takes at most %d shifts", kMaxShift - 1);
     CLG_CHECK_ARG(prec == CLG_F64 || prec == CLG_F32, "multi-shift precision must be FP64 or FP32");
     CLG_CHECK_ARG(b->kind == p->kind && b->subset == CLG_FULL && b->prec == CLG_F64, "b must be a FULL FP64 field of params.kind");
     CLG_CHECK_ARG(gauge->prec == CLG_F64 && gauge->kind == CLG_GAUGE, "gauge must be the FP64 18-real field");
@@ -570,6 +692,41 @@ int clg_multishift_solve(clg_context ctx, const clg_solver_params* p, int kind_s
     }
     CLG_TRY(check_same_lattice(b, gauge));
     (void)hipSetDevice(ctx->device);
+    if (kind_solver == MS_NESTED) {
+        // CMultiShiftNested::Solve (CMultiShiftNested.cpp:69-150): staggered fermions and real shifts
+        // only; one solve of the operator with the shifted mass per shift (D: m + c, DD^dagger:
+        // sqrt(m^2 + c)), params acting as the nested solver's
+        CLG_CHECK_ARG(p->kind == CLG_KS, "MS-Nested supports staggered fermions only");
+        CLG_CHECK_ARG(op == EFO_F_D || op == EFO_F_Ddagger || op == EFO_F_DDdagger, "MS-Nested operator not supported");
+        const double t0 = now_s();
+        clg_solver_params q = *p;
+        if (prec == CLG_F32 && q.inner_prec == CLG_F64) q.inner_prec = CLG_F32;
+        int cycles = 0, applies = 0, st_all = CLG_OK;
+        double worst = 0;
+        for (int k = 0; k < nshift; ++k) {
+            CLG_CHECK_ARG(std::fabs(sh[k].y) <= 1e-7, "MS-Nested takes real shifts only");
+            const double m2 = op == EFO_F_DDdagger ? param * param + sh[k].x : param + sh[k].x;
+            CLG_CHECK_ARG(m2 >= 1e-7, "MS-Nested needs mass + shift > 0");
+            const double mass = op == EFO_F_DDdagger ? std::sqrt(m2) : m2;
+            clg_solve_report r{};
+            const int st = clg_solve(ctx, &q, X[k], b, gauge, mass, op + 4, nullptr, &r);
+            if (st != CLG_OK && st != CLG_NOT_CONVERGED) return st;
+            if (st != CLG_OK) st_all = CLG_NOT_CONVERGED;
+            cycles += r.cycles;
+            applies += r.operator_applies;
+            worst = std::max(worst, r.true_residual);
+        }
+        if (rep) {
+            std::memset(rep, 0, sizeof(*rep));
+            rep->converged = st_all == CLG_OK;
+            rep->cycles = cycles;
+            rep->outer = nshift;
+            rep->operator_applies = applies;
+            rep->true_residual = worst;
+            rep->seconds = now_s() - t0;
+        }
+        return st_all;
+    }
     return run_multishift(ctx, p, kind_solver, prec, X, sh, b, gauge, param, op, rep);
 }
 

@@ -224,7 +224,12 @@ int clg_gcrodr_solve(clg_gcrodr solver, clg_field x, clg_field b, clg_field gaug
                      int phase, clg_field start, clg_solve_report* report);
 
 /* ---- multi-shift solvers and the rational approximation (RHMC) ---- */
-typedef enum { CLG_MS_FOM = 0, CLG_MS_BICGSTAB = 1 } clg_multishift_kind;
+typedef enum {
+    CLG_MS_FOM = 0,      /* CMultiShiftFOM */
+    CLG_MS_BICGSTAB = 1, /* CMultiShiftBiCGStab */
+    CLG_MS_GMRES = 2,    /* CMultiShiftGMRES (CMultiShiftGMRES.cpp:112-330), at most 31 shifts */
+    CLG_MS_NESTED = 3    /* CMultiShiftNested (CMultiShiftNested.cpp:69-150): staggered, real shifts */
+} clg_multishift_kind;
 
 /* CMultiShiftSolver::Solve (CMultiShiftSolver.h:38-47): xs[k] = (A + c_k)^-1 b for all k, with
  * A = op of the params.kind fermion and c_k = shifts[2k] + i shifts[2k+1].  CLG_MS_FOM follows

@@ -40,7 +40,7 @@ def apply_shifted(kind, op, U, x, s, m):
     return orc.as_complex(Ax) + s * orc.as_complex(x)
 
 
-@pytest.mark.parametrize("solver", [clg.MS_FOM, clg.MS_BICGSTAB])
+@pytest.mark.parametrize("solver", [clg.MS_FOM, clg.MS_BICGSTAB, clg.MS_GMRES])
 def test_wilson_complex_shifts(ctx, solver):
     # TestMultiShiftSolver shift set (TestMultiShiftSolver.cpp:12-111)
     U, b, g, fb = setup(ctx, clg.WILSON)
@@ -55,7 +55,7 @@ def test_wilson_complex_shifts(ctx, solver):
         assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-9
 
 
-@pytest.mark.parametrize("solver", [clg.MS_FOM, clg.MS_BICGSTAB])
+@pytest.mark.parametrize("solver", [clg.MS_FOM, clg.MS_BICGSTAB, clg.MS_GMRES, clg.MS_NESTED])
 @pytest.mark.parametrize("prec", [clg.F64, clg.F32])
 def test_ks_ddagger_remez_shifts_vs_oracle(ctx, solver, prec):
     U, b, g, fb = setup(ctx, clg.KS)
@@ -65,6 +65,10 @@ def test_ks_ddagger_remez_shifts_vs_oracle(ctx, solver, prec):
     acc = 1e-11 if prec == clg.F64 else 1e-5
     p = clg.solver_params(kind=clg.KS, max_dim=30, restart=50, max_step=400, deviation_step=5, accuracy=acc,
                           absolute_accuracy=1)
+    if solver == clg.MS_NESTED:
+        # the nested solver: relative true-residual GMRES per shifted mass
+        p = clg.solver_params(kind=clg.KS, max_dim=30, restart=50, accuracy=acc * 1e-1,
+                              stop_rule=clg.STOP_TRUE_RESIDUAL, inner_accuracy=1e-4)
     st, rep = clg.multishift_solve(ctx, p, xs, shifts, fb, g, 0.1, clg.EFO_F_DDdagger, kind=solver, prec=prec)
     assert st == clg.OK
     xo, rc, _, _ = orc.msfom(1, orc.DDDAG, DIMS, orc.BC_FERMION, 0.1, U, b, shifts, max_dim=30, restart=50,
