@@ -1,0 +1,40 @@
+"""CLGTest sections of the reference's test suite replayed through the C++ facade (CLGLib class
+names over the HIP library), judged by the reference tests' own criteria and expected values."""
+import json
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RUNNER = os.path.join(ROOT, "cudalatticegauge_amd", "clg_test_runner")
+YAML = os.path.join(ROOT, "tests", "golden", "clgtest_sections.yaml")
+
+
+def run(section, test):
+    print(f"running {section} {test}", flush=True)
+    out = subprocess.run([RUNNER, YAML, section, test], capture_output=True, timeout=240)
+    assert out.returncode == 0, out.stderr.decode()[-2000:]
+    res = json.loads(out.stdout.decode().strip().splitlines()[-1])
+    print(res)
+    return res
+
+
+@pytest.mark.parametrize("section", ["TestSolverBiCGStab", "TestSolverGCRODR"])
+def test_solver_sections(section):
+    # TestSolver (TestSolver.cpp:12-121): every |.|^2 error below ExpectedErr (default 1e-4)
+    r = run(section, "solver")
+    assert all(e < r["expected"] for e in r["errors"]), r
+
+
+@pytest.mark.parametrize("section", ["TestFermionUpdatorKS", "TestFermionUpdatorKSNestedForceGradient",
+                                     "TestFermionUpdatorNestedForceGradient"])
+def test_updator_sections(section):
+    # TestUpdateCommon (TestUpdator.cpp:12-140): plaquette within 0.15 of ExpectedRes, at most
+    # ExpectMiss rejections, H_diff < 0.12 and |last H_diff| < 0.1
+    r = run(section, "update")
+    assert abs(r["plaquette_avg"] - r["expected"]) < 0.15, r
+    assert r["accepted"] >= r["expect_accept"], r
+    assert r["hdiff"] < 0.12 and r["last_hdiff"] < 0.1, r
