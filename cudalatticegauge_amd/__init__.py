@@ -45,7 +45,10 @@ EXPORTED = [
     "clg_gauge_normalize", "clg_gauge_plaquette_energy", "clg_gauge_force", "clg_wilson_prepare_hmc",
     "clg_ks_prepare_hmc", "clg_hmc_params_default", "clg_hmc_evolve", "clg_hmc_trajectory",
     "clg_wilson_hop_batch", "clg_solve_batch", "clg_gcrodr_params_default", "clg_gcrodr_create",
-    "clg_gcrodr_destroy", "clg_gcrodr_reset", "clg_gcrodr_solve",
+    "clg_gcrodr_destroy", "clg_gcrodr_reset", "clg_gcrodr_solve", "clg_multidot", "clg_multiaxpy",
+    "clg_vector_multiply_matrix", "clg_autotune_enable", "clg_autotune_force", "clg_autotune_reset",
+    "clg_autotune_count", "clg_autotune_entry", "clg_graph_begin", "clg_graph_end", "clg_graph_launch",
+    "clg_graph_nodes", "clg_graph_destroy",
 ]
 
 
@@ -176,6 +179,19 @@ def lib():
             "clg_gcrodr_destroy": (C.c_int, [vp]),
             "clg_gcrodr_reset": (C.c_int, [vp]),
             "clg_gcrodr_solve": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, vp, C.POINTER(SolveReport)]),
+            "clg_multidot": (C.c_int, [vp, C.c_int, C.POINTER(vp), vp, dp]),
+            "clg_multiaxpy": (C.c_int, [vp, vp, C.c_int, C.POINTER(vp), dp]),
+            "clg_vector_multiply_matrix": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.c_int, C.POINTER(vp), dp]),
+            "clg_autotune_enable": (C.c_int, [C.c_int]),
+            "clg_autotune_force": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+            "clg_autotune_reset": (C.c_int, []),
+            "clg_autotune_count": (C.c_int, []),
+            "clg_autotune_entry": (C.c_int, [C.c_int, C.c_char_p, C.c_int]),
+            "clg_graph_begin": (C.c_int, [vp]),
+            "clg_graph_end": (C.c_int, [vp, C.POINTER(vp)]),
+            "clg_graph_launch": (C.c_int, [vp]),
+            "clg_graph_nodes": (C.c_int, [vp]),
+            "clg_graph_destroy": (C.c_int, [vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -202,6 +218,7 @@ class Context:
         check(lib().clg_init(device, C.byref(h)))
         self.h = h
         self._fields = weakref.WeakSet()
+        self._graphs = weakref.WeakSet()
 
     def sync(self):
         check(lib().clg_synchronize(self.h))
@@ -214,6 +231,8 @@ class Context:
     def close(self):
         """Frees every field of this context (views first), then the context itself."""
         if self.h:
+            for g in list(self._graphs):
+                g.close()
             fields = list(self._fields)
             for f in fields:
                 if f._parent is not None:
@@ -346,6 +365,93 @@ def norm2(ctx, a):
     out = C.c_double()
     check(lib().clg_norm2(ctx.h, a.h, C.byref(out)))
     return out.value
+
+
+def _handles(fields):
+    return (C.c_void_p * len(fields))(*[f.h for f in fields])
+
+
+def _cplx(values):
+    v = np.asarray(values, dtype=np.complex128).ravel()
+    return (C.c_double * (2 * len(v)))(*np.column_stack([v.real, v.imag]).ravel().tolist())
+
+
+def multidot(ctx, V, w):
+    """[<V_k, w>] for the fields V (one pass over w per 16 vectors, FP64 sums)."""
+    out = (C.c_double * (2 * len(V)))()
+    check(lib().clg_multidot(ctx.h, len(V), _handles(V), w.h, out))
+    return np.array(out[0::2]) + 1j * np.array(out[1::2])
+
+
+def multiaxpy(ctx, w, V, coeffs):
+    """w += sum_k coeffs[k] V_k."""
+    check(lib().clg_multiaxpy(ctx.h, w.h, len(V), _handles(V), _cplx(coeffs)))
+
+
+def vector_multiply_matrix(ctx, res, left, matrix):
+    """CFieldMatrixOperation::VectorMultiplyMatrix: with W = res + left, res_i <- sum_j W_j M[j, i]."""
+    M = np.asarray(matrix, dtype=np.complex128)
+    if M.shape != (len(res) + len(left), len(res)):
+        raise CLGError(f"matrix must be {(len(res) + len(left), len(res))}, got {M.shape}")
+    lh = _handles(left) if left else None
+    check(lib().clg_vector_multiply_matrix(ctx.h, len(res), _handles(res), len(left), lh, _cplx(M)))
+
+
+def autotune_enable(mode):
+    check(lib().clg_autotune_enable(int(mode)))
+
+
+def autotune_force(pin_mode=0, block=256, xcd=8):
+    check(lib().clg_autotune_force(int(pin_mode), int(block), int(xcd)))
+
+
+def autotune_reset():
+    check(lib().clg_autotune_reset())
+
+
+def autotune_entries():
+    out = []
+    buf = C.create_string_buffer(512)
+    for i in range(lib().clg_autotune_count()):
+        check(lib().clg_autotune_entry(i, buf, 512))
+        out.append(buf.value.decode())
+    return out
+
+
+class Graph:
+    """Launches recorded between Graph.record(ctx) and end(), replayed by launch()."""
+
+    def __init__(self, ctx, handle):
+        self.ctx, self.h = ctx, handle
+        ctx._graphs.add(self)
+
+    @staticmethod
+    def begin(ctx):
+        check(lib().clg_graph_begin(ctx.h))
+
+    @staticmethod
+    def end(ctx):
+        h = C.c_void_p()
+        check(lib().clg_graph_end(ctx.h, C.byref(h)))
+        return Graph(ctx, h)
+
+    def launch(self):
+        check(lib().clg_graph_launch(self.h))
+
+    @property
+    def nodes(self):
+        return lib().clg_graph_nodes(self.h)
+
+    def close(self):
+        if self.h and self.ctx.h:
+            lib().clg_graph_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def solver_params(**kw):

@@ -351,8 +351,7 @@ int batch_hop(clg_context ctx, const HopBatch& B, bool dag, double br, int tile)
         if (it != btune().end()) cfg = it->second;
         else need = true;
     }
-    const char* e = std::getenv("CLG_TUNE");
-    const bool tuning = !(e && e[0] == '0');
+    const bool tuning = tuning_enabled();
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     (void)hipStreamIsCapturing(st, &cs);
     if (need && tuning && !aliased && cs == hipStreamCaptureStatusNone) {
@@ -993,6 +992,41 @@ struct BatchSolver {
 };
 
 }  // namespace
+
+namespace clg {
+
+int batch_tune_count() {
+    std::lock_guard<std::mutex> lk(btune_mutex());
+    return (int)btune().size();
+}
+
+// key layout: prec, gauge prec, gauge kind, dagger, x, parities, pbase, in subset, out subset, then
+// the distinct lattice shapes (4 extents each)
+int batch_tune_entry(int index, char* buf, int len) {
+    std::lock_guard<std::mutex> lk(btune_mutex());
+    CLG_CHECK_ARG(index >= 0 && index < (int)btune().size(), "no tuned entry %d", index);
+    auto it = btune().begin();
+    std::advance(it, index);
+    const std::vector<long>& k = it->first;
+    std::string shapes;
+    for (size_t i = 9; i + 3 < k.size(); i += 4) {
+        char t[64];
+        std::snprintf(t, sizeof(t), "%s%ldx%ldx%ldx%ld", shapes.empty() ? "" : ",", k[i], k[i + 1], k[i + 2], k[i + 3]);
+        shapes += t;
+    }
+    std::snprintf(buf, (size_t)len,
+                  "wilson_hop_batch prec=%ld gauge_prec=%ld dagger=%ld x=%ld parities=%ld dims=%s -> tile=%d block=%d "
+                  "xcd=%d",
+                  k[0], k[1], k[3], k[4], k[5], shapes.c_str(), it->second.nr, it->second.block, it->second.xcd);
+    return CLG_OK;
+}
+
+void batch_tune_reset() {
+    std::lock_guard<std::mutex> lk(btune_mutex());
+    btune().clear();
+}
+
+}  // namespace clg
 
 // ================================================================== C ABI
 extern "C" {

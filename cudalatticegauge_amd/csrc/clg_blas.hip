@@ -10,6 +10,7 @@
 #include "clg_internal.h"
 
 #include <algorithm>
+#include <cstring>
 
 using namespace clg;
 using namespace clgd;
@@ -18,6 +19,9 @@ namespace {
 
 const int kMaxRedGrid = 2048;
 const int kMaxK = 16;  // vectors per multi-dot / multi-axpy launch
+// the context's 4096-double device result buffer: results from 0, multi-dot staging at 2048
+// (2 kMaxK + 1 doubles), host-supplied coefficients of the BLAS entry points from 2112 (1984)
+const int kMultidotStage = 2048, kHostArgs = 2112;
 
 // ---- 16-byte packets: NC complex numbers per lane and load, values handled as doubles
 enum { PK_D = 0, PK_F4 = 1, PK_F2 = 2 };
@@ -248,6 +252,44 @@ __global__ __launch_bounds__(kRedBlock) void k_multiaxpy(long n, Ptrs<P> V, cons
     }
 }
 
+// out_i = sum_{j<X} in_j M[j*Y + i] (i < Y <= YM) for FP64 complex vectors of n elements, M in
+// LDS; out may be the first Y inputs (each thread reads every input of its element before it
+// writes).  One pass: X reads and Y writes per element.
+constexpr int kMaxVmmIn = 32, kMaxVmmOut = 31;
+struct VmmArgs {
+    const double2* in[kMaxVmmIn];
+    double2* out[kMaxVmmOut];
+    int X, Y;
+    long n;
+};
+template <int YM>
+__global__ __launch_bounds__(256) void k_vmm(VmmArgs a, const double* __restrict__ M) {
+    __shared__ double2 sm[kMaxVmmIn * YM];
+    const int tot = a.X * a.Y;
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) sm[i] = make_double2(M[2 * i], M[2 * i + 1]);
+    __syncthreads();
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < a.n; e += (long)gridDim.x * blockDim.x) {
+        double2 acc[YM];
+#pragma unroll
+        for (int i = 0; i < YM; ++i) acc[i] = make_double2(0.0, 0.0);
+        for (int j = 0; j < a.X; ++j) {
+            const double2 v = a.in[j][e];
+            const double2* row = sm + j * a.Y;
+#pragma unroll
+            for (int i = 0; i < YM; ++i) {
+                if (i < a.Y) {
+                    const double2 m = row[i];
+                    acc[i].x = fma(v.x, m.x, fma(-v.y, m.y, acc[i].x));
+                    acc[i].y = fma(v.x, m.y, fma(v.y, m.x, acc[i].y));
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < YM; ++i)
+            if (i < a.Y) a.out[i][e] = acc[i];
+    }
+}
+
 unsigned red_grid(clg_context ctx, long n) {
     long g = (n + kRedBlock * 4 - 1) / (kRedBlock * 4);
     const long cap = std::min<long>(kMaxRedGrid, (long)ctx->num_cu * 4);
@@ -274,6 +316,8 @@ int ensure_reduction_scratch(clg_context ctx, size_t doubles) {
     ctx->red_partial_cap = doubles;
     return CLG_OK;
 }
+
+int reduction_ready(clg_context ctx) { return ensure_reduction_scratch(ctx, (size_t)kMaxRedGrid * (2 * kMaxK + 1)); }
 
 long n_complex(const clg_field_s* f) {
     const long nblk = f->subset == CLG_FULL ? 2 : 1;
@@ -407,7 +451,7 @@ template <int P>
 static int multidot_p(clg_context ctx, clg_field_s* const* V, int nv, const clg_field_s* w, double* out, double* wnorm,
                       const int* cond, long n) {
     const unsigned g = red_grid(ctx, n);
-    double* tmp = ctx->red_result + 2048;  // staging of the 2K+1 sums
+    double* tmp = ctx->red_result + kMultidotStage;  // staging of the 2K+1 sums
     for (int k0 = 0; k0 < nv; k0 += kMaxK) {
         const int kk = std::min(kMaxK, nv - k0);
         Ptrs<P> Pp;
@@ -487,9 +531,35 @@ int blas_multiaxpy_dev(clg_context ctx, clg_field_s* w, clg_field_s* const* V, i
     return multiaxpy_p<PK_F2>(ctx, w, V, nv, c, s, nrm, cond, ref, eta, flag, n);
 }
 
+// out_i = sum_{j<nin} in_j M[j*nout + i] over FP64 fields of one shape (M: device, interleaved
+// complex, nin x nout row-major); out may be the first nout inputs.
+int blas_vmm_dev(clg_context ctx, clg_field_s* const* in, int nin, clg_field_s* const* out, int nout, const double* M) {
+    CLG_CHECK_ARG(nin >= 1 && nin <= kMaxVmmIn && nout >= 1 && nout <= kMaxVmmOut && nout <= nin,
+                  "need 1 <= nout <= nin, nin <= %d, nout <= %d", kMaxVmmIn, kMaxVmmOut);
+    for (int j = 0; j < nin; ++j) {
+        CLG_TRY(check_pair(in[j], in[0]));
+        CLG_CHECK_ARG(in[j]->prec == CLG_F64, "field combinations take FP64 fields");
+    }
+    for (int i = 0; i < nout; ++i) CLG_TRY(check_pair(out[i], in[0]));
+    VmmArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.X = nin;
+    a.Y = nout;
+    a.n = n_complex(in[0]);
+    for (int j = 0; j < nin; ++j) a.in[j] = (const double2*)in[j]->data;
+    for (int i = 0; i < nout; ++i) a.out[i] = (double2*)out[i]->data;
+    const unsigned grid = (unsigned)std::max<long>(1, std::min<long>((a.n + 255) / 256, (long)ctx->num_cu * 8));
+    if (nout <= 8) k_vmm<8><<<grid, 256, 0, ctx->stream>>>(a, M);
+    else if (nout <= 16) k_vmm<16><<<grid, 256, 0, ctx->stream>>>(a, M);
+    else k_vmm<kMaxVmmOut><<<grid, 256, 0, ctx->stream>>>(a, M);
+    CLG_HIP(hipGetLastError());
+    return CLG_OK;
+}
+
 // copies `count` doubles from device to the context's pinned buffer and waits
 int fetch_doubles(clg_context ctx, const double* dev, int count, double* host) {
     CLG_CHECK_ARG(count > 0 && count <= 4096, "bad fetch size");
+    CLG_CHECK_ARG(!stream_capturing(ctx), "a blocking call (a reduction read back to the host) while recording a graph");
     CLG_HIP(hipMemcpyAsync(ctx->host_result, dev, sizeof(double) * count, hipMemcpyDeviceToHost, ctx->stream));
     CLG_HIP(hipStreamSynchronize(ctx->stream));
     for (int i = 0; i < count; ++i) host[i] = ctx->host_result[i];
@@ -531,6 +601,7 @@ int clg_scale(clg_context ctx, clg_field y, double ar, double ai) {
 
 int clg_dot(clg_context ctx, clg_field a, clg_field b, double out[2]) {
     CLG_CHECK_ARG(ctx && out, "null argument");
+    CLG_CHECK_ARG(!stream_capturing(ctx), "blocking call while recording a graph");
     (void)hipSetDevice(ctx->device);
     CLG_TRY(blas_dot_dev(ctx, a, b, ctx->red_result));
     return fetch_doubles(ctx, ctx->red_result, 2, out);
@@ -538,9 +609,54 @@ int clg_dot(clg_context ctx, clg_field a, clg_field b, double out[2]) {
 
 int clg_norm2(clg_context ctx, clg_field a, double* out) {
     CLG_CHECK_ARG(ctx && out, "null argument");
+    CLG_CHECK_ARG(!stream_capturing(ctx), "blocking call while recording a graph");
     (void)hipSetDevice(ctx->device);
     CLG_TRY(blas_norm2_dev(ctx, a, ctx->red_result));
     return fetch_doubles(ctx, ctx->red_result, 1, out);
+}
+
+int clg_multidot(clg_context ctx, int n, clg_field* V, clg_field w, double* out) {
+    CLG_CHECK_ARG(ctx && V && w && out, "null argument");
+    CLG_CHECK_ARG(!stream_capturing(ctx), "blocking call while recording a graph");
+    CLG_CHECK_ARG(n >= 1 && n <= 1000, "n must be in [1, 1000]");  // results below the staging area
+    (void)hipSetDevice(ctx->device);
+    CLG_TRY(blas_multidot_dev(ctx, V, n, w, ctx->red_result, nullptr, nullptr));
+    return fetch_doubles(ctx, ctx->red_result, 2 * n, out);
+}
+
+int clg_multiaxpy(clg_context ctx, clg_field w, int n, clg_field* V, const double* coeffs) {
+    CLG_CHECK_ARG(ctx && V && w && coeffs, "null argument");
+    CLG_CHECK_ARG(n >= 1 && n <= 992, "n must be in [1, 992]");
+    for (int k = 0; k < n; ++k) CLG_CHECK_ARG(V[k] && V[k]->data != w->data, "V[%d] must be a field other than w", k);
+    (void)hipSetDevice(ctx->device);
+    CLG_CHECK_ARG(!stream_capturing(ctx), "clg_multiaxpy uploads host coefficients; not while recording a graph");
+    double* c = ctx->red_result + kHostArgs;
+    CLG_HIP(hipMemcpyAsync(c, coeffs, sizeof(double) * 2 * n, hipMemcpyHostToDevice, ctx->stream));
+    CLG_TRY(blas_multiaxpy_dev(ctx, w, V, n, c, 1.0, nullptr, nullptr, nullptr, 0.0, nullptr));
+    CLG_HIP(hipStreamSynchronize(ctx->stream));  // coeffs is the caller's host memory
+    return CLG_OK;
+}
+
+int clg_vector_multiply_matrix(clg_context ctx, int nres, clg_field* res, int nleft, clg_field* left,
+                               const double* matrix) {
+    CLG_CHECK_ARG(ctx && res && matrix && (nleft == 0 || left), "null argument");
+    CLG_CHECK_ARG(nres >= 1 && nleft >= 0 && nres + nleft <= kMaxVmmIn && nres <= kMaxVmmOut,
+                  "need 1 <= nres <= %d and nres + nleft <= %d", kMaxVmmOut, kMaxVmmIn);
+    CLG_CHECK_ARG(!stream_capturing(ctx), "clg_vector_multiply_matrix uploads host data; not while recording a graph");
+    (void)hipSetDevice(ctx->device);
+    std::vector<clg_field_s*> in;
+    for (int j = 0; j < nres; ++j) in.push_back(res[j]);
+    for (int j = 0; j < nleft; ++j) in.push_back(left[j]);
+    for (size_t j = 0; j < in.size(); ++j) {
+        CLG_CHECK_ARG(in[j], "null field %d", (int)j);
+        for (size_t k = 0; k < j; ++k) CLG_CHECK_ARG(in[k]->data != in[j]->data, "fields %d and %d alias", (int)k, (int)j);
+    }
+    const int nin = nres + nleft;
+    double* m = ctx->red_result + kHostArgs;
+    CLG_HIP(hipMemcpyAsync(m, matrix, sizeof(double) * 2 * nin * nres, hipMemcpyHostToDevice, ctx->stream));
+    CLG_TRY(blas_vmm_dev(ctx, in.data(), nin, in.data(), nres, m));
+    CLG_HIP(hipStreamSynchronize(ctx->stream));
+    return CLG_OK;
 }
 
 }  // extern "C"

@@ -39,6 +39,7 @@ int blas_multiaxpy_dev(clg_context, clg_field_s*, clg_field_s* const*, int, cons
                        const double*, double, int*);
 int fetch_doubles(clg_context, const double*, int, double*);
 int workspace(clg_context, size_t, double**);
+int blas_vmm_dev(clg_context, clg_field_s* const*, int, clg_field_s* const*, int, const double*);
 int linop_apply(clg_context, int, int, const clg_field_s*, double, clg_field_s*, const clg_field_s*);
 int wilson_hop_half(clg_context, clg_field_s*, const clg_field_s*, const clg_field_s*, bool, double, double,
                     const clg_field_s*, double, double);
@@ -57,41 +58,6 @@ struct clg_gcrodr_s {
 namespace {
 
 constexpr int kMaxDimDR = 31;  // MDim limit of the reference (_kMaxDimDR, CSolverGCRODR.h)
-
-// out_i = sum_{j<X} in_j M[j*Y + i] (i < Y) for FP64 complex vectors of n elements; out may be the
-// first Y inputs (each thread reads all inputs of its element before writing)
-struct VmmArgs {
-    const double2* in[kMaxDimDR + 1];
-    double2* out[kMaxDimDR];
-    int X, Y;
-    long n;
-};
-__global__ __launch_bounds__(256) void k_vmm(VmmArgs a, const double* __restrict__ M) {
-    __shared__ double2 sm[(kMaxDimDR + 1) * kMaxDimDR];
-    const int tot = a.X * a.Y;
-    for (int i = threadIdx.x; i < tot; i += blockDim.x) sm[i] = make_double2(M[2 * i], M[2 * i + 1]);
-    __syncthreads();
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < a.n; e += (long)gridDim.x * blockDim.x) {
-        double2 acc[kMaxDimDR];
-#pragma unroll
-        for (int i = 0; i < kMaxDimDR; ++i) acc[i] = make_double2(0.0, 0.0);
-        for (int j = 0; j < a.X; ++j) {
-            const double2 v = a.in[j][e];
-            const double2* row = sm + j * a.Y;
-#pragma unroll
-            for (int i = 0; i < kMaxDimDR; ++i) {
-                if (i < a.Y) {
-                    const double2 m = row[i];
-                    acc[i].x = fma(v.x, m.x, fma(-v.y, m.y, acc[i].x));
-                    acc[i].y = fma(v.x, m.y, fma(v.y, m.x, acc[i].y));
-                }
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < kMaxDimDR; ++i)
-            if (i < a.Y) a.out[i][e] = acc[i];
-    }
-}
 
 // column j of the device G (column-major, ld rows): rows 0..nr-1 = c1 (+ c2 when the second pass
 // ran), row j+1 = |w|
@@ -167,20 +133,10 @@ struct Gcro {
     }
     // out_i = sum_j in_j M(j, i), in place allowed when out are the first inputs
     int vmm(const std::vector<clg_field_s*>& in, const std::vector<clg_field_s*>& out, const Mat& M) {
-        VmmArgs a;
-        std::memset(&a, 0, sizeof(a));
-        a.X = (int)in.size();
-        a.Y = (int)out.size();
-        CLG_CHECK_ARG(a.X <= kMaxDimDR + 1 && a.Y <= kMaxDimDR && M.r == a.X && M.c == a.Y, "bad VMM shape");
-        a.n = (long)(in[0]->bytes / sizeof(double2));
-        for (int j = 0; j < a.X; ++j) a.in[j] = (const double2*)in[j]->data;
-        for (int i = 0; i < a.Y; ++i) a.out[i] = (double2*)out[i]->data;
+        CLG_CHECK_ARG(M.r == (int)in.size() && M.c == (int)out.size(), "bad VMM shape");
         std::vector<cx> mv(M.a.begin(), M.a.end());
         CLG_TRY(upload(mv, dws + offM));
-        const unsigned grid = (unsigned)std::max<long>(1, std::min<long>((a.n + 255) / 256, (long)ctx->num_cu * 8));
-        k_vmm<<<grid, 256, 0, ctx->stream>>>(a, dws + offM);
-        CLG_HIP(hipGetLastError());
-        return CLG_OK;
+        return blas_vmm_dev(ctx, in.data(), (int)in.size(), out.data(), (int)out.size(), dws + offM);
     }
     // w <- w - sum_j W_j (W_j^H w) over W_0..W_{nw-1} with a conditional second pass; column j of G
     int orthogonalise(clg_field_s* w, int nw, int j) {

@@ -135,9 +135,19 @@ inline int field_parity(const clg_field_s* f) { return f->subset == CLG_ODD ? 1 
 
 // reductions (clg_blas.hip)
 int ensure_reduction_scratch(clg_context ctx, size_t doubles);
+// allocates the partial-sum buffer every reduction kernel uses (its final size)
+int reduction_ready(clg_context ctx);
 // scratch field of the same shape as `like` (kind/prec/subset/dims), index slot
 int scratch_field(clg_context ctx, const clg_field_s* like, int kind, int prec, int subset,
                   int slot, clg_field_s** out);
+// true while the context stream records a graph (blocking calls are refused then)
+bool stream_capturing(clg_context ctx);
+
+// launch autotuner (clg_wilson.hip, clg_batch.hip): process-wide switch and tuned-shape tables
+bool tuning_enabled();
+int batch_tune_count();
+int batch_tune_entry(int index, char* buf, int len);
+void batch_tune_reset();
 
 // launch helpers
 inline unsigned grid_for(long n, int block) { return (unsigned)((n + block - 1) / block); }

@@ -1,6 +1,7 @@
 // clg_runtime.hip -- context, fields, host<->device layout conversion, synthetic inputs.
 #include "clg_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -48,6 +49,16 @@ using namespace clgd;
 
 // ============================================================== device kernels
 namespace {
+
+// zero fill with 16-byte stores (a kernel rather than a memset node, so that recorded graphs
+// replay it like any other launch)
+__global__ void k_zero16(uint4* p, long n16) {
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (long)gridDim.x * blockDim.x) p[i] = z;
+}
+__global__ void k_zero4(unsigned* p, long n4) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x) p[i] = 0u;
+}
 
 // reunitarise(exp(i eps H)) with Taylor scaling-and-squaring (same recipe as the oracle)
 __device__ void random_link(unsigned link, double eps, unsigned seed, cd* out) {
@@ -344,7 +355,20 @@ int scratch_field(clg_context ctx, const clg_field_s* like, int kind, int prec, 
     *out = f;
     return CLG_OK;
 }
+
+bool stream_capturing(clg_context ctx) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(ctx->stream, &cs) != hipSuccess) return false;
+    return cs != hipStreamCaptureStatusNone;
+}
 }  // namespace clg
+
+struct clg_graph_s {
+    clg_context ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int nodes = 0;
+};
 
 // ============================================================== C ABI
 extern "C" {
@@ -407,8 +431,72 @@ int clg_finalize(clg_context c) {
 
 int clg_synchronize(clg_context c) {
     CLG_CHECK_ARG(c, "null context");
+    CLG_CHECK_ARG(!stream_capturing(c), "clg_synchronize while the context records a graph");
     CLG_HIP(hipStreamSynchronize(c->stream));
     CLG_HIP(hipGetLastError());
+    return CLG_OK;
+}
+
+// ---- graphs: record the launches of non-blocking entry points on the context stream, replay them
+int clg_graph_begin(clg_context c) {
+    CLG_CHECK_ARG(c, "null context");
+    CLG_CHECK_ARG(!stream_capturing(c), "the context already records a graph");
+    (void)hipSetDevice(c->device);
+    // recorded reductions keep the partial-sum buffer they were recorded with: allocate it first
+    CLG_TRY(reduction_ready(c));
+    CLG_HIP(hipStreamSynchronize(c->stream));
+    CLG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    return CLG_OK;
+}
+
+int clg_graph_end(clg_context c, clg_graph* out) {
+    CLG_CHECK_ARG(c && out, "null argument");
+    CLG_CHECK_ARG(stream_capturing(c), "the context records no graph");
+    (void)hipSetDevice(c->device);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (e != hipSuccess || !g) {
+        (void)hipGetLastError();
+        if (g) (void)hipGraphDestroy(g);
+        set_error("recording failed (%s): a blocking call or an allocation-growing call was made while recording",
+                  hipGetErrorString(e));
+        return CLG_HIP_ERROR;
+    }
+    size_t nodes = 0;
+    (void)hipGraphGetNodes(g, nullptr, &nodes);
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        set_error("hipGraphInstantiate: %s", hipGetErrorString(ei));
+        return CLG_HIP_ERROR;
+    }
+    clg_graph_s* r = new clg_graph_s();
+    r->ctx = c;
+    r->graph = g;
+    r->exec = ex;
+    r->nodes = (int)nodes;
+    *out = r;
+    return CLG_OK;
+}
+
+int clg_graph_launch(clg_graph g) {
+    CLG_CHECK_ARG(g && g->exec, "null graph");
+    CLG_CHECK_ARG(!stream_capturing(g->ctx), "cannot replay a graph while recording");
+    (void)hipSetDevice(g->ctx->device);
+    CLG_HIP(hipGraphLaunch(g->exec, g->ctx->stream));
+    return CLG_OK;
+}
+
+int clg_graph_nodes(clg_graph g) { return g ? g->nodes : -1; }
+
+int clg_graph_destroy(clg_graph g) {
+    if (!g) return CLG_OK;
+    (void)hipSetDevice(g->ctx->device);
+    (void)hipStreamSynchronize(g->ctx->stream);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
     return CLG_OK;
 }
 
@@ -519,8 +607,22 @@ int clg_field_zero(clg_field f) {
     CLG_CHECK_ARG(f, "null field");
     (void)hipSetDevice(f->ctx->device);
     const size_t nblk = is_gauge(f->kind) ? 8 : (f->subset == CLG_FULL ? 2 : 1);
-    CLG_HIP(hipMemsetAsync(f->data, 0, f->block_bytes * nblk, f->ctx->stream));
-    if (f->norm) CLG_HIP(hipMemsetAsync(f->norm, 0, sizeof(float) * f->Vh * nblk, f->ctx->stream));
+    hipStream_t st = f->ctx->stream;
+    const size_t bytes = f->block_bytes * nblk;
+    const unsigned cap = (unsigned)f->ctx->num_cu * 16;
+    if (bytes % 16 == 0 && ((size_t)f->data & 15) == 0) {
+        const long n = (long)(bytes / 16);
+        k_zero16<<<std::max(1u, std::min(cap, (unsigned)((n + 255) / 256))), 256, 0, st>>>((uint4*)f->data, n);
+    } else {
+        const long n = (long)(bytes / 4);  // block sizes are multiples of 4 bytes
+        k_zero4<<<std::max(1u, std::min(cap, (unsigned)((n + 255) / 256))), 256, 0, st>>>((unsigned*)f->data, n);
+    }
+    CLG_HIP(hipGetLastError());
+    if (f->norm) {
+        const long n = (long)(f->Vh * nblk);
+        k_zero4<<<std::max(1u, std::min(cap, (unsigned)((n + 255) / 256))), 256, 0, st>>>((unsigned*)f->norm, n);
+        CLG_HIP(hipGetLastError());
+    }
     return CLG_OK;
 }
 

@@ -594,6 +594,7 @@ int clg_solver_params_default(clg_solver_params* p) {
 int clg_solve(clg_context ctx, const clg_solver_params* p, clg_field x, clg_field b, clg_field gauge,
               double param, int op, clg_field start, clg_solve_report* rep) {
     CLG_CHECK_ARG(ctx && x && b && gauge, "null argument");
+    CLG_CHECK_ARG(!stream_capturing(ctx), "a solve blocks; not while recording a graph");
     CLG_TRY(validate_params(p));
     CLG_CHECK_ARG(b->kind == p->kind && x->kind == p->kind, "field kind does not match params.kind");
     CLG_CHECK_ARG(b->subset == CLG_FULL && x->subset == CLG_FULL, "solve takes FULL fields");

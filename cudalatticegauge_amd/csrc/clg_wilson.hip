@@ -14,8 +14,11 @@
 #include "clg_internal.h"
 #include "clg_wilson_common.h"
 
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
+#include <iterator>
 #include <map>
 #include <mutex>
 
@@ -173,18 +176,25 @@ std::mutex& tune_mutex() {
     static std::mutex m;
     return m;
 }
-bool tuning_enabled() {
-    const char* e = std::getenv("CLG_TUNE");
-    return !(e && e[0] == '0');
+bool valid_cfg(int pm, int blk, int xcd) {
+    return (pm == 1 || pm == 2) && (blk == 64 || blk == 128 || blk == 256) && (xcd == 0 || xcd == 8);
 }
-// CLG_TUNE_FORCE="pin,block,xcd" fixes the launch configuration of every hop launch (used to
-// profile one configuration under counters, where event timings are not representative)
+// process-wide overrides set through clg_autotune_enable / clg_autotune_force
+std::atomic<int> g_tune_mode{-1};  // -1: CLG_TUNE environment (default on), 0 off, 1 on
+std::atomic<int> g_forced{0};      // packed pin | block << 4 | xcd << 16, 0 = none
+// CLG_TUNE_FORCE="pin,block,xcd" (or clg_autotune_force) fixes the launch configuration of every
+// hop launch (used to profile one configuration under counters, where event timings are not
+// representative)
 bool forced_cfg(HopCfg& c) {
+    const int f = g_forced.load();
+    if (f) {
+        c = HopCfg{f & 15, (f >> 4) & 4095, f >> 16};
+        return true;
+    }
     const char* e = std::getenv("CLG_TUNE_FORCE");
     if (!e) return false;
     int pm = 0, blk = 0, xcd = 0;
-    if (std::sscanf(e, "%d,%d,%d", &pm, &blk, &xcd) != 3) return false;
-    if ((pm != 1 && pm != 2) || (blk != 64 && blk != 128 && blk != 256) || (xcd != 0 && xcd != 8)) return false;
+    if (std::sscanf(e, "%d,%d,%d", &pm, &blk, &xcd) != 3 || !valid_cfg(pm, blk, xcd)) return false;
     c = HopCfg{pm, blk, xcd};
     return true;
 }
@@ -256,6 +266,13 @@ HopArgs base_args(const clg_field_s* out, const clg_field_s* gauge) {
 }  // namespace
 
 namespace clg {
+
+bool tuning_enabled() {
+    const int m = g_tune_mode.load();
+    if (m >= 0) return m != 0;
+    const char* e = std::getenv("CLG_TUNE");
+    return !(e && e[0] == '0');
+}
 
 // out(full) = coeff * (in - kappa H in) with dagger projectors when dag
 int wilson_full(clg_context ctx, clg_field_s* out, const clg_field_s* in, const clg_field_s* gauge, double kappa,
@@ -421,6 +438,61 @@ int clg_time_wilson(clg_context ctx, clg_field out, clg_field in, clg_field gaug
     CLG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     *ms_avg = (double)ms / reps;
     return CLG_OK;
+}
+
+// ---- launch autotuner control
+int clg_autotune_enable(int mode) {
+    CLG_CHECK_ARG(mode >= -1 && mode <= 1, "mode must be -1 (environment), 0 or 1");
+    g_tune_mode.store(mode);
+    return CLG_OK;
+}
+
+int clg_autotune_force(int pin_mode, int block, int xcd) {
+    if (pin_mode == 0) {
+        g_forced.store(0);
+        return CLG_OK;
+    }
+    CLG_CHECK_ARG(valid_cfg(pin_mode, block, xcd), "pin_mode 1|2, block 64|128|256, xcd 0|8");
+    g_forced.store(pin_mode | block << 4 | xcd << 16);
+    return CLG_OK;
+}
+
+int clg_autotune_reset(void) {
+    {
+        std::lock_guard<std::mutex> lk(tune_mutex());
+        tune_table().clear();
+    }
+    batch_tune_reset();
+    return CLG_OK;
+}
+
+int clg_autotune_count(void) {
+    int n = 0;
+    {
+        std::lock_guard<std::mutex> lk(tune_mutex());
+        n = (int)tune_table().size();
+    }
+    return n + batch_tune_count();
+}
+
+int clg_autotune_entry(int index, char* buf, int len) {
+    CLG_CHECK_ARG(buf && len > 0 && index >= 0, "bad arguments");
+    {
+        std::lock_guard<std::mutex> lk(tune_mutex());
+        if (index < (int)tune_table().size()) {
+            auto it = tune_table().begin();
+            std::advance(it, index);
+            const TuneKey& k = it->first;
+            const HopCfg& c = it->second;
+            std::snprintf(buf, (size_t)len,
+                          "wilson_hop prec=%d gauge_prec=%d recon=%d dagger=%d x=%d parities=%d dims=%dx%dx%dx%d "
+                          "-> pin=%d block=%d xcd=%d",
+                          k.prec, k.gprec, k.recon, k.dag, k.hasx, k.nblk, k.L0, k.L1, k.L2, k.L3, c.pm, c.block, c.xcd);
+            return CLG_OK;
+        }
+        index -= (int)tune_table().size();
+    }
+    return batch_tune_entry(index, buf, len);
 }
 
 }  // extern "C"

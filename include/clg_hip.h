@@ -144,6 +144,19 @@ int clg_scale(clg_context ctx, clg_field y, double a_re, double a_im);
  * the host, as the reference's Dot does. */
 int clg_dot(clg_context ctx, clg_field a, clg_field b, double out[2]);
 int clg_norm2(clg_context ctx, clg_field a, double* out);
+/* Krylov BLAS of the solvers, FP64 accumulation, one pass over w per 16 vectors:
+ * out[2k..2k+1] = <V_k, w> for k < n <= 1000 -- the Arnoldi inner products the reference takes
+ * as n separate CField::Dot calls (CSolverGMRES.cpp:175-189).  Blocks. */
+int clg_multidot(clg_context ctx, int n, clg_field* V, clg_field w, double* out);
+/* w += sum_k c_k V_k, c_k = coeffs[2k] + i coeffs[2k+1], n <= 992 -- the solution update the
+ * reference runs as n CField::Axpy calls (CSolverGMRES.cpp:222-229).  No V_k may be w. */
+int clg_multiaxpy(clg_context ctx, clg_field w, int n, clg_field* V, const double* coeffs);
+/* CFieldMatrixOperation::VectorMultiplyMatrix (CField.h:372-380; _kernelMatrixMultiply,
+ * CFieldFermionWilsonSquareSU3.cu:1533-1560): with W = (res_0..res_{nres-1}, left_0..left_{nleft-1}),
+ * res_i <- sum_j W_j M[j*nres + i], M a host (nres+nleft) x nres complex matrix (interleaved
+ * re/im, row j = input j).  FP64 fields of one shape, nres <= 31, nres + nleft <= 32. */
+int clg_vector_multiply_matrix(clg_context ctx, int nres, clg_field* res, int nleft, clg_field* left,
+                               const double* matrix);
 
 /* ============================== solvers ============================== */
 typedef enum { CLG_SOLVER_GMRES = 0, CLG_SOLVER_BICGSTAB = 1 } clg_solver_kind;
@@ -384,6 +397,39 @@ int clg_wilson_hop_batch(clg_context ctx, int n, clg_field* out, clg_field* in, 
  * (CFieldFermionWilsonSquareSU3.cu:1222-1255).  reports (nullable) holds n entries. */
 int clg_solve_batch(clg_context ctx, const clg_solver_params* params, int n, clg_field* x, clg_field* b,
                     clg_field* gauge, double kappa, int op, clg_solve_report* reports);
+
+/* ============================== launch autotuning ============================== */
+/* The Wilson hop launches (single and batched) time their candidate launch shapes (pin mode,
+ * workgroup size, XCD remap; batched: right-hand sides per thread) on the first call of each
+ * operator shape and keep the fastest.  The reference fixes one decomposition at start-up
+ * (_HC_DecompX.., CCommonData.h:72-91); these calls inspect and steer the choice. */
+/* mode 1 tune, 0 use defaults, -1 follow the CLG_TUNE environment variable (default on) */
+int clg_autotune_enable(int mode);
+/* fix every single-system hop launch: pin_mode 1|2, block 64|128|256, xcd 0|8; pin_mode 0 clears */
+int clg_autotune_force(int pin_mode, int block, int xcd);
+/* forget every tuned shape (the next call of each shape tunes again) */
+int clg_autotune_reset(void);
+/* number of tuned shapes, and a one-line description of entry `index` */
+int clg_autotune_count(void);
+int clg_autotune_entry(int index, char* buf, int len);
+
+/* ============================== graphs ============================== */
+/* Record the launches of non-blocking entry points made on a context between begin and end
+ * (operators, hops, batched hops, axpy/scale, copies, zero, gauge updates and forces), then
+ * replay them with one launch.  Entry points that read results back to the host or upload host
+ * data (dot, norm2, multi-dot, solves, energies, HMC) fail with CLG_BAD_ARG while recording; the
+ * recording stays open (clg_dot, clg_norm2, clg_multidot and clg_solve refuse before launching
+ * anything, the others may leave launches of their own in it).  A recorded operator keeps the launch shape chosen before recording (tune
+ * outside first).  Fields used by a graph must outlive it.  No reference counterpart: the
+ * reference launches every kernel from the host (CSolverGMRES.cpp:150-247). */
+typedef struct clg_graph_s* clg_graph;
+int clg_graph_begin(clg_context ctx);
+int clg_graph_end(clg_context ctx, clg_graph* out);
+/* enqueue the recorded work on the context stream; returns without waiting (clg_synchronize) */
+int clg_graph_launch(clg_graph g);
+/* recorded operations (graph nodes) */
+int clg_graph_nodes(clg_graph g);
+int clg_graph_destroy(clg_graph g);
 
 /* ============================== measurement ============================== */
 /* Times `reps` back-to-back launches of one Wilson operator on the context stream with HIP
