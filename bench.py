@@ -154,6 +154,10 @@ def main():
         extra["solve_speedup_mixed_fp16link_vs_fp64"] = round(
             results["fp64"].seconds / results["mixed_fp16link"].seconds, 3)
 
+        del x
+        # harder system: 32^3x32, kappa 0.137, mixed precision against the FP64 solve
+        extra.update(mixed_solves_32x32x32x32(ctx, clg, np))
+
     # ---------------- batched solves: 12 point sources + 4 uneven lattices in one batch
     if not args.no_batch:
         extra.update(batched_solves(ctx, clg, np))
@@ -185,8 +189,7 @@ def main():
 
     achieved = BYTES_D[8] * V / (k64 * 1e-3) / 1e9
     line = {
-        "metric": "Wilson Dslash GB/s (algorithmic bytes, % of 8 TB/s HBM peak) at 32^3x16; "
-                  "GMRES(12) solve time to 1e-10 mixed vs FP64",
+        "metric": "Dslash GB/s (% HBM peak); 32\u00b3\u00d716 solve time to 1e-10: mixed/batched vs FP64",
         "value": round(gbs, 1),
         "unit": "GB/s",
         "n_gpus": 1,
@@ -262,6 +265,49 @@ def rhmc_trajectory(ctx, clg, np):
     return out
 
 
+def mixed_solves_32x32x32x32(ctx, clg, np):
+    """EO-GMRES(12) to |r|/|b| < 1e-10 at 32^3x32, kappa 0.137, eps 0.5 gauge, Gaussian source: plain FP64
+    against FP32-inner and FP16-link-inner defect correction; the mixed solutions are compared with the
+    FP64 one (|x - x64| / |x64|)."""
+    dims, kappa = (32, 32, 32, 32), 0.137
+    g = clg.Field(ctx, clg.GAUGE, clg.F64, dims)
+    g.gauge_random(0.5, 1)
+    b = clg.Field(ctx, clg.WILSON, clg.F64, dims, bc=(1, 1, 1, -1))
+    b.gaussian(2)
+    x = clg.Field(ctx, clg.WILSON, clg.F64, dims, bc=(1, 1, 1, -1))
+    out, sols, times = {}, {}, {}
+    for name, inner, inner_acc in (("fp64", clg.F64, 0.0), ("mixed_fp32", clg.F32, 1e-4),
+                                   ("mixed_fp16link", clg.F16, 1e-3)):
+        p = clg.solver_params(max_dim=12, restart=2000, accuracy=1e-10, stop_rule=clg.STOP_TRUE_RESIDUAL, even_odd=1,
+                              inner_prec=inner, inner_accuracy=inner_acc, max_outer=80)
+        clg.solve(ctx, p, x, b, g, kappa, clg.EFO_F_InverseD)  # warm (tuning, scratch)
+        best = None
+        for _ in range(2):
+            st, rep = clg.solve(ctx, p, x, b, g, kappa, clg.EFO_F_InverseD)
+            if best is None or rep.seconds < best[1].seconds:
+                best = (st, rep)
+        st, rep = best
+        sols[name] = x.download()
+        times[name] = rep.seconds
+        tag = f"c2_{name}"
+        out[f"{tag}_s"] = round(rep.seconds, 4)
+        out[f"{tag}_cycles"] = rep.cycles
+        out[f"{tag}_outer"] = rep.outer
+        out[f"{tag}_applies"] = rep.operator_applies
+        out[f"{tag}_true_res"] = float(f"{rep.true_residual:.3e}")
+        msg = f"32^3x32 kappa {kappa} solve {name}: {rep.seconds * 1e3:.1f} ms, cycles {rep.cycles}, outer {rep.outer}, "
+        msg += f"res {rep.true_residual:.2e}, status {st}"
+        if name != "fp64":
+            err = float(np.linalg.norm(sols[name] - sols["fp64"]) / np.linalg.norm(sols["fp64"]))
+            out[f"{tag}_err_vs_fp64"] = float(f"{err:.3e}")
+            out[f"{tag}_speedup_vs_fp64"] = round(times["fp64"] / rep.seconds, 3)
+            msg += f", |x-x64|/|x64| {err:.2e}, x{times['fp64'] / rep.seconds:.2f} vs FP64"
+        log(msg)
+    out["c2_config"] = "32^3x32 Wilson EO-GMRES(12), kappa 0.137, eps 0.5 gauge (seed 1), Gaussian source (seed 2)"
+    del g, b, x
+    return out
+
+
 def batched_solves(ctx, clg, np):
     """12 point sources (every spin x colour at the origin) on 24^3x48 plus one Gaussian source each on
     8^4, 12^4, 16^3x32 and 20^3x40, kappa 0.1355, eps 0.5 gauge fields: one batched EO-GMRES(12)
@@ -306,12 +352,27 @@ def batched_solves(ctx, clg, np):
         sep_res = max(sep_res, rep1.true_residual)
     ctx.sync()
     t_sep = time.perf_counter() - t0
+    # and as plain FP64 solves (no inner precision), one at a time
+    p64 = clg.solver_params(max_dim=12, restart=2000, accuracy=1e-10, stop_rule=clg.STOP_TRUE_RESIDUAL, even_odd=1,
+                            inner_prec=clg.F64, max_outer=1)
+    for i in range(len(xs)):
+        clg.solve(ctx, p64, xs[i], bs[i], gl[i], KAPPA, clg.EFO_F_InverseD)
+    t0 = time.perf_counter()
+    res64 = 0.0
+    for i in range(len(xs)):
+        st2, rep2 = clg.solve(ctx, p64, xs[i], bs[i], gl[i], KAPPA, clg.EFO_F_InverseD)
+        res64 = max(res64, rep2.true_residual)
+    ctx.sync()
+    t_64 = time.perf_counter() - t0
     max_res = max(r.true_residual for r in reps)
-    log(f"batched solve: {len(xs)} systems, {site_rhs} site-RHS, {t_batch * 1e3:.1f} ms (separate {t_sep * 1e3:.1f} ms, "
-        f"x{t_sep / t_batch:.2f}), max |r|/|b| {max_res:.2e}, status {st}")
+    log(f"batched solve: {len(xs)} systems, {site_rhs} site-RHS, {t_batch * 1e3:.1f} ms (separate mixed "
+        f"{t_sep * 1e3:.1f} ms x{t_sep / t_batch:.2f}; separate FP64 {t_64 * 1e3:.1f} ms x{t_64 / t_batch:.2f}), "
+        f"max |r|/|b| {max_res:.2e}, status {st}")
     out = {"batch_systems": len(xs), "batch_site_rhs": site_rhs, "batch_solve_s": round(t_batch, 4),
            "batch_separate_s": round(t_sep, 4), "batch_speedup_vs_separate": round(t_sep / t_batch, 3),
+           "batch_separate_fp64_s": round(t_64, 4), "batch_throughput_vs_fp64": round(t_64 / t_batch, 3),
            "batch_max_true_res": float(f"{max_res:.3e}"), "batch_separate_max_true_res": float(f"{sep_res:.3e}"),
+           "batch_separate_fp64_max_true_res": float(f"{res64:.3e}"),
            "batch_site_rhs_per_s": round(site_rhs / t_batch, 1)}
     del xs, bs, gl, gauges
     return out

@@ -29,6 +29,7 @@ template <int P> struct Pk;
 template <> struct Pk<PK_D> {
     typedef double2 T;
     static constexpr int NC = 1;
+    __device__ static void cvt(const T& a, double* v) { v[0] = a.x; v[1] = a.y; }
     __device__ static void ld(const T* p, long i, double* v) {
         const double2 a = p[i];
         v[0] = a.x; v[1] = a.y;
@@ -38,6 +39,7 @@ template <> struct Pk<PK_D> {
 template <> struct Pk<PK_F4> {
     typedef float4 T;
     static constexpr int NC = 2;
+    __device__ static void cvt(const T& a, double* v) { v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; }
     __device__ static void ld(const T* p, long i, double* v) {
         const float4 a = p[i];
         v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
@@ -52,6 +54,7 @@ template <> struct Pk<PK_F4> {
 template <> struct Pk<PK_F2> {
     typedef float2 T;
     static constexpr int NC = 1;
+    __device__ static void cvt(const T& a, double* v) { v[0] = a.x; v[1] = a.y; }
     __device__ static void ld(const T* p, long i, double* v) {
         const float2 a = p[i];
         v[0] = a.x; v[1] = a.y;
@@ -192,17 +195,25 @@ __global__ __launch_bounds__(kRedBlock) void k_multidot(long n, Ptrs<P> V, const
 #pragma unroll
     for (int k = 0; k < 2 * K + 1; ++k) v[k] = 0;
     GRID_STRIDE(i, n) {
+        // issue every packet load of this element first (raw 16-byte packets), then convert: the
+        // loads stay in flight together instead of one vector at a time
+        typename Pk<P>::T raw[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) raw[k] = V.p[k][i];
         double y[2 * NC];
         Pk<P>::ld(w, i, y);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             double x[2 * NC];
-            Pk<P>::ld(V.p[k], i, x);
+            Pk<P>::cvt(raw[k], x);
+            double re = 0.0, im = 0.0;
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
-                v[2 * k] += x[2 * c] * y[2 * c] + x[2 * c + 1] * y[2 * c + 1];
-                v[2 * k + 1] += x[2 * c] * y[2 * c + 1] - x[2 * c + 1] * y[2 * c];
+                re += x[2 * c] * y[2 * c] + x[2 * c + 1] * y[2 * c + 1];
+                im += x[2 * c] * y[2 * c + 1] - x[2 * c + 1] * y[2 * c];
             }
+            v[2 * k] += re;
+            v[2 * k + 1] += im;
         }
         if constexpr (WN) {
 #pragma unroll
@@ -229,12 +240,15 @@ __global__ __launch_bounds__(kRedBlock) void k_multiaxpy(long n, Ptrs<P> V, cons
     for (int k = 0; k < K; ++k) { cr[k] = s * cc[2 * k]; ci[k] = s * cc[2 * k + 1]; }
     double v[1] = {0};
     GRID_STRIDE(i, n) {
+        typename Pk<P>::T raw[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) raw[k] = V.p[k][i];
         double y[2 * NC];
         Pk<P>::ld(w, i, y);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             double x[2 * NC];
-            Pk<P>::ld(V.p[k], i, x);
+            Pk<P>::cvt(raw[k], x);
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
                 y[2 * c] += cr[k] * x[2 * c] - ci[k] * x[2 * c + 1];

@@ -353,7 +353,7 @@ int msgmres(clg_context ctx, OpRef& A, std::vector<clg_field_s*>& X, const std::
     clg_field_s *xb = nullptr, *w = V[m];
     CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 2100, &xb));
     const int ld = m + 1;
-    const size_t oH = 0, oY = oH + 2 * (size_t)ld * m, oS = oY + 2 * (size_t)m * std::max(ns, 1), oC = oS + 16,
+    const size_t oH = 0, oY = oH + 2 * (size_t)ld * m, oS = oY + 2 * (size_t)m * (ns + 1), oC = oS + 16,
                  tot = oC + 2 * (size_t)(m + 1);
     double* ws = nullptr;
     CLG_TRY(workspace(ctx, tot, &ws));
@@ -392,8 +392,9 @@ int msgmres(clg_context ctx, OpRef& A, std::vector<clg_field_s*>& X, const std::
         std::vector<double> hh(2 * (size_t)ld * m);
         CLG_TRY(fetch_doubles(ctx, H, (int)hh.size(), hh.data()));
         Mat Hb(m + 1, m);
+        // columns hold rows 0..j+1 only; the device entries below the sub-diagonal are stale
         for (int j = 0; j < m; ++j)
-            for (int i = 0; i <= m; ++i) Hb(i, j) = zc(hh[2 * (j * ld + i)], hh[2 * (j * ld + i) + 1]);
+            for (int i = 0; i <= j + 1; ++i) Hb(i, j) = zc(hh[2 * (j * ld + i)], hh[2 * (j * ld + i) + 1]);
         // unshifted least squares, x_base += V y, z = beta e_1 - Hbar y
         std::vector<zc> rhs(m + 1, zc(0.0, 0.0)), y;
         rhs[0] = beta;

@@ -549,14 +549,16 @@ UINT CHMC::Update(UINT iSteps, UBOOL bMeasure) {
     for (auto& a : m_pOwner->m_pActionList) acts.push_back(a->Describe());
     for (UINT i = 0; i < iSteps; ++i) {
         clg_hmc_report rep{};
+        // the reference measures H_diff in its test mode but then accepts every trajectory, and
+        // records H_diff only in that mode (CHMC.cpp:100-140)
         clg_hmc_params hp = m_params;
-        if (m_bTestHDiff) hp.metropolis = 1;
+        if (m_bTestHDiff) hp.metropolis = 0;
         check(clg_hmc_trajectory(m_pOwner->m_ctx, m_pOwner->m_pGaugeField->Handle(), &hp, acts.data(), (int)acts.size(),
                                  m_uiSeed * 1000003ull + (++m_uiTrajectory), &rep),
               "HMC trajectory");
         m_fLastHDiff = rep.h_diff;
-        if (m_params.metropolis || m_bTestHDiff) m_lstHDiff.push_back(rep.h_diff);
-        const bool accepted = (m_params.metropolis || m_bTestHDiff) ? rep.accepted != 0 : true;
+        if (m_bTestHDiff) m_lstHDiff.push_back(rep.h_diff);
+        const bool accepted = rep.accepted != 0;
         if (accepted) ++m_iAcceptedConfigurationCount;
         std::fprintf(stderr, " HMC: step = %u, H_diff = %f (%f - %f) %s\n", i + 1, rep.h_diff, rep.h_before, rep.h_after,
                      accepted ? "accepted" : "rejected");

@@ -353,7 +353,8 @@ typedef struct {
     int nested_steps;    /* NestedStep (CIntegrator.cpp:281-297) */
     int inner_leapfrog;  /* InnerLeapfrog: nested leapfrog inside the force-gradient steps */
     int exp_precision;   /* ExponentialPrecision: 0 quick exponential, n > 0 Taylor order n */
-    int metropolis;      /* Metropolis */
+    int metropolis;      /* Metropolis: 1 accept on exp(h_diff), 0 accept every trajectory (h_diff is
+                            reported either way) */
 } clg_hmc_params;
 typedef struct {
     double h_before, h_after; /* kinetic + every action, FP64 (CIntegrator::GetEnergy) */
