@@ -441,7 +441,7 @@ __global__ __launch_bounds__(kRedBlock) void k_bnorm2(const BTab* T, Act act, in
 #pragma unroll
         for (int c = 0; c < 2 * BP<P>::NC; ++c) acc[0] += a[c] * a[c];
     }
-    reduce_finish<1>(acc, partial + (size_t)blockIdx.y * gridDim.x, counter + s, out + (size_t)s * ostride);
+    block_partial<1>(acc, partial + (size_t)blockIdx.y * gridDim.x);
 }
 
 // dst = src / sqrt(n2[s * nstride])
@@ -496,12 +496,7 @@ __global__ __launch_bounds__(kRedBlock) void k_bmultidot(const BTab* T, Act act,
             for (int c = 0; c < 2 * NC; ++c) acc[2 * K] += wv[c] * wv[c];
         }
     }
-    double* dst = out + (size_t)s * ostride;
-    __shared__ double stage[NR];
-    if (reduce_finish<NR>(acc, partial + (size_t)blockIdx.y * gridDim.x * NR, counter + s, stage)) {
-        for (int r = 0; r < 2 * K; ++r) dst[r] = stage[r];
-        if constexpr (WN) wn[(size_t)s * wstride] = stage[2 * K];
-    }
+    block_partial<NR>(acc, partial + (size_t)blockIdx.y * gridDim.x * NR);
 }
 
 // y = target (V[s][tsel] or X), y -= sgn * sum_j c_j V_j with c at c[s*cstride + 2j]; NORM: |y|^2 into
@@ -546,13 +541,40 @@ __global__ __launch_bounds__(kRedBlock) void k_bmultiaxpy(const BTab* T, Act act
             for (int c = 0; c < 2 * NC; ++c) acc[0] += q[c] * q[c];
         }
     }
-    if constexpr (NORM) {
-        __shared__ double stage[1];
-        if (reduce_finish<1>(acc, partial + (size_t)blockIdx.y * gridDim.x, counter + s, stage)) {
-            nrm[(size_t)s * nstride] = stage[0];
-            if (flag) flag[(size_t)s * nstride] = stage[0] < eta * ref[(size_t)s * nstride] ? 1.0 : 0.0;
-        }
+    if constexpr (NORM) block_partial<1>(acc, partial + (size_t)blockIdx.y * gridDim.x);
+}
+
+// Second stage of the batched reductions: one workgroup per active system (blockIdx.y) sums its
+// nblk partials (nr values each) in a fixed order; totals r < nout go to out[s*stride + r], the one
+// after them to out2[s*stride]; flag (nullable) gets total_0 < eta * ref[s*stride].  Systems with
+// cond[s*stride] == 0 are skipped, as in the first stage.
+__global__ __launch_bounds__(kRedBlock) void k_bred_finish(Act act, const double* __restrict__ partial, int nblk,
+                                                           int nr, double* out, int nout, double* out2,
+                                                           const double* cond, const double* ref, double eta,
+                                                           double* flag, int stride) {
+    const int s = act.idx[blockIdx.y];
+    if (cond && cond[(size_t)s * stride] == 0.0) return;
+    const double* pp = partial + (size_t)blockIdx.y * nblk * nr;
+    __shared__ double sm[kRedBlock / 64][2 * kMaxM + 1];
+    __shared__ double tot[2 * kMaxM + 1];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int r = 0; r < nr; ++r) {
+        double a = 0;
+        for (int b = threadIdx.x; b < nblk; b += blockDim.x) a += pp[(size_t)b * nr + r];
+        a = wave_sum(a);
+        if (lane == 0) sm[wv][r] = a;
     }
+    __syncthreads();
+    if (threadIdx.x < nr) {
+        const int r = threadIdx.x;
+        double t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sm[w][r];
+        tot[r] = t;
+        if (r < nout) out[(size_t)s * stride + r] = t;
+        else if (out2) out2[(size_t)s * stride] = t;
+    }
+    __syncthreads();
+    if (flag && threadIdx.x == 0) flag[(size_t)s * stride] = tot[0] < eta * ref[(size_t)s * stride] ? 1.0 : 0.0;
 }
 
 // y = a y + b x over (V or X) selections, real coefficients
@@ -810,6 +832,10 @@ struct BatchSolver {
                 dtab, A, m, ctx->red_partial, ctx->red_counter, out_base + ostride_off, (int)L.stride, nullptr,
                 (int)L.stride, cond, (int)L.stride);
         CLG_HIP(hipGetLastError());
+        k_bred_finish<<<dim3(1, A.n), kRedBlock, 0, ctx->stream>>>(A, ctx->red_partial, (int)g, 2 * K + (wn ? 1 : 0),
+                                                                   out_base + ostride_off, 2 * K, wn, cond, nullptr,
+                                                                   0.0, nullptr, (int)L.stride);
+        CLG_HIP(hipGetLastError());
         return CLG_OK;
     }
     template <int P, int K>
@@ -826,6 +852,11 @@ struct BatchSolver {
                                                                            ctx->red_partial, ctx->red_counter, nullptr,
                                                                            (int)L.stride, cond, ref, eta, flag);
         CLG_HIP(hipGetLastError());
+        if (nrm) {
+            k_bred_finish<<<dim3(1, A.n), kRedBlock, 0, ctx->stream>>>(A, ctx->red_partial, (int)g, 1, nrm, 1, nullptr,
+                                                                       cond, ref, eta, flag, (int)L.stride);
+            CLG_HIP(hipGetLastError());
+        }
         return CLG_OK;
     }
     // dispatch on the runtime vector count k (1..kMaxM)
@@ -877,6 +908,9 @@ struct BatchSolver {
             CLG_HIP(hipGetLastError());
             k_bnorm2<P><<<rgrid, kRedBlock, 0, ctx->stream>>>(dtab, A, 0, ctx->red_partial, ctx->red_counter,
                                                               ws + L.offS, (int)L.stride);
+            k_bred_finish<<<dim3(1, A.n), kRedBlock, 0, ctx->stream>>>(A, ctx->red_partial, (int)rg, 1, ws + L.offS, 1,
+                                                                       nullptr, nullptr, nullptr, 0.0, nullptr,
+                                                                       (int)L.stride);
             k_bscale<P><<<egrid, 256, 0, ctx->stream>>>(dtab, A, 0, 0, ws + L.offS, (int)L.stride);
             CLG_HIP(hipGetLastError());
             for (int s : act) cycles[s]++;

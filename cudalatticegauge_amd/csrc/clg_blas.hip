@@ -19,9 +19,9 @@ namespace {
 
 const int kMaxRedGrid = 2048;
 const int kMaxK = 16;  // vectors per multi-dot / multi-axpy launch
-// the context's 4096-double device result buffer: results from 0, multi-dot staging at 2048
-// (2 kMaxK + 1 doubles), host-supplied coefficients of the BLAS entry points from 2112 (1984)
-const int kMultidotStage = 2048, kHostArgs = 2112;
+// the context's 4096-double device result buffer: results from 0, host-supplied coefficients of
+// the BLAS entry points from 2112 (1984 doubles)
+const int kHostArgs = 2112;
 
 // ---- 16-byte packets: NC complex numbers per lane and load, values handled as doubles
 enum { PK_D = 0, PK_F4 = 1, PK_F2 = 2 };
@@ -150,8 +150,7 @@ __global__ void k_conj(long n, const typename Pk<P>::T* __restrict__ a, typename
 // out = sum conj(a) b  (2 doubles)
 template <int P>
 __global__ __launch_bounds__(kRedBlock) void k_dot(long n, const typename Pk<P>::T* __restrict__ a,
-                                                    const typename Pk<P>::T* __restrict__ b, double* partial,
-                                                    unsigned* counter, double* out) {
+                                                    const typename Pk<P>::T* __restrict__ b, double* partial) {
     constexpr int NC = Pk<P>::NC;
     double v[2] = {0, 0};
     stride4(n, [&](long i) {
@@ -164,11 +163,10 @@ __global__ __launch_bounds__(kRedBlock) void k_dot(long n, const typename Pk<P>:
             v[1] += x[2 * c] * y[2 * c + 1] - x[2 * c + 1] * y[2 * c];
         }
     });
-    reduce_finish<2>(v, partial, counter, out);
+    block_partial<2>(v, partial);
 }
 template <int P>
-__global__ __launch_bounds__(kRedBlock) void k_norm2(long n, const typename Pk<P>::T* __restrict__ a, double* partial,
-                                                      unsigned* counter, double* out) {
+__global__ __launch_bounds__(kRedBlock) void k_norm2(long n, const typename Pk<P>::T* __restrict__ a, double* partial) {
     constexpr int NC = Pk<P>::NC;
     double v[1] = {0};
     stride4(n, [&](long i) {
@@ -177,18 +175,17 @@ __global__ __launch_bounds__(kRedBlock) void k_norm2(long n, const typename Pk<P
 #pragma unroll
         for (int c = 0; c < 2 * NC; ++c) v[0] += x[c] * x[c];
     });
-    reduce_finish<1>(v, partial, counter, out);
+    block_partial<1>(v, partial);
 }
 
 template <int P> struct Ptrs {
     const typename Pk<P>::T* p[kMaxK];
 };
-// out[2k..2k+1] = sum conj(V_k) w  for k < K; WN: also |w|^2 -> wnorm[0].
-// cond (nullable): when *cond == 0 the whole grid exits at once (a skipped pass).
+// partials of sum conj(V_k) w for k < K (2K values) and, WN, of |w|^2 (one more); k_red_finish
+// completes them.  cond (nullable): when *cond == 0 the whole grid exits at once (a skipped pass).
 template <int P, int K, bool WN>
 __global__ __launch_bounds__(kRedBlock) void k_multidot(long n, Ptrs<P> V, const typename Pk<P>::T* __restrict__ w,
-                                                         double* partial, unsigned* counter, double* tmp, double* out,
-                                                         double* wnorm, const int* cond) {
+                                                         double* partial, const int* cond) {
     constexpr int NC = Pk<P>::NC;
     if (cond && *cond == 0) return;
     double v[2 * K + 1];
@@ -220,19 +217,14 @@ __global__ __launch_bounds__(kRedBlock) void k_multidot(long n, Ptrs<P> V, const
             for (int c = 0; c < 2 * NC; ++c) v[2 * K] += y[c] * y[c];
         }
     }
-    if (reduce_finish<2 * K + 1>(v, partial, counter, tmp)) {
-        for (int k = 0; k < 2 * K; ++k) out[k] = tmp[k];
-        if constexpr (WN) wnorm[0] = tmp[2 * K];
-    }
+    block_partial<2 * K + 1>(v, partial);
 }
-// w += s * sum_k c_k V_k  (c on device, 2K doubles); NORM: |w|^2 of the stored result ->
-// nrm_out[0] and, if flag_out != null, *flag_out = (|w|^2 < eta * ref[0]) (the Gram-Schmidt
-// reorthogonalisation test).  cond (nullable): skip the pass when *cond == 0.
+// w += s * sum_k c_k V_k  (c on device, 2K doubles); NORM: partials of |w|^2 of the stored result
+// (k_red_finish completes them and sets the Gram-Schmidt reorthogonalisation flag).  cond
+// (nullable): skip the pass when *cond == 0.
 template <int P, int K, bool NORM>
 __global__ __launch_bounds__(kRedBlock) void k_multiaxpy(long n, Ptrs<P> V, const double* __restrict__ cc, double s,
-                                                          typename Pk<P>::T* w, double* partial, unsigned* counter,
-                                                          double* nrm_out, const int* cond, const double* ref,
-                                                          double eta, int* flag_out) {
+                                                          typename Pk<P>::T* w, double* partial, const int* cond) {
     constexpr int NC = Pk<P>::NC;
     if (cond && *cond == 0) return;
     double cr[K], ci[K];
@@ -261,9 +253,38 @@ __global__ __launch_bounds__(kRedBlock) void k_multiaxpy(long n, Ptrs<P> V, cons
             for (int c = 0; c < 2 * NC; ++c) v[0] += y[c] * y[c];
         }
     }
-    if constexpr (NORM) {
-        if (reduce_finish<1>(v, partial, counter, nrm_out) && flag_out) *flag_out = nrm_out[0] < eta * ref[0] ? 1 : 0;
+    if constexpr (NORM) block_partial<1>(v, partial);
+}
+
+// Second stage of the BLAS reductions (one workgroup): total_r = sum over nblk partials of
+// partial[b * nr + r] in a fixed order; totals r < nout go to out[r], the rest to out2[r - nout].
+// flag (nullable): *flag = (total_0 < eta * ref[0]).  cond (nullable): skip when *cond == 0, as the
+// first stage did.
+constexpr int kMaxRedNR = 2 * kMaxK + 1;
+__global__ __launch_bounds__(kRedBlock) void k_red_finish(const double* __restrict__ partial, int nblk, int nr,
+                                                          double* out, int nout, double* out2, const int* cond,
+                                                          const double* ref, double eta, int* flag) {
+    if (cond && *cond == 0) return;
+    __shared__ double sm[kRedBlock / 64][kMaxRedNR];
+    __shared__ double tot[kMaxRedNR];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int r = 0; r < nr; ++r) {
+        double acc = 0;
+        for (int b = threadIdx.x; b < nblk; b += blockDim.x) acc += partial[(size_t)b * nr + r];
+        acc = wave_sum(acc);
+        if (lane == 0) sm[wv][r] = acc;
     }
+    __syncthreads();
+    if (threadIdx.x < nr) {
+        const int r = threadIdx.x;
+        double t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sm[w][r];
+        tot[r] = t;
+        if (r < nout) out[r] = t;
+        else if (out2) out2[r - nout] = t;
+    }
+    __syncthreads();
+    if (flag && threadIdx.x == 0) *flag = tot[0] < eta * ref[0] ? 1 : 0;
 }
 
 // out_i = sum_{j<X} in_j M[j*Y + i] (i < Y <= YM) for FP64 complex vectors of n elements, M in
@@ -420,19 +441,27 @@ int blas_conj(clg_context ctx, clg_field_s* dst, const clg_field_s* src) {
     return CLG_OK;
 }
 
+// second stage of a reduction whose first stage wrote nblk * nr partials to ctx->red_partial
+static int red_finish(clg_context ctx, unsigned nblk, int nr, double* out, int nout, double* out2, const int* cond,
+                      const double* ref, double eta, int* flag) {
+    k_red_finish<<<1, kRedBlock, 0, ctx->stream>>>(ctx->red_partial, (int)nblk, nr, out, nout, out2, cond, ref, eta,
+                                                   flag);
+    CLG_HIP(hipGetLastError());
+    return CLG_OK;
+}
+
 int blas_dot_dev(clg_context ctx, const clg_field_s* a, const clg_field_s* b, double* out) {
     CLG_TRY(check_pair(a, b));
     CLG_TRY(ensure_reduction_scratch(ctx, (size_t)kMaxRedGrid * (2 * kMaxK + 1)));
     long n;
     const int pk = pk_of(a, n);
     const unsigned g = red_grid(ctx, n);
-#define L(P)                                                                                                   \
-    k_dot<P><<<g, kRedBlock, 0, ctx->stream>>>(n, (const Pk<P>::T*)a->data, (const Pk<P>::T*)b->data, ctx->red_partial, \
-                                               ctx->red_counter, out);
+#define L(P) \
+    k_dot<P><<<g, kRedBlock, 0, ctx->stream>>>(n, (const Pk<P>::T*)a->data, (const Pk<P>::T*)b->data, ctx->red_partial);
     PK_DISPATCH(pk, L)
 #undef L
     CLG_HIP(hipGetLastError());
-    return CLG_OK;
+    return red_finish(ctx, g, 2, out, 2, nullptr, nullptr, nullptr, 0.0, nullptr);
 }
 
 int blas_norm2_dev(clg_context ctx, const clg_field_s* a, double* out) {
@@ -441,12 +470,11 @@ int blas_norm2_dev(clg_context ctx, const clg_field_s* a, double* out) {
     long n;
     const int pk = pk_of(a, n);
     const unsigned g = red_grid(ctx, n);
-#define L(P) \
-    k_norm2<P><<<g, kRedBlock, 0, ctx->stream>>>(n, (const Pk<P>::T*)a->data, ctx->red_partial, ctx->red_counter, out);
+#define L(P) k_norm2<P><<<g, kRedBlock, 0, ctx->stream>>>(n, (const Pk<P>::T*)a->data, ctx->red_partial);
     PK_DISPATCH(pk, L)
 #undef L
     CLG_HIP(hipGetLastError());
-    return CLG_OK;
+    return red_finish(ctx, g, 1, out, 1, nullptr, nullptr, nullptr, 0.0, nullptr);
 }
 
 #define SWITCH_K(K, CALL)                                         \
@@ -465,7 +493,6 @@ template <int P>
 static int multidot_p(clg_context ctx, clg_field_s* const* V, int nv, const clg_field_s* w, double* out, double* wnorm,
                       const int* cond, long n) {
     const unsigned g = red_grid(ctx, n);
-    double* tmp = ctx->red_result + kMultidotStage;  // staging of the 2K+1 sums
     for (int k0 = 0; k0 < nv; k0 += kMaxK) {
         const int kk = std::min(kMaxK, nv - k0);
         Ptrs<P> Pp;
@@ -473,16 +500,16 @@ static int multidot_p(clg_context ctx, clg_field_s* const* V, int nv, const clg_
         const typename Pk<P>::T* wp = (const typename Pk<P>::T*)w->data;
         double* wn = k0 == 0 ? wnorm : nullptr;
         double* o = out + 2 * k0;
-#define CALL(KK)                                                                                                   \
-    if (wn)                                                                                                        \
-        k_multidot<P, KK, true><<<g, kRedBlock, 0, ctx->stream>>>(n, Pp, wp, ctx->red_partial, ctx->red_counter, tmp, o, \
-                                                                  wn, cond);                                       \
-    else                                                                                                           \
-        k_multidot<P, KK, false><<<g, kRedBlock, 0, ctx->stream>>>(n, Pp, wp, ctx->red_partial, ctx->red_counter, tmp,  \
-                                                                   o, nullptr, cond);
+#define CALL(KK)                                                                                    \
+    if (wn)                                                                                         \
+        k_multidot<P, KK, true><<<g, kRedBlock, 0, ctx->stream>>>(n, Pp, wp, ctx->red_partial, cond); \
+    else                                                                                            \
+        k_multidot<P, KK, false><<<g, kRedBlock, 0, ctx->stream>>>(n, Pp, wp, ctx->red_partial, cond);
         SWITCH_K(kk, CALL)
 #undef CALL
         CLG_HIP(hipGetLastError());
+        // the first stage writes 2K + 1 partials per block (the last is |w|^2, kept only when asked)
+        CLG_TRY(red_finish(ctx, g, 2 * kk + 1, o, 2 * kk, wn, cond, nullptr, 0.0, nullptr));
     }
     return CLG_OK;
 }
@@ -513,16 +540,14 @@ static int multiaxpy_p(clg_context ctx, clg_field_s* w, clg_field_s* const* V, i
         const double* ck = c + 2 * k0;
         if (want) {
             const unsigned g = red_grid(ctx, n);
-#define CALL(KK)                                                                                                       \
-    k_multiaxpy<P, KK, true><<<g, kRedBlock, 0, ctx->stream>>>(n, Pp, ck, s, wp, ctx->red_partial, ctx->red_counter, nrm, \
-                                                               cond, ref, eta, flag);
+#define CALL(KK) k_multiaxpy<P, KK, true><<<g, kRedBlock, 0, ctx->stream>>>(n, Pp, ck, s, wp, ctx->red_partial, cond);
             SWITCH_K(kk, CALL)
 #undef CALL
+            CLG_HIP(hipGetLastError());
+            CLG_TRY(red_finish(ctx, g, 1, nrm, 1, nullptr, cond, ref, eta, flag));
         } else {
             const unsigned g = ew_grid(ctx, n);
-#define CALL(KK)                                                                                                 \
-    k_multiaxpy<P, KK, false><<<g, kRedBlock, 0, ctx->stream>>>(n, Pp, ck, s, wp, ctx->red_partial, ctx->red_counter, \
-                                                                nullptr, cond, nullptr, 0.0, nullptr);
+#define CALL(KK) k_multiaxpy<P, KK, false><<<g, kRedBlock, 0, ctx->stream>>>(n, Pp, ck, s, wp, nullptr, cond);
             SWITCH_K(kk, CALL)
 #undef CALL
         }

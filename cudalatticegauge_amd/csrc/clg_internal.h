@@ -644,6 +644,30 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Block-level stage of a two-kernel grid reduction: sums NR per-lane FP64 values over the
+// workgroup (blockDim.x <= kRedBlock, a multiple of 64) and writes them to
+// partial[blockIdx.x * NR + r].  No fence or counter: the finishing kernel launched after it on the
+// same stream sees every partial.  (A device-scope release fence per workgroup costs a
+// cross-XCD L2 write-back on gfx950: tens of microseconds over a 1024-block grid.)
+template <int NR>
+__device__ void block_partial(double (&v)[NR], double* partial) {
+    __shared__ double sm[kRedBlock / 64][NR];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) v[r] = wave_sum(v[r]);
+    if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) sm[wv][r] = v[r];
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < NR; r += blockDim.x) {
+        double s = 0;
+        for (int w = 0; w < nw; ++w) s += sm[w][r];
+        partial[(size_t)blockIdx.x * NR + r] = s;
+    }
+}
+
 // Reduce NR per-lane FP64 values over the grid (blockDim.x <= kRedBlock, a multiple of 64);
 // the last workgroup to finish writes out[0..NR), resets the counter and returns true on its
 // thread 0.  partial holds gridDim.x * NR doubles.
