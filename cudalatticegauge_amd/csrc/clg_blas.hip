@@ -268,11 +268,22 @@ __global__ __launch_bounds__(kRedBlock) void k_red_finish(const double* __restri
     __shared__ double sm[kRedBlock / 64][kMaxRedNR];
     __shared__ double tot[kMaxRedNR];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int r = 0; r < nr; ++r) {
-        double acc = 0;
-        for (int b = threadIdx.x; b < nblk; b += blockDim.x) acc += partial[(size_t)b * nr + r];
-        acc = wave_sum(acc);
-        if (lane == 0) sm[wv][r] = acc;
+    // block-major sweep: every load of a thread is independent, so they all stay in flight
+    double acc[kMaxRedNR];
+#pragma unroll
+    for (int r = 0; r < kMaxRedNR; ++r) acc[r] = 0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+        const double* pb = partial + (size_t)b * nr;
+#pragma unroll
+        for (int r = 0; r < kMaxRedNR; ++r)
+            if (r < nr) acc[r] += pb[r];
+    }
+#pragma unroll
+    for (int r = 0; r < kMaxRedNR; ++r) {
+        if (r < nr) {
+            const double a = wave_sum(acc[r]);
+            if (lane == 0) sm[wv][r] = a;
+        }
     }
     __syncthreads();
     if (threadIdx.x < nr) {
@@ -322,6 +333,162 @@ __global__ __launch_bounds__(256) void k_vmm(VmmArgs a, const double* __restrict
 #pragma unroll
         for (int i = 0; i < YM; ++i)
             if (i < a.Y) a.out[i][e] = acc[i];
+    }
+}
+
+// ---- fused Gram-Schmidt step of the Arnoldi process (see blas_gs_step)
+// w' = w - sum_k h_k V_k (h = the first-pass projections, on device) is stored, and the same pass
+// yields the partials of V_k^H w' (2K values) and |w'|^2 (the last one).
+template <int P, int K>
+__global__ __launch_bounds__(kRedBlock) void k_gs_project(long n, Ptrs<P> V, const double* __restrict__ h,
+                                                           typename Pk<P>::T* w, double* partial) {
+    constexpr int NC = Pk<P>::NC;
+    double hr[K], hi[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) { hr[k] = h[2 * k]; hi[k] = h[2 * k + 1]; }
+    double v[2 * K + 1];
+#pragma unroll
+    for (int k = 0; k < 2 * K + 1; ++k) v[k] = 0;
+    GRID_STRIDE(i, n) {
+        typename Pk<P>::T raw[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) raw[k] = V.p[k][i];
+        double y[2 * NC];
+        Pk<P>::ld(w, i, y);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double x[2 * NC];
+            Pk<P>::cvt(raw[k], x);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                y[2 * c] -= hr[k] * x[2 * c] - hi[k] * x[2 * c + 1];
+                y[2 * c + 1] -= hr[k] * x[2 * c + 1] + hi[k] * x[2 * c];
+            }
+        }
+        Pk<P>::st(w, i, y);  // y now holds the stored (rounded) values
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double x[2 * NC];
+            Pk<P>::cvt(raw[k], x);
+            double re = 0.0, im = 0.0;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                re += x[2 * c] * y[2 * c] + x[2 * c + 1] * y[2 * c + 1];
+                im += x[2 * c] * y[2 * c + 1] - x[2 * c + 1] * y[2 * c];
+            }
+            v[2 * k] += re;
+            v[2 * k + 1] += im;
+        }
+#pragma unroll
+        for (int c = 0; c < 2 * NC; ++c) v[2 * K] += y[c] * y[c];
+    }
+    block_partial<2 * K + 1>(v, partial);
+}
+
+// Completes k_gs_project (one workgroup) and updates Hessenberg column j (hcol, interleaved complex,
+// rows 0..j+1) with the Daniel-Gragg-Kaufman-Stewart test: the second pass counts only when the
+// first cancelled more than half of |w|^2 (|w'|^2 < |w|^2 / 2).  Then
+//   H(i, j) = h_i + flag c2_i (shifted: delta_ij - that), |w''|^2 = |w'|^2 - flag |c2|^2
+// (V orthonormal), H(j+1, j) = +-|w''|.  sc: [0] |w|^2 (in), [1] |w''|^2, [2] 1/|w''|, [3] flag,
+// [4..4+2K) c2.
+template <int K>
+__global__ __launch_bounds__(kRedBlock) void k_gs_finish(const double* __restrict__ partial, int nblk, double* hcol,
+                                                          int j, int shifted, double* sc) {
+    constexpr int NR = 2 * K + 1;
+    __shared__ double sm[kRedBlock / 64][NR];
+    __shared__ double tot[NR];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = 0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[r] += partial[(size_t)b * NR + r];
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = wave_sum(acc[r]);
+    if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) sm[wv][r] = acc[r];
+    }
+    __syncthreads();
+    if (threadIdx.x < NR) {
+        double t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sm[w][threadIdx.x];
+        tot[threadIdx.x] = t;
+    }
+    __syncthreads();
+    const double w2 = tot[2 * K];
+    const bool flag = w2 < 0.5 * sc[0];
+    const int i = threadIdx.x;
+    if (i < K) {
+        double re = hcol[2 * i], im = hcol[2 * i + 1];
+        if (flag) {
+            re += tot[2 * i];
+            im += tot[2 * i + 1];
+        }
+        if (shifted) {
+            re = (i == j ? 1.0 : 0.0) - re;
+            im = -im;
+        }
+        hcol[2 * i] = re;
+        hcol[2 * i + 1] = im;
+        sc[4 + 2 * i] = tot[2 * i];
+        sc[4 + 2 * i + 1] = tot[2 * i + 1];
+    }
+    if (i == 0) {
+        double c2 = 0;
+        for (int k = 0; k < 2 * K; ++k) c2 += tot[k] * tot[k];
+        double f2 = flag ? w2 - c2 : w2;
+        if (!(f2 > 1e-300)) f2 = 1e-300;
+        const double f = sqrt(f2);
+        hcol[2 * (j + 1)] = shifted ? -f : f;
+        hcol[2 * (j + 1) + 1] = 0.0;
+        sc[1] = f2;
+        sc[2] = 1.0 / f;
+        sc[3] = flag ? 1.0 : 0.0;
+    }
+}
+
+// vnext = (w' - flag sum_k c2_k V_k) / |w''| with flag, c2 and 1/|w''| from k_gs_finish
+template <int P, int K>
+__global__ __launch_bounds__(kRedBlock) void k_gs_finalize(long n, Ptrs<P> V, const typename Pk<P>::T* __restrict__ w,
+                                                            typename Pk<P>::T* vnext, const double* __restrict__ sc) {
+    constexpr int NC = Pk<P>::NC;
+    const bool flag = sc[3] != 0.0;
+    const double inv = sc[2];
+    if (flag) {
+        double cr[K], ci[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) { cr[k] = sc[4 + 2 * k]; ci[k] = sc[4 + 2 * k + 1]; }
+        GRID_STRIDE(i, n) {
+            typename Pk<P>::T raw[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) raw[k] = V.p[k][i];
+            double y[2 * NC];
+            Pk<P>::ld(w, i, y);
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                double x[2 * NC];
+                Pk<P>::cvt(raw[k], x);
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    y[2 * c] -= cr[k] * x[2 * c] - ci[k] * x[2 * c + 1];
+                    y[2 * c + 1] -= cr[k] * x[2 * c + 1] + ci[k] * x[2 * c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 2 * NC; ++c) y[c] *= inv;
+            Pk<P>::st(vnext, i, y);
+        }
+    } else {
+        GRID_STRIDE(i, n) {
+            double y[2 * NC];
+            Pk<P>::ld(w, i, y);
+#pragma unroll
+            for (int c = 0; c < 2 * NC; ++c) y[c] *= inv;
+            Pk<P>::st(vnext, i, y);
+        }
     }
 }
 
@@ -593,6 +760,59 @@ int blas_vmm_dev(clg_context ctx, clg_field_s* const* in, int nin, clg_field_s* 
     else k_vmm<kMaxVmmOut><<<grid, 256, 0, ctx->stream>>>(a, M);
     CLG_HIP(hipGetLastError());
     return CLG_OK;
+}
+
+template <int P>
+static int gs_step_p(clg_context ctx, clg_field_s* const* V, int nv, clg_field_s* w, double* hcol, int j, bool shifted,
+                     clg_field_s* vnext, double* sc, long n) {
+    const unsigned g = red_grid(ctx, n);
+    Ptrs<P> Pp;
+    for (int k = 0; k < nv; ++k) Pp.p[k] = (const typename Pk<P>::T*)V[k]->data;
+    const typename Pk<P>::T* wc = (const typename Pk<P>::T*)w->data;
+    typename Pk<P>::T* wp = (typename Pk<P>::T*)w->data;
+    // (1) h = V^H w into the column, |w|^2 into sc[0]
+#define CALL(KK) k_multidot<P, KK, true><<<g, kRedBlock, 0, ctx->stream>>>(n, Pp, wc, ctx->red_partial, nullptr);
+    SWITCH_K(nv, CALL)
+#undef CALL
+    CLG_HIP(hipGetLastError());
+    CLG_TRY(red_finish(ctx, g, 2 * nv + 1, hcol, 2 * nv, sc, nullptr, nullptr, 0.0, nullptr));
+    // (2) w' = w - V h with V^H w' and |w'|^2, then the column
+#define CALL(KK)                                                                                     \
+    k_gs_project<P, KK><<<g, kRedBlock, 0, ctx->stream>>>(n, Pp, hcol, wp, ctx->red_partial);       \
+    k_gs_finish<KK><<<1, kRedBlock, 0, ctx->stream>>>(ctx->red_partial, (int)g, hcol, j, shifted ? 1 : 0, sc);
+    SWITCH_K(nv, CALL)
+#undef CALL
+    CLG_HIP(hipGetLastError());
+    // (3) the next basis vector
+    if (vnext) {
+        const unsigned ge = ew_grid(ctx, n);
+        typename Pk<P>::T* vp = (typename Pk<P>::T*)vnext->data;
+#define CALL(KK) k_gs_finalize<P, KK><<<ge, kRedBlock, 0, ctx->stream>>>(n, Pp, wc, vp, sc);
+        SWITCH_K(nv, CALL)
+#undef CALL
+        CLG_HIP(hipGetLastError());
+    }
+    return CLG_OK;
+}
+
+// One Arnoldi orthogonalisation: w against the orthonormal V_0..V_{nv-1} (nv <= 16) by classical
+// Gram-Schmidt with the DGKS-conditioned second pass, in three streaming passes (projections;
+// subtraction fused with the second-pass projections; normalisation, which applies the second pass
+// when it is needed) plus two one-workgroup finishes.  hcol: column j of the Hessenberg matrix
+// (device, interleaved complex, rows 0..j+1), written with h + [flag] c2 (shifted: delta_ij minus
+// that) and H(j+1, j) = +-|w''|.  vnext (nullable) receives w'' / |w''|; w keeps w'.  sc: device
+// scratch of 4 + 32 doubles.
+int blas_gs_step(clg_context ctx, clg_field_s* const* V, int nv, clg_field_s* w, double* hcol, int j, bool shifted,
+                 clg_field_s* vnext, double* sc) {
+    CLG_CHECK_ARG(nv >= 1 && nv <= kMaxK, "fused Gram-Schmidt takes 1..%d vectors", kMaxK);
+    for (int k = 0; k < nv; ++k) CLG_TRY(check_pair(V[k], w));
+    if (vnext) CLG_TRY(check_pair(vnext, w));
+    CLG_TRY(ensure_reduction_scratch(ctx, (size_t)kMaxRedGrid * (2 * kMaxK + 1)));
+    long n;
+    const int pk = pk_of(w, n);
+    if (pk == PK_D) return gs_step_p<PK_D>(ctx, V, nv, w, hcol, j, shifted, vnext, sc, n);
+    if (pk == PK_F4) return gs_step_p<PK_F4>(ctx, V, nv, w, hcol, j, shifted, vnext, sc, n);
+    return gs_step_p<PK_F2>(ctx, V, nv, w, hcol, j, shifted, vnext, sc, n);
 }
 
 // copies `count` doubles from device to the context's pinned buffer and waits

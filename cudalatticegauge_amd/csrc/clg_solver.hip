@@ -31,6 +31,7 @@ int blas_norm2_dev(clg_context, const clg_field_s*, double*);
 int blas_multidot_dev(clg_context, clg_field_s* const*, int, const clg_field_s*, double*, double*, const int*);
 int blas_multiaxpy_dev(clg_context, clg_field_s*, clg_field_s* const*, int, const double*, double, double*, const int*,
                        const double*, double, int*);
+int blas_gs_step(clg_context, clg_field_s* const*, int, clg_field_s*, double*, int, bool, clg_field_s*, double*);
 int fetch_doubles(clg_context, const double*, int, double*);
 int workspace(clg_context, size_t, double**);
 int blas_conj(clg_context, clg_field_s*, const clg_field_s*);
@@ -230,9 +231,9 @@ int gmres(clg_context ctx, LinOp& A, clg_field_s* x, const clg_field_s* b, int m
     const int ld = m + 1;
     // device workspace: H, c2, g, y, scalars
     const size_t offH = 0, offC = offH + 2 * (size_t)ld * m, offG = offC + 2 * (size_t)m,
-                 offY = offG + 2 * (size_t)(m + 1), offS = offY + 2 * (size_t)m;
+                 offY = offG + 2 * (size_t)(m + 1), offS = offY + 2 * (size_t)m, offGS = offS + 16;
     double* wsp = nullptr;
-    CLG_TRY(workspace(ctx, offS + 16, &wsp));
+    CLG_TRY(workspace(ctx, offGS + 40, &wsp));
     double* H = wsp + offH;
     double* c2 = wsp + offC;
     double* g = wsp + offG;
@@ -265,6 +266,11 @@ int gmres(clg_context ctx, LinOp& A, clg_field_s* x, const clg_field_s* b, int m
             double* Hj = H + 2 * (size_t)j * ld;
             // classical Gram-Schmidt, repeated only when the first pass cancelled more than half
             // of |w|^2 (|w'|^2 < |w|^2 / 2, the Daniel-Gragg-Kaufman-Stewart test)
+            if (m <= 16) {  // fused form: three passes, V[j+1] written normalised
+                CLG_TRY(blas_gs_step(ctx, V.data(), j + 1, w, Hj, j, shifted, j < m - 1 ? V[j + 1] : nullptr,
+                                     wsp + offGS));
+                continue;
+            }
             CLG_TRY(blas_multidot_dev(ctx, V.data(), j + 1, w, Hj, sc + 4, nullptr));
             CLG_TRY(blas_multiaxpy_dev(ctx, w, V.data(), j + 1, Hj, -1.0, sc + 1, nullptr, sc + 4, 0.5, reorth));
             CLG_TRY(blas_multidot_dev(ctx, V.data(), j + 1, w, c2, nullptr, reorth));
