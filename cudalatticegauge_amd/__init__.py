@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(_HERE, "libclg_hip.so")
 # enums mirrored from include/clg_hip.h
 WILSON, KS, GAUGE, GAUGE12 = 0, 1, 2, 3
 F64, F32, F16 = 0, 1, 2
+INNER_F16_VECTORS = 3  # inner_prec: FP16 storage of the inner Krylov vectors and links
 FULL, EVEN, ODD = 0, 1, 2
 EFO_F_D, EFO_F_Ddagger, EFO_F_DD, EFO_F_DDdagger = 0, 1, 2, 3
 EFO_F_InverseD, EFO_F_InverseDdagger, EFO_F_InverseDD, EFO_F_InverseDDdagger = 4, 5, 6, 7

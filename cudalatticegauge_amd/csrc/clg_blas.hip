@@ -257,36 +257,34 @@ __global__ __launch_bounds__(kRedBlock) void k_multiaxpy(long n, Ptrs<P> V, cons
 }
 
 // Second stage of the BLAS reductions (one workgroup): total_r = sum over nblk partials of
-// partial[b * nr + r] in a fixed order; totals r < nout go to out[r], the rest to out2[r - nout].
-// flag (nullable): *flag = (total_0 < eta * ref[0]).  cond (nullable): skip when *cond == 0, as the
-// first stage did.
+// partial[b * NR + r] in a fixed order; totals r < nout go to out[r], the one after them to
+// out2[0] (when given).  flag (nullable): *flag = (total_0 < eta * ref[0]).  cond (nullable): skip
+// when *cond == 0, as the first stage did.  Every load of a thread is independent (block-major
+// sweep with NR accumulators), so they all stay in flight.
 constexpr int kMaxRedNR = 2 * kMaxK + 1;
-__global__ __launch_bounds__(kRedBlock) void k_red_finish(const double* __restrict__ partial, int nblk, int nr,
-                                                          double* out, int nout, double* out2, const int* cond,
-                                                          const double* ref, double eta, int* flag) {
+template <int NR>
+__global__ __launch_bounds__(kRedBlock) void k_red_finish(const double* __restrict__ partial, int nblk, double* out,
+                                                          int nout, double* out2, const int* cond, const double* ref,
+                                                          double eta, int* flag) {
     if (cond && *cond == 0) return;
-    __shared__ double sm[kRedBlock / 64][kMaxRedNR];
-    __shared__ double tot[kMaxRedNR];
+    __shared__ double sm[kRedBlock / 64][NR];
+    __shared__ double tot[NR];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    // block-major sweep: every load of a thread is independent, so they all stay in flight
-    double acc[kMaxRedNR];
+    double acc[NR];
 #pragma unroll
-    for (int r = 0; r < kMaxRedNR; ++r) acc[r] = 0;
+    for (int r = 0; r < NR; ++r) acc[r] = 0;
     for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
-        const double* pb = partial + (size_t)b * nr;
 #pragma unroll
-        for (int r = 0; r < kMaxRedNR; ++r)
-            if (r < nr) acc[r] += pb[r];
+        for (int r = 0; r < NR; ++r) acc[r] += partial[(size_t)b * NR + r];
     }
 #pragma unroll
-    for (int r = 0; r < kMaxRedNR; ++r) {
-        if (r < nr) {
-            const double a = wave_sum(acc[r]);
-            if (lane == 0) sm[wv][r] = a;
-        }
+    for (int r = 0; r < NR; ++r) acc[r] = wave_sum(acc[r]);
+    if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) sm[wv][r] = acc[r];
     }
     __syncthreads();
-    if (threadIdx.x < nr) {
+    if (threadIdx.x < NR) {
         const int r = threadIdx.x;
         double t = 0;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sm[w][r];
@@ -294,8 +292,10 @@ __global__ __launch_bounds__(kRedBlock) void k_red_finish(const double* __restri
         if (r < nout) out[r] = t;
         else if (out2) out2[r - nout] = t;
     }
-    __syncthreads();
-    if (flag && threadIdx.x == 0) *flag = tot[0] < eta * ref[0] ? 1 : 0;
+    if (flag) {
+        __syncthreads();
+        if (threadIdx.x == 0) *flag = tot[0] < eta * ref[0] ? 1 : 0;
+    }
 }
 
 // out_i = sum_{j<X} in_j M[j*Y + i] (i < Y <= YM) for FP64 complex vectors of n elements, M in
@@ -507,6 +507,13 @@ unsigned ew_grid(clg_context ctx, long n) {
 
 namespace clg {
 
+// FP16-storage Wilson fields (clg_blas16.hip)
+int h16_axpby(clg_context, clg_field_s*, double, double, const clg_field_s*, double, double);
+int h16_norm2_dev(clg_context, const clg_field_s*, double*);
+int h16_scale_inv_sqrt(clg_context, clg_field_s*, const clg_field_s*, const double*);
+int h16_multiaxpy(clg_context, clg_field_s*, clg_field_s* const*, int, const double*, double);
+int h16_gs_step(clg_context, clg_field_s* const*, int, clg_field_s*, double*, int, bool, clg_field_s*, double*);
+
 int ensure_reduction_scratch(clg_context ctx, size_t doubles) {
     if (ctx->red_partial_cap >= doubles) return CLG_OK;
     if (ctx->red_partial) {
@@ -556,6 +563,7 @@ static int check_pair(const clg_field_s* a, const clg_field_s* b) {
     }
 
 int blas_axpby(clg_context ctx, clg_field_s* y, double ar, double ai, const clg_field_s* x, double br, double bi) {
+    if (y && y->prec == CLG_F16) return h16_axpby(ctx, y, ar, ai, x, br, bi);
     CLG_TRY(check_pair(y, x));
     long n;
     const int pk = pk_of(y, n);
@@ -585,6 +593,7 @@ int blas_axpy_dev(clg_context ctx, clg_field_s* y, const double* c, double s, co
 }
 
 int blas_scale_inv_sqrt(clg_context ctx, clg_field_s* dst, const clg_field_s* src, const double* nrm2) {
+    if (dst && dst->prec == CLG_F16) return h16_scale_inv_sqrt(ctx, dst, src, nrm2);
     CLG_TRY(check_pair(dst, src));
     long n;
     const int pk = pk_of(dst, n);
@@ -611,8 +620,17 @@ int blas_conj(clg_context ctx, clg_field_s* dst, const clg_field_s* src) {
 // second stage of a reduction whose first stage wrote nblk * nr partials to ctx->red_partial
 static int red_finish(clg_context ctx, unsigned nblk, int nr, double* out, int nout, double* out2, const int* cond,
                       const double* ref, double eta, int* flag) {
-    k_red_finish<<<1, kRedBlock, 0, ctx->stream>>>(ctx->red_partial, (int)nblk, nr, out, nout, out2, cond, ref, eta,
-                                                   flag);
+#define FIN(N)                                                                                                  \
+    case N:                                                                                                     \
+        k_red_finish<N><<<1, kRedBlock, 0, ctx->stream>>>(ctx->red_partial, (int)nblk, out, nout, out2, cond, ref, \
+                                                          eta, flag);                                            \
+        break;
+    switch (nr) {
+        FIN(1) FIN(2) FIN(3) FIN(5) FIN(7) FIN(9) FIN(11) FIN(13) FIN(15) FIN(17) FIN(19) FIN(21) FIN(23) FIN(25)
+        FIN(27) FIN(29) FIN(31) FIN(33)
+        default: CLG_CHECK_ARG(false, "no reduction finish for %d values", nr);
+    }
+#undef FIN
     CLG_HIP(hipGetLastError());
     return CLG_OK;
 }
@@ -632,6 +650,7 @@ int blas_dot_dev(clg_context ctx, const clg_field_s* a, const clg_field_s* b, do
 }
 
 int blas_norm2_dev(clg_context ctx, const clg_field_s* a, double* out) {
+    if (a && a->prec == CLG_F16) return h16_norm2_dev(ctx, a, out);
     CLG_TRY(check_flat(a));
     CLG_TRY(ensure_reduction_scratch(ctx, (size_t)kMaxRedGrid * (2 * kMaxK + 1)));
     long n;
@@ -728,6 +747,10 @@ static int multiaxpy_p(clg_context ctx, clg_field_s* w, clg_field_s* const* V, i
 int blas_multiaxpy_dev(clg_context ctx, clg_field_s* w, clg_field_s* const* V, int nv, const double* c, double s,
                        double* nrm, const int* cond, const double* ref, double eta, int* flag) {
     CLG_CHECK_ARG(nv >= 1, "need at least one vector");
+    if (w && w->prec == CLG_F16) {
+        CLG_CHECK_ARG(!nrm && !cond, "FP16 multi-axpy has no norm or condition");
+        return h16_multiaxpy(ctx, w, V, nv, c, s);
+    }
     for (int k = 0; k < nv; ++k) CLG_TRY(check_pair(V[k], w));
     CLG_TRY(ensure_reduction_scratch(ctx, (size_t)kMaxRedGrid * (2 * kMaxK + 1)));
     long n;
@@ -795,6 +818,19 @@ static int gs_step_p(clg_context ctx, clg_field_s* const* V, int nv, clg_field_s
     return CLG_OK;
 }
 
+// the reduction finish and the Gram-Schmidt finish for kernels of other files
+int red_finish_partials(clg_context ctx, unsigned nblk, int nr, double* out, int nout, double* out2, const int* cond,
+                        const double* ref, double eta, int* flag) {
+    return red_finish(ctx, nblk, nr, out, nout, out2, cond, ref, eta, flag);
+}
+int gs_finish_launch(clg_context ctx, unsigned nblk, int nv, double* hcol, int j, bool shifted, double* sc) {
+#define CALL(KK) k_gs_finish<KK><<<1, kRedBlock, 0, ctx->stream>>>(ctx->red_partial, (int)nblk, hcol, j, shifted ? 1 : 0, sc);
+    SWITCH_K(nv, CALL)
+#undef CALL
+    CLG_HIP(hipGetLastError());
+    return CLG_OK;
+}
+
 // One Arnoldi orthogonalisation: w against the orthonormal V_0..V_{nv-1} (nv <= 16) by classical
 // Gram-Schmidt with the DGKS-conditioned second pass, in three streaming passes (projections;
 // subtraction fused with the second-pass projections; normalisation, which applies the second pass
@@ -805,6 +841,7 @@ static int gs_step_p(clg_context ctx, clg_field_s* const* V, int nv, clg_field_s
 int blas_gs_step(clg_context ctx, clg_field_s* const* V, int nv, clg_field_s* w, double* hcol, int j, bool shifted,
                  clg_field_s* vnext, double* sc) {
     CLG_CHECK_ARG(nv >= 1 && nv <= kMaxK, "fused Gram-Schmidt takes 1..%d vectors", kMaxK);
+    if (w && w->prec == CLG_F16) return h16_gs_step(ctx, V, nv, w, hcol, j, shifted, vnext, sc);
     for (int k = 0; k < nv; ++k) CLG_TRY(check_pair(V[k], w));
     if (vnext) CLG_TRY(check_pair(vnext, w));
     CLG_TRY(ensure_reduction_scratch(ctx, (size_t)kMaxRedGrid * (2 * kMaxK + 1)));

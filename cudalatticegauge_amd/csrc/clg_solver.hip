@@ -380,7 +380,10 @@ int validate_params(const clg_solver_params* p) {
     CLG_CHECK_ARG(p->max_dim >= 1 && p->max_dim <= 64, "MaxDim must be in [1,64]");
     CLG_CHECK_ARG(p->restart >= 1, "Restart must be >= 1");
     CLG_CHECK_ARG(p->accuracy > 0, "Accuracy must be > 0");
-    CLG_CHECK_ARG(p->inner_prec >= CLG_F64 && p->inner_prec <= CLG_F16, "bad inner precision");
+    CLG_CHECK_ARG(p->inner_prec >= CLG_F64 && p->inner_prec <= CLG_INNER_F16_VECTORS, "bad inner precision");
+    if (p->inner_prec == CLG_INNER_F16_VECTORS)
+        CLG_CHECK_ARG(p->kind == CLG_WILSON && p->solver == CLG_SOLVER_GMRES && p->max_dim <= 16 && p->even_odd != 2,
+                      "FP16-vector inner solves take Wilson GMRES with MaxDim <= 16 (no normal equations)");
     return CLG_OK;
 }
 
@@ -510,12 +513,14 @@ struct Solve1 {
     // FP64 defect correction around FP32 / FP16-storage inner GMRES
     int run_mixed(LinOp& A, clg_field_s* x, const clg_field_s* b, bool x_zero, double bn, bool& conv) {
         const int lp = p->inner_prec;
-        // inner gauge: FP32 (for FP32) or FP16 12-real (for FP16 storage)
+        // inner gauge: FP32 (for FP32) or FP16 12-real (FP16 links, FP16 links and vectors)
         clg_field_s* g_lo = nullptr;
-        const int gk = lp == CLG_F16 ? CLG_GAUGE12 : gauge->kind;
-        CLG_TRY(scratch_field(ctx, gauge, gk, lp, CLG_FULL, 0, &g_lo));
+        const int gprec = lp == CLG_F32 ? CLG_F32 : CLG_F16;
+        const int gk = gprec == CLG_F16 ? CLG_GAUGE12 : gauge->kind;
+        CLG_TRY(scratch_field(ctx, gauge, gk, gprec, CLG_FULL, 0, &g_lo));
         CLG_TRY(gauge_convert(ctx, g_lo, gauge));
-        const int vprec = CLG_F32;  // Krylov vectors of the inner solve
+        // Krylov vectors of the inner solve
+        const int vprec = lp == CLG_INNER_F16_VECTORS ? CLG_F16 : CLG_F32;
         clg_field_s *r, *r_lo, *e_lo, *e_hi;
         CLG_TRY(scratch_field(ctx, b, b->kind, CLG_F64, b->subset, 33, &r));
         CLG_TRY(scratch_field(ctx, b, b->kind, vprec, b->subset, 34, &r_lo));
@@ -524,7 +529,6 @@ struct Solve1 {
         LinOp Alo = A;
         Alo.gauge = g_lo;
         Alo.applies = 0;
-        if (lp == CLG_F16) Alo.gauge = g_lo;
         double* d = ctx->red_result + 16;
         conv = false;
         if (x_zero) CLG_TRY(blas_axpby(ctx, x, 0.0, 0.0, b, 0.0, 0.0));

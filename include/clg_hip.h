@@ -46,6 +46,10 @@ typedef enum {
 } clg_kind;
 
 typedef enum { CLG_F64 = 0, CLG_F32 = 1, CLG_F16 = 2 } clg_prec;
+/* clg_solver_params.inner_prec takes a clg_prec (CLG_F16 there: FP32 vectors with FP16 12-real
+ * links) or this value: FP16 storage of the inner Krylov vectors as well as the links, FP32
+ * arithmetic (Wilson GMRES, MaxDim <= 16) */
+typedef enum { CLG_INNER_F16_VECTORS = 3 } clg_inner_prec;
 typedef enum { CLG_FULL = 0, CLG_EVEN = 1, CLG_ODD = 2 } clg_subset;
 
 /* EFieldOperator (CField.h:37-56), same numbering */
@@ -177,7 +181,7 @@ typedef struct {
     int absolute_accuracy; /* YAML AbsoluteAccuracy */
     int stop_rule;         /* clg_stop_rule */
     int even_odd;          /* 1: solve through the even-site Schur complement (Wilson) */
-    int inner_prec;        /* CLG_F64: plain solve; CLG_F32 / CLG_F16: defect correction */
+    int inner_prec;        /* CLG_F64: plain solve; CLG_F32 / CLG_F16 / CLG_INNER_F16_VECTORS: defect correction */
     double inner_accuracy; /* relative residual target of each inner solve (e.g. 1e-5) */
     int max_outer;         /* defect-correction outer iterations (mixed precision) */
     int kind;              /* CLG_WILSON or CLG_KS */

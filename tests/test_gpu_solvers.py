@@ -60,7 +60,7 @@ def test_gmres12_f64_c0(ctx, c0, eo):
     assert relerr(x, x_ref) < 1e-8   # |D^-1| <~ 1/(1-8 kappa) ~ 10 on this field
 
 
-@pytest.mark.parametrize("inner", [clg.F32, clg.F16])
+@pytest.mark.parametrize("inner", [clg.F32, clg.F16, clg.INNER_F16_VECTORS])
 def test_mixed_precision_defect_correction(ctx, c0, inner):
     U, b, x_ref = c0
     g, fb, fx = fields(ctx, U, b)
@@ -70,6 +70,35 @@ def test_mixed_precision_defect_correction(ctx, c0, inner):
     print(f"inner={inner} outer={rep.outer} cycles={rep.cycles} res={rep.true_residual:.2e}")
     assert st == clg.OK and rep.true_residual < 1e-10
     assert relerr(fx.download(), x_ref) < 1e-8
+
+
+@pytest.mark.parametrize("eo,op", [(0, clg.EFO_F_InverseD), (1, clg.EFO_F_InverseDdagger), (0, clg.EFO_F_InverseDdagger)])
+def test_fp16_vector_inner_solve_variants(ctx, c0, eo, op):
+    # FP16-storage inner Krylov vectors (full lattice and Schur, D and D^dagger): the FP64 outer
+    # loop must still reach |b - A x| / |b| < 1e-10, checked with the GPU FP64 operator
+    U, b, x_ref = c0
+    g, fb, fx = fields(ctx, U, b)
+    p = clg.solver_params(max_dim=12, restart=50, accuracy=1e-10, stop_rule=clg.STOP_TRUE_RESIDUAL, even_odd=eo,
+                          inner_prec=clg.INNER_F16_VECTORS, inner_accuracy=1e-2, max_outer=40)
+    st, rep = clg.solve(ctx, p, fx, fb, g, KAPPA, op)
+    print(f"eo={eo} op={op} outer={rep.outer} cycles={rep.cycles} res={rep.true_residual:.2e}")
+    assert st == clg.OK and rep.true_residual < 1e-10
+    ax = clg.Field(ctx, clg.WILSON, clg.F64, DIMS0, bc=orc.BC_FERMION)
+    clg.wilson_apply(ctx, ax, fx, g, KAPPA, clg.EFO_F_D if op == clg.EFO_F_InverseD else clg.EFO_F_Ddagger)
+    assert np.linalg.norm(ax.download() - b) / np.linalg.norm(b) < 1e-10
+    if op == clg.EFO_F_InverseD:
+        assert relerr(fx.download(), x_ref) < 1e-8
+
+
+def test_fp16_vector_inner_rejects_unsupported(ctx, c0):
+    U, b, _ = c0
+    g, fb, fx = fields(ctx, U, b)
+    for kw in (dict(max_dim=20), dict(even_odd=2), dict(solver=clg.SOLVER_BICGSTAB)):
+        args = dict(max_dim=12, restart=50, accuracy=1e-10, stop_rule=clg.STOP_TRUE_RESIDUAL, even_odd=1,
+                    inner_prec=clg.INNER_F16_VECTORS, inner_accuracy=1e-2)
+        args.update(kw)
+        with pytest.raises(clg.CLGError):
+            clg.solve(ctx, clg.solver_params(**args), fx, fb, g, KAPPA, clg.EFO_F_InverseD)
 
 
 def test_reference_stop_rule_cycles_match_oracle(ctx, c0):
