@@ -133,7 +133,8 @@ def main():
         x = clg.Field(ctx, clg.WILSON, clg.F64, DIMS, bc=(1, 1, 1, -1))
         results = {}
         for name, inner, inner_acc in (("fp64", clg.F64, 0.0), ("mixed_fp32", clg.F32, 1e-4),
-                                       ("mixed_fp16link", clg.F16, 1e-3)):
+                                       ("mixed_fp16link", clg.F16, 1e-3),
+                                       ("mixed_fp16", clg.INNER_F16_VECTORS, 1e-2)):
             p = clg.solver_params(max_dim=12, restart=500, accuracy=1e-10, stop_rule=clg.STOP_TRUE_RESIDUAL,
                                   even_odd=1, inner_prec=inner, inner_accuracy=inner_acc, max_outer=60)
             clg.solve(ctx, p, x, src64, gauge64, KAPPA, clg.EFO_F_InverseD)  # warm (tuning, scratch)
@@ -153,6 +154,7 @@ def main():
         extra["solve_speedup_mixed_fp32_vs_fp64"] = round(results["fp64"].seconds / results["mixed_fp32"].seconds, 3)
         extra["solve_speedup_mixed_fp16link_vs_fp64"] = round(
             results["fp64"].seconds / results["mixed_fp16link"].seconds, 3)
+        extra["solve_speedup_mixed_fp16_vs_fp64"] = round(results["fp64"].seconds / results["mixed_fp16"].seconds, 3)
 
         del x
         # harder system: 32^3x32, kappa 0.137, mixed precision against the FP64 solve
@@ -267,8 +269,8 @@ def rhmc_trajectory(ctx, clg, np):
 
 def mixed_solves_32x32x32x32(ctx, clg, np):
     """EO-GMRES(12) to |r|/|b| < 1e-10 at 32^3x32, kappa 0.137, eps 0.5 gauge, Gaussian source: plain FP64
-    against FP32-inner and FP16-link-inner defect correction; the mixed solutions are compared with the
-    FP64 one (|x - x64| / |x64|)."""
+    against defect correction with FP32 inner vectors and links, FP32 vectors with FP16 links, and FP16
+    vectors and links; the mixed solutions are compared with the FP64 one (|x - x64| / |x64|)."""
     dims, kappa = (32, 32, 32, 32), 0.137
     g = clg.Field(ctx, clg.GAUGE, clg.F64, dims)
     g.gauge_random(0.5, 1)
@@ -277,7 +279,7 @@ def mixed_solves_32x32x32x32(ctx, clg, np):
     x = clg.Field(ctx, clg.WILSON, clg.F64, dims, bc=(1, 1, 1, -1))
     out, sols, times = {}, {}, {}
     for name, inner, inner_acc in (("fp64", clg.F64, 0.0), ("mixed_fp32", clg.F32, 1e-4),
-                                   ("mixed_fp16link", clg.F16, 1e-3)):
+                                   ("mixed_fp16link", clg.F16, 1e-3), ("mixed_fp16", clg.INNER_F16_VECTORS, 1e-2)):
         p = clg.solver_params(max_dim=12, restart=2000, accuracy=1e-10, stop_rule=clg.STOP_TRUE_RESIDUAL, even_odd=1,
                               inner_prec=inner, inner_accuracy=inner_acc, max_outer=80)
         clg.solve(ctx, p, x, b, g, kappa, clg.EFO_F_InverseD)  # warm (tuning, scratch)
