@@ -313,7 +313,8 @@ def mixed_solves_32x32x32x32(ctx, clg, np):
 def batched_solves(ctx, clg, np):
     """12 point sources (every spin x colour at the origin) on 24^3x48 plus one Gaussian source each on
     8^4, 12^4, 16^3x32 and 20^3x40, kappa 0.1355, eps 0.5 gauge fields: one batched EO-GMRES(12)
-    solve with FP32 inner / FP64 outer to |r|/|b| < 1e-10, against the 16 separate solves."""
+    solve with FP32 (or FP16-storage) inner / FP64 outer to |r|/|b| < 1e-10, against the same 16
+    systems solved one at a time with the same precision and in plain FP64."""
     lattices = [((24, 24, 24, 48), 12), ((8, 8, 8, 8), 1), ((12, 12, 12, 12), 1), ((16, 16, 16, 32), 1),
                 ((20, 20, 20, 40), 1)]
     gauges, xs, bs, gl = {}, [], [], []
@@ -332,50 +333,46 @@ def batched_solves(ctx, clg, np):
             xs.append(clg.Field(ctx, clg.WILSON, clg.F64, dims, bc=(1, 1, 1, -1)))
             gl.append(g)
             site_rhs += int(np.prod(dims))
-    p = clg.solver_params(max_dim=12, restart=500, accuracy=1e-10, stop_rule=clg.STOP_TRUE_RESIDUAL, even_odd=1,
-                          inner_prec=clg.F32, inner_accuracy=1e-4, max_outer=60)
-    clg.solve_batch(ctx, p, xs, bs, gl, KAPPA)  # warm: tuning and scratch
-    best = None
-    for _ in range(2):
-        t0 = time.perf_counter()
-        st, reps = clg.solve_batch(ctx, p, xs, bs, gl, KAPPA)
-        ctx.sync()
-        dt = time.perf_counter() - t0
-        if best is None or dt < best[0]:
-            best = (dt, st, reps)
-    t_batch, st, reps = best
-    # the same systems one at a time
-    for i in range(len(xs)):
-        clg.solve(ctx, p, xs[i], bs[i], gl[i], KAPPA, clg.EFO_F_InverseD)  # warm per shape
-    t0 = time.perf_counter()
-    sep_res = 0.0
-    for i in range(len(xs)):
-        st1, rep1 = clg.solve(ctx, p, xs[i], bs[i], gl[i], KAPPA, clg.EFO_F_InverseD)
-        sep_res = max(sep_res, rep1.true_residual)
-    ctx.sync()
-    t_sep = time.perf_counter() - t0
-    # and as plain FP64 solves (no inner precision), one at a time
+    def timed(fn):
+        fn()  # warm: tuning and scratch
+        best = None
+        for _ in range(2):
+            t0 = time.perf_counter()
+            r = fn()
+            ctx.sync()
+            dt = time.perf_counter() - t0
+            if best is None or dt < best[0]:
+                best = (dt, r)
+        return best
+
+    def separate(p):
+        res = 0.0
+        for i in range(len(xs)):
+            _, rep1 = clg.solve(ctx, p, xs[i], bs[i], gl[i], KAPPA, clg.EFO_F_InverseD)
+            res = max(res, rep1.true_residual)
+        return res
+
+    # plain FP64 solves, one at a time: the reference point
     p64 = clg.solver_params(max_dim=12, restart=2000, accuracy=1e-10, stop_rule=clg.STOP_TRUE_RESIDUAL, even_odd=1,
                             inner_prec=clg.F64, max_outer=1)
-    for i in range(len(xs)):
-        clg.solve(ctx, p64, xs[i], bs[i], gl[i], KAPPA, clg.EFO_F_InverseD)
-    t0 = time.perf_counter()
-    res64 = 0.0
-    for i in range(len(xs)):
-        st2, rep2 = clg.solve(ctx, p64, xs[i], bs[i], gl[i], KAPPA, clg.EFO_F_InverseD)
-        res64 = max(res64, rep2.true_residual)
-    ctx.sync()
-    t_64 = time.perf_counter() - t0
-    max_res = max(r.true_residual for r in reps)
-    log(f"batched solve: {len(xs)} systems, {site_rhs} site-RHS, {t_batch * 1e3:.1f} ms (separate mixed "
-        f"{t_sep * 1e3:.1f} ms x{t_sep / t_batch:.2f}; separate FP64 {t_64 * 1e3:.1f} ms x{t_64 / t_batch:.2f}), "
-        f"max |r|/|b| {max_res:.2e}, status {st}")
-    out = {"batch_systems": len(xs), "batch_site_rhs": site_rhs, "batch_solve_s": round(t_batch, 4),
-           "batch_separate_s": round(t_sep, 4), "batch_speedup_vs_separate": round(t_sep / t_batch, 3),
-           "batch_separate_fp64_s": round(t_64, 4), "batch_throughput_vs_fp64": round(t_64 / t_batch, 3),
-           "batch_max_true_res": float(f"{max_res:.3e}"), "batch_separate_max_true_res": float(f"{sep_res:.3e}"),
-           "batch_separate_fp64_max_true_res": float(f"{res64:.3e}"),
-           "batch_site_rhs_per_s": round(site_rhs / t_batch, 1)}
+    t_64, res64 = timed(lambda: separate(p64))
+    out = {"batch_systems": len(xs), "batch_site_rhs": site_rhs, "batch_separate_fp64_s": round(t_64, 4),
+           "batch_separate_fp64_max_true_res": float(f"{res64:.3e}")}
+    for tag, inner, acc in (("fp32", clg.F32, 1e-4), ("fp16", clg.INNER_F16_VECTORS, 1e-2)):
+        p = clg.solver_params(max_dim=12, restart=500, accuracy=1e-10, stop_rule=clg.STOP_TRUE_RESIDUAL, even_odd=1,
+                              inner_prec=inner, inner_accuracy=acc, max_outer=60)
+        t_batch, (st, reps) = timed(lambda: clg.solve_batch(ctx, p, xs, bs, gl, KAPPA))
+        t_sep, sep_res = timed(lambda: separate(p))
+        max_res = max(r.true_residual for r in reps)
+        log(f"batched solve {tag}: {len(xs)} systems, {site_rhs} site-RHS, {t_batch * 1e3:.1f} ms (separate mixed "
+            f"{t_sep * 1e3:.1f} ms x{t_sep / t_batch:.2f}; separate FP64 {t_64 * 1e3:.1f} ms x{t_64 / t_batch:.2f}), "
+            f"max |r|/|b| {max_res:.2e}, status {st}")
+        out.update({f"batch_{tag}_solve_s": round(t_batch, 4), f"batch_{tag}_separate_s": round(t_sep, 4),
+                    f"batch_{tag}_speedup_vs_separate": round(t_sep / t_batch, 3),
+                    f"batch_{tag}_throughput_vs_fp64": round(t_64 / t_batch, 3),
+                    f"batch_{tag}_max_true_res": float(f"{max_res:.3e}"),
+                    f"batch_{tag}_separate_max_true_res": float(f"{sep_res:.3e}"),
+                    f"batch_{tag}_site_rhs_per_s": round(site_rhs / t_batch, 1)})
     del xs, bs, gl, gauges
     return out
 

@@ -54,6 +54,9 @@ struct BHopArgs {
     const char* in[kMaxSys];
     char* out[kMaxSys];
     const char* x[kMaxSys];
+    const float* inn[kMaxSys];  // per-site scales of FP16-storage fields (null otherwise)
+    float* outn[kMaxSys];
+    const float* xn[kMaxSys];
     int nseg;
     int in_full, out_full;  // 1: the field holds both parity blocks
     int pbase, nblk;        // output parities
@@ -74,7 +77,7 @@ __device__ __forceinline__ void bhop_mu(const BHopArgs& a, const BSeg& sg, const
         for (int r = 0; r < NR; ++r) {
             if (r < sg.nr) {
                 C2<R> psi[12], hh[6], uh[6];
-                SIO::load(a.in[sg.slot0 + r], nullptr, Vh, pe * SIO::NPL * Vh + n.h, 0u, psi);
+                SIO::load(a.in[sg.slot0 + r], a.inn[sg.slot0 + r], Vh, pe * SIO::NPL * Vh + n.h, pe * Vh + n.h, psi);
                 project<MU, DAG ? 0 : 2>(psi, hh);
                 if (n.sign < 0) {
 #pragma unroll
@@ -96,7 +99,7 @@ __device__ __forceinline__ void bhop_mu(const BHopArgs& a, const BSeg& sg, const
         for (int r = 0; r < NR; ++r) {
             if (r < sg.nr) {
                 C2<R> psi[12], hh[6], uh[6];
-                SIO::load(a.in[sg.slot0 + r], nullptr, Vh, pe * SIO::NPL * Vh + n.h, 0u, psi);
+                SIO::load(a.in[sg.slot0 + r], a.inn[sg.slot0 + r], Vh, pe * SIO::NPL * Vh + n.h, pe * Vh + n.h, psi);
                 project<MU, DAG ? 2 : 0>(psi, hh);
                 if (n.sign < 0) {
 #pragma unroll
@@ -140,7 +143,7 @@ __global__ __launch_bounds__(256) void k_wilson_hop_batch(BHopArgs a) {
     bhop_mu<2, DAG, NR, R, SIO, LIO>(a, sg, s, p, h, acc);
     bhop_mu<3, DAG, NR, R, SIO, LIO>(a, sg, s, p, h, acc);
     const unsigned po = a.out_full ? (unsigned)p : 0u;
-    const unsigned eo = po * SIO::NPL * Vh + h;
+    const unsigned eo = po * SIO::NPL * Vh + h, no = po * Vh + h;
     const R ca = (R)sg.ar;
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
@@ -148,7 +151,7 @@ __global__ __launch_bounds__(256) void k_wilson_hop_batch(BHopArgs a) {
             C2<R> o[12];
             if constexpr (HASX) {
                 C2<R> xv[12];
-                SIO::load(a.x[sg.slot0 + r], nullptr, Vh, eo, 0u, xv);
+                SIO::load(a.x[sg.slot0 + r], a.xn[sg.slot0 + r], Vh, eo, no, xv);
                 const R cb = (R)a.br;
 #pragma unroll
                 for (int k = 0; k < 12; ++k) o[k] = cmk<R>(cb * xv[k].x + ca * acc[r][k].x, cb * xv[k].y + ca * acc[r][k].y);
@@ -156,7 +159,7 @@ __global__ __launch_bounds__(256) void k_wilson_hop_batch(BHopArgs a) {
 #pragma unroll
                 for (int k = 0; k < 12; ++k) o[k] = cmk<R>(ca * acc[r][k].x, ca * acc[r][k].y);
             }
-            SIO::store(a.out[sg.slot0 + r], nullptr, Vh, eo, 0u, o);
+            SIO::store(a.out[sg.slot0 + r], a.outn[sg.slot0 + r], Vh, eo, no, o);
         }
     }
 }
@@ -235,6 +238,9 @@ int build_args(const HopBatch& B, int nr, int block, int xcd, bool hasx, double 
             a.in[slot + r] = (const char*)B.in[sys]->data;
             a.out[slot + r] = (char*)B.out[sys]->data;
             a.x[slot + r] = hasx ? (const char*)B.x[sys]->data : nullptr;
+            a.inn[slot + r] = B.in[sys]->norm;
+            a.outn[slot + r] = B.out[sys]->norm;
+            a.xn[slot + r] = hasx ? B.x[sys]->norm : nullptr;
         }
         slot += cnt;
         blk += (int)grid_for((long)nblk * sg.g.Vh, block);
@@ -261,6 +267,7 @@ int launch_batch_cfg(const HopBatch& B, bool dag, bool hasx, double br, int pbas
     const clg_field_s* gf = B.gauge[0];
     const int recon = gf->kind == CLG_GAUGE12 ? 12 : 18;
     if (f->prec == CLG_F64) return launch_dx<double, CLG_F64, CLG_F64, 18>(a, dag, hasx, grid, c.block, c.nr, st);
+    if (f->prec == CLG_F16) return launch_dx<float, CLG_F16, CLG_F16, 12>(a, dag, hasx, grid, c.block, c.nr, st);
     if (gf->prec == CLG_F16) return launch_dx<float, CLG_F32, CLG_F16, 12>(a, dag, hasx, grid, c.block, c.nr, st);
     (void)recon;
     return launch_dx<float, CLG_F32, CLG_F32, 18>(a, dag, hasx, grid, c.block, c.nr, st);
@@ -286,10 +293,11 @@ int check_batch(const HopBatch& B) {
         CLG_CHECK_ARG(in->prec == f0->prec && out->prec == f0->prec && in->subset == f0->subset &&
                           out->subset == B.out[0]->subset,
                       "all batch slots need one precision and subset layout");
-        CLG_CHECK_ARG(in->prec == CLG_F64 || in->prec == CLG_F32, "batched hops take FP64 or FP32 spinors");
         CLG_CHECK_ARG(is_gauge(g->kind), "slot %d: need a gauge field", i);
         if (in->prec == CLG_F64)
             CLG_CHECK_ARG(g->prec == CLG_F64 && g->kind == CLG_GAUGE, "FP64 batch needs FP64 18-real links");
+        else if (in->prec == CLG_F16)
+            CLG_CHECK_ARG(g->prec == CLG_F16 && g->kind == CLG_GAUGE12, "FP16 batch needs FP16 12-real links");
         else
             CLG_CHECK_ARG((g->prec == CLG_F32 && g->kind == CLG_GAUGE) || (g->prec == CLG_F16 && g->kind == CLG_GAUGE12),
                           "FP32 batch needs FP32 18-real or FP16 12-real links");
@@ -327,7 +335,7 @@ int batch_hop(clg_context ctx, const HopBatch& B, bool dag, double br, int tile)
     hipStream_t st = ctx->stream;
     const bool f64 = in0->prec == CLG_F64;
     if (tile > 0) {
-        CLG_CHECK_ARG(tile == 1 || tile == 2 || (tile == 4 && !f64), "tile must be 1, 2 or 4 (FP32)");
+        CLG_CHECK_ARG(tile == 1 || tile == 2 || (tile == 4 && !f64), "tile must be 1, 2 or 4 (not FP64)");
         return launch_batch_cfg(B, dag, hasx, br, pbase, nblk, BCfg{tile, 128, 8}, st);
     }
     std::vector<long> key = {in0->prec, B.gauge[0]->prec, B.gauge[0]->kind, dag, hasx, nblk, pbase,
@@ -394,6 +402,10 @@ struct BTab {
     long npk[kMaxSys];
     void* V[kMaxSys][kMaxM + 1];
     void* X[kMaxSys][2];
+    // FP16-storage vectors: per-site scales and the sites per vector (one parity block)
+    float* Vn[kMaxSys][kMaxM + 1];
+    float* Xn[kMaxSys][2];
+    unsigned vh[kMaxSys];
 };
 struct Act {
     int n;
@@ -609,14 +621,15 @@ __global__ void k_bzero(const BTab* T, Act act, int ysel) {
 // Per-system workspace: H (2 ld m), c2 (2m), g (2(m+1)), y (2m), sc (16) doubles, then an int area.
 struct WsLayout {
     int m, ld;
-    size_t offH, offC, offG, offY, offS, stride;
+    size_t offH, offC, offG, offY, offS, offGS, stride;
     WsLayout(int m_) : m(m_), ld(m_ + 1) {
         offH = 0;
         offC = offH + 2 * (size_t)ld * m;
         offG = offC + 2 * (size_t)m;
         offY = offG + 2 * (size_t)(m + 1);
         offS = offY + 2 * (size_t)m;
-        stride = offS + 16;
+        offGS = offS + 16;
+        stride = offGS + 40;
     }
 };
 // sc: [0] |r|^2, [1] |w'|^2, [2] dev, [3] |dev|, [4] |w|^2 before GS, [5] |b|^2, [6] |r_outer|^2,
@@ -688,6 +701,317 @@ __global__ void k_bgmres_lsq(double* ws, size_t stride, WsLayout L, Act act) {
     }
 }
 
+// ---- batched BLAS on FP16-storage vectors (one parity block each; see clg_blas16.hip for the
+//      format and the arithmetic), one workgroup row per active system (blockIdx.y)
+__device__ __forceinline__ uint4* hvec(const BTab* T, int s, int sel) {
+    return (uint4*)(sel >= 0 ? T->V[s][sel] : T->X[s][-sel - 1]);
+}
+__device__ __forceinline__ float* hnrm(const BTab* T, int s, int sel) {
+    return sel >= 0 ? T->Vn[s][sel] : T->Xn[s][-sel - 1];
+}
+#define BSITE_LOOP(i, n) for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
+
+__global__ __launch_bounds__(kRedBlock) void k_bh_norm2(const BTab* T, Act act, int sel, double* partial) {
+    const int s = act.idx[blockIdx.y];
+    const unsigned n = T->vh[s];
+    const uint4* c = hvec(T, s, sel);
+    const float* sn = hnrm(T, s, sel);
+    double acc[1] = {0.0};
+    BSITE_LOOP(i, n) {
+        float v[24];
+        h16_load_site(c, sn, n, i, v);
+        float t = 0.f;
+#pragma unroll
+        for (int q = 0; q < 24; ++q) t += v[q] * v[q];
+        acc[0] += t;
+    }
+    block_partial<1>(acc, partial + (size_t)blockIdx.y * gridDim.x);
+}
+// dst = src / sqrt(n2[s * stride])
+__global__ __launch_bounds__(256) void k_bh_scale(const BTab* T, Act act, int dsel, int ssel, const double* n2, int stride) {
+    const int s = act.idx[blockIdx.y];
+    const unsigned n = T->vh[s];
+    const float f = (float)(1.0 / sqrt(n2[(size_t)s * stride]));
+    const uint4* sc = hvec(T, s, ssel);
+    const float* sn = hnrm(T, s, ssel);
+    uint4* dc = hvec(T, s, dsel);
+    float* dn = hnrm(T, s, dsel);
+    BSITE_LOOP(i, n) {
+        float v[24];
+        h16_load_site(sc, sn, n, i, v);
+#pragma unroll
+        for (int q = 0; q < 24; ++q) v[q] *= f;
+        h16_store_site(dc, dn, n, i, v);
+    }
+}
+// y = a y + b x (real); a == 0 overwrites y without reading it; b == 0 with a == 0 zeroes y
+__global__ __launch_bounds__(256) void k_bh_axpby(const BTab* T, Act act, int ysel, float a, int xsel, float b) {
+    const int s = act.idx[blockIdx.y];
+    const unsigned n = T->vh[s];
+    uint4* yc = hvec(T, s, ysel);
+    float* yn = hnrm(T, s, ysel);
+    const uint4* xc = hvec(T, s, xsel);
+    const float* xn = hnrm(T, s, xsel);
+    BSITE_LOOP(i, n) {
+        float y[24], x[24];
+        if (b != 0.f) {
+            h16_load_site(xc, xn, n, i, x);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 24; ++q) x[q] = 0.f;
+        }
+        if (a != 0.f) {
+            h16_load_site(yc, yn, n, i, y);
+#pragma unroll
+            for (int q = 0; q < 24; ++q) y[q] = a * y[q] + b * x[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 24; ++q) y[q] = b * x[q];
+        }
+        h16_store_site(yc, yn, n, i, y);
+    }
+}
+// partials of V_k^H w (2K) and |w|^2 (one more), w = V[s][wsel]
+template <int K>
+__global__ __launch_bounds__(kRedBlock) void k_bh_multidot(const BTab* T, Act act, int wsel, double* partial) {
+    const int s = act.idx[blockIdx.y];
+    const unsigned n = T->vh[s];
+    const uint4* wc = hvec(T, s, wsel);
+    const float* wn = hnrm(T, s, wsel);
+    double acc[2 * K + 1];
+#pragma unroll
+    for (int r = 0; r < 2 * K + 1; ++r) acc[r] = 0.0;
+    BSITE_LOOP(i, n) {
+        float sv[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) sv[k] = T->Vn[s][k][i];
+        const float sw = wn[i];
+        float re[K], im[K], ww = 0.f;
+#pragma unroll
+        for (int k = 0; k < K; ++k) re[k] = im[k] = 0.f;
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            const size_t ci = h16_chunk(i, n, ch);
+            uint4 raw[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) raw[k] = ((const uint4*)T->V[s][k])[ci];
+            float y[8];
+            h16_unpack8(wc[ci], sw, y);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) ww += y[q] * y[q];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                float x[8];
+                h16_unpack8(raw[k], sv[k], x);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    re[k] += x[2 * q] * y[2 * q] + x[2 * q + 1] * y[2 * q + 1];
+                    im[k] += x[2 * q] * y[2 * q + 1] - x[2 * q + 1] * y[2 * q];
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) { acc[2 * k] += re[k]; acc[2 * k + 1] += im[k]; }
+        acc[2 * K] += ww;
+    }
+    block_partial<2 * K + 1>(acc, partial + (size_t)blockIdx.y * gridDim.x * (2 * K + 1));
+}
+// w' = w - sum_k h_k V_k (stored) with the partials of V_k^H w' and |w'|^2; h at ws[s*stride + offh]
+template <int K>
+__global__ __launch_bounds__(kRedBlock) void k_bh_gs_project(const BTab* T, Act act, int wsel, const double* ws,
+                                                              int stride, int offh, double* partial) {
+    const int s = act.idx[blockIdx.y];
+    const unsigned n = T->vh[s];
+    const double* h = ws + (size_t)s * stride + offh;
+    uint4* wc = hvec(T, s, wsel);
+    float* wn = hnrm(T, s, wsel);
+    float hr[K], hi[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) { hr[k] = (float)h[2 * k]; hi[k] = (float)h[2 * k + 1]; }
+    double acc[2 * K + 1];
+#pragma unroll
+    for (int r = 0; r < 2 * K + 1; ++r) acc[r] = 0.0;
+    BSITE_LOOP(i, n) {
+        float sv[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) sv[k] = T->Vn[s][k][i];
+        const float sw = wn[i];
+        float y[24];
+        float ww = 0.f;
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            const size_t ci = h16_chunk(i, n, ch);
+            uint4 raw[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) raw[k] = ((const uint4*)T->V[s][k])[ci];
+            float* yc = y + 8 * ch;
+            h16_unpack8(wc[ci], sw, yc);
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                float x[8];
+                h16_unpack8(raw[k], sv[k], x);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    yc[2 * q] -= hr[k] * x[2 * q] - hi[k] * x[2 * q + 1];
+                    yc[2 * q + 1] -= hr[k] * x[2 * q + 1] + hi[k] * x[2 * q];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                float x[8];
+                h16_unpack8(raw[k], sv[k], x);
+                float re = 0.f, im = 0.f;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    re += x[2 * q] * yc[2 * q] + x[2 * q + 1] * yc[2 * q + 1];
+                    im += x[2 * q] * yc[2 * q + 1] - x[2 * q + 1] * yc[2 * q];
+                }
+                acc[2 * k] += re;
+                acc[2 * k + 1] += im;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) ww += yc[q] * yc[q];
+        }
+        h16_store_site(wc, wn, n, i, y);
+        acc[2 * K] += ww;
+    }
+    block_partial<2 * K + 1>(acc, partial + (size_t)blockIdx.y * gridDim.x * (2 * K + 1));
+}
+// per system: completes k_bh_gs_project and updates Hessenberg column j of the Arnoldi process on
+// K = 1 - A (the batched inner solve): H(i, j) = delta_ij - (h_i + flag c2_i), H(j+1, j) = -|w''|.
+// gs (per system, 36 doubles): [0] |w|^2 (in), [1] |w''|^2, [2] 1/|w''|, [3] flag, [4..) c2.
+template <int K>
+__global__ __launch_bounds__(kRedBlock) void k_bgs_finish(Act act, const double* __restrict__ partial, int nblk,
+                                                           double* ws, int stride, int offh, int offgs) {
+    constexpr int NR = 2 * K + 1;
+    const int s = act.idx[blockIdx.y];
+    const double* pp = partial + (size_t)blockIdx.y * nblk * NR;
+    double* hcol = ws + (size_t)s * stride + offh;
+    double* gs = ws + (size_t)s * stride + offgs;
+    const int j = K - 1;
+    __shared__ double sm[kRedBlock / 64][NR];
+    __shared__ double tot[NR];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = 0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[r] += pp[(size_t)b * NR + r];
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = wave_sum(acc[r]);
+    if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) sm[wv][r] = acc[r];
+    }
+    __syncthreads();
+    if (threadIdx.x < NR) {
+        double t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sm[w][threadIdx.x];
+        tot[threadIdx.x] = t;
+    }
+    __syncthreads();
+    const double w2 = tot[2 * K];
+    const bool flag = w2 < 0.5 * gs[0];
+    const int i = threadIdx.x;
+    if (i < K) {
+        double re = hcol[2 * i], im = hcol[2 * i + 1];
+        if (flag) {
+            re += tot[2 * i];
+            im += tot[2 * i + 1];
+        }
+        hcol[2 * i] = (i == j ? 1.0 : 0.0) - re;
+        hcol[2 * i + 1] = -im;
+        gs[4 + 2 * i] = tot[2 * i];
+        gs[4 + 2 * i + 1] = tot[2 * i + 1];
+    }
+    if (i == 0) {
+        double c2 = 0;
+        for (int k = 0; k < 2 * K; ++k) c2 += tot[k] * tot[k];
+        double f2 = flag ? w2 - c2 : w2;
+        if (!(f2 > 1e-300)) f2 = 1e-300;
+        const double f = sqrt(f2);
+        hcol[2 * (j + 1)] = -f;
+        hcol[2 * (j + 1) + 1] = 0.0;
+        gs[1] = f2;
+        gs[2] = 1.0 / f;
+        gs[3] = flag ? 1.0 : 0.0;
+    }
+}
+// V[s][dsel] = (w' - flag sum_k c2_k V_k) / |w''|, w' = V[s][wsel]
+template <int K>
+__global__ __launch_bounds__(256) void k_bh_gs_finalize(const BTab* T, Act act, int dsel, int wsel, const double* ws, int stride, int offgs) {
+    const int s = act.idx[blockIdx.y];
+    const unsigned n = T->vh[s];
+    const double* gs = ws + (size_t)s * stride + offgs;
+    const bool flag = gs[3] != 0.0;
+    const float inv = (float)gs[2];
+    float cr[K], ci[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) { cr[k] = (float)gs[4 + 2 * k]; ci[k] = (float)gs[4 + 2 * k + 1]; }
+    const uint4* wc = hvec(T, s, wsel);
+    const float* wn = hnrm(T, s, wsel);
+    uint4* dc = hvec(T, s, dsel);
+    float* dn = hnrm(T, s, dsel);
+    BSITE_LOOP(i, n) {
+        float y[24];
+        h16_load_site(wc, wn, n, i, y);
+        if (flag) {
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) {
+                const size_t cidx = h16_chunk(i, n, ch);
+                float* yc = y + 8 * ch;
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    float x[8];
+                    h16_unpack8(((const uint4*)T->V[s][k])[cidx], T->Vn[s][k][i], x);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        yc[2 * q] -= cr[k] * x[2 * q] - ci[k] * x[2 * q + 1];
+                        yc[2 * q + 1] -= cr[k] * x[2 * q + 1] + ci[k] * x[2 * q];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 24; ++q) y[q] *= inv;
+        h16_store_site(dc, dn, n, i, y);
+    }
+}
+// y += sum_k c_k V_k, y = V[s][tsel] or X, c at ws[s*stride + offc]
+template <int K>
+__global__ __launch_bounds__(256) void k_bh_multiaxpy(const BTab* T, Act act, int tsel, const double* ws, int stride, int offc) {
+    const int s = act.idx[blockIdx.y];
+    const unsigned n = T->vh[s];
+    const double* c = ws + (size_t)s * stride + offc;
+    float cr[K], ci[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) { cr[k] = (float)c[2 * k]; ci[k] = (float)c[2 * k + 1]; }
+    uint4* yc = hvec(T, s, tsel);
+    float* yn = hnrm(T, s, tsel);
+    BSITE_LOOP(i, n) {
+        float y[24];
+        h16_load_site(yc, yn, n, i, y);
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            const size_t cidx = h16_chunk(i, n, ch);
+            float* yp = y + 8 * ch;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                float x[8];
+                h16_unpack8(((const uint4*)T->V[s][k])[cidx], T->Vn[s][k][i], x);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    yp[2 * q] += cr[k] * x[2 * q] - ci[k] * x[2 * q + 1];
+                    yp[2 * q + 1] += cr[k] * x[2 * q + 1] + ci[k] * x[2 * q];
+                }
+            }
+        }
+        h16_store_site(yc, yn, n, i, y);
+    }
+}
+
 unsigned red_grid_b(clg_context ctx, long n, int nact) {
     long g = (n + kRedBlock * 4 - 1) / (kRedBlock * 4);
     const long cap = std::max<long>(8, (long)ctx->num_cu * 4 / std::max(1, nact));
@@ -732,7 +1056,7 @@ struct BatchSolver {
     int setup(clg_field_s* const* xs, clg_field_s* const* bs, clg_field_s* const* gs) {
         m = p->max_dim;
         L = WsLayout(m);
-        vprec = p->inner_prec == CLG_F64 ? CLG_F64 : CLG_F32;
+        vprec = p->inner_prec == CLG_F64 ? CLG_F64 : p->inner_prec == CLG_INNER_F16_VECTORS ? CLG_F16 : CLG_F32;
         // inner links, one conversion per distinct gauge field
         std::vector<std::pair<const clg_field_s*, clg_field_s*>> conv_g;
         for (int s = 0; s < ns; ++s) {
@@ -762,7 +1086,7 @@ struct BatchSolver {
                 for (auto& pr : conv_g)
                     if (pr.first == gauge[s]) glo[s] = pr.second;
                 if (!glo[s]) {
-                    const int lp = p->inner_prec == CLG_F16 ? CLG_F16 : CLG_F32;
+                    const int lp = p->inner_prec == CLG_F32 ? CLG_F32 : CLG_F16;
                     const int gk = lp == CLG_F16 ? CLG_GAUGE12 : CLG_GAUGE;
                     clg_field_s* g = nullptr;
                     CLG_TRY(scratch_field(ctx, gauge[s], gk, lp, CLG_FULL, 20000 + (int)conv_g.size(), &g));
@@ -784,9 +1108,15 @@ struct BatchSolver {
             const long nc = 12L * V[s][0]->Vh;
             h.npk[s] = vprec == CLG_F64 ? nc : nc / 2;
             max_npk = std::max(max_npk, h.npk[s]);
-            for (int k = 0; k <= m; ++k) h.V[s][k] = V[s][k]->data;
+            for (int k = 0; k <= m; ++k) {
+                h.V[s][k] = V[s][k]->data;
+                h.Vn[s][k] = V[s][k]->norm;
+            }
             h.X[s][0] = elo[s]->data;
             h.X[s][1] = rlo[s]->data;
+            h.Xn[s][0] = elo[s]->norm;
+            h.Xn[s][1] = rlo[s]->norm;
+            h.vh[s] = (unsigned)V[s][0]->Vh;
         }
         CLG_HIP(hipMemcpyAsync(dtab, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
         CLG_HIP(hipStreamSynchronize(ctx->stream));  // h is a stack object
@@ -948,6 +1278,85 @@ struct BatchSolver {
         return CLG_OK;
     }
 
+    // ---- inner batched GMRES(m) with FP16-storage vectors: the fused Gram-Schmidt step (projections;
+    //      subtraction with the second-pass projections; normalisation) per Arnoldi step
+    int inner16(std::vector<int> act, const double* tol) {
+        const int maxc = p->restart;
+        const double k2 = kappa * kappa;
+        std::vector<clg_field_s*> Vj(kMaxSys), W(kMaxSys);
+        const int stride = (int)L.stride;
+        for (int cyc = 0; cyc < maxc && !act.empty(); ++cyc) {
+            Act A = make_act(act);
+            const unsigned eg = ew_grid_b(ctx, max_npk, A.n);
+            dim3 egrid(eg, A.n);
+            const unsigned rg = red_grid_b(ctx, max_npk, A.n);
+            dim3 rgrid(rg, A.n);
+            if (cyc == 0) {
+                k_bh_axpby<<<egrid, 256, 0, ctx->stream>>>(dtab, A, -1, 0.f, -1, 0.f);  // e = 0
+                k_bh_axpby<<<egrid, 256, 0, ctx->stream>>>(dtab, A, 0, 0.f, -2, 1.f);   // V0 = r
+            } else {
+                // V0 = kappa^2 H_eo H_oe e - e (= -Mhat e), then V0 += r
+                for (int s : act) Vj[s] = V[s][0];
+                CLG_TRY(hop(act, tlo, elo, nullptr, 1.0, 0.0, true));
+                CLG_TRY(hop(act, Vj.data(), tlo, elo, k2, -1.0, true));
+                k_bh_axpby<<<egrid, 256, 0, ctx->stream>>>(dtab, A, 0, 1.f, -2, 1.f);
+            }
+            CLG_HIP(hipGetLastError());
+            k_bh_norm2<<<rgrid, kRedBlock, 0, ctx->stream>>>(dtab, A, 0, ctx->red_partial);
+            k_bred_finish<<<dim3(1, A.n), kRedBlock, 0, ctx->stream>>>(A, ctx->red_partial, (int)rg, 1, ws + L.offS, 1,
+                                                                       nullptr, nullptr, nullptr, 0.0, nullptr, stride);
+            k_bh_scale<<<egrid, 256, 0, ctx->stream>>>(dtab, A, 0, 0, ws + L.offS, stride);
+            CLG_HIP(hipGetLastError());
+            for (int s : act) cycles[s]++;
+            for (int j = 0; j < m; ++j) {
+                // w = K V_j = kappa^2 H_eo H_oe V_j
+                for (int s : act) { Vj[s] = V[s][j]; W[s] = V[s][m]; }
+                CLG_TRY(hop(act, tlo, Vj.data(), nullptr, 1.0, 0.0, true));
+                CLG_TRY(hop(act, W.data(), tlo, nullptr, k2, 0.0, true));
+                const int offh = (int)(L.offH + 2 * (size_t)j * L.ld);
+                const int nv = j + 1;
+#define CALL(KK)                                                                                                 \
+    k_bh_multidot<KK><<<rgrid, kRedBlock, 0, ctx->stream>>>(dtab, A, m, ctx->red_partial);                       \
+    k_bred_finish<<<dim3(1, A.n), kRedBlock, 0, ctx->stream>>>(A, ctx->red_partial, (int)rg, 2 * KK + 1,         \
+                                                               ws + offh, 2 * KK, ws + L.offGS, nullptr, nullptr, \
+                                                               0.0, nullptr, stride);                            \
+    k_bh_gs_project<KK><<<rgrid, kRedBlock, 0, ctx->stream>>>(dtab, A, m, ws, stride, offh, ctx->red_partial);   \
+    k_bgs_finish<KK><<<dim3(1, A.n), kRedBlock, 0, ctx->stream>>>(A, ctx->red_partial, (int)rg, ws, stride, offh, \
+                                                                  (int)L.offGS);                                 \
+    if (j < m - 1)                                                                                               \
+        k_bh_gs_finalize<KK><<<egrid, 256, 0, ctx->stream>>>(dtab, A, j + 1, m, ws, stride, (int)L.offGS);
+                switch (nv) {
+                    case 1: CALL(1) break; case 2: CALL(2) break; case 3: CALL(3) break; case 4: CALL(4) break;
+                    case 5: CALL(5) break; case 6: CALL(6) break; case 7: CALL(7) break; case 8: CALL(8) break;
+                    case 9: CALL(9) break; case 10: CALL(10) break; case 11: CALL(11) break; case 12: CALL(12) break;
+                    case 13: CALL(13) break; case 14: CALL(14) break; case 15: CALL(15) break; default: CALL(16) break;
+                }
+#undef CALL
+                CLG_HIP(hipGetLastError());
+            }
+            k_bgmres_lsq<<<dim3(1, A.n), 64, 0, ctx->stream>>>(ws, L.stride, L, A);
+            CLG_HIP(hipGetLastError());
+            // e += V y
+#define CALL(KK) k_bh_multiaxpy<KK><<<egrid, 256, 0, ctx->stream>>>(dtab, A, -1, ws, stride, (int)L.offY);
+            switch (m) {
+                case 1: CALL(1) break; case 2: CALL(2) break; case 3: CALL(3) break; case 4: CALL(4) break;
+                case 5: CALL(5) break; case 6: CALL(6) break; case 7: CALL(7) break; case 8: CALL(8) break;
+                case 9: CALL(9) break; case 10: CALL(10) break; case 11: CALL(11) break; case 12: CALL(12) break;
+                case 13: CALL(13) break; case 14: CALL(14) break; case 15: CALL(15) break; default: CALL(16) break;
+            }
+#undef CALL
+            CLG_HIP(hipGetLastError());
+            std::vector<int> next;
+            for (int s : act) {
+                double d[2];
+                CLG_TRY(fetch_doubles(ctx, sc(s) + 2, 2, d));
+                if (!(d[1] < tol[s]) && std::isfinite(d[0])) next.push_back(s);
+            }
+            act.swap(next);
+        }
+        return CLG_OK;
+    }
+
     // ---- outer FP64 defect correction on Mhat x_e = bhat for all systems, then x_o
     int run(double* true_res) {
         std::vector<int> all;
@@ -999,6 +1408,7 @@ struct BatchSolver {
                 CLG_TRY(clg_field_copy(rlo[s], r64[s]));
             }
             if (vprec == CLG_F64) CLG_TRY(inner<CLG_F64>(act, tol.data()));
+            else if (vprec == CLG_F16) CLG_TRY(inner16(act, tol.data()));
             else CLG_TRY(inner<CLG_F32>(act, tol.data()));
             // x_e += e
             for (int s : act) {

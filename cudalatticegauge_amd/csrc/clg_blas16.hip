@@ -38,53 +38,16 @@ struct HTab {
     HPtr v[kMaxV];
 };
 
-// chunk k of site i (blocks of Vh sites, three planes per block)
-__device__ __forceinline__ size_t chunk_index(unsigned i, unsigned Vh, int k) {
-    const unsigned b = i / Vh, h = i - b * Vh;
-    return (size_t)(3 * b + k) * Vh + h;
-}
-__device__ __forceinline__ void unpack8(uint4 a, float s, float* v) {
-    float2 f;
-    f = unpack_h2(a.x); v[0] = f.x * s; v[1] = f.y * s;
-    f = unpack_h2(a.y); v[2] = f.x * s; v[3] = f.y * s;
-    f = unpack_h2(a.z); v[4] = f.x * s; v[5] = f.y * s;
-    f = unpack_h2(a.w); v[6] = f.x * s; v[7] = f.y * s;
-}
-// all 24 components of site i
-__device__ __forceinline__ void load_site(const uint4* c, const float* s, unsigned Vh, unsigned i, float* v) {
-    const float sc = s[i];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) unpack8(c[chunk_index(i, Vh, k)], sc, v + 8 * k);
-}
-// stores site i with a new scale; v is left holding the stored (rounded) values
-__device__ __forceinline__ void store_site(uint4* c, float* s, unsigned Vh, unsigned i, float* v) {
-    float m = 0.f;
-#pragma unroll
-    for (int q = 0; q < 24; ++q) m = fmaxf(m, fabsf(v[q]));
-    const float inv = m > 0.f ? 1.0f / m : 0.f;
-    s[i] = m;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        uint4 a;
-        a.x = pack_h2(v[8 * k + 0] * inv, v[8 * k + 1] * inv);
-        a.y = pack_h2(v[8 * k + 2] * inv, v[8 * k + 3] * inv);
-        a.z = pack_h2(v[8 * k + 4] * inv, v[8 * k + 5] * inv);
-        a.w = pack_h2(v[8 * k + 6] * inv, v[8 * k + 7] * inv);
-        c[chunk_index(i, Vh, k)] = a;
-        unpack8(a, m, v + 8 * k);
-    }
-}
-
 #define SITE_LOOP(i, n) for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
 
 // y = a y + b x (complex a, b); a == 0 overwrites y without reading it
 template <bool READY>
-__global__ void k_h_axpby(unsigned n, unsigned Vh, double ar, double ai, HPtr x, double br, double bi, uint4* yc,
+__global__ __launch_bounds__(256) void k_h_axpby(unsigned n, unsigned Vh, double ar, double ai, HPtr x, double br, double bi, uint4* yc,
                           float* ys) {
     SITE_LOOP(i, n) {
         float xv[24], yv[24];
-        load_site(x.c, x.s, Vh, i, xv);
-        if (READY) load_site(yc, ys, Vh, i, yv);
+        h16_load_site(x.c, x.s, Vh, i, xv);
+        if (READY) h16_load_site(yc, ys, Vh, i, yv);
 #pragma unroll
         for (int q = 0; q < 12; ++q) {
             const float xr = xv[2 * q], xi = xv[2 * q + 1];
@@ -97,7 +60,7 @@ __global__ void k_h_axpby(unsigned n, unsigned Vh, double ar, double ai, HPtr x,
             yv[2 * q] = re;
             yv[2 * q + 1] = im;
         }
-        store_site(yc, ys, Vh, i, yv);
+        h16_store_site(yc, ys, Vh, i, yv);
     }
 }
 
@@ -105,7 +68,7 @@ __global__ __launch_bounds__(kRedBlock) void k_h_norm2(unsigned n, unsigned Vh, 
     double acc[1] = {0.0};
     SITE_LOOP(i, n) {
         float v[24];
-        load_site(x.c, x.s, Vh, i, v);
+        h16_load_site(x.c, x.s, Vh, i, v);
         float t = 0.f;
 #pragma unroll
         for (int q = 0; q < 24; ++q) t += v[q] * v[q];
@@ -115,15 +78,15 @@ __global__ __launch_bounds__(kRedBlock) void k_h_norm2(unsigned n, unsigned Vh, 
 }
 
 // dst = src / sqrt(nrm2[0])
-__global__ void k_h_scale_inv_sqrt(unsigned n, unsigned Vh, const double* __restrict__ nrm2, HPtr src, uint4* dc,
+__global__ __launch_bounds__(256) void k_h_scale_inv_sqrt(unsigned n, unsigned Vh, const double* __restrict__ nrm2, HPtr src, uint4* dc,
                                    float* ds) {
     const float f = (float)(1.0 / sqrt(nrm2[0]));
     SITE_LOOP(i, n) {
         float v[24];
-        load_site(src.c, src.s, Vh, i, v);
+        h16_load_site(src.c, src.s, Vh, i, v);
 #pragma unroll
         for (int q = 0; q < 24; ++q) v[q] *= f;
-        store_site(dc, ds, Vh, i, v);
+        h16_store_site(dc, ds, Vh, i, v);
     }
 }
 
@@ -143,18 +106,18 @@ __global__ __launch_bounds__(kRedBlock) void k_h_multidot(unsigned n, unsigned V
         for (int k = 0; k < K; ++k) re[k] = im[k] = 0.f;
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
-            const size_t ci = chunk_index(i, Vh, ch);
+            const size_t ci = h16_chunk(i, Vh, ch);
             uint4 raw[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) raw[k] = V.v[k].c[ci];
             float y[8];
-            unpack8(w.c[ci], sw, y);
+            h16_unpack8(w.c[ci], sw, y);
 #pragma unroll
             for (int q = 0; q < 8; ++q) ww += y[q] * y[q];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 float x[8];
-                unpack8(raw[k], sv[k], x);
+                h16_unpack8(raw[k], sv[k], x);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     re[k] += x[2 * q] * y[2 * q] + x[2 * q + 1] * y[2 * q + 1];
@@ -190,16 +153,16 @@ __global__ __launch_bounds__(kRedBlock) void k_h_gs_project(unsigned n, unsigned
         for (int k = 0; k < K; ++k) re[k] = im[k] = 0.f;
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
-            const size_t ci = chunk_index(i, Vh, ch);
+            const size_t ci = h16_chunk(i, Vh, ch);
             uint4 raw[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) raw[k] = V.v[k].c[ci];
             float* yc = y + 8 * ch;
-            unpack8(wc[ci], sw, yc);
+            h16_unpack8(wc[ci], sw, yc);
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 float x[8];
-                unpack8(raw[k], sv[k], x);
+                h16_unpack8(raw[k], sv[k], x);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     yc[2 * q] -= hr[k] * x[2 * q] - hi[k] * x[2 * q + 1];
@@ -209,7 +172,7 @@ __global__ __launch_bounds__(kRedBlock) void k_h_gs_project(unsigned n, unsigned
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 float x[8];
-                unpack8(raw[k], sv[k], x);
+                h16_unpack8(raw[k], sv[k], x);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     re[k] += x[2 * q] * yc[2 * q] + x[2 * q + 1] * yc[2 * q + 1];
@@ -219,7 +182,7 @@ __global__ __launch_bounds__(kRedBlock) void k_h_gs_project(unsigned n, unsigned
 #pragma unroll
             for (int q = 0; q < 8; ++q) ww += yc[q] * yc[q];
         }
-        store_site(wc, ws, Vh, i, y);
+        h16_store_site(wc, ws, Vh, i, y);
 #pragma unroll
         for (int k = 0; k < K; ++k) { acc[2 * k] += re[k]; acc[2 * k + 1] += im[k]; }
         acc[2 * K] += ww;
@@ -229,7 +192,7 @@ __global__ __launch_bounds__(kRedBlock) void k_h_gs_project(unsigned n, unsigned
 
 // vnext = (w' - flag sum_k c2_k V_k) / |w''|; sc as written by the Gram-Schmidt finish
 template <int K>
-__global__ void k_h_gs_finalize(unsigned n, unsigned Vh, HTab V, HPtr w, uint4* vc, float* vs,
+__global__ __launch_bounds__(256) void k_h_gs_finalize(unsigned n, unsigned Vh, HTab V, HPtr w, uint4* vc, float* vs,
                                 const double* __restrict__ sc) {
     const bool flag = sc[3] != 0.0;
     const float inv = (float)sc[2];
@@ -238,19 +201,19 @@ __global__ void k_h_gs_finalize(unsigned n, unsigned Vh, HTab V, HPtr w, uint4* 
     for (int k = 0; k < K; ++k) { cr[k] = (float)sc[4 + 2 * k]; ci[k] = (float)sc[4 + 2 * k + 1]; }
     SITE_LOOP(i, n) {
         float y[24];
-        load_site(w.c, w.s, Vh, i, y);
+        h16_load_site(w.c, w.s, Vh, i, y);
         if (flag) {
             float sv[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) sv[k] = V.v[k].s[i];
 #pragma unroll
             for (int ch = 0; ch < 3; ++ch) {
-                const size_t cidx = chunk_index(i, Vh, ch);
+                const size_t cidx = h16_chunk(i, Vh, ch);
                 float* yc = y + 8 * ch;
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     float x[8];
-                    unpack8(V.v[k].c[cidx], sv[k], x);
+                    h16_unpack8(V.v[k].c[cidx], sv[k], x);
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         yc[2 * q] -= cr[k] * x[2 * q] - ci[k] * x[2 * q + 1];
@@ -261,31 +224,31 @@ __global__ void k_h_gs_finalize(unsigned n, unsigned Vh, HTab V, HPtr w, uint4* 
         }
 #pragma unroll
         for (int q = 0; q < 24; ++q) y[q] *= inv;
-        store_site(vc, vs, Vh, i, y);
+        h16_store_site(vc, vs, Vh, i, y);
     }
 }
 
 // y += s * sum_k c_k V_k (c on device)
 template <int K>
-__global__ void k_h_multiaxpy(unsigned n, unsigned Vh, HTab V, const double* __restrict__ c, double s, uint4* yc,
+__global__ __launch_bounds__(256) void k_h_multiaxpy(unsigned n, unsigned Vh, HTab V, const double* __restrict__ c, double s, uint4* yc,
                               float* ys) {
     float cr[K], ci[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) { cr[k] = (float)(s * c[2 * k]); ci[k] = (float)(s * c[2 * k + 1]); }
     SITE_LOOP(i, n) {
         float y[24];
-        load_site(yc, ys, Vh, i, y);
+        h16_load_site(yc, ys, Vh, i, y);
         float sv[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) sv[k] = V.v[k].s[i];
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
-            const size_t cidx = chunk_index(i, Vh, ch);
+            const size_t cidx = h16_chunk(i, Vh, ch);
             float* yp = y + 8 * ch;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 float x[8];
-                unpack8(V.v[k].c[cidx], sv[k], x);
+                h16_unpack8(V.v[k].c[cidx], sv[k], x);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     yp[2 * q] += cr[k] * x[2 * q] - ci[k] * x[2 * q + 1];
@@ -293,7 +256,7 @@ __global__ void k_h_multiaxpy(unsigned n, unsigned Vh, HTab V, const double* __r
                 }
             }
         }
-        store_site(yc, ys, Vh, i, y);
+        h16_store_site(yc, ys, Vh, i, y);
     }
 }
 

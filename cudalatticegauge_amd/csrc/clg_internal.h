@@ -305,6 +305,45 @@ __device__ __forceinline__ float2 unpack_h2(unsigned u) {
     return __half22float2(h);
 }
 
+// FP16-storage Wilson sites as flat BLAS sees them: site i of a field of Vh-site parity blocks has
+// its three 16-byte chunks at chunk index (3 * (i / Vh) + k) * Vh + i % Vh and its scale at s[i].
+__device__ __forceinline__ size_t h16_chunk(unsigned i, unsigned Vh, int k) {
+    const unsigned b = i / Vh, h = i - b * Vh;
+    return (size_t)(3 * b + k) * Vh + h;
+}
+__device__ __forceinline__ void h16_unpack8(uint4 a, float s, float* v) {
+    float2 f;
+    f = unpack_h2(a.x); v[0] = f.x * s; v[1] = f.y * s;
+    f = unpack_h2(a.y); v[2] = f.x * s; v[3] = f.y * s;
+    f = unpack_h2(a.z); v[4] = f.x * s; v[5] = f.y * s;
+    f = unpack_h2(a.w); v[6] = f.x * s; v[7] = f.y * s;
+}
+// all 24 components of site i
+__device__ __forceinline__ void h16_load_site(const uint4* c, const float* s, unsigned Vh, unsigned i, float* v) {
+    const float sc = s[i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) h16_unpack8(c[h16_chunk(i, Vh, k)], sc, v + 8 * k);
+}
+// stores site i with a new scale; v is left holding the stored (rounded) values
+__device__ __forceinline__ void h16_store_site(uint4* c, float* s, unsigned Vh, unsigned i, float* v) {
+    float m = 0.f;
+#pragma unroll
+    for (int q = 0; q < 24; ++q) m = fmaxf(m, fabsf(v[q]));
+    const float inv = m > 0.f ? 1.0f / m : 0.f;
+    s[i] = m;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        uint4 a;
+        a.x = pack_h2(v[8 * k + 0] * inv, v[8 * k + 1] * inv);
+        a.y = pack_h2(v[8 * k + 2] * inv, v[8 * k + 3] * inv);
+        a.z = pack_h2(v[8 * k + 4] * inv, v[8 * k + 5] * inv);
+        a.w = pack_h2(v[8 * k + 6] * inv, v[8 * k + 7] * inv);
+        c[h16_chunk(i, Vh, k)] = a;
+        h16_unpack8(a, m, v + 8 * k);
+    }
+}
+
+
 // Wilson spinor: 12 complex numbers per site.
 //   F64: 12 planes of double2; F32: 6 planes of float4 (two complex); F16: 3 planes of 8
 //   halves plus a per-site float scale (max |component|), element index ne for the scale.

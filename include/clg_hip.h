@@ -388,9 +388,10 @@ int clg_hmc_trajectory(clg_context ctx, clg_field gauge, const clg_hmc_params* p
 /* ============================== batched operators and solves ============================== */
 /* Batched Wilson hop: out_i = a_i H in_i + b x_i for n <= 16 slots in one launch (slots may live on
  * lattices of different extents; x may be NULL).  Same parity/layout rules as clg_wilson_hop for
- * half fields; FULL fields give the hop over both parities.  FP64 spinors with FP64 links, or FP32
- * spinors with FP32 18-real or FP16 12-real links.  tile = right-hand sides per thread sharing one
- * link load (1, 2, 4 (FP32 only)); 0 lets the autotuner choose.  The reference applies D to one
+ * half fields; FULL fields give the hop over both parities.  FP64 spinors with FP64 links, FP32
+ * spinors with FP32 18-real or FP16 12-real links, or FP16-storage spinors with FP16 12-real links.
+ * tile = right-hand sides per thread sharing one link load (1, 2, 4 (not FP64)); 0 lets the
+ * autotuner choose.  The reference applies D to one
  * field per launch (CFieldFermionWilsonSquareSU3.cu:909-1014); this is its batched form. */
 int clg_wilson_hop_batch(clg_context ctx, int n, clg_field* out, clg_field* in, clg_field* gauge, int dagger,
                          const double* a, clg_field* x, double b, int tile);

@@ -63,6 +63,7 @@ def gauge_in(ctx, gauges, dims, prec, recon12=False):
 @pytest.mark.parametrize("prec,links,tile", [
     (clg.F64, "f64", 1), (clg.F64, "f64", 2),
     (clg.F32, "f32", 1), (clg.F32, "f32", 2), (clg.F32, "f32", 4), (clg.F32, "f16r12", 4), (clg.F32, "f32", 0),
+    (clg.F16, "f16r12", 1), (clg.F16, "f16r12", 2), (clg.F16, "f16r12", 4), (clg.F16, "f16r12", 0),
 ])
 @pytest.mark.parametrize("dagger", [False, True])
 def test_batched_half_hop_matches_single(ctx, gauges, prec, links, tile, dagger):
@@ -72,7 +73,8 @@ def test_batched_half_hop_matches_single(ctx, gauges, prec, links, tile, dagger)
     outs = [field(ctx, d, prec, clg.EVEN) for d in SHAPES]
     a = [0.3 + 0.1 * i for i in range(len(SHAPES))]
     clg.wilson_hop_batch(ctx, outs, ins, [gs[d] for d in SHAPES], a, dagger=dagger, xs=xs, b=-0.7, tile=tile)
-    tol = 1e-14 if prec == clg.F64 else 2e-6
+    # FP16 storage: both paths compute in FP32 and round the outputs the same way
+    tol = 1e-14 if prec == clg.F64 else 2e-6 if prec == clg.F32 else 2e-3
     for i, d in enumerate(SHAPES):
         ref = field(ctx, d, prec, clg.EVEN)
         clg.wilson_hop(ctx, ref, ins[i], gs[d], dagger=dagger, a=a[i], x=xs[i], b=-0.7)
@@ -92,7 +94,7 @@ def test_batched_full_hop_is_wilson_d(ctx, gauges, tile):
         assert rel(outs[i].download(), want) < 1e-12, (i, d)
 
 
-@pytest.mark.parametrize("inner", [clg.F32, clg.F64, clg.F16])
+@pytest.mark.parametrize("inner", [clg.F32, clg.F64, clg.F16, clg.INNER_F16_VECTORS])
 def test_solve_batch_matches_single_solves(ctx, gauges, inner):
     bs, xs = [], []
     for i, d in enumerate(SHAPES):
@@ -104,7 +106,8 @@ def test_solve_batch_matches_single_solves(ctx, gauges, inner):
         bs.append(b)
         xs.append(clg.Field(ctx, clg.WILSON, clg.F64, d, bc=BC))
     p = clg.solver_params(max_dim=12, restart=100, accuracy=1e-10, stop_rule=clg.STOP_TRUE_RESIDUAL, even_odd=1,
-                          inner_prec=inner, inner_accuracy=1e-4 if inner != clg.F64 else 1e-11, max_outer=40)
+                          inner_prec=inner, inner_accuracy={clg.F64: 1e-11, clg.INNER_F16_VECTORS: 1e-2}.get(inner, 1e-4),
+                          max_outer=40)
     st, reps = clg.solve_batch(ctx, p, xs, bs, [gauges[d] for d in SHAPES], KAPPA)
     print([(r.cycles, r.outer, r.true_residual) for r in reps])
     assert st == clg.OK
