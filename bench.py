@@ -381,9 +381,10 @@ def wilson_hmc_trajectory(ctx, clg):
     """One Wilson Nf=2 HMC trajectory at 32^3x16 with the reference's published set-up (plaquette
     beta 2.5, kappa 0.1355, nested force gradient 8 outer x 6 nested steps, exponential order 8),
     on the synthetic eps 0.5 gauge field: even-odd GMRES(12) force solves to |r|/|b| < 1e-8, in
-    plain FP64 and with FP32 inner defect correction; energies to 1e-10."""
+    plain FP64 and with FP32 or FP16-storage inner defect correction; energies to 1e-10."""
     out = {}
-    for tag, inner, inner_acc in (("fp64", clg.F64, 0.0), ("mixed_fp32", clg.F32, 1e-4)):
+    for tag, inner, inner_acc in (("fp64", clg.F64, 0.0), ("mixed_fp32", clg.F32, 1e-4),
+                                  ("mixed_fp16", clg.INNER_F16_VECTORS, 1e-2)):
         g = clg.Field(ctx, clg.GAUGE, clg.F64, DIMS)
         g.gauge_random(0.5, 1)
         phi = clg.Field(ctx, clg.WILSON, clg.F64, DIMS, bc=(1, 1, 1, -1))

@@ -1012,6 +1012,12 @@ __global__ __launch_bounds__(256) void k_bh_multiaxpy(const BTab* T, Act act, in
     }
 }
 
+// out[i * cnt + c] = ws[s_i * stride + off + c] for the systems s_i of act (one read-back per step)
+__global__ void k_bgather(const double* ws, int stride, int off, int cnt, Act act, double* out) {
+    const int i = blockIdx.x, c = threadIdx.x;
+    if (i < act.n && c < cnt) out[(size_t)i * cnt + c] = ws[(size_t)act.idx[i] * stride + off + c];
+}
+
 unsigned red_grid_b(clg_context ctx, long n, int nact) {
     long g = (n + kRedBlock * 4 - 1) / (kRedBlock * 4);
     const long cap = std::max<long>(8, (long)ctx->num_cu * 4 / std::max(1, nact));
@@ -1099,7 +1105,7 @@ struct BatchSolver {
         // device tables: BTab then the per-system workspace
         const size_t tab_doubles = (sizeof(BTab) + sizeof(double) - 1) / sizeof(double);
         double* w = nullptr;
-        CLG_TRY(workspace(ctx, tab_doubles + L.stride * kMaxSys + 64, &w));
+        CLG_TRY(workspace(ctx, tab_doubles + L.stride * kMaxSys + 256, &w));  // + read-back staging
         dtab = reinterpret_cast<BTab*>(w);
         ws = w + tab_doubles;
         BTab h;
@@ -1148,6 +1154,16 @@ struct BatchSolver {
         return a;
     }
     double* sc(int s) const { return ws + (size_t)s * L.stride + L.offS; }
+    // scalars sc(s)[off .. off+cnt) of the systems in act, one device-to-host copy
+    int fetch_sc(const std::vector<int>& act, int off, int cnt, std::vector<double>& out) {
+        Act A = make_act(act);
+        CLG_CHECK_ARG(cnt >= 1 && cnt <= 16, "bad read-back size");
+        double* stage = ws + L.stride * kMaxSys;  // staging after the per-system blocks (256 doubles)
+        k_bgather<<<A.n, 64, 0, ctx->stream>>>(ws, (int)L.stride, (int)L.offS + off, cnt, A, stage);
+        CLG_HIP(hipGetLastError());
+        out.resize((size_t)A.n * cnt);
+        return fetch_doubles(ctx, stage, A.n * cnt, out.data());
+    }
 
     template <int P, int K>
     int multidot(const Act& A, double* out_base, size_t ostride_off, double* wn, const double* cond) {
@@ -1268,10 +1284,11 @@ struct BatchSolver {
             std::vector<double> dev(ns * 2, 0.0);
             CLG_HIP(hipStreamSynchronize(ctx->stream));
             std::vector<int> next;
-            for (int s : act) {
-                double d[2];
-                CLG_TRY(fetch_doubles(ctx, sc(s) + 2, 2, d));
-                if (!(d[1] < tol[s]) && std::isfinite(d[0])) next.push_back(s);
+            std::vector<double> dv;
+            CLG_TRY(fetch_sc(act, 2, 2, dv));
+            for (size_t q = 0; q < act.size(); ++q) {
+                const int s = act[q];
+                if (!(dv[2 * q + 1] < tol[s]) && std::isfinite(dv[2 * q])) next.push_back(s);
             }
             act.swap(next);
         }
@@ -1347,10 +1364,11 @@ struct BatchSolver {
 #undef CALL
             CLG_HIP(hipGetLastError());
             std::vector<int> next;
-            for (int s : act) {
-                double d[2];
-                CLG_TRY(fetch_doubles(ctx, sc(s) + 2, 2, d));
-                if (!(d[1] < tol[s]) && std::isfinite(d[0])) next.push_back(s);
+            std::vector<double> dv;
+            CLG_TRY(fetch_sc(act, 2, 2, dv));
+            for (size_t q = 0; q < act.size(); ++q) {
+                const int s = act[q];
+                if (!(dv[2 * q + 1] < tol[s]) && std::isfinite(dv[2 * q])) next.push_back(s);
             }
             act.swap(next);
         }
@@ -1368,10 +1386,10 @@ struct BatchSolver {
         }
         // |b|^2 of the full systems
         for (int s = 0; s < ns; ++s) CLG_TRY(blas_norm2_dev(ctx, b[s], sc(s) + 5));
-        for (int s = 0; s < ns; ++s) {
-            double v;
-            CLG_TRY(fetch_doubles(ctx, sc(s) + 5, 1, &v));
-            bnorm[s] = std::sqrt(v);
+        {
+            std::vector<double> v;
+            CLG_TRY(fetch_sc(all, 5, 1, v));
+            for (int s = 0; s < ns; ++s) bnorm[s] = std::sqrt(v[s]);
         }
         // bhat = b_e + kappa H_eo b_o ; x_e = 0
         CLG_TRY(hop(all, bhat, bo, be, kappa, 1.0, false));
@@ -1391,9 +1409,11 @@ struct BatchSolver {
             }
             for (int s : act) CLG_TRY(blas_norm2_dev(ctx, r64[s], sc(s) + 6));
             std::vector<int> next;
-            for (int s : act) {
-                double r2;
-                CLG_TRY(fetch_doubles(ctx, sc(s) + 6, 1, &r2));
+            std::vector<double> rv;
+            CLG_TRY(fetch_sc(act, 6, 1, rv));
+            for (size_t q = 0; q < act.size(); ++q) {
+                const int s = act[q];
+                const double r2 = rv[q];
                 if (std::sqrt(r2) < p->accuracy * bnorm[s]) {
                     conv[s] = true;
                 } else if (std::isfinite(r2)) {
@@ -1426,11 +1446,9 @@ struct BatchSolver {
             CLG_TRY(blas_axpby(ctx, full[s], 1.0, 0.0, b[s], 1.0, 0.0));
             CLG_TRY(blas_norm2_dev(ctx, full[s], sc(s) + 7));
         }
-        for (int s = 0; s < ns; ++s) {
-            double r2;
-            CLG_TRY(fetch_doubles(ctx, sc(s) + 7, 1, &r2));
-            true_res[s] = bnorm[s] > 0 ? std::sqrt(r2) / bnorm[s] : std::sqrt(r2);
-        }
+        std::vector<double> r2;
+        CLG_TRY(fetch_sc(all, 7, 1, r2));
+        for (int s = 0; s < ns; ++s) true_res[s] = bnorm[s] > 0 ? std::sqrt(r2[s]) / bnorm[s] : std::sqrt(r2[s]);
         return CLG_OK;
     }
 };

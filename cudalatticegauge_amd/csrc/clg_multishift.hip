@@ -171,49 +171,70 @@ __global__ void k_ms_hess(double* H, int ld, int j, const double* c2, const doub
         H[2 * (j * ld + j + 1) + 1] = 0.0;
     }
 }
-// per-shift FOM solve (CMultiShiftFOM RotateHSolveY, CMultiShiftFOM.cpp:272-330): thread k copies
-// H (m x m, column-major, ld = m+1) into its scratch, adds c_k on the diagonal, applies m-1
-// Givens rotations to g = beta0_k e1, back-substitutes, then beta0_k <- -h_{m,m-1} y_{m-1}.
-__global__ void k_msfom_lsq(int ns, int m, const double* __restrict__ H, const double* __restrict__ shifts,
-                            double* Hc_all, double* g_all, double* y, double* beta0, double* errs) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+// per-shift FOM solve (CMultiShiftFOM RotateHSolveY, CMultiShiftFOM.cpp:272-330): workgroup k (one
+// wave) copies H (m x m, column-major, ld = m+1) into LDS, adds c_k on the diagonal, applies the m-1
+// Givens rotations to g = beta0_k e1 (each rotation spread over the columns it touches), back-
+// substitutes with wave-wide sums, then beta0_k <- -h_{m,m-1} y_{m-1}.
+__global__ __launch_bounds__(64) void k_msfom_lsq(int ns, int m, const double* __restrict__ H,
+                                                  const double* __restrict__ shifts, double* y, double* beta0,
+                                                  double* errs) {
+    const int k = blockIdx.x;
     if (k >= ns) return;
-    const int ld = m + 1;
-    cx* Hc = reinterpret_cast<cx*>(Hc_all + (size_t)k * 2 * m * m);
-    cx* G = reinterpret_cast<cx*>(g_all + (size_t)k * 2 * m);
-    cx* Y = reinterpret_cast<cx*>(y + (size_t)k * 2 * m);
+    extern __shared__ double lds[];
+    cx* Hc = reinterpret_cast<cx*>(lds);  // m x m, column-major
+    __shared__ cx G[64], Y[64], rot[2];
+    const int t = threadIdx.x, ld = m + 1;
     const cx* Hs = reinterpret_cast<const cx*>(H);
-    auto h = [&](int r, int col) -> cx& { return Hc[col * m + r]; };
-    for (int col = 0; col < m; ++col)
-        for (int r = 0; r < m; ++r) h(r, col) = Hs[col * ld + r];
-    for (int d = 0; d < m; ++d) h(d, d) = cadd(h(d, d), cx{shifts[2 * k], shifts[2 * k + 1]});
-    for (int d = 0; d < m; ++d) G[d] = cx{0.0, 0.0};
-    G[0] = cx{beta0[2 * k], beta0[2 * k + 1]};
+    for (int e = t; e < m * m; e += 64) {
+        const int col = e / m, r = e - col * m;
+        cx v = Hs[col * ld + r];
+        if (r == col) v = cadd(v, cx{shifts[2 * k], shifts[2 * k + 1]});
+        Hc[e] = v;
+    }
+    for (int d = t; d < m; d += 64) G[d] = cx{0.0, 0.0};
+    __syncthreads();
+    if (t == 0) G[0] = cx{beta0[2 * k], beta0[2 * k + 1]};
     for (int i = 0; i < m - 1; ++i) {
-        const cx hii = h(i, i), hi1 = h(i + 1, i);
-        const double den = 1.0 / sqrt(hii.x * hii.x + hii.y * hii.y + hi1.x * hi1.x + hi1.y * hi1.y);
-        const cx c = cscale(hii, den), s = cscale(hi1, den);
-        const cx ch = cconj(c), sh = cconj(s);
-        for (int j = i; j < m; ++j) {
-            const cx hij = h(i, j), h1j = h(i + 1, j);
-            h(i, j) = cadd(cmul(ch, hij), cmul(sh, h1j));
-            h(i + 1, j) = csub(cmul(c, h1j), cmul(s, hij));
+        __syncthreads();
+        if (t == 0) {
+            const cx hii = Hc[i * m + i], hi1 = Hc[i * m + i + 1];
+            const double den = 1.0 / sqrt(hii.x * hii.x + hii.y * hii.y + hi1.x * hi1.x + hi1.y * hi1.y);
+            rot[0] = cscale(hii, den);
+            rot[1] = cscale(hi1, den);
+            const cx mg = cx{-G[i].x, -G[i].y};
+            G[i] = cmul(cconj(rot[0]), G[i]);
+            G[i + 1] = cmul(rot[1], mg);
         }
-        const cx mg = cx{-G[i].x, -G[i].y};
-        G[i] = cmul(ch, G[i]);
-        G[i + 1] = cmul(s, mg);
+        __syncthreads();
+        const cx c = rot[0], sn = rot[1], ch = cconj(c), sh = cconj(sn);
+        for (int jj = i + t; jj < m; jj += 64) {
+            const cx hij = Hc[jj * m + i], h1j = Hc[jj * m + i + 1];
+            Hc[jj * m + i] = cadd(cmul(ch, hij), cmul(sh, h1j));
+            Hc[jj * m + i + 1] = csub(cmul(c, h1j), cmul(sn, hij));
+        }
     }
+    __syncthreads();
     for (int i = m - 1; i >= 0; --i) {
-        cx acc = G[i];
-        for (int j = i + 1; j < m; ++j) acc = csub(acc, cmul(h(i, j), Y[j]));
-        const cx d = h(i, i);
-        const double dd = d.x * d.x + d.y * d.y;
-        Y[i] = cx{(acc.x * d.x + acc.y * d.y) / dd, (acc.y * d.x - acc.x * d.y) / dd};
+        cx part{0.0, 0.0};
+        for (int jj = i + 1 + t; jj < m; jj += 64) part = cadd(part, cmul(Hc[jj * m + i], Y[jj]));
+        part.x = wave_sum(part.x);
+        part.y = wave_sum(part.y);
+        if (t == 0) {
+            const cx acc = csub(G[i], part);
+            const cx d = Hc[i * m + i];
+            const double dd = d.x * d.x + d.y * d.y;
+            Y[i] = cx{(acc.x * d.x + acc.y * d.y) / dd, (acc.y * d.x - acc.x * d.y) / dd};
+        }
+        __syncthreads();
     }
-    const double hm = Hs[(m - 1) * ld + m].x;  // h_{m, m-1}, real and positive
-    beta0[2 * k] = -hm * Y[m - 1].x;
-    beta0[2 * k + 1] = -hm * Y[m - 1].y;
-    errs[k] = sqrt(beta0[2 * k] * beta0[2 * k] + beta0[2 * k + 1] * beta0[2 * k + 1]);
+    cx* Yk = reinterpret_cast<cx*>(y + (size_t)k * 2 * m);
+    for (int d = t; d < m; d += 64) Yk[d] = Y[d];
+    if (t == 0) {
+        const double hm = Hs[(m - 1) * ld + m].x;  // h_{m, m-1}, real and positive
+        beta0[2 * k] = -hm * Y[m - 1].x;
+        beta0[2 * k + 1] = -hm * Y[m - 1].y;
+        errs[k] = sqrt(beta0[2 * k] * beta0[2 * k] + beta0[2 * k + 1] * beta0[2 * k + 1]);
+    }
 }
 
 unsigned ew_grid(clg_context ctx, long n) {
@@ -252,7 +273,7 @@ struct OpRef {
 int msfom(clg_context ctx, OpRef& A, std::vector<clg_field_s*>& X, const std::vector<cx>& shifts,
           const clg_field_s* b, int m, int restart, double accuracy, double blen, int& cycles, double& maxerr) {
     const int ns = (int)shifts.size();
-    CLG_CHECK_ARG(m >= 2 && m <= 63, "MS-FOM MaxDim must be in [2, 63]");
+    CLG_CHECK_ARG(m >= 2 && m <= 60, "MS-FOM MaxDim must be in [2, 60]");  // the shifted Hessenberg sits in LDS
     // V[0..m-1] basis, V[m] = w, vnext = v_m seeding the next cycle
     std::vector<clg_field_s*> V(m + 1);
     for (int k = 0; k <= m; ++k) CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 2000 + k, &V[k]));
@@ -260,9 +281,8 @@ int msfom(clg_context ctx, OpRef& A, std::vector<clg_field_s*>& X, const std::ve
     CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 59, &vnext));
     clg_field_s* w = V[m];
     const int ld = m + 1;
-    const size_t oH = 0, oHc = oH + 2 * (size_t)ld * m, oG = oHc + 2 * (size_t)m * m * ns, oY = oG + 2 * (size_t)m * ns,
-                 oB = oY + 2 * (size_t)m * ns, oE = oB + 2 * (size_t)ns, oSh = oE + ns, oS = oSh + 2 * (size_t)ns,
-                 oC = oS + 16, tot = oC + 2 * (size_t)m;
+    const size_t oH = 0, oY = oH + 2 * (size_t)ld * m, oB = oY + 2 * (size_t)m * ns, oE = oB + 2 * (size_t)ns,
+                 oSh = oE + ns, oS = oSh + 2 * (size_t)ns, oC = oS + 16, tot = oC + 2 * (size_t)m;
     double* ws = nullptr;
     CLG_TRY(workspace(ctx, tot, &ws));
     double* H = ws + oH;
@@ -304,8 +324,7 @@ int msfom(clg_context ctx, OpRef& A, std::vector<clg_field_s*>& X, const std::ve
         }
         CLG_HIP(hipMemcpyAsync(ws + oSh, bsh.data(), sizeof(double) * 2 * na, hipMemcpyHostToDevice, ctx->stream));
         CLG_HIP(hipMemcpyAsync(ws + oB, bb.data(), sizeof(double) * 2 * na, hipMemcpyHostToDevice, ctx->stream));
-        k_msfom_lsq<<<(na + 63) / 64, 64, 0, ctx->stream>>>(na, m, H, ws + oSh, ws + oHc, ws + oG, ws + oY, ws + oB,
-                                                             ws + oE);
+        k_msfom_lsq<<<na, 64, sizeof(cx) * (size_t)m * m, ctx->stream>>>(na, m, H, ws + oSh, ws + oY, ws + oB, ws + oE);
         CLG_HIP(hipGetLastError());
         ShiftPtrs sp;
         VPtrs vp;

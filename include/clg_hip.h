@@ -250,7 +250,7 @@ typedef enum {
 
 /* CMultiShiftSolver::Solve (CMultiShiftSolver.h:38-47): xs[k] = (A + c_k)^-1 b for all k, with
  * A = op of the params.kind fermion and c_k = shifts[2k] + i shifts[2k+1].  CLG_MS_FOM follows
- * CMultiShiftFOM.cpp:97-270 (params max_dim, restart, accuracy, absolute_accuracy), CLG_MS_BICGSTAB
+ * CMultiShiftFOM.cpp:97-270 (params max_dim <= 60, restart, accuracy, absolute_accuracy), CLG_MS_BICGSTAB
  * follows CMultiShiftBiCGStab.cpp:73-270 (max_step, deviation_step, accuracy, absolute_accuracy).
  * prec CLG_F32 runs the whole solve on FP32 copies (solutions written back as FP64).  The xs must
  * not alias b.  Returns CLG_NOT_CONVERGED with the iterates written when the budget runs out. */
