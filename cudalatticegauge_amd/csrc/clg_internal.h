@@ -295,6 +295,29 @@ __device__ __forceinline__ void stp(char* base, int Vh, int k, unsigned e, T v) 
     *reinterpret_cast<T*>(base + (size_t)k * (size_t)Vh * sizeof(T) + (unsigned)(e * (unsigned)sizeof(T))) = v;
 }
 
+// Non-temporal form of ldp (the "nt" cache policy: the line is marked for early eviction), for data
+// a kernel reads exactly once, so it does not displace data that is read again (neighbour spinors).
+template <typename T> struct NtVec;
+template <> struct NtVec<double2> { typedef double V __attribute__((ext_vector_type(2))); };
+template <> struct NtVec<float2> { typedef float V __attribute__((ext_vector_type(2))); };
+template <> struct NtVec<float4> { typedef float V __attribute__((ext_vector_type(4))); };
+template <> struct NtVec<uint2> { typedef unsigned V __attribute__((ext_vector_type(2))); };
+template <> struct NtVec<unsigned> { typedef unsigned V; };
+template <typename T, bool NT>
+__device__ __forceinline__ T ldq(const char* base, int Vh, int k, unsigned e) {
+    if constexpr (NT) {
+        typedef typename NtVec<T>::V V;
+        const V* p = reinterpret_cast<const V*>(base + (size_t)k * (size_t)Vh * sizeof(T) +
+                                                (unsigned)(e * (unsigned)sizeof(T)));
+        const V v = __builtin_nontemporal_load(p);
+        T out;
+        __builtin_memcpy(&out, &v, sizeof(T));
+        return out;
+    } else {
+        return ldp<T>(base, Vh, k, e);
+    }
+}
+
 // FP16 storage helpers
 __device__ __forceinline__ unsigned pack_h2(float a, float b) {
     __half2 h = __floats2half2_rn(a, b);
@@ -484,10 +507,10 @@ __device__ __forceinline__ void rebuild_row2(C2<R>* m) {
 template <int PREC, int RECON> struct LinkIO;
 template <> struct LinkIO<CLG_F64, 18> {
     static constexpr int NPL = 9;
-    template <typename R>
+    template <bool NT = false, typename R>
     __device__ __forceinline__ static void load(const char* b, int Vh, unsigned e, C2<R>* m) {
 #pragma unroll
-        for (int k = 0; k < 9; ++k) { const double2 a = ldp<double2>(b, Vh, k, e); m[k] = C2<R>{(R)a.x, (R)a.y}; }
+        for (int k = 0; k < 9; ++k) { const double2 a = ldq<double2, NT>(b, Vh, k, e); m[k] = C2<R>{(R)a.x, (R)a.y}; }
     }
     __device__ __forceinline__ static void store(char* b, int Vh, unsigned e, const cd* m) {
 #pragma unroll
@@ -496,10 +519,10 @@ template <> struct LinkIO<CLG_F64, 18> {
 };
 template <> struct LinkIO<CLG_F64, 12> {
     static constexpr int NPL = 6;
-    template <typename R>
+    template <bool NT = false, typename R>
     __device__ __forceinline__ static void load(const char* b, int Vh, unsigned e, C2<R>* m) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) { const double2 a = ldp<double2>(b, Vh, k, e); m[k] = C2<R>{(R)a.x, (R)a.y}; }
+        for (int k = 0; k < 6; ++k) { const double2 a = ldq<double2, NT>(b, Vh, k, e); m[k] = C2<R>{(R)a.x, (R)a.y}; }
         rebuild_row2(m);
     }
     __device__ __forceinline__ static void store(char* b, int Vh, unsigned e, const cd* m) {
@@ -509,10 +532,10 @@ template <> struct LinkIO<CLG_F64, 12> {
 };
 template <> struct LinkIO<CLG_F32, 18> {
     static constexpr int NPL = 9;
-    template <typename R>
+    template <bool NT = false, typename R>
     __device__ __forceinline__ static void load(const char* b, int Vh, unsigned e, C2<R>* m) {
 #pragma unroll
-        for (int k = 0; k < 9; ++k) { const float2 a = ldp<float2>(b, Vh, k, e); m[k] = C2<R>{(R)a.x, (R)a.y}; }
+        for (int k = 0; k < 9; ++k) { const float2 a = ldq<float2, NT>(b, Vh, k, e); m[k] = C2<R>{(R)a.x, (R)a.y}; }
     }
     __device__ __forceinline__ static void store(char* b, int Vh, unsigned e, const cd* m) {
 #pragma unroll
@@ -521,11 +544,11 @@ template <> struct LinkIO<CLG_F32, 18> {
 };
 template <> struct LinkIO<CLG_F32, 12> {
     static constexpr int NPL = 3;
-    template <typename R>
+    template <bool NT = false, typename R>
     __device__ __forceinline__ static void load(const char* b, int Vh, unsigned e, C2<R>* m) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const float4 a = ldp<float4>(b, Vh, k, e);
+            const float4 a = ldq<float4, NT>(b, Vh, k, e);
             m[2 * k] = C2<R>{(R)a.x, (R)a.y};
             m[2 * k + 1] = C2<R>{(R)a.z, (R)a.w};
         }
@@ -540,10 +563,10 @@ template <> struct LinkIO<CLG_F32, 12> {
 };
 template <> struct LinkIO<CLG_F16, 18> {
     static constexpr int NPL = 9;
-    template <typename R>
+    template <bool NT = false, typename R>
     __device__ __forceinline__ static void load(const char* b, int Vh, unsigned e, C2<R>* m) {
 #pragma unroll
-        for (int k = 0; k < 9; ++k) { const float2 f = unpack_h2(ldp<unsigned>(b, Vh, k, e)); m[k] = C2<R>{(R)f.x, (R)f.y}; }
+        for (int k = 0; k < 9; ++k) { const float2 f = unpack_h2(ldq<unsigned, NT>(b, Vh, k, e)); m[k] = C2<R>{(R)f.x, (R)f.y}; }
     }
     __device__ __forceinline__ static void store(char* b, int Vh, unsigned e, const cd* m) {
 #pragma unroll
@@ -552,11 +575,11 @@ template <> struct LinkIO<CLG_F16, 18> {
 };
 template <> struct LinkIO<CLG_F16, 12> {
     static constexpr int NPL = 3;
-    template <typename R>
+    template <bool NT = false, typename R>
     __device__ __forceinline__ static void load(const char* b, int Vh, unsigned e, C2<R>* m) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const uint2 a = ldp<uint2>(b, Vh, k, e);
+            const uint2 a = ldq<uint2, NT>(b, Vh, k, e);
             const float2 f0 = unpack_h2(a.x), f1 = unpack_h2(a.y);
             m[2 * k] = C2<R>{(R)f0.x, (R)f0.y};
             m[2 * k + 1] = C2<R>{(R)f1.x, (R)f1.y};

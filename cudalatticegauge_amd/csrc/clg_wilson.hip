@@ -56,12 +56,12 @@ __device__ __forceinline__ void hop_mu(const HopArgs& a, const Site& s, int p, i
 #pragma unroll
             for (int k = 0; k < 6; ++k) hh[k] = cmk(-hh[k].x, -hh[k].y);
         }
-        LIO::load(a.gauge, Vh, (unsigned)(p * 4 + MU) * LIO::NPL * Vh + h, u);
+        LIO::template load<(PM >= 3)>(a.gauge, Vh, (unsigned)(p * 4 + MU) * LIO::NPL * Vh + h, u);
         su3_mv(u, hh, uh);
         su3_mv(u, hh + 3, uh + 3);
         reconstruct_add<MU, DAG ? 0 : 2>(acc, uh);
     }
-    if constexpr (PM == 1) pin_acc(acc);
+    if constexpr (PM == 1 || PM == 3) pin_acc(acc);
     // backward: (1 + g_mu) U_mu(x-mu)^dag psi(x-mu)  [dagger: (1 - g_mu)]
     {
         const Nbr n = bwd_nbr<MU>(a.g, s, p);
@@ -72,7 +72,7 @@ __device__ __forceinline__ void hop_mu(const HopArgs& a, const Site& s, int p, i
 #pragma unroll
             for (int k = 0; k < 6; ++k) hh[k] = cmk(-hh[k].x, -hh[k].y);
         }
-        LIO::load(a.gauge, Vh, (unsigned)(n.p * 4 + MU) * LIO::NPL * Vh + n.h, u);
+        LIO::template load<(PM >= 3)>(a.gauge, Vh, (unsigned)(n.p * 4 + MU) * LIO::NPL * Vh + n.h, u);
         su3_mdv(u, hh, uh);
         su3_mdv(u, hh + 3, uh + 3);
         reconstruct_add<MU, DAG ? 2 : 0>(acc, uh);
@@ -115,7 +115,8 @@ __global__ __launch_bounds__(256) void k_wilson_hop(HopArgs a) {
 }
 
 struct HopCfg {
-    int pm;     // pin mode
+    int pm;     // pin mode: 1 after each half of a direction, 2 after each direction; 3 / 4 the same
+                // with non-temporal link loads (links read once per launch stream past the caches)
     int block;  // threads per workgroup
     int xcd;    // 0: hardware order, 8: XCD-contiguous slabs (+ parity interleave for full D)
 };
@@ -137,7 +138,9 @@ void launch_pm(const HopArgs& a0, bool dag, bool hasx, hipStream_t st, int block
 
 template <typename R, int SP, int GP, int RECON>
 int launch_t(const HopArgs& a, bool dag, bool hasx, hipStream_t st, HopCfg c) {
-    if (c.pm == 2) launch_pm<R, SP, GP, RECON, 2>(a, dag, hasx, st, c.block, c.xcd);
+    if (c.pm == 4) launch_pm<R, SP, GP, RECON, 4>(a, dag, hasx, st, c.block, c.xcd);
+    else if (c.pm == 3) launch_pm<R, SP, GP, RECON, 3>(a, dag, hasx, st, c.block, c.xcd);
+    else if (c.pm == 2) launch_pm<R, SP, GP, RECON, 2>(a, dag, hasx, st, c.block, c.xcd);
     else launch_pm<R, SP, GP, RECON, 1>(a, dag, hasx, st, c.block, c.xcd);
     CLG_HIP(hipGetLastError());
     return CLG_OK;
@@ -177,7 +180,7 @@ std::mutex& tune_mutex() {
     return m;
 }
 bool valid_cfg(int pm, int blk, int xcd) {
-    return (pm == 1 || pm == 2) && (blk == 64 || blk == 128 || blk == 256) && (xcd == 0 || xcd == 8);
+    return pm >= 1 && pm <= 4 && (blk == 64 || blk == 128 || blk == 256) && (xcd == 0 || xcd == 8);
 }
 // process-wide overrides set through clg_autotune_enable / clg_autotune_force
 std::atomic<int> g_tune_mode{-1};  // -1: CLG_TUNE environment (default on), 0 off, 1 on
@@ -219,7 +222,8 @@ int launch_hop(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool 
     (void)hipStreamIsCapturing(st, &cs);
     if (need_tune && tuning_enabled() && !aliased && cs == hipStreamCaptureStatusNone && (long)a.nblk * a.g.Vh >= 4096) {
         const HopCfg cands[] = {{1, 64, 0}, {1, 128, 0}, {1, 256, 0}, {2, 64, 0}, {2, 128, 0}, {2, 256, 0},
-                                {1, 64, 8}, {1, 128, 8}, {1, 256, 8}, {2, 64, 8}, {2, 128, 8}, {2, 256, 8}};
+                                {1, 64, 8}, {1, 128, 8}, {1, 256, 8}, {2, 64, 8}, {2, 128, 8}, {2, 256, 8},
+                                {3, 64, 8}, {3, 128, 8}, {3, 256, 8}, {4, 64, 8}, {4, 128, 8}, {4, 256, 8}};
         hipEvent_t e0, e1;
         CLG_HIP(hipEventCreate(&e0));
         CLG_HIP(hipEventCreate(&e1));
@@ -452,7 +456,7 @@ int clg_autotune_force(int pin_mode, int block, int xcd) {
         g_forced.store(0);
         return CLG_OK;
     }
-    CLG_CHECK_ARG(valid_cfg(pin_mode, block, xcd), "pin_mode 1|2, block 64|128|256, xcd 0|8");
+    CLG_CHECK_ARG(valid_cfg(pin_mode, block, xcd), "pin_mode 1..4, block 64|128|256, xcd 0|8");
     g_forced.store(pin_mode | block << 4 | xcd << 16);
     return CLG_OK;
 }

@@ -411,7 +411,8 @@ int clg_solve_batch(clg_context ctx, const clg_solver_params* params, int n, clg
  * (_HC_DecompX.., CCommonData.h:72-91); these calls inspect and steer the choice. */
 /* mode 1 tune, 0 use defaults, -1 follow the CLG_TUNE environment variable (default on) */
 int clg_autotune_enable(int mode);
-/* fix every single-system hop launch: pin_mode 1|2, block 64|128|256, xcd 0|8; pin_mode 0 clears */
+/* fix every single-system hop launch: pin_mode 1|2 (register pinning per half / per direction), 3|4 (the
+ * same with non-temporal link loads), block 64|128|256, xcd 0|8; pin_mode 0 clears */
 int clg_autotune_force(int pin_mode, int block, int xcd);
 /* forget every tuned shape (the next call of each shape tunes again) */
 int clg_autotune_reset(void);
