@@ -47,3 +47,21 @@ def test_solver_defaults_follow_reference_yaml_defaults():
     p = clg.solver_params()
     assert p.max_dim == 20 and p.restart == 3 and p.accuracy == pytest.approx(1e-8)
     assert p.stop_rule == clg.STOP_REFERENCE and p.inner_prec == clg.F64
+
+
+def test_tuner_and_graph_entry_points_check_arguments_without_a_device():
+    L = clg.lib()
+    assert L.clg_autotune_enable(2) == 2 and L.clg_autotune_enable(-2) == 2
+    assert L.clg_autotune_force(5, 256, 8) == 2 and L.clg_autotune_force(1, 96, 8) == 2
+    assert L.clg_autotune_force(1, 128, 3) == 2
+    assert L.clg_autotune_force(0, 0, 0) == 0   # clearing needs no device
+    assert L.clg_autotune_enable(-1) == 0
+    assert L.clg_autotune_count() >= 0
+    buf = C.create_string_buffer(64)
+    assert L.clg_autotune_entry(-1, buf, 64) == 2
+    assert L.clg_graph_begin(None) == 2
+    assert L.clg_graph_launch(None) == 2
+    assert L.clg_graph_nodes(None) == -1
+    assert L.clg_graph_destroy(None) == 0
+    assert L.clg_multidot(None, 1, None, None, None) == 2
+    assert L.clg_vector_multiply_matrix(None, 1, None, 0, None, None) == 2
