@@ -77,12 +77,8 @@ __device__ __forceinline__ void bhop_mu(const BHopArgs& a, const BSeg& sg, const
         for (int r = 0; r < NR; ++r) {
             if (r < sg.nr) {
                 C2<R> psi[12], hh[6], uh[6];
-                SIO::load(a.in[sg.slot0 + r], a.inn[sg.slot0 + r], Vh, pe * SIO::NPL * Vh + n.h, pe * Vh + n.h, psi);
-                project<MU, DAG ? 0 : 2>(psi, hh);
-                if (n.sign < 0) {
-#pragma unroll
-                    for (int k = 0; k < 6; ++k) hh[k] = cmk(-hh[k].x, -hh[k].y);
-                }
+                load_projected<SIO, MU, DAG ? 0 : 2>(a.in[sg.slot0 + r], a.inn[sg.slot0 + r], Vh, pe * SIO::NPL * Vh + n.h,
+                                              pe * Vh + n.h, n.sign, psi, hh);
                 su3_mv(u, hh, uh);
                 su3_mv(u, hh + 3, uh + 3);
                 reconstruct_add<MU, DAG ? 0 : 2>(acc[r], uh);
@@ -99,12 +95,8 @@ __device__ __forceinline__ void bhop_mu(const BHopArgs& a, const BSeg& sg, const
         for (int r = 0; r < NR; ++r) {
             if (r < sg.nr) {
                 C2<R> psi[12], hh[6], uh[6];
-                SIO::load(a.in[sg.slot0 + r], a.inn[sg.slot0 + r], Vh, pe * SIO::NPL * Vh + n.h, pe * Vh + n.h, psi);
-                project<MU, DAG ? 2 : 0>(psi, hh);
-                if (n.sign < 0) {
-#pragma unroll
-                    for (int k = 0; k < 6; ++k) hh[k] = cmk(-hh[k].x, -hh[k].y);
-                }
+                load_projected<SIO, MU, DAG ? 2 : 0>(a.in[sg.slot0 + r], a.inn[sg.slot0 + r], Vh, pe * SIO::NPL * Vh + n.h,
+                                              pe * Vh + n.h, n.sign, psi, hh);
                 su3_mdv(u, hh, uh);
                 su3_mdv(u, hh + 3, uh + 3);
                 reconstruct_add<MU, DAG ? 2 : 0>(acc[r], uh);

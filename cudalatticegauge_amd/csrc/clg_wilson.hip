@@ -50,12 +50,7 @@ __device__ __forceinline__ void hop_mu(const HopArgs& a, const Site& s, int p, i
     {
         const Nbr n = fwd_nbr<MU>(a.g, s, p);
         const unsigned pe = a.in_full ? (unsigned)n.p : 0u;
-        SIO::load(a.in, a.inn, Vh, pe * SIO::NPL * Vh + n.h, pe * Vh + n.h, psi);
-        project<MU, DAG ? 0 : 2>(psi, hh);
-        if (n.sign < 0) {
-#pragma unroll
-            for (int k = 0; k < 6; ++k) hh[k] = cmk(-hh[k].x, -hh[k].y);
-        }
+        load_projected<SIO, MU, DAG ? 0 : 2>(a.in, a.inn, Vh, pe * SIO::NPL * Vh + n.h, pe * Vh + n.h, n.sign, psi, hh);
         LIO::template load<(PM >= 3)>(a.gauge, Vh, (unsigned)(p * 4 + MU) * LIO::NPL * Vh + h, u);
         su3_mv(u, hh, uh);
         su3_mv(u, hh + 3, uh + 3);
@@ -66,12 +61,7 @@ __device__ __forceinline__ void hop_mu(const HopArgs& a, const Site& s, int p, i
     {
         const Nbr n = bwd_nbr<MU>(a.g, s, p);
         const unsigned pe = a.in_full ? (unsigned)n.p : 0u;
-        SIO::load(a.in, a.inn, Vh, pe * SIO::NPL * Vh + n.h, pe * Vh + n.h, psi);
-        project<MU, DAG ? 2 : 0>(psi, hh);
-        if (n.sign < 0) {
-#pragma unroll
-            for (int k = 0; k < 6; ++k) hh[k] = cmk(-hh[k].x, -hh[k].y);
-        }
+        load_projected<SIO, MU, DAG ? 2 : 0>(a.in, a.inn, Vh, pe * SIO::NPL * Vh + n.h, pe * Vh + n.h, n.sign, psi, hh);
         LIO::template load<(PM >= 3)>(a.gauge, Vh, (unsigned)(n.p * 4 + MU) * LIO::NPL * Vh + n.h, u);
         su3_mdv(u, hh, uh);
         su3_mdv(u, hh + 3, uh + 3);

@@ -28,6 +28,27 @@ __device__ __forceinline__ void project(const C2<R>* psi, C2<R>* hh) {
         hh[3 + c] = cadd(psi[3 + c], mulz4<G<MU, 1>::z + SG>(psi[G<MU, 1>::idx * 3 + c]));
     }
 }
+// neighbour spinor -> spin-projected half-spinor with the boundary sign (and, for FP16 storage,
+// the site scale) applied to the 6 projected components rather than the 12 loaded ones
+template <class SIO, int MU, int S, typename R>
+__device__ __forceinline__ void load_projected(const char* b, const float* nb, int Vh, unsigned e, unsigned ne, int sign,
+                                               C2<R>* psi, C2<R>* hh) {
+    if constexpr (SIO::SCALED) {
+        const float sc = SIO::load_unscaled(b, nb, Vh, e, ne, psi);
+        project<MU, S>(psi, hh);
+        const R f = (R)(sign < 0 ? -sc : sc);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) hh[k] = cmk(f * hh[k].x, f * hh[k].y);
+    } else {
+        SIO::load(b, nb, Vh, e, ne, psi);
+        project<MU, S>(psi, hh);
+        if (sign < 0) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) hh[k] = cmk(-hh[k].x, -hh[k].y);
+        }
+    }
+}
+
 template <int MU, int SG, typename R>
 __device__ __forceinline__ void reconstruct_add(C2<R>* acc, const C2<R>* uh) {
 #pragma unroll
