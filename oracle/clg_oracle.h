@@ -16,7 +16,7 @@
  * Every function returns 0 on success and a negative number on bad input.
  * Functions whose results no reference-held artefact pins are pinned by the
  * invariant tests in tests/test_oracle_*.py (plane waves, gamma5-hermiticity,
- * gauge covariance, dense-matrix brute force); see the table in DESIGN.md.
+ * gauge covariance, dense-matrix brute force, finite differences of the actions).
  */
 #ifndef CLG_ORACLE_H
 #define CLG_ORACLE_H
