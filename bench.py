@@ -32,6 +32,7 @@ BYTES_D = {8: 384 * 8, 4: 384 * 4}          # full Wilson D, 18-real links
 BYTES_D12 = {8: 336 * 8, 4: 336 * 4}        # 12-real links
 BYTES_SCHUR = {8: 744 * 8, 4: 744 * 4}      # per even site
 FLOPS_D = 1368
+BYTES_KS = {8: 204 * 8, 4: 204 * 4}         # full staggered D: 8 neighbour vectors, 8 links, site in/out
 
 
 def log(msg):
@@ -127,6 +128,28 @@ def main():
         extra[name + "_GBs"] = round(nb * (V // 2) / (ms * 1e-3) / 1e9, 1)
         log(f"{name}: {ms * 1e3:.1f} us  {extra[name + '_GBs']:.0f} GB/s")
         del e_in, e_out, g, s, o
+
+    # ---------------- staggered full D (_kernelDFermionKS) at this lattice and at the RHMC lattice
+    for tag, dims, prec in (("ks_fp64_D", DIMS, clg.F64), ("ks_fp32_D", DIMS, clg.F32),
+                            ("ks_fp64_D_24x24x24x8", (24, 24, 24, 8), clg.F64)):
+        g64 = clg.Field(ctx, clg.GAUGE, clg.F64, dims)
+        g64.gauge_random(0.5, 1)
+        s64 = clg.Field(ctx, clg.KS, clg.F64, dims, bc=(1, 1, 1, -1))
+        s64.gaussian(2)
+        if prec == clg.F64:
+            g, s = g64, s64
+        else:
+            g = clg.Field(ctx, clg.GAUGE, prec, dims)
+            g.copy_from(g64)
+            s = clg.Field(ctx, clg.KS, prec, dims, bc=(1, 1, 1, -1))
+            s.copy_from(s64)
+        o = clg.Field(ctx, clg.KS, prec, dims, bc=(1, 1, 1, -1))
+        ms = clg.time_ks(ctx, o, s, g, 0.02, args.steps)
+        vk = int(np.prod(dims))
+        extra[tag + "_us"] = round(ms * 1e3, 2)
+        extra[tag + "_GBs"] = round(BYTES_KS[8 if prec == clg.F64 else 4] * vk / (ms * 1e-3) / 1e9, 1)
+        log(f"{tag}: {ms * 1e3:.1f} us  {extra[tag + '_GBs']:.0f} GB/s")
+        del g, s, o, g64, s64
 
     # ---------------- solves to |r|/|b| < 1e-10 (EO-preconditioned GMRES(12))
     if not args.no_solve:

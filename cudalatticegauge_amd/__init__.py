@@ -39,7 +39,7 @@ EXPORTED = [
     "clg_field_parity_view", "clg_field_host_doubles", "clg_field_bytes", "clg_gauge_random",
     "clg_gauge_unit", "clg_field_gaussian", "clg_field_point_source", "clg_wilson_apply",
     "clg_wilson_hop", "clg_wilson_schur", "clg_ks_apply", "clg_axpy", "clg_scale", "clg_dot",
-    "clg_norm2", "clg_solve", "clg_solver_params_default", "clg_time_wilson",
+    "clg_norm2", "clg_solve", "clg_solver_params_default", "clg_time_wilson", "clg_time_ks",
     "clg_multishift_solve", "clg_rational_apply", "clg_wilson_force_kernel", "clg_ks_force_kernel",
     "clg_wilson_force", "clg_ks_force", "clg_wilson_pf_energy", "clg_ks_pf_energy",
     "clg_gauge_momentum_random", "clg_gauge_kinetic_energy", "clg_gauge_axpy", "clg_gauge_exp_mult",
@@ -142,6 +142,7 @@ def lib():
                                     C.POINTER(SolveReport)]),
             "clg_solver_params_default": (C.c_int, [C.POINTER(SolverParams)]),
             "clg_time_wilson": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, dp]),
+            "clg_time_ks": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_int, dp]),
             "clg_multishift_solve": (C.c_int, [vp, C.POINTER(SolverParams), C.c_int, C.c_int, C.POINTER(vp), dp,
                                                C.c_int, vp, vp, C.c_double, C.c_int, C.POINTER(SolveReport)]),
             "clg_rational_apply": (C.c_int, [vp, C.POINTER(SolverParams), C.c_int, C.c_int, vp, vp, vp, C.c_double,
@@ -476,6 +477,12 @@ def solve(ctx, params, x, b, gauge, kappa_or_mass, op=EFO_F_InverseD, start=None
 def time_wilson(ctx, out, inp, gauge, kappa, variant, reps):
     ms = C.c_double()
     check(lib().clg_time_wilson(ctx.h, out.h, inp.h, gauge.h, kappa, variant, reps, C.byref(ms)))
+    return ms.value
+
+
+def time_ks(ctx, out, inp, gauge, two_am, reps):
+    ms = C.c_double()
+    check(lib().clg_time_ks(ctx.h, out.h, inp.h, gauge.h, two_am, reps, C.byref(ms)))
     return ms.value
 
 

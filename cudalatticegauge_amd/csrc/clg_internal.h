@@ -148,6 +148,9 @@ bool tuning_enabled();
 int batch_tune_count();
 int batch_tune_entry(int index, char* buf, int len);
 void batch_tune_reset();
+int ks_tune_count();
+int ks_tune_entry(int index, char* buf, int len);
+void ks_tune_reset();
 
 // launch helpers
 inline unsigned grid_for(long n, int block) { return (unsigned)((n + block - 1) / block); }

@@ -5,9 +5,19 @@
 //   D^dagger flips the sign of the hopping sum (D(m=0) is anti-Hermitian).
 // eta_mu(x) = (-1)^(x_0 + ... + x_{mu-1}) is computed from the coordinates instead of the
 // reference's per-site byte table (CIndexSquare.cu:257-267).
+//
+// The links (576 bytes per site in FP64) outweigh the colour vectors (48 bytes), and every link
+// is read twice: forward by x and backward by x+mu.  The launch therefore keeps both users of a
+// link in one L2 (XCD-contiguous slabs with the two parities interleaved block by block) and reads
+// links through the cache; a small per-shape tuner picks the workgroup size and the schedule.
 #include "clg_internal.h"
 
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <iterator>
+#include <map>
+#include <mutex>
 
 using namespace clg;
 using namespace clgd;
@@ -26,6 +36,7 @@ struct KSArgs {
     double hs;       // +1 for D, -1 for D^dagger
     double cr, ci;   // overall coefficient
     int pbase, nblk;
+    int xcd, pinter;  // work decomposition (clgd::map_site)
 };
 
 template <int MU>
@@ -41,7 +52,9 @@ __device__ __forceinline__ void pin3(C2<R>* a) {
     asm volatile("" : "+v"(a[0].x), "+v"(a[0].y), "+v"(a[1].x), "+v"(a[1].y), "+v"(a[2].x), "+v"(a[2].y)::"memory");
 }
 
-template <int MU, typename R, class SIO, class LIO>
+// PM 2 pins the accumulator after each half of a direction (bounded register use, in-order
+// loads); PM 0 leaves the schedule to the compiler
+template <int MU, int PM, typename R, class SIO, class LIO>
 __device__ __forceinline__ void ks_mu(const KSArgs& a, const Site& s, int p, int h, C2<R>* acc) {
     C2<R> v[3], u[9], t[3];
     const int Vh = a.g.Vh;
@@ -56,7 +69,7 @@ __device__ __forceinline__ void ks_mu(const KSArgs& a, const Site& s, int p, int
 #pragma unroll
         for (int c = 0; c < 3; ++c) { acc[c].x += f * t[c].x; acc[c].y += f * t[c].y; }
     }
-    pin3(acc);
+    if constexpr (PM == 2) pin3(acc);
     {
         const Nbr n = bwd_nbr<MU>(a.g, s, p);
         const unsigned pe = a.in_full ? (unsigned)n.p : 0u;
@@ -67,26 +80,23 @@ __device__ __forceinline__ void ks_mu(const KSArgs& a, const Site& s, int p, int
 #pragma unroll
         for (int c = 0; c < 3; ++c) { acc[c].x += f * t[c].x; acc[c].y += f * t[c].y; }
     }
-    pin3(acc);
+    if constexpr (PM == 2) pin3(acc);
 }
 
-template <typename R, int SP, int RECON>
+template <typename R, int SP, int RECON, int PM>
 __global__ __launch_bounds__(256) void k_ks(KSArgs a) {
     typedef KSIO<SP> SIO;
     typedef LinkIO<SP, RECON> LIO;
-    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= (long)a.nblk * a.g.Vh) return;
-    const int blk = (int)(gid / a.g.Vh);
-    const int h = (int)(gid - (long)blk * a.g.Vh);
-    const int p = a.nblk == 2 ? blk : a.pbase;
+    int p, h;
+    if (!map_site(a.g.Vh, a.nblk, a.pbase, a.xcd, a.pinter, p, h)) return;
     const Site s = site_of(a.g, p, h);
     C2<R> acc[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) acc[c] = cmk<R>(0, 0);
-    ks_mu<0, R, SIO, LIO>(a, s, p, h, acc);
-    ks_mu<1, R, SIO, LIO>(a, s, p, h, acc);
-    ks_mu<2, R, SIO, LIO>(a, s, p, h, acc);
-    ks_mu<3, R, SIO, LIO>(a, s, p, h, acc);
+    ks_mu<0, PM, R, SIO, LIO>(a, s, p, h, acc);
+    ks_mu<1, PM, R, SIO, LIO>(a, s, p, h, acc);
+    ks_mu<2, PM, R, SIO, LIO>(a, s, p, h, acc);
+    ks_mu<3, PM, R, SIO, LIO>(a, s, p, h, acc);
     const unsigned po = a.out_full ? (unsigned)p : 0u;
     const unsigned eo = po * SIO::NPL * a.g.Vh + h, no = po * a.g.Vh + h;
     const unsigned pi = a.in_full ? (unsigned)p : 0u;
@@ -102,9 +112,134 @@ __global__ __launch_bounds__(256) void k_ks(KSArgs a) {
     SIO::store(a.out, a.outn, a.g.Vh, eo, no, o);
 }
 
+struct KSCfg {
+    int block;  // threads per workgroup
+    int xcd;    // 0: hardware order; 8: XCD-contiguous slabs, parities interleaved block by block
+    int pm;     // 0 or 2 (ks_mu)
+};
+
+template <typename R, int SP, int RECON, int PM>
+void ks_launch_pm(const KSArgs& a0, hipStream_t st, const KSCfg& c) {
+    KSArgs a = a0;
+    a.xcd = c.xcd;
+    a.pinter = c.xcd > 1 ? 1 : 0;
+    const unsigned grid = sched_grid(a.g.Vh, a.nblk, c.block, a.xcd, a.pinter);
+    k_ks<R, SP, RECON, PM><<<grid, c.block, 0, st>>>(a);
+}
+
+template <typename R, int SP, int RECON>
+void ks_launch_t(const KSArgs& a, hipStream_t st, const KSCfg& c) {
+    if (c.pm == 2) ks_launch_pm<R, SP, RECON, 2>(a, st, c);
+    else ks_launch_pm<R, SP, RECON, 0>(a, st, c);
+}
+
+int ks_launch_cfg(const KSArgs& a, int prec, bool r12, hipStream_t st, const KSCfg& c) {
+    if (prec == CLG_F64) {
+        if (r12) ks_launch_t<double, CLG_F64, 12>(a, st, c);
+        else ks_launch_t<double, CLG_F64, 18>(a, st, c);
+    } else if (prec == CLG_F32) {
+        if (r12) ks_launch_t<float, CLG_F32, 12>(a, st, c);
+        else ks_launch_t<float, CLG_F32, 18>(a, st, c);
+    } else {
+        if (r12) ks_launch_t<float, CLG_F16, 12>(a, st, c);
+        else ks_launch_t<float, CLG_F16, 18>(a, st, c);
+    }
+    CLG_HIP(hipGetLastError());
+    return CLG_OK;
+}
+
+// per-shape tuner: the first launch of a (precision, links, lattice) key times each candidate with
+// HIP events on the launch stream and keeps the fastest (every candidate writes the same output)
+struct KSKey {
+    int prec, r12, L0, L1, L2, L3;
+    bool operator<(const KSKey& o) const { return std::memcmp(this, &o, sizeof(KSKey)) < 0; }
+};
+std::map<KSKey, KSCfg>& ks_table() {
+    static std::map<KSKey, KSCfg> t;
+    return t;
+}
+std::mutex& ks_mutex() {
+    static std::mutex m;
+    return m;
+}
+
+bool ks_valid(const KSCfg& c) {
+    return (c.block == 64 || c.block == 128 || c.block == 256) && (c.xcd == 0 || c.xcd == 8) && (c.pm == 0 || c.pm == 2);
+}
+
+int ks_launch(const KSArgs& a, int prec, bool r12, hipStream_t st) {
+    KSCfg cfg{256, 8, 0};
+    // CLG_TUNE_FORCE_KS="block,xcd,pin" fixes the launch configuration (testing and profiling)
+    if (const char* e = std::getenv("CLG_TUNE_FORCE_KS")) {
+        KSCfg f{0, 0, 0};
+        if (std::sscanf(e, "%d,%d,%d", &f.block, &f.xcd, &f.pm) == 3 && ks_valid(f)) return ks_launch_cfg(a, prec, r12, st, f);
+    }
+    KSKey k;
+    std::memset(&k, 0, sizeof(k));
+    k.prec = prec; k.r12 = r12; k.L0 = a.g.L[0]; k.L1 = a.g.L[1]; k.L2 = a.g.L[2]; k.L3 = a.g.L[3];
+    bool need_tune = false;
+    {
+        std::lock_guard<std::mutex> lk(ks_mutex());
+        auto it = ks_table().find(k);
+        if (it != ks_table().end()) cfg = it->second;
+        else need_tune = true;
+    }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    (void)hipStreamIsCapturing(st, &cs);
+    if (need_tune && tuning_enabled() && cs == hipStreamCaptureStatusNone && (long)a.nblk * a.g.Vh >= 4096) {
+        const KSCfg cands[] = {{64, 0, 2}, {128, 0, 2}, {256, 0, 2}, {64, 8, 2}, {128, 8, 2}, {256, 8, 2},
+                               {64, 8, 0}, {128, 8, 0}, {256, 8, 0}};
+        hipEvent_t e0, e1;
+        CLG_HIP(hipEventCreate(&e0));
+        CLG_HIP(hipEventCreate(&e1));
+        float best = 1e30f;
+        for (const KSCfg& c : cands) {
+            CLG_TRY(ks_launch_cfg(a, prec, r12, st, c));  // warm
+            CLG_HIP(hipEventRecord(e0, st));
+            for (int r = 0; r < 3; ++r) CLG_TRY(ks_launch_cfg(a, prec, r12, st, c));
+            CLG_HIP(hipEventRecord(e1, st));
+            CLG_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            CLG_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) { best = ms; cfg = c; }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        const char* verbose = std::getenv("CLG_TUNE_VERBOSE");
+        if (verbose && verbose[0] == '1')
+            std::fprintf(stderr, "[clg tune] staggered D prec=%d recon=%d L=%dx%dx%dx%d -> block=%d xcd=%d pin=%d (%.1f us)\n",
+                         prec, r12 ? 12 : 18, a.g.L[0], a.g.L[1], a.g.L[2], a.g.L[3], cfg.block, cfg.xcd, cfg.pm,
+                         best * 1e3 / 3);
+        std::lock_guard<std::mutex> lk(ks_mutex());
+        ks_table()[k] = cfg;
+    }
+    return ks_launch_cfg(a, prec, r12, st, cfg);
+}
+
 }  // namespace
 
 namespace clg {
+
+int ks_tune_count() {
+    std::lock_guard<std::mutex> lk(ks_mutex());
+    return (int)ks_table().size();
+}
+
+int ks_tune_entry(int index, char* buf, int len) {
+    std::lock_guard<std::mutex> lk(ks_mutex());
+    CLG_CHECK_ARG(index >= 0 && index < (int)ks_table().size(), "no tuned entry %d", index);
+    auto it = ks_table().begin();
+    std::advance(it, index);
+    const KSKey& k = it->first;
+    std::snprintf(buf, (size_t)len, "staggered_d prec=%d recon=%d dims=%dx%dx%dx%d -> block=%d xcd=%d pin=%d", k.prec,
+                  k.r12 ? 12 : 18, k.L0, k.L1, k.L2, k.L3, it->second.block, it->second.xcd, it->second.pm);
+    return CLG_OK;
+}
+
+void ks_tune_reset() {
+    std::lock_guard<std::mutex> lk(ks_mutex());
+    ks_table().clear();
+}
 
 int ks_full(clg_context ctx, clg_field_s* out, const clg_field_s* in, const clg_field_s* gauge, double two_am, bool dag,
             double cr, double ci) {
@@ -124,21 +259,7 @@ int ks_full(clg_context ctx, clg_field_s* out, const clg_field_s* in, const clg_
     a.ci = ci;
     a.pbase = 0;
     a.nblk = 2;
-    const unsigned grid = grid_for(2L * a.g.Vh, 256);
-    hipStream_t st = ctx->stream;
-    const bool r12 = gauge->kind == CLG_GAUGE12;
-    if (out->prec == CLG_F64) {
-        if (r12) k_ks<double, CLG_F64, 12><<<grid, 256, 0, st>>>(a);
-        else k_ks<double, CLG_F64, 18><<<grid, 256, 0, st>>>(a);
-    } else if (out->prec == CLG_F32) {
-        if (r12) k_ks<float, CLG_F32, 12><<<grid, 256, 0, st>>>(a);
-        else k_ks<float, CLG_F32, 18><<<grid, 256, 0, st>>>(a);
-    } else {
-        if (r12) k_ks<float, CLG_F16, 12><<<grid, 256, 0, st>>>(a);
-        else k_ks<float, CLG_F16, 18><<<grid, 256, 0, st>>>(a);
-    }
-    CLG_HIP(hipGetLastError());
-    return CLG_OK;
+    return ks_launch(a, out->prec, gauge->kind == CLG_GAUGE12, ctx->stream);
 }
 
 }  // namespace clg
@@ -172,6 +293,21 @@ int clg_ks_apply(clg_context ctx, clg_field out, clg_field in, clg_field gauge, 
     CLG_TRY(scratch_field(ctx, in, CLG_KS, in->prec, CLG_FULL, 0, &tmp));
     CLG_TRY(ks_full(ctx, tmp, in, gauge, two_am, op == EFO_F_DDdagger, 1.0, 0.0));
     return ks_full(ctx, out, tmp, gauge, two_am, false, fr, fi);
+}
+
+int clg_time_ks(clg_context ctx, clg_field out, clg_field in, clg_field gauge, double two_am, int reps, double* ms_avg) {
+    CLG_CHECK_ARG(ctx && ms_avg && reps > 0, "bad arguments");
+    (void)hipSetDevice(ctx->device);
+    auto one = [&]() -> int { return clg_ks_apply(ctx, out, in, gauge, two_am, EFO_F_D, EOCT_None, 1.0, 0.0); };
+    CLG_TRY(one());  // validates arguments, tunes, warms up
+    CLG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    for (int r = 0; r < reps; ++r) CLG_TRY(one());
+    CLG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    CLG_HIP(hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    CLG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    *ms_avg = (double)ms / reps;
+    return CLG_OK;
 }
 
 }  // extern "C"

@@ -108,14 +108,15 @@ struct HopCfg {
     int pm;     // pin mode: 1 after each half of a direction, 2 after each direction; 3 / 4 the same
                 // with non-temporal link loads (links read once per launch stream past the caches)
     int block;  // threads per workgroup
-    int xcd;    // 0: hardware order, 8: XCD-contiguous slabs (+ parity interleave for full D)
+    int xcd;    // 0: hardware order; 8: XCD-contiguous slabs, for full D with the two parities
+                // interleaved block by block; 1: XCD-contiguous slabs, parities one after the other
 };
 
 template <typename R, int SP, int GP, int RECON, int PM>
 void launch_pm(const HopArgs& a0, bool dag, bool hasx, hipStream_t st, int block, int xcd) {
     HopArgs a = a0;
-    a.xcd = xcd;
-    a.pinter = xcd > 1 ? 1 : 0;
+    a.xcd = xcd ? 8 : 0;
+    a.pinter = xcd == 8 ? 1 : 0;
     const unsigned grid = sched_grid(a.g.Vh, a.nblk, block, a.xcd, a.pinter);
     if (dag) {
         if (hasx) k_wilson_hop<R, SP, GP, RECON, true, true, PM><<<grid, block, 0, st>>>(a);
@@ -170,7 +171,7 @@ std::mutex& tune_mutex() {
     return m;
 }
 bool valid_cfg(int pm, int blk, int xcd) {
-    return pm >= 1 && pm <= 4 && (blk == 64 || blk == 128 || blk == 256) && (xcd == 0 || xcd == 8);
+    return pm >= 1 && pm <= 4 && (blk == 64 || blk == 128 || blk == 256) && (xcd == 0 || xcd == 1 || xcd == 8);
 }
 // process-wide overrides set through clg_autotune_enable / clg_autotune_force
 std::atomic<int> g_tune_mode{-1};  // -1: CLG_TUNE environment (default on), 0 off, 1 on
@@ -213,12 +214,15 @@ int launch_hop(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool 
     if (need_tune && tuning_enabled() && !aliased && cs == hipStreamCaptureStatusNone && (long)a.nblk * a.g.Vh >= 4096) {
         const HopCfg cands[] = {{1, 64, 0}, {1, 128, 0}, {1, 256, 0}, {2, 64, 0}, {2, 128, 0}, {2, 256, 0},
                                 {1, 64, 8}, {1, 128, 8}, {1, 256, 8}, {2, 64, 8}, {2, 128, 8}, {2, 256, 8},
-                                {3, 64, 8}, {3, 128, 8}, {3, 256, 8}, {4, 64, 8}, {4, 128, 8}, {4, 256, 8}};
+                                {3, 64, 8}, {3, 128, 8}, {3, 256, 8}, {4, 64, 8}, {4, 128, 8}, {4, 256, 8},
+                                {1, 128, 1}, {1, 256, 1}, {2, 128, 1}, {2, 256, 1},
+                                {3, 64, 1}, {3, 128, 1}, {3, 256, 1}, {4, 64, 1}, {4, 128, 1}, {4, 256, 1}};
         hipEvent_t e0, e1;
         CLG_HIP(hipEventCreate(&e0));
         CLG_HIP(hipEventCreate(&e1));
         float best = 1e30f;
         for (const HopCfg& c : cands) {
+            if (c.xcd == 1 && a.nblk != 2) continue;  // one parity: the same launch as xcd 8
             CLG_TRY(launch_cfg(a, prec, gprec, recon, dag, hasx, st, c));  // warm
             CLG_HIP(hipEventRecord(e0, st));
             for (int r = 0; r < 3; ++r) CLG_TRY(launch_cfg(a, prec, gprec, recon, dag, hasx, st, c));
@@ -446,7 +450,7 @@ int clg_autotune_force(int pin_mode, int block, int xcd) {
         g_forced.store(0);
         return CLG_OK;
     }
-    CLG_CHECK_ARG(valid_cfg(pin_mode, block, xcd), "pin_mode 1..4, block 64|128|256, xcd 0|8");
+    CLG_CHECK_ARG(valid_cfg(pin_mode, block, xcd), "pin_mode 1..4, block 64|128|256, xcd 0|1|8");
     g_forced.store(pin_mode | block << 4 | xcd << 16);
     return CLG_OK;
 }
@@ -457,6 +461,7 @@ int clg_autotune_reset(void) {
         tune_table().clear();
     }
     batch_tune_reset();
+    ks_tune_reset();
     return CLG_OK;
 }
 
@@ -466,7 +471,7 @@ int clg_autotune_count(void) {
         std::lock_guard<std::mutex> lk(tune_mutex());
         n = (int)tune_table().size();
     }
-    return n + batch_tune_count();
+    return n + batch_tune_count() + ks_tune_count();
 }
 
 int clg_autotune_entry(int index, char* buf, int len) {
@@ -486,7 +491,9 @@ int clg_autotune_entry(int index, char* buf, int len) {
         }
         index -= (int)tune_table().size();
     }
-    return batch_tune_entry(index, buf, len);
+    const int nb = batch_tune_count();
+    if (index < nb) return batch_tune_entry(index, buf, len);
+    return ks_tune_entry(index - nb, buf, len);
 }
 
 }  // extern "C"

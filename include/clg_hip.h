@@ -412,8 +412,12 @@ int clg_solve_batch(clg_context ctx, const clg_solver_params* params, int n, clg
 /* mode 1 tune, 0 use defaults, -1 follow the CLG_TUNE environment variable (default on) */
 int clg_autotune_enable(int mode);
 /* fix every single-system hop launch: pin_mode 1|2 (register pinning per half / per direction), 3|4 (the
- * same with non-temporal link loads), block 64|128|256, xcd 0|8; pin_mode 0 clears */
+ * same with non-temporal link loads), block 64|128|256, xcd 0 (hardware workgroup order) | 8 (each XCD
+ * sweeps a contiguous slab of sites; full D interleaves the parities block by block) | 1 (XCD slabs,
+ * parities one after the other); pin_mode 0 clears */
 int clg_autotune_force(int pin_mode, int block, int xcd);
+/* The staggered D tunes its own launch (block 64|128|256, hardware or XCD-slab order, pinning);
+ * the environment variable CLG_TUNE_FORCE_KS="block,xcd,pin" (xcd 0|8, pin 0|2) fixes it. */
 /* forget every tuned shape (the next call of each shape tunes again) */
 int clg_autotune_reset(void);
 /* number of tuned shapes, and a one-line description of entry `index` */
@@ -444,6 +448,10 @@ int clg_graph_destroy(clg_graph g);
  * 1 half-lattice hop, 2 Schur operator.  No reference counterpart (bench support). */
 int clg_time_wilson(clg_context ctx, clg_field out, clg_field in, clg_field gauge, double kappa,
                     int variant, int reps, double* ms_avg);
+/* The same for the full-lattice staggered D (clg_ks_apply, EFO_F_D) with mass term two_am.
+ * No reference counterpart (bench support). */
+int clg_time_ks(clg_context ctx, clg_field out, clg_field in, clg_field gauge, double two_am, int reps,
+                double* ms_avg);
 
 #ifdef __cplusplus
 }

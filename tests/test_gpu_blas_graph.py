@@ -101,7 +101,7 @@ def test_autotune_controls(ctx):
     ref = cvec(y0)
     ent = clg.autotune_entries()
     assert len(ent) >= 1 and any(e.startswith("wilson_hop") and "dims=8x8x8x16" in e for e in ent)
-    for pm, blk, xcd in [(1, 64, 0), (2, 128, 8), (1, 256, 8), (3, 128, 8), (4, 256, 0)]:
+    for pm, blk, xcd in [(1, 64, 0), (2, 128, 8), (1, 256, 8), (3, 128, 8), (4, 256, 0), (3, 128, 1), (4, 64, 1), (1, 256, 1)]:
         clg.autotune_force(pm, blk, xcd)
         y = clg.Field(ctx, clg.WILSON, clg.F64, dims)
         clg.wilson_apply(ctx, y, x, g, 0.12, clg.EFO_F_D)

@@ -202,3 +202,30 @@ def test_ks_solve_ddagger(ctx):
     x = fx.download()
     r = chi - orc.ks_op(dims, orc.BC_FERMION, 0.1, U, x, orc.DDDAG)
     assert np.linalg.norm(r) / np.linalg.norm(chi) < 1e-9
+
+
+def test_ks_launch_configurations(ctx, monkeypatch):
+    """Every staggered launch the tuner may pick gives the oracle's D (8^3x16 is large enough to tune)."""
+    dims = (8, 8, 8, 16)
+    U = orc.gauge_random(dims, 0.4, 5)
+    chi = orc.ks_gaussian(dims, 6)
+    g = clg.Field(ctx, clg.GAUGE, clg.F64, dims)
+    g.upload(U)
+    fi = clg.Field(ctx, clg.KS, clg.F64, dims, bc=orc.BC_FERMION)
+    fi.upload(chi)
+    ref = orc.ks_op(dims, orc.BC_FERMION, 0.07, U, chi, clg.EFO_F_D)
+    for blk in (64, 128, 256):
+        for xcd in (0, 8):
+            for pm in (0, 2):
+                monkeypatch.setenv("CLG_TUNE_FORCE_KS", f"{blk},{xcd},{pm}")
+                fo = clg.Field(ctx, clg.KS, clg.F64, dims)
+                clg.ks_apply(ctx, fo, fi, g, 0.07, clg.EFO_F_D)
+                assert np.abs(fo.download() - ref).max() / np.abs(ref).max() < 1e-12, (blk, xcd, pm)
+    monkeypatch.delenv("CLG_TUNE_FORCE_KS")
+    clg.autotune_reset()
+    clg.autotune_enable(1)
+    fo = clg.Field(ctx, clg.KS, clg.F64, dims)
+    clg.ks_apply(ctx, fo, fi, g, 0.07, clg.EFO_F_D)
+    assert np.abs(fo.download() - ref).max() / np.abs(ref).max() < 1e-12
+    assert any(e.startswith("staggered_d") and "dims=8x8x8x16" in e for e in clg.autotune_entries())
+    clg.autotune_enable(-1)
