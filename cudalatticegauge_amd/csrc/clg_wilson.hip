@@ -21,6 +21,7 @@
 #include <iterator>
 #include <map>
 #include <mutex>
+#include <vector>
 
 using namespace clg;
 using namespace clgd;
@@ -42,15 +43,24 @@ struct HopArgs {
     int xcd, pinter;     // work decomposition (clgd::map_site)
 };
 
-template <int MU, bool DAG, int PM, typename R, class SIO, class LIO>
-__device__ __forceinline__ void hop_mu(const HopArgs& a, const Site& s, int p, int h, C2<R>* acc) {
+// Neighbour spinors come from the input field, or (STAGED, z and t only) from the copy of this
+// (x,y) plane's other-parity sites in LDS: `st`/`stn` hold NPL planes of blockDim.x 16-byte chunks
+// (and the FP16 site scales); the site with index h sits at local index h - hbase.
+
+template <int MU, bool DAG, int PM, bool STAGED, typename R, class SIO, class LIO>
+__device__ __forceinline__ void hop_mu(const HopArgs& a, const Site& s, int p, int h, C2<R>* acc, const char* st,
+                                       const float* stn, int hbase) {
     C2<R> psi[12], hh[6], uh[6], u[9];
     const int Vh = a.g.Vh;
+    constexpr bool LDS = STAGED && MU >= 2;
     // forward: (1 - g_mu) U_mu(x) psi(x+mu)  [dagger: (1 + g_mu)]
     {
         const Nbr n = fwd_nbr<MU>(a.g, s, p);
         const unsigned pe = a.in_full ? (unsigned)n.p : 0u;
-        load_projected<SIO, MU, DAG ? 0 : 2>(a.in, a.inn, Vh, pe * SIO::NPL * Vh + n.h, pe * Vh + n.h, n.sign, psi, hh);
+        if constexpr (LDS)
+            load_projected<SIO, MU, DAG ? 0 : 2>(st, stn, (int)blockDim.x, (unsigned)(n.h - hbase), (unsigned)(n.h - hbase), n.sign, psi, hh);
+        else
+            load_projected<SIO, MU, DAG ? 0 : 2>(a.in, a.inn, Vh, pe * SIO::NPL * Vh + n.h, pe * Vh + n.h, n.sign, psi, hh);
         LIO::template load<(PM >= 3)>(a.gauge, Vh, (unsigned)(p * 4 + MU) * LIO::NPL * Vh + h, u);
         su3_mv(u, hh, uh);
         su3_mv(u, hh + 3, uh + 3);
@@ -61,7 +71,10 @@ __device__ __forceinline__ void hop_mu(const HopArgs& a, const Site& s, int p, i
     {
         const Nbr n = bwd_nbr<MU>(a.g, s, p);
         const unsigned pe = a.in_full ? (unsigned)n.p : 0u;
-        load_projected<SIO, MU, DAG ? 2 : 0>(a.in, a.inn, Vh, pe * SIO::NPL * Vh + n.h, pe * Vh + n.h, n.sign, psi, hh);
+        if constexpr (LDS)
+            load_projected<SIO, MU, DAG ? 2 : 0>(st, stn, (int)blockDim.x, (unsigned)(n.h - hbase), (unsigned)(n.h - hbase), n.sign, psi, hh);
+        else
+            load_projected<SIO, MU, DAG ? 2 : 0>(a.in, a.inn, Vh, pe * SIO::NPL * Vh + n.h, pe * Vh + n.h, n.sign, psi, hh);
         LIO::template load<(PM >= 3)>(a.gauge, Vh, (unsigned)(n.p * 4 + MU) * LIO::NPL * Vh + n.h, u);
         su3_mdv(u, hh, uh);
         su3_mdv(u, hh + 3, uh + 3);
@@ -70,28 +83,58 @@ __device__ __forceinline__ void hop_mu(const HopArgs& a, const Site& s, int p, i
     if constexpr (PM >= 1) pin_acc(acc);
 }
 
+// PM 1..4: register pinning and link cache policy (hop_mu); PM 5..8: PM 1..4 with the z and t
+// neighbours staged in LDS.  A staged launch needs one workgroup per (x,y) plane of
+// one parity (blockDim = L2*L3/2, L3 even, no padded workgroups): a plane's z and t neighbours
+// then all lie in the same plane of the other parity, which the workgroup copies into LDS with
+// one coalesced read before any hop (stage_bytes of dynamic LDS).
 template <typename R, int SP, int GP, int RECON, bool DAG, bool HASX, int PM>
 __global__ __launch_bounds__(256) void k_wilson_hop(HopArgs a) {
     typedef WilsonIO<SP> SIO;
     typedef LinkIO<GP, RECON> LIO;
+    constexpr bool STAGED = PM >= 5;
+    constexpr int PMH = STAGED ? PM - 4 : PM;
     int p, h;
-    if (!map_site(a.g.Vh, a.nblk, a.pbase, a.xcd, a.pinter, p, h)) return;
+    const bool valid = map_site(a.g.Vh, a.nblk, a.pbase, a.xcd, a.pinter, p, h);
+    const char* st = nullptr;
+    const float* stn = nullptr;
+    int hbase = 0;
+    if constexpr (STAGED) {
+        extern __shared__ uint4 stage[];
+        const int B = (int)blockDim.x;
+        float* stage_n = reinterpret_cast<float*>(stage + SIO::NPL * B);
+        // padded workgroups (xcd remap) are whole: every lane of a workgroup agrees on valid
+        if (!valid) return;
+        hbase = h - (int)threadIdx.x;
+        const unsigned pe = a.in_full ? (unsigned)(1 - p) : 0u;
+#pragma unroll
+        for (int k = 0; k < SIO::NPL; ++k) stage[k * B + threadIdx.x] = ldp<uint4>(a.in, a.g.Vh, k, pe * SIO::NPL * a.g.Vh + h);
+        if constexpr (SIO::SCALED) stage_n[threadIdx.x] = a.inn[pe * a.g.Vh + h];
+        __syncthreads();
+        st = reinterpret_cast<const char*>(stage);
+        stn = stage_n;
+    } else {
+        if (!valid) return;
+    }
     const Site s = site_of(a.g, p, h);
+    const unsigned po = a.out_full ? (unsigned)p : 0u;
+    const unsigned eo = po * SIO::NPL * a.g.Vh + h, no = po * a.g.Vh + h;
+    // a staged workgroup's own-parity sites have usually left L2 by the end of its hops (its other
+    // reads all go to the stage or to other planes): read the x term first
+    C2<R> xv[12];
+    if constexpr (HASX && STAGED) SIO::load(a.x, a.xn, a.g.Vh, eo, no, xv);
     C2<R> acc[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[k] = cmk<R>(0, 0);
-    hop_mu<0, DAG, PM, R, SIO, LIO>(a, s, p, h, acc);
-    hop_mu<1, DAG, PM, R, SIO, LIO>(a, s, p, h, acc);
-    hop_mu<2, DAG, PM, R, SIO, LIO>(a, s, p, h, acc);
-    hop_mu<3, DAG, PM, R, SIO, LIO>(a, s, p, h, acc);
+    hop_mu<0, DAG, PMH, STAGED, R, SIO, LIO>(a, s, p, h, acc, st, stn, hbase);
+    hop_mu<1, DAG, PMH, STAGED, R, SIO, LIO>(a, s, p, h, acc, st, stn, hbase);
+    hop_mu<2, DAG, PMH, STAGED, R, SIO, LIO>(a, s, p, h, acc, st, stn, hbase);
+    hop_mu<3, DAG, PMH, STAGED, R, SIO, LIO>(a, s, p, h, acc, st, stn, hbase);
     const C2<R> ca = cmk<R>((R)a.ar, (R)a.ai);
-    const unsigned po = a.out_full ? (unsigned)p : 0u;
-    const unsigned eo = po * SIO::NPL * a.g.Vh + h, no = po * a.g.Vh + h;
     C2<R> o[12];
     if constexpr (HASX) {
         const C2<R> cb = cmk<R>((R)a.br, (R)a.bi);
-        C2<R> xv[12];
-        SIO::load(a.x, a.xn, a.g.Vh, eo, no, xv);
+        if constexpr (!STAGED) SIO::load(a.x, a.xn, a.g.Vh, eo, no, xv);
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
             o[k] = cmul(cb, xv[k]);
@@ -106,7 +149,8 @@ __global__ __launch_bounds__(256) void k_wilson_hop(HopArgs a) {
 
 struct HopCfg {
     int pm;     // pin mode: 1 after each half of a direction, 2 after each direction; 3 / 4 the same
-                // with non-temporal link loads (links read once per launch stream past the caches)
+                // with non-temporal link loads (links read once per launch stream past the caches);
+                // 5..8 as 1..4 with the z and t neighbours staged in LDS (block = L2*L3/2)
     int block;  // threads per workgroup
     int xcd;    // 0: hardware order; 8: XCD-contiguous slabs, for full D with the two parities
                 // interleaved block by block; 1: XCD-contiguous slabs, parities one after the other
@@ -118,18 +162,24 @@ void launch_pm(const HopArgs& a0, bool dag, bool hasx, hipStream_t st, int block
     a.xcd = xcd ? 8 : 0;
     a.pinter = xcd == 8 ? 1 : 0;
     const unsigned grid = sched_grid(a.g.Vh, a.nblk, block, a.xcd, a.pinter);
+    // staged launches: NPL 16-byte chunks per site of the plane, plus the FP16 site scales
+    const size_t shm = PM >= 5 ? (size_t)block * (16 * WilsonIO<SP>::NPL + (WilsonIO<SP>::SCALED ? 4 : 0)) : 0;
     if (dag) {
-        if (hasx) k_wilson_hop<R, SP, GP, RECON, true, true, PM><<<grid, block, 0, st>>>(a);
-        else k_wilson_hop<R, SP, GP, RECON, true, false, PM><<<grid, block, 0, st>>>(a);
+        if (hasx) k_wilson_hop<R, SP, GP, RECON, true, true, PM><<<grid, block, shm, st>>>(a);
+        else k_wilson_hop<R, SP, GP, RECON, true, false, PM><<<grid, block, shm, st>>>(a);
     } else {
-        if (hasx) k_wilson_hop<R, SP, GP, RECON, false, true, PM><<<grid, block, 0, st>>>(a);
-        else k_wilson_hop<R, SP, GP, RECON, false, false, PM><<<grid, block, 0, st>>>(a);
+        if (hasx) k_wilson_hop<R, SP, GP, RECON, false, true, PM><<<grid, block, shm, st>>>(a);
+        else k_wilson_hop<R, SP, GP, RECON, false, false, PM><<<grid, block, shm, st>>>(a);
     }
 }
 
 template <typename R, int SP, int GP, int RECON>
 int launch_t(const HopArgs& a, bool dag, bool hasx, hipStream_t st, HopCfg c) {
-    if (c.pm == 4) launch_pm<R, SP, GP, RECON, 4>(a, dag, hasx, st, c.block, c.xcd);
+    if (c.pm == 8) launch_pm<R, SP, GP, RECON, 8>(a, dag, hasx, st, c.block, c.xcd);
+    else if (c.pm == 7) launch_pm<R, SP, GP, RECON, 7>(a, dag, hasx, st, c.block, c.xcd);
+    else if (c.pm == 6) launch_pm<R, SP, GP, RECON, 6>(a, dag, hasx, st, c.block, c.xcd);
+    else if (c.pm == 5) launch_pm<R, SP, GP, RECON, 5>(a, dag, hasx, st, c.block, c.xcd);
+    else if (c.pm == 4) launch_pm<R, SP, GP, RECON, 4>(a, dag, hasx, st, c.block, c.xcd);
     else if (c.pm == 3) launch_pm<R, SP, GP, RECON, 3>(a, dag, hasx, st, c.block, c.xcd);
     else if (c.pm == 2) launch_pm<R, SP, GP, RECON, 2>(a, dag, hasx, st, c.block, c.xcd);
     else launch_pm<R, SP, GP, RECON, 1>(a, dag, hasx, st, c.block, c.xcd);
@@ -137,7 +187,20 @@ int launch_t(const HopArgs& a, bool dag, bool hasx, hipStream_t st, HopCfg c) {
     return CLG_OK;
 }
 
+// the workgroup size of an LDS-staged launch on this lattice (one (x,y) plane of one parity), or 0
+int staged_block(const Geo& g) {
+    if (g.L[3] % 2) return 0;
+    const int b = g.L[2] * g.L[3] / 2;
+    return (b == 64 || b == 128 || b == 256) ? b : 0;
+}
+// a staged configuration that does not fit the lattice runs as its unstaged pin mode
+HopCfg fit_cfg(const HopArgs& a, HopCfg c) {
+    if (c.pm >= 5 && (c.block != staged_block(a.g) || c.block == 0)) c.pm -= 4;
+    return c;
+}
+
 int launch_cfg(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool hasx, hipStream_t st, HopCfg c) {
+    c = fit_cfg(a, c);
     if (prec == CLG_F64) {
         return recon == 12 ? launch_t<double, CLG_F64, CLG_F64, 12>(a, dag, hasx, st, c)
                            : launch_t<double, CLG_F64, CLG_F64, 18>(a, dag, hasx, st, c);
@@ -171,7 +234,7 @@ std::mutex& tune_mutex() {
     return m;
 }
 bool valid_cfg(int pm, int blk, int xcd) {
-    return pm >= 1 && pm <= 4 && (blk == 64 || blk == 128 || blk == 256) && (xcd == 0 || xcd == 1 || xcd == 8);
+    return pm >= 1 && pm <= 8 && (blk == 64 || blk == 128 || blk == 256) && (xcd == 0 || xcd == 1 || xcd == 8);
 }
 // process-wide overrides set through clg_autotune_enable / clg_autotune_force
 std::atomic<int> g_tune_mode{-1};  // -1: CLG_TUNE environment (default on), 0 off, 1 on
@@ -221,7 +284,11 @@ int launch_hop(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool 
         CLG_HIP(hipEventCreate(&e0));
         CLG_HIP(hipEventCreate(&e1));
         float best = 1e30f;
-        for (const HopCfg& c : cands) {
+        std::vector<HopCfg> all(std::begin(cands), std::end(cands));
+        if (const int sb = staged_block(a.g)) {
+            for (int pm = 5; pm <= 8; ++pm) all.push_back(HopCfg{pm, sb, 8});
+        }
+        for (const HopCfg& c : all) {
             if (c.xcd == 1 && a.nblk != 2) continue;  // one parity: the same launch as xcd 8
             CLG_TRY(launch_cfg(a, prec, gprec, recon, dag, hasx, st, c));  // warm
             CLG_HIP(hipEventRecord(e0, st));
@@ -450,7 +517,7 @@ int clg_autotune_force(int pin_mode, int block, int xcd) {
         g_forced.store(0);
         return CLG_OK;
     }
-    CLG_CHECK_ARG(valid_cfg(pin_mode, block, xcd), "pin_mode 1..4, block 64|128|256, xcd 0|1|8");
+    CLG_CHECK_ARG(valid_cfg(pin_mode, block, xcd), "pin_mode 1..8, block 64|128|256, xcd 0|1|8");
     g_forced.store(pin_mode | block << 4 | xcd << 16);
     return CLG_OK;
 }

@@ -412,7 +412,8 @@ int clg_solve_batch(clg_context ctx, const clg_solver_params* params, int n, clg
 /* mode 1 tune, 0 use defaults, -1 follow the CLG_TUNE environment variable (default on) */
 int clg_autotune_enable(int mode);
 /* fix every single-system hop launch: pin_mode 1|2 (register pinning per half / per direction), 3|4 (the
- * same with non-temporal link loads), block 64|128|256, xcd 0 (hardware workgroup order) | 8 (each XCD
+ * same with non-temporal link loads), 5..8 (as 1..4 with the z and t neighbours staged in LDS; needs
+ * block = L2*L3/2 with L3 even, otherwise the launch runs as 1..4), block 64|128|256, xcd 0 (hardware workgroup order) | 8 (each XCD
  * sweeps a contiguous slab of sites; full D interleaves the parities block by block) | 1 (XCD slabs,
  * parities one after the other); pin_mode 0 clears */
 int clg_autotune_force(int pin_mode, int block, int xcd);

@@ -101,14 +101,15 @@ def test_autotune_controls(ctx):
     ref = cvec(y0)
     ent = clg.autotune_entries()
     assert len(ent) >= 1 and any(e.startswith("wilson_hop") and "dims=8x8x8x16" in e for e in ent)
-    for pm, blk, xcd in [(1, 64, 0), (2, 128, 8), (1, 256, 8), (3, 128, 8), (4, 256, 0), (3, 128, 1), (4, 64, 1), (1, 256, 1)]:
+    for pm, blk, xcd in [(1, 64, 0), (2, 128, 8), (1, 256, 8), (3, 128, 8), (4, 256, 0), (3, 128, 1), (4, 64, 1), (1, 256, 1),
+                         (5, 64, 8), (6, 64, 1)]:
         clg.autotune_force(pm, blk, xcd)
         y = clg.Field(ctx, clg.WILSON, clg.F64, dims)
         clg.wilson_apply(ctx, y, x, g, 0.12, clg.EFO_F_D)
         assert np.max(np.abs(cvec(y) - ref)) <= 1e-13 * np.max(np.abs(ref))
     clg.autotune_force(0)
     with pytest.raises(clg.CLGError):
-        clg.autotune_force(5, 256, 8)
+        clg.autotune_force(9, 256, 8)
     with pytest.raises(clg.CLGError):
         clg.autotune_enable(2)
     clg.autotune_enable(-1)

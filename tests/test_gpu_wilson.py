@@ -198,3 +198,43 @@ def test_precision_conversion(ctx, c0):
     h.copy_from(a)
     assert rel(b.download(), psi) < 1e-6
     assert rel(h.download(), psi) < 2e-3
+
+
+@pytest.mark.parametrize("dims", [(4, 6, 8, 16), (4, 4, 16, 16)])
+def test_lds_staged_launches(ctx, dims):
+    """Pin modes 5..8 stage the z and t neighbours in LDS (one workgroup per (x,y) plane of one
+    parity, block = L2*L3/2: 64 and 128 here); a staged mode whose block does not fit runs unstaged."""
+    plane = dims[2] * dims[3] // 2
+    U = orc.gauge_random(dims, 0.3, 11)
+    psi = orc.wilson_gaussian(dims, 12)
+    refs = {dag: orc.wilson_op(dims, orc.BC_FERMION, KAPPA, U, psi, orc.DDAG if dag else orc.D)
+            for dag in (False, True)}
+    g64 = upload(ctx, clg.GAUGE, clg.F64, dims, U)
+    f64 = upload(ctx, clg.WILSON, clg.F64, dims, psi)
+    g32 = clg.Field(ctx, clg.GAUGE12, clg.F32, dims)
+    g32.copy_from(g64)
+    f32 = clg.Field(ctx, clg.WILSON, clg.F32, dims, bc=orc.BC_FERMION)
+    f32.copy_from(f64)
+    g16 = clg.Field(ctx, clg.GAUGE12, clg.F16, dims)
+    g16.copy_from(g64)
+    f16 = clg.Field(ctx, clg.WILSON, clg.F16, dims, bc=orc.BC_FERMION)
+    f16.copy_from(f64)
+    cases = ((f64, g64, 1e-12), (f32, g32, 1e-6), (f16, g16, 1e-2))
+    ref_eo = refs[False].reshape(-1, 24)
+    try:
+        for pm, blk, xcd in ((1, plane, 8), (5, plane, 8), (6, plane, 8), (6, plane, 0), (5, plane, 1),
+                             (7, plane, 8), (8, plane, 0), (5, 256, 8), (8, 256, 8)):
+            clg.autotune_force(pm, blk, xcd)
+            for f, g, tol in cases:
+                for dag in (False, True):
+                    out = clg.Field(ctx, clg.WILSON, f.prec, dims)
+                    clg.wilson_apply(ctx, out, f, g, KAPPA, clg.EFO_F_Ddagger if dag else clg.EFO_F_D)
+                    assert rel(out.download(), refs[dag]) < tol, (pm, blk, xcd, f.prec, dag)
+            # even-odd hops read a half field: out_p = x_p - kappa H psi_(1-p)
+            for out_sub, par in ((clg.EVEN, 0), (clg.ODD, 1)):
+                in_sub = clg.ODD if out_sub == clg.EVEN else clg.EVEN
+                out = clg.Field(ctx, clg.WILSON, clg.F64, dims, subset=out_sub, bc=orc.BC_FERMION)
+                clg.wilson_hop(ctx, out, f64.parity(in_sub), g64, dagger=False, a=-KAPPA, x=f64.parity(out_sub), b=1.0)
+                assert rel(out.download().reshape(-1, 24), ref_eo[orc.eo_sites(dims, par)]) < 1e-12, (pm, par)
+    finally:
+        clg.autotune_force(0)
