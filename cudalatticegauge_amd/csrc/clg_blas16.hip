@@ -132,10 +132,12 @@ __global__ __launch_bounds__(kRedBlock) void k_h_multidot(unsigned n, unsigned V
     block_partial<2 * K + 1>(acc, partial);
 }
 
-// w' = w - sum_k h_k V_k (stored), with the partials of V_k^H w' and |w'|^2 of the unrounded w'
+// w' = w - sum_k h_k V_k, stored to (oc, os) (w itself or the next basis vector), with the
+// partials of V_k^H w' and |w'|^2 of the unrounded w'
 template <int K>
 __global__ __launch_bounds__(kRedBlock) void k_h_gs_project(unsigned n, unsigned Vh, HTab V, const double* __restrict__ h,
-                                                             uint4* wc, float* ws, double* partial) {
+                                                             const uint4* wc, const float* ws, uint4* oc, float* os,
+                                                             double* partial) {
     float hr[K], hi[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) { hr[k] = (float)h[2 * k]; hi[k] = (float)h[2 * k + 1]; }
@@ -182,7 +184,7 @@ __global__ __launch_bounds__(kRedBlock) void k_h_gs_project(unsigned n, unsigned
 #pragma unroll
             for (int q = 0; q < 8; ++q) ww += yc[q] * yc[q];
         }
-        h16_store_site(wc, ws, Vh, i, y);
+        h16_store_site(oc, os, Vh, i, y);
 #pragma unroll
         for (int k = 0; k < K; ++k) { acc[2 * k] += re[k]; acc[2 * k + 1] += im[k]; }
         acc[2 * K] += ww;
@@ -190,19 +192,25 @@ __global__ __launch_bounds__(kRedBlock) void k_h_gs_project(unsigned n, unsigned
     block_partial<2 * K + 1>(acc, partial);
 }
 
-// vnext = (w' - flag sum_k c2_k V_k) / |w''|; sc as written by the Gram-Schmidt finish
+// in place on vnext, which holds w': vnext = (w' - flag sum_k c2_k V_k) / |w''| (sc as written by
+// the Gram-Schmidt finish).  Without the second pass only the per-site scales change: the stored
+// halves of w'/|w'| are those of w'.
 template <int K>
-__global__ __launch_bounds__(256) void k_h_gs_finalize(unsigned n, unsigned Vh, HTab V, HPtr w, uint4* vc, float* vs,
+__global__ __launch_bounds__(256) void k_h_gs_finalize(unsigned n, unsigned Vh, HTab V, uint4* vc, float* vs,
                                 const double* __restrict__ sc) {
     const bool flag = sc[3] != 0.0;
     const float inv = (float)sc[2];
+    if (!flag) {
+        SITE_LOOP(i, n) vs[i] *= inv;
+        return;
+    }
     float cr[K], ci[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) { cr[k] = (float)sc[4 + 2 * k]; ci[k] = (float)sc[4 + 2 * k + 1]; }
     SITE_LOOP(i, n) {
         float y[24];
-        h16_load_site(w.c, w.s, Vh, i, y);
-        if (flag) {
+        h16_load_site(vc, vs, Vh, i, y);
+        {
             float sv[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) sv[k] = V.v[k].s[i];
@@ -277,8 +285,11 @@ unsigned nsites(const clg_field_s* f) { return (unsigned)((f->subset == CLG_FULL
 unsigned ew_grid16(clg_context ctx, unsigned n) {
     return std::max(1u, std::min((unsigned)ctx->num_cu * 8, (n + 255) / 256));
 }
+// reductions: two or more sites per lane at solver sizes (one site per lane spends as long on the
+// 2K+1-value workgroup reduction as on the loads; measured 3.5 % of an FP16-vector solve at 32^3x16)
 unsigned red_grid16(clg_context ctx, unsigned n) {
-    return std::max(1u, std::min(std::min((unsigned)kMaxGrid, (unsigned)ctx->num_cu * 4), (n + 255) / 256));
+    const unsigned cap = std::min((unsigned)kMaxGrid, (unsigned)ctx->num_cu * 2);
+    return std::max(1u, std::min(cap, (n + 255) / 256));
 }
 int check16(const clg_field_s* a, const clg_field_s* b) {
     CLG_CHECK_ARG(a && a->kind == CLG_WILSON && a->prec == CLG_F16 && a->norm,
@@ -364,16 +375,18 @@ int h16_gs_step(clg_context ctx, clg_field_s* const* V, int nv, clg_field_s* w, 
 #undef CALL
     CLG_HIP(hipGetLastError());
     CLG_TRY(red_finish_partials(ctx, g, 2 * nv + 1, hcol, 2 * nv, sc, nullptr, nullptr, 0.0, nullptr));
-#define CALL(KK) \
-    k_h_gs_project<KK><<<g, kRedBlock, 0, ctx->stream>>>(n, Vh, t, hcol, (uint4*)w->data, w->norm, ctx->red_partial);
+    // w' goes straight into the next basis vector when there is one (finalize then works in place)
+    clg_field_s* o = vnext ? vnext : w;
+#define CALL(KK)                                                                                             \
+    k_h_gs_project<KK><<<g, kRedBlock, 0, ctx->stream>>>(n, Vh, t, hcol, (const uint4*)w->data, w->norm, \
+                                                         (uint4*)o->data, o->norm, ctx->red_partial);
     SWITCH_V(nv, CALL)
 #undef CALL
     CLG_HIP(hipGetLastError());
     CLG_TRY(gs_finish_launch(ctx, g, nv, hcol, j, shifted, sc));
     if (vnext) {
         const unsigned ge = ew_grid16(ctx, n);
-#define CALL(KK) \
-    k_h_gs_finalize<KK><<<ge, 256, 0, ctx->stream>>>(n, Vh, t, hptr(w), (uint4*)vnext->data, vnext->norm, sc);
+#define CALL(KK) k_h_gs_finalize<KK><<<ge, 256, 0, ctx->stream>>>(n, Vh, t, (uint4*)vnext->data, vnext->norm, sc);
         SWITCH_V(nv, CALL)
 #undef CALL
         CLG_HIP(hipGetLastError());
