@@ -185,7 +185,7 @@ def lib():
             "clg_multiaxpy": (C.c_int, [vp, vp, C.c_int, C.POINTER(vp), dp]),
             "clg_vector_multiply_matrix": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.c_int, C.POINTER(vp), dp]),
             "clg_autotune_enable": (C.c_int, [C.c_int]),
-            "clg_autotune_force": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+            "clg_autotune_force": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
             "clg_autotune_reset": (C.c_int, []),
             "clg_autotune_count": (C.c_int, []),
             "clg_autotune_entry": (C.c_int, [C.c_int, C.c_char_p, C.c_int]),
@@ -403,8 +403,8 @@ def autotune_enable(mode):
     check(lib().clg_autotune_enable(int(mode)))
 
 
-def autotune_force(pin_mode=0, block=256, xcd=8):
-    check(lib().clg_autotune_force(int(pin_mode), int(block), int(xcd)))
+def autotune_force(pin_mode=0, block=256, xcd=8, sites_per_thread=1):
+    check(lib().clg_autotune_force(int(pin_mode), int(block), int(xcd), int(sites_per_thread)))
 
 
 def autotune_reset():

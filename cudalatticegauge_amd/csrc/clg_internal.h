@@ -154,9 +154,11 @@ void ks_tune_reset();
 
 // launch helpers
 inline unsigned grid_for(long n, int block) { return (unsigned)((n + block - 1) / block); }
-// grid for clgd::map_site: nblk parity blocks of Vh sites, optional parity interleave and XCD padding
-inline unsigned sched_grid(long Vh, int nblk, int block, int xcd, int pinter) {
+// grid for clgd::map_site: nblk parity blocks of Vh sites, optional parity interleave, spt logical
+// blocks per workgroup and XCD padding
+inline unsigned sched_grid(long Vh, int nblk, int block, int xcd, int pinter, int spt = 1) {
     long g = (nblk == 2 && pinter) ? 2 * ((Vh + block - 1) / block) : (nblk * Vh + block - 1) / block;
+    g = (g + spt - 1) / spt;
     if (xcd > 1) g = (g + xcd - 1) / xcd * xcd;
     return (unsigned)g;
 }
@@ -265,23 +267,32 @@ __device__ __forceinline__ Nbr bwd_nbr(const clg::Geo& g, const Site& s, int p) 
 // then meet in one L2.  With pinter = 1 a two-parity launch alternates parities block by
 // block, so a link's forward user (parity p) and backward user (parity 1-p) are co-resident.
 // The grid must be padded to a multiple of xcd (sched_grid) for the remap to be a bijection.
-__device__ __forceinline__ bool map_site(int Vh, int nblk, int pbase, int xcd, int pinter, int& p, int& h) {
+// A launch with spt logical blocks per workgroup (sites per thread) gives workgroup w the
+// logical blocks w*spt .. w*spt+spt-1 after the remap, so each XCD still sweeps one range.
+__device__ __forceinline__ unsigned logical_wg(int xcd) {
     unsigned wg = blockIdx.x;
     if (xcd > 1) {
         const unsigned per = gridDim.x / (unsigned)xcd;
         wg = (wg % (unsigned)xcd) * per + wg / (unsigned)xcd;
     }
+    return wg;
+}
+// site (p, h) of this thread in logical block lb of blockDim.x sites
+__device__ __forceinline__ bool block_site(unsigned lb, int Vh, int nblk, int pbase, int pinter, int& p, int& h) {
     if (nblk == 2 && pinter) {
-        p = (int)(wg & 1u);
-        h = (int)((wg >> 1) * blockDim.x + threadIdx.x);
+        p = (int)(lb & 1u);
+        h = (int)((lb >> 1) * blockDim.x + threadIdx.x);
         return h < Vh;
     }
-    const long gid = (long)wg * blockDim.x + threadIdx.x;
+    const long gid = (long)lb * blockDim.x + threadIdx.x;
     if (gid >= (long)nblk * Vh) return false;
     const int blk = (int)(gid / Vh);
     h = (int)(gid - (long)blk * Vh);
     p = nblk == 2 ? blk : pbase;
     return true;
+}
+__device__ __forceinline__ bool map_site(int Vh, int nblk, int pbase, int xcd, int pinter, int& p, int& h) {
+    return block_site(logical_wg(xcd), Vh, nblk, pbase, pinter, p, h);
 }
 
 // ------------------------------------------------------------------ storage traits

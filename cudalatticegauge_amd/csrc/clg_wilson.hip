@@ -41,6 +41,7 @@ struct HopArgs {
     double ar, ai, br, bi;
     int pbase, nblk;     // output parities handled
     int xcd, pinter;     // work decomposition (clgd::map_site)
+    int spt;             // logical blocks (sites per thread) per workgroup
 };
 
 // Neighbour spinors come from the input field, or (STAGED, z and t only) from the copy of this
@@ -94,8 +95,10 @@ __global__ __launch_bounds__(256) void k_wilson_hop(HopArgs a) {
     typedef LinkIO<GP, RECON> LIO;
     constexpr bool STAGED = PM >= 5;
     constexpr int PMH = STAGED ? PM - 4 : PM;
+    const unsigned lw = logical_wg(a.xcd);
+    for (int vs = 0; vs < a.spt; ++vs) {
     int p, h;
-    const bool valid = map_site(a.g.Vh, a.nblk, a.pbase, a.xcd, a.pinter, p, h);
+    const bool valid = block_site(lw * (unsigned)a.spt + (unsigned)vs, a.g.Vh, a.nblk, a.pbase, a.pinter, p, h);
     const char* st = nullptr;
     const float* stn = nullptr;
     int hbase = 0;
@@ -103,8 +106,9 @@ __global__ __launch_bounds__(256) void k_wilson_hop(HopArgs a) {
         extern __shared__ uint4 stage[];
         const int B = (int)blockDim.x;
         float* stage_n = reinterpret_cast<float*>(stage + SIO::NPL * B);
-        // padded workgroups (xcd remap) are whole: every lane of a workgroup agrees on valid
-        if (!valid) return;
+        // padded blocks (xcd remap, spt) are whole: every lane of a workgroup agrees on valid,
+        // and blocks past the end stay past it
+        if (!valid) break;
         hbase = h - (int)threadIdx.x;
         const unsigned pe = a.in_full ? (unsigned)(1 - p) : 0u;
 #pragma unroll
@@ -114,7 +118,7 @@ __global__ __launch_bounds__(256) void k_wilson_hop(HopArgs a) {
         st = reinterpret_cast<const char*>(stage);
         stn = stage_n;
     } else {
-        if (!valid) return;
+        if (!valid) continue;
     }
     const Site s = site_of(a.g, p, h);
     const unsigned po = a.out_full ? (unsigned)p : 0u;
@@ -145,6 +149,8 @@ __global__ __launch_bounds__(256) void k_wilson_hop(HopArgs a) {
         for (int k = 0; k < 12; ++k) o[k] = cmul(ca, acc[k]);
     }
     SIO::store(a.out, a.outn, a.g.Vh, eo, no, o);
+    if constexpr (STAGED) __syncthreads();  // the next block rewrites the stage
+    }
 }
 
 struct HopCfg {
@@ -154,14 +160,16 @@ struct HopCfg {
     int block;  // threads per workgroup
     int xcd;    // 0: hardware order; 8: XCD-contiguous slabs, for full D with the two parities
                 // interleaved block by block; 1: XCD-contiguous slabs, parities one after the other
+    int spt = 1;  // sites per thread: consecutive logical blocks per workgroup (1, 2 or 4)
 };
 
 template <typename R, int SP, int GP, int RECON, int PM>
-void launch_pm(const HopArgs& a0, bool dag, bool hasx, hipStream_t st, int block, int xcd) {
+void launch_pm(const HopArgs& a0, bool dag, bool hasx, hipStream_t st, int block, int xcd, int spt) {
     HopArgs a = a0;
     a.xcd = xcd ? 8 : 0;
     a.pinter = xcd == 8 ? 1 : 0;
-    const unsigned grid = sched_grid(a.g.Vh, a.nblk, block, a.xcd, a.pinter);
+    a.spt = spt;
+    const unsigned grid = sched_grid(a.g.Vh, a.nblk, block, a.xcd, a.pinter, spt);
     // staged launches: NPL 16-byte chunks per site of the plane, plus the FP16 site scales
     const size_t shm = PM >= 5 ? (size_t)block * (16 * WilsonIO<SP>::NPL + (WilsonIO<SP>::SCALED ? 4 : 0)) : 0;
     if (dag) {
@@ -175,14 +183,14 @@ void launch_pm(const HopArgs& a0, bool dag, bool hasx, hipStream_t st, int block
 
 template <typename R, int SP, int GP, int RECON>
 int launch_t(const HopArgs& a, bool dag, bool hasx, hipStream_t st, HopCfg c) {
-    if (c.pm == 8) launch_pm<R, SP, GP, RECON, 8>(a, dag, hasx, st, c.block, c.xcd);
-    else if (c.pm == 7) launch_pm<R, SP, GP, RECON, 7>(a, dag, hasx, st, c.block, c.xcd);
-    else if (c.pm == 6) launch_pm<R, SP, GP, RECON, 6>(a, dag, hasx, st, c.block, c.xcd);
-    else if (c.pm == 5) launch_pm<R, SP, GP, RECON, 5>(a, dag, hasx, st, c.block, c.xcd);
-    else if (c.pm == 4) launch_pm<R, SP, GP, RECON, 4>(a, dag, hasx, st, c.block, c.xcd);
-    else if (c.pm == 3) launch_pm<R, SP, GP, RECON, 3>(a, dag, hasx, st, c.block, c.xcd);
-    else if (c.pm == 2) launch_pm<R, SP, GP, RECON, 2>(a, dag, hasx, st, c.block, c.xcd);
-    else launch_pm<R, SP, GP, RECON, 1>(a, dag, hasx, st, c.block, c.xcd);
+    if (c.pm == 8) launch_pm<R, SP, GP, RECON, 8>(a, dag, hasx, st, c.block, c.xcd, c.spt);
+    else if (c.pm == 7) launch_pm<R, SP, GP, RECON, 7>(a, dag, hasx, st, c.block, c.xcd, c.spt);
+    else if (c.pm == 6) launch_pm<R, SP, GP, RECON, 6>(a, dag, hasx, st, c.block, c.xcd, c.spt);
+    else if (c.pm == 5) launch_pm<R, SP, GP, RECON, 5>(a, dag, hasx, st, c.block, c.xcd, c.spt);
+    else if (c.pm == 4) launch_pm<R, SP, GP, RECON, 4>(a, dag, hasx, st, c.block, c.xcd, c.spt);
+    else if (c.pm == 3) launch_pm<R, SP, GP, RECON, 3>(a, dag, hasx, st, c.block, c.xcd, c.spt);
+    else if (c.pm == 2) launch_pm<R, SP, GP, RECON, 2>(a, dag, hasx, st, c.block, c.xcd, c.spt);
+    else launch_pm<R, SP, GP, RECON, 1>(a, dag, hasx, st, c.block, c.xcd, c.spt);
     CLG_HIP(hipGetLastError());
     return CLG_OK;
 }
@@ -233,26 +241,28 @@ std::mutex& tune_mutex() {
     static std::mutex m;
     return m;
 }
-bool valid_cfg(int pm, int blk, int xcd) {
-    return pm >= 1 && pm <= 8 && (blk == 64 || blk == 128 || blk == 256) && (xcd == 0 || xcd == 1 || xcd == 8);
+bool valid_cfg(int pm, int blk, int xcd, int spt) {
+    return pm >= 1 && pm <= 8 && (blk == 64 || blk == 128 || blk == 256) && (xcd == 0 || xcd == 1 || xcd == 8) &&
+           (spt == 1 || spt == 2 || spt == 4);
 }
 // process-wide overrides set through clg_autotune_enable / clg_autotune_force
 std::atomic<int> g_tune_mode{-1};  // -1: CLG_TUNE environment (default on), 0 off, 1 on
-std::atomic<int> g_forced{0};      // packed pin | block << 4 | xcd << 16, 0 = none
-// CLG_TUNE_FORCE="pin,block,xcd" (or clg_autotune_force) fixes the launch configuration of every
-// hop launch (used to profile one configuration under counters, where event timings are not
-// representative)
+std::atomic<int> g_forced{0};      // packed pin | block << 4 | xcd << 16 | spt << 20, 0 = none
+// CLG_TUNE_FORCE="pin,block,xcd[,spt]" (or clg_autotune_force) fixes the launch configuration of
+// every hop launch (used to profile one configuration under counters, where event timings are
+// not representative)
 bool forced_cfg(HopCfg& c) {
     const int f = g_forced.load();
     if (f) {
-        c = HopCfg{f & 15, (f >> 4) & 4095, f >> 16};
+        c = HopCfg{f & 15, (f >> 4) & 4095, (f >> 16) & 15, f >> 20};
         return true;
     }
     const char* e = std::getenv("CLG_TUNE_FORCE");
     if (!e) return false;
-    int pm = 0, blk = 0, xcd = 0;
-    if (std::sscanf(e, "%d,%d,%d", &pm, &blk, &xcd) != 3 || !valid_cfg(pm, blk, xcd)) return false;
-    c = HopCfg{pm, blk, xcd};
+    int pm = 0, blk = 0, xcd = 0, spt = 1;
+    const int got = std::sscanf(e, "%d,%d,%d,%d", &pm, &blk, &xcd, &spt);
+    if (got < 3 || !valid_cfg(pm, blk, xcd, spt)) return false;
+    c = HopCfg{pm, blk, xcd, spt};
     return true;
 }
 
@@ -285,8 +295,13 @@ int launch_hop(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool 
         CLG_HIP(hipEventCreate(&e1));
         float best = 1e30f;
         std::vector<HopCfg> all(std::begin(cands), std::end(cands));
+        // several sites per thread for the XCD-slab launches
+        for (int spt : {2, 4})
+            for (int pm : {2, 4})
+                for (int blk : {128, 256}) all.push_back(HopCfg{pm, blk, 8, spt});
         if (const int sb = staged_block(a.g)) {
             for (int pm = 5; pm <= 8; ++pm) all.push_back(HopCfg{pm, sb, 8});
+            for (int pm : {6, 8}) all.push_back(HopCfg{pm, sb, 8, 2});
         }
         for (const HopCfg& c : all) {
             if (c.xcd == 1 && a.nblk != 2) continue;  // one parity: the same launch as xcd 8
@@ -303,9 +318,9 @@ int launch_hop(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool 
         (void)hipEventDestroy(e1);
         const char* verbose = std::getenv("CLG_TUNE_VERBOSE");
         if (verbose && verbose[0] == '1')
-            std::fprintf(stderr, "[clg tune] wilson hop prec=%d gprec=%d recon=%d dag=%d x=%d nblk=%d L=%dx%dx%dx%d -> pin=%d block=%d xcd=%d (%.1f us)\n",
+            std::fprintf(stderr, "[clg tune] wilson hop prec=%d gprec=%d recon=%d dag=%d x=%d nblk=%d L=%dx%dx%dx%d -> pin=%d block=%d xcd=%d spt=%d (%.1f us)\n",
                          prec, gprec, recon, (int)dag, (int)hasx, a.nblk, a.g.L[0], a.g.L[1], a.g.L[2], a.g.L[3],
-                         cfg.pm, cfg.block, cfg.xcd, best * 1e3 / 3);
+                         cfg.pm, cfg.block, cfg.xcd, cfg.spt, best * 1e3 / 3);
         std::lock_guard<std::mutex> lk(tune_mutex());
         tune_table()[k] = cfg;
     }
@@ -512,13 +527,15 @@ int clg_autotune_enable(int mode) {
     return CLG_OK;
 }
 
-int clg_autotune_force(int pin_mode, int block, int xcd) {
+int clg_autotune_force(int pin_mode, int block, int xcd, int sites_per_thread) {
     if (pin_mode == 0) {
         g_forced.store(0);
         return CLG_OK;
     }
-    CLG_CHECK_ARG(valid_cfg(pin_mode, block, xcd), "pin_mode 1..8, block 64|128|256, xcd 0|1|8");
-    g_forced.store(pin_mode | block << 4 | xcd << 16);
+    if (sites_per_thread == 0) sites_per_thread = 1;
+    CLG_CHECK_ARG(valid_cfg(pin_mode, block, xcd, sites_per_thread),
+                  "pin_mode 1..8, block 64|128|256, xcd 0|1|8, sites_per_thread 1|2|4");
+    g_forced.store(pin_mode | block << 4 | xcd << 16 | sites_per_thread << 20);
     return CLG_OK;
 }
 
@@ -552,8 +569,9 @@ int clg_autotune_entry(int index, char* buf, int len) {
             const HopCfg& c = it->second;
             std::snprintf(buf, (size_t)len,
                           "wilson_hop prec=%d gauge_prec=%d recon=%d dagger=%d x=%d parities=%d dims=%dx%dx%dx%d "
-                          "-> pin=%d block=%d xcd=%d",
-                          k.prec, k.gprec, k.recon, k.dag, k.hasx, k.nblk, k.L0, k.L1, k.L2, k.L3, c.pm, c.block, c.xcd);
+                          "-> pin=%d block=%d xcd=%d spt=%d",
+                          k.prec, k.gprec, k.recon, k.dag, k.hasx, k.nblk, k.L0, k.L1, k.L2, k.L3, c.pm, c.block, c.xcd,
+                          c.spt);
             return CLG_OK;
         }
         index -= (int)tune_table().size();

@@ -413,10 +413,11 @@ int clg_solve_batch(clg_context ctx, const clg_solver_params* params, int n, clg
 int clg_autotune_enable(int mode);
 /* fix every single-system hop launch: pin_mode 1|2 (register pinning per half / per direction), 3|4 (the
  * same with non-temporal link loads), 5..8 (as 1..4 with the z and t neighbours staged in LDS; needs
- * block = L2*L3/2 with L3 even, otherwise the launch runs as 1..4), block 64|128|256, xcd 0 (hardware workgroup order) | 8 (each XCD
- * sweeps a contiguous slab of sites; full D interleaves the parities block by block) | 1 (XCD slabs,
- * parities one after the other); pin_mode 0 clears */
-int clg_autotune_force(int pin_mode, int block, int xcd);
+ * block = L2*L3/2 with L3 even, otherwise the launch runs as 1..4), block 64|128|256, xcd 0 (hardware
+ * workgroup order) | 8 (each XCD sweeps a contiguous slab of sites; full D interleaves the parities
+ * block by block) | 1 (XCD slabs, parities one after the other), sites_per_thread 1|2|4 (0 = 1:
+ * consecutive blocks of sites per workgroup); pin_mode 0 clears */
+int clg_autotune_force(int pin_mode, int block, int xcd, int sites_per_thread);
 /* The staggered D tunes its own launch (block 64|128|256, hardware or XCD-slab order, pinning);
  * the environment variable CLG_TUNE_FORCE_KS="block,xcd,pin" (xcd 0|8, pin 0|2) fixes it. */
 /* forget every tuned shape (the next call of each shape tunes again) */

@@ -52,9 +52,9 @@ def test_solver_defaults_follow_reference_yaml_defaults():
 def test_tuner_and_graph_entry_points_check_arguments_without_a_device():
     L = clg.lib()
     assert L.clg_autotune_enable(2) == 2 and L.clg_autotune_enable(-2) == 2
-    assert L.clg_autotune_force(9, 256, 8) == 2 and L.clg_autotune_force(1, 96, 8) == 2
-    assert L.clg_autotune_force(1, 128, 3) == 2
-    assert L.clg_autotune_force(0, 0, 0) == 0   # clearing needs no device
+    assert L.clg_autotune_force(9, 256, 8, 1) == 2 and L.clg_autotune_force(1, 96, 8, 1) == 2
+    assert L.clg_autotune_force(1, 128, 3, 1) == 2 and L.clg_autotune_force(1, 128, 8, 3) == 2
+    assert L.clg_autotune_force(0, 0, 0, 0) == 0   # clearing needs no device
     assert L.clg_autotune_enable(-1) == 0
     assert L.clg_autotune_count() >= 0
     buf = C.create_string_buffer(64)

@@ -101,9 +101,10 @@ def test_autotune_controls(ctx):
     ref = cvec(y0)
     ent = clg.autotune_entries()
     assert len(ent) >= 1 and any(e.startswith("wilson_hop") and "dims=8x8x8x16" in e for e in ent)
-    for pm, blk, xcd in [(1, 64, 0), (2, 128, 8), (1, 256, 8), (3, 128, 8), (4, 256, 0), (3, 128, 1), (4, 64, 1), (1, 256, 1),
-                         (5, 64, 8), (6, 64, 1)]:
-        clg.autotune_force(pm, blk, xcd)
+    for pm, blk, xcd, spt in [(1, 64, 0, 1), (2, 128, 8, 1), (1, 256, 8, 1), (3, 128, 8, 1), (4, 256, 0, 1),
+                              (3, 128, 1, 1), (4, 64, 1, 1), (1, 256, 1, 1), (5, 64, 8, 1), (6, 64, 1, 1),
+                              (2, 128, 8, 2), (4, 256, 8, 4), (1, 64, 0, 4), (6, 64, 8, 2), (8, 64, 8, 4)]:
+        clg.autotune_force(pm, blk, xcd, spt)
         y = clg.Field(ctx, clg.WILSON, clg.F64, dims)
         clg.wilson_apply(ctx, y, x, g, 0.12, clg.EFO_F_D)
         assert np.max(np.abs(cvec(y) - ref)) <= 1e-13 * np.max(np.abs(ref))

@@ -203,7 +203,8 @@ def test_precision_conversion(ctx, c0):
 @pytest.mark.parametrize("dims", [(4, 6, 8, 16), (4, 4, 16, 16)])
 def test_lds_staged_launches(ctx, dims):
     """Pin modes 5..8 stage the z and t neighbours in LDS (one workgroup per (x,y) plane of one
-    parity, block = L2*L3/2: 64 and 128 here); a staged mode whose block does not fit runs unstaged."""
+    parity, block = L2*L3/2: 64 and 128 here), also with several planes per workgroup in turn
+    (sites per thread); a staged mode whose block does not fit runs unstaged."""
     plane = dims[2] * dims[3] // 2
     U = orc.gauge_random(dims, 0.3, 11)
     psi = orc.wilson_gaussian(dims, 12)
@@ -222,14 +223,15 @@ def test_lds_staged_launches(ctx, dims):
     cases = ((f64, g64, 1e-12), (f32, g32, 1e-6), (f16, g16, 1e-2))
     ref_eo = refs[False].reshape(-1, 24)
     try:
-        for pm, blk, xcd in ((1, plane, 8), (5, plane, 8), (6, plane, 8), (6, plane, 0), (5, plane, 1),
-                             (7, plane, 8), (8, plane, 0), (5, 256, 8), (8, 256, 8)):
-            clg.autotune_force(pm, blk, xcd)
+        for pm, blk, xcd, spt in ((1, plane, 8, 1), (5, plane, 8, 1), (6, plane, 8, 1), (6, plane, 0, 1),
+                                  (5, plane, 1, 1), (7, plane, 8, 1), (8, plane, 0, 1), (5, 256, 8, 1),
+                                  (8, 256, 8, 1), (6, plane, 8, 2), (8, plane, 8, 4), (5, plane, 0, 4)):
+            clg.autotune_force(pm, blk, xcd, spt)
             for f, g, tol in cases:
                 for dag in (False, True):
                     out = clg.Field(ctx, clg.WILSON, f.prec, dims)
                     clg.wilson_apply(ctx, out, f, g, KAPPA, clg.EFO_F_Ddagger if dag else clg.EFO_F_D)
-                    assert rel(out.download(), refs[dag]) < tol, (pm, blk, xcd, f.prec, dag)
+                    assert rel(out.download(), refs[dag]) < tol, (pm, blk, xcd, spt, f.prec, dag)
             # even-odd hops read a half field: out_p = x_p - kappa H psi_(1-p)
             for out_sub, par in ((clg.EVEN, 0), (clg.ODD, 1)):
                 in_sub = clg.ODD if out_sub == clg.EVEN else clg.EVEN
