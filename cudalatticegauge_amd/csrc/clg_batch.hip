@@ -274,6 +274,10 @@ std::mutex& btune_mutex() {
     static std::mutex m;
     return m;
 }
+std::string& btune_loaded() {  // the tuning-cache file whose entries are in btune()
+    static std::string p;
+    return p;
+}
 
 int check_batch(const HopBatch& B) {
     CLG_CHECK_ARG(B.n >= 1 && B.n <= kMaxSys, "batch needs 1..%d systems", kMaxSys);
@@ -345,9 +349,23 @@ int batch_hop(clg_context ctx, const HopBatch& B, bool dag, double br, int tile)
     }
     BCfg cfg{2, 128, 8};
     bool need = false;
+    const std::string cpath = tune_cache_path();
     {
         std::lock_guard<std::mutex> lk(btune_mutex());
         auto it = btune().find(key);
+        if (it == btune().end() && !cpath.empty() && cpath != btune_loaded()) {
+            std::vector<std::pair<std::vector<long>, std::vector<long>>> ent;
+            tune_cache_read(cpath, "wilson_hop_batch", ent);
+            for (const auto& e : ent) {
+                const std::vector<long>& c = e.second;
+                if (e.first.size() < 13 || (e.first.size() - 9) % 4 || c.size() != 3) continue;
+                const bool okc = (c[0] == 1 || c[0] == 2 || (c[0] == 4 && e.first[0] != CLG_F64)) &&
+                                 (c[1] == 64 || c[1] == 128 || c[1] == 256) && (c[2] == 0 || c[2] == 8);
+                if (okc) btune()[e.first] = BCfg{(int)c[0], (int)c[1], (int)c[2]};
+            }
+            btune_loaded() = cpath;
+            it = btune().find(key);
+        }
         if (it != btune().end()) cfg = it->second;
         else need = true;
     }
@@ -383,6 +401,7 @@ int batch_hop(clg_context ctx, const HopBatch& B, bool dag, double br, int tile)
                          B.n, in0->prec, (int)dag, (int)hasx, nblk, cfg.nr, cfg.block, cfg.xcd, best * 1e3 / 3);
         std::lock_guard<std::mutex> lk(btune_mutex());
         btune()[key] = cfg;
+        tune_cache_append(cpath, "wilson_hop_batch", key, {cfg.nr, cfg.block, cfg.xcd});
     }
     return launch_batch_cfg(B, dag, hasx, br, pbase, nblk, cfg, st);
 }
@@ -1478,6 +1497,7 @@ int batch_tune_entry(int index, char* buf, int len) {
 void batch_tune_reset() {
     std::lock_guard<std::mutex> lk(btune_mutex());
     btune().clear();
+    btune_loaded().clear();
 }
 
 }  // namespace clg

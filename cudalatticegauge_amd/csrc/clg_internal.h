@@ -148,6 +148,16 @@ bool tuning_enabled();
 int batch_tune_count();
 int batch_tune_entry(int index, char* buf, int len);
 void batch_tune_reset();
+// Persistent tuning cache (CLG_TUNE_CACHE=<file>): one text line per tuned key,
+//   "<device arch> <tuner> <key integers> : <configuration integers>".
+// A tuner reads the lines of its name and of the current device's architecture the first time it
+// misses a key under a cache path (again after clg_autotune_reset), and appends every key it tunes.
+std::string tune_cache_path();
+std::string device_arch();
+int tune_cache_read(const std::string& path, const char* tuner,
+                    std::vector<std::pair<std::vector<long>, std::vector<long>>>& out);
+void tune_cache_append(const std::string& path, const char* tuner, const std::vector<long>& key,
+                       const std::vector<long>& cfg);
 int ks_tune_count();
 int ks_tune_entry(int index, char* buf, int len);
 void ks_tune_reset();

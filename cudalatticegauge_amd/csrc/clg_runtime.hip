@@ -5,7 +5,14 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <cctype>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
 #include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
 
 namespace clg {
 
@@ -360,6 +367,77 @@ bool stream_capturing(clg_context ctx) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(ctx->stream, &cs) != hipSuccess) return false;
     return cs != hipStreamCaptureStatusNone;
+}
+
+// ---- persistent tuning cache
+std::string tune_cache_path() {
+    const char* e = std::getenv("CLG_TUNE_CACHE");
+    return e ? std::string(e) : std::string();
+}
+
+std::string device_arch() {
+    static std::mutex m;
+    static std::map<int, std::string> names;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(m);
+    auto it = names.find(dev);
+    if (it != names.end()) return it->second;
+    hipDeviceProp_t prop;
+    std::string name = "unknown";
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+        name = prop.gcnArchName;
+        for (char& c : name)
+            if (std::isspace((unsigned char)c)) c = '_';
+    }
+    names[dev] = name;
+    return name;
+}
+
+static std::mutex& cache_file_mutex() {
+    static std::mutex m;
+    return m;
+}
+
+int tune_cache_read(const std::string& path, const char* tuner, std::vector<std::pair<std::vector<long>, std::vector<long>>>& out) {
+    out.clear();
+    std::lock_guard<std::mutex> lk(cache_file_mutex());
+    std::FILE* f = std::fopen(path.c_str(), "r");
+    if (!f) return 0;
+    const std::string arch = device_arch();
+    char line[2048];
+    while (std::fgets(line, sizeof(line), f)) {
+        std::istringstream in(line);
+        std::string a, t, tok;
+        if (!(in >> a >> t) || a != arch || t != tuner) continue;
+        std::vector<long> key, cfg;
+        bool colon = false, ok = true;
+        while (in >> tok) {
+            if (tok == ":") { colon = true; continue; }
+            char* end = nullptr;
+            const long v = std::strtol(tok.c_str(), &end, 10);
+            if (!end || *end) { ok = false; break; }
+            (colon ? cfg : key).push_back(v);
+        }
+        if (ok && colon && !key.empty() && !cfg.empty()) out.emplace_back(key, cfg);
+    }
+    std::fclose(f);
+    return (int)out.size();
+}
+
+void tune_cache_append(const std::string& path, const char* tuner, const std::vector<long>& key, const std::vector<long>& cfg) {
+    if (path.empty()) return;
+    std::ostringstream o;
+    o << device_arch() << ' ' << tuner;
+    for (long v : key) o << ' ' << v;
+    o << " :";
+    for (long v : cfg) o << ' ' << v;
+    o << '\n';
+    std::lock_guard<std::mutex> lk(cache_file_mutex());
+    if (std::FILE* f = std::fopen(path.c_str(), "a")) {
+        std::fputs(o.str().c_str(), f);
+        std::fclose(f);
+    }
 }
 }  // namespace clg
 

@@ -162,6 +162,10 @@ std::mutex& ks_mutex() {
     static std::mutex m;
     return m;
 }
+std::string& ks_loaded() {  // the tuning-cache file whose entries are in ks_table()
+    static std::string p;
+    return p;
+}
 
 bool ks_valid(const KSCfg& c) {
     return (c.block == 64 || c.block == 128 || c.block == 256) && (c.xcd == 0 || c.xcd == 8) && (c.pm == 0 || c.pm == 2);
@@ -178,9 +182,26 @@ int ks_launch(const KSArgs& a, int prec, bool r12, hipStream_t st) {
     std::memset(&k, 0, sizeof(k));
     k.prec = prec; k.r12 = r12; k.L0 = a.g.L[0]; k.L1 = a.g.L[1]; k.L2 = a.g.L[2]; k.L3 = a.g.L[3];
     bool need_tune = false;
+    const std::string cpath = tune_cache_path();
     {
         std::lock_guard<std::mutex> lk(ks_mutex());
         auto it = ks_table().find(k);
+        if (it == ks_table().end() && !cpath.empty() && cpath != ks_loaded()) {
+            std::vector<std::pair<std::vector<long>, std::vector<long>>> ent;
+            tune_cache_read(cpath, "staggered_d", ent);
+            for (const auto& e : ent) {
+                const std::vector<long>& v = e.first;
+                const std::vector<long>& c = e.second;
+                const KSCfg kc{(int)c[0], c.size() > 1 ? (int)c[1] : -1, c.size() > 2 ? (int)c[2] : -1};
+                if (v.size() != 6 || c.size() != 3 || !ks_valid(kc)) continue;
+                KSKey t;
+                std::memset(&t, 0, sizeof(t));
+                t.prec = (int)v[0]; t.r12 = (int)v[1]; t.L0 = (int)v[2]; t.L1 = (int)v[3]; t.L2 = (int)v[4]; t.L3 = (int)v[5];
+                ks_table()[t] = kc;
+            }
+            ks_loaded() = cpath;
+            it = ks_table().find(k);
+        }
         if (it != ks_table().end()) cfg = it->second;
         else need_tune = true;
     }
@@ -212,6 +233,7 @@ int ks_launch(const KSArgs& a, int prec, bool r12, hipStream_t st) {
                          best * 1e3 / 3);
         std::lock_guard<std::mutex> lk(ks_mutex());
         ks_table()[k] = cfg;
+        tune_cache_append(cpath, "staggered_d", {k.prec, k.r12, k.L0, k.L1, k.L2, k.L3}, {cfg.block, cfg.xcd, cfg.pm});
     }
     return ks_launch_cfg(a, prec, r12, st, cfg);
 }
@@ -239,6 +261,7 @@ int ks_tune_entry(int index, char* buf, int len) {
 void ks_tune_reset() {
     std::lock_guard<std::mutex> lk(ks_mutex());
     ks_table().clear();
+    ks_loaded().clear();
 }
 
 int ks_full(clg_context ctx, clg_field_s* out, const clg_field_s* in, const clg_field_s* gauge, double two_am, bool dag,

@@ -241,6 +241,14 @@ std::mutex& tune_mutex() {
     static std::mutex m;
     return m;
 }
+// the tuning-cache file whose entries are in tune_table() (under tune_mutex)
+std::string& tune_loaded() {
+    static std::string p;
+    return p;
+}
+std::vector<long> key_vec(const TuneKey& k) {
+    return {k.prec, k.gprec, k.recon, k.dag, k.hasx, k.nblk, k.L0, k.L1, k.L2, k.L3};
+}
 bool valid_cfg(int pm, int blk, int xcd, int spt) {
     return pm >= 1 && pm <= 8 && (blk == 64 || blk == 128 || blk == 256) && (xcd == 0 || xcd == 1 || xcd == 8) &&
            (spt == 1 || spt == 2 || spt == 4);
@@ -275,9 +283,26 @@ int launch_hop(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool 
     HopCfg cfg = dflt;
     if (forced_cfg(cfg)) return launch_cfg(a, prec, gprec, recon, dag, hasx, st, cfg);
     bool need_tune = false;
+    const std::string cpath = tune_cache_path();
     {
         std::lock_guard<std::mutex> lk(tune_mutex());
         auto it = tune_table().find(k);
+        if (it == tune_table().end() && !cpath.empty() && cpath != tune_loaded()) {
+            std::vector<std::pair<std::vector<long>, std::vector<long>>> ent;
+            tune_cache_read(cpath, "wilson_hop", ent);
+            for (const auto& e : ent) {
+                const std::vector<long>& v = e.first;
+                const std::vector<long>& c = e.second;
+                if (v.size() != 10 || c.size() != 4 || !valid_cfg((int)c[0], (int)c[1], (int)c[2], (int)c[3])) continue;
+                TuneKey t;
+                std::memset(&t, 0, sizeof(t));
+                t.prec = (int)v[0]; t.gprec = (int)v[1]; t.recon = (int)v[2]; t.dag = (int)v[3]; t.hasx = (int)v[4];
+                t.nblk = (int)v[5]; t.L0 = (int)v[6]; t.L1 = (int)v[7]; t.L2 = (int)v[8]; t.L3 = (int)v[9];
+                tune_table()[t] = HopCfg{(int)c[0], (int)c[1], (int)c[2], (int)c[3]};
+            }
+            tune_loaded() = cpath;
+            it = tune_table().find(k);
+        }
         if (it != tune_table().end()) cfg = it->second;
         else need_tune = true;
     }
@@ -323,6 +348,7 @@ int launch_hop(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool 
                          cfg.pm, cfg.block, cfg.xcd, cfg.spt, best * 1e3 / 3);
         std::lock_guard<std::mutex> lk(tune_mutex());
         tune_table()[k] = cfg;
+        tune_cache_append(cpath, "wilson_hop", key_vec(k), {cfg.pm, cfg.block, cfg.xcd, cfg.spt});
     }
     return launch_cfg(a, prec, gprec, recon, dag, hasx, st, cfg);
 }
@@ -543,6 +569,7 @@ int clg_autotune_reset(void) {
     {
         std::lock_guard<std::mutex> lk(tune_mutex());
         tune_table().clear();
+        tune_loaded().clear();
     }
     batch_tune_reset();
     ks_tune_reset();

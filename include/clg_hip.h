@@ -418,6 +418,11 @@ int clg_autotune_enable(int mode);
  * block by block) | 1 (XCD slabs, parities one after the other), sites_per_thread 1|2|4 (0 = 1:
  * consecutive blocks of sites per workgroup); pin_mode 0 clears */
 int clg_autotune_force(int pin_mode, int block, int xcd, int sites_per_thread);
+/* Tuned configurations persist across processes when the environment variable CLG_TUNE_CACHE names a
+ * file: every tuned key is appended as one line "<device arch> <tuner> <key> : <configuration>", and
+ * the tuners take the lines of the current device architecture from the file before timing a key
+ * (again after clg_autotune_reset).  Lines that do not parse, or configurations no launch accepts,
+ * are skipped. */
 /* The staggered D tunes its own launch (block 64|128|256, hardware or XCD-slab order, pinning);
  * the environment variable CLG_TUNE_FORCE_KS="block,xcd,pin" (xcd 0|8, pin 0|2) fixes it. */
 /* forget every tuned shape (the next call of each shape tunes again) */

@@ -168,3 +168,53 @@ def test_graph_replays_match_eager(ctx):
         _ = clg.norm2(ctx, x)
         clg.wilson_apply(ctx, t, x, g, 0.2, clg.EFO_F_Ddagger)
     graph.close()
+
+
+def test_autotune_cache_file(ctx, tmp_path, monkeypatch):
+    """CLG_TUNE_CACHE: every tuned shape is appended to the file (keyed by the device architecture),
+    and a fresh tuner table takes its configurations from the file instead of timing again."""
+    path = tmp_path / "tune.txt"
+    monkeypatch.setenv("CLG_TUNE_CACHE", str(path))
+    dims = (8, 8, 8, 16)
+    g = clg.Field(ctx, clg.GAUGE, clg.F64, dims)
+    g.gauge_random(0.5, 3)
+    x = clg.Field(ctx, clg.WILSON, clg.F64, dims)
+    x.gaussian(4)
+    k = clg.Field(ctx, clg.KS, clg.F64, dims)
+    k.gaussian(5)
+    try:
+        clg.autotune_reset()
+        clg.autotune_enable(1)
+        y0 = clg.Field(ctx, clg.WILSON, clg.F64, dims)
+        clg.wilson_apply(ctx, y0, x, g, 0.12, clg.EFO_F_D)
+        k0 = clg.Field(ctx, clg.KS, clg.F64, dims)
+        clg.ks_apply(ctx, k0, k, g, 0.05, clg.EFO_F_D)
+        lines = path.read_text().splitlines()
+        arch = ctx.info().split()[0].split(":")[0]
+        assert any(" wilson_hop " in ln and " 8 8 8 16 :" in ln for ln in lines), lines
+        assert any(" staggered_d " in ln for ln in lines), lines
+        assert all(ln.startswith(arch) for ln in lines), (arch, lines)
+        # replace the tuned configurations; a fresh table loads these instead of tuning
+        new = []
+        for ln in lines:
+            head = ln.split(" :")[0]
+            if " wilson_hop " in ln:
+                ln = head + " : 1 64 0 2"
+            elif " staggered_d " in ln:
+                ln = head + " : 64 0 2"
+            new.append(ln)
+        path.write_text("\n".join(new) + "\nnot a cache line\n")
+        clg.autotune_reset()
+        y1 = clg.Field(ctx, clg.WILSON, clg.F64, dims)
+        clg.wilson_apply(ctx, y1, x, g, 0.12, clg.EFO_F_D)
+        k1 = clg.Field(ctx, clg.KS, clg.F64, dims)
+        clg.ks_apply(ctx, k1, k, g, 0.05, clg.EFO_F_D)
+        ent = clg.autotune_entries()
+        assert any(e.startswith("wilson_hop") and "pin=1 block=64 xcd=0 spt=2" in e for e in ent), ent
+        assert any(e.startswith("staggered_d") and "block=64 xcd=0 pin=2" in e for e in ent), ent
+        assert len(path.read_text().splitlines()) == len(new) + 1   # nothing was tuned again
+        assert np.max(np.abs(cvec(y1) - cvec(y0))) <= 1e-13 * np.max(np.abs(cvec(y0)))
+        assert np.max(np.abs(k1.download() - k0.download())) <= 1e-13 * np.max(np.abs(k0.download()))
+    finally:
+        clg.autotune_enable(-1)
+        clg.autotune_reset()
