@@ -38,7 +38,7 @@ EXPORTED = [
     "clg_field_upload", "clg_field_download", "clg_field_copy", "clg_field_zero",
     "clg_field_parity_view", "clg_field_host_doubles", "clg_field_bytes", "clg_gauge_random",
     "clg_gauge_unit", "clg_field_gaussian", "clg_field_point_source", "clg_wilson_apply",
-    "clg_wilson_hop", "clg_wilson_schur", "clg_ks_apply", "clg_axpy", "clg_scale", "clg_dot",
+    "clg_wilson_hop", "clg_wilson_schur", "clg_ks_apply", "clg_ks_hop", "clg_ks_schur", "clg_axpy", "clg_scale", "clg_dot",
     "clg_norm2", "clg_solve", "clg_solver_params_default", "clg_time_wilson", "clg_time_ks",
     "clg_multishift_solve", "clg_rational_apply", "clg_wilson_force_kernel", "clg_ks_force_kernel",
     "clg_wilson_force", "clg_ks_force", "clg_wilson_pf_energy", "clg_ks_pf_energy",
@@ -134,6 +134,8 @@ def lib():
             "clg_wilson_hop": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_double, C.c_double, vp, C.c_double, C.c_double]),
             "clg_wilson_schur": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_int]),
             "clg_ks_apply": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double]),
+            "clg_ks_hop": (C.c_int, [vp, vp, vp, vp, C.c_double, vp, C.c_double]),
+            "clg_ks_schur": (C.c_int, [vp, vp, vp, vp, C.c_double]),
             "clg_axpy": (C.c_int, [vp, vp, C.c_double, C.c_double, vp]),
             "clg_scale": (C.c_int, [vp, vp, C.c_double, C.c_double]),
             "clg_dot": (C.c_int, [vp, vp, vp, dp]),
@@ -345,6 +347,14 @@ def wilson_schur(ctx, out, inp, gauge, kappa, op=EFO_F_D):
 
 def ks_apply(ctx, out, inp, gauge, two_am, op=EFO_F_D, coeff_type=EOCT_None, cre=1.0, cim=0.0):
     check(lib().clg_ks_apply(ctx.h, out.h, inp.h, gauge.h, two_am, op, coeff_type, cre, cim))
+
+
+def ks_hop(ctx, out, inp, gauge, a=1.0, x=None, b=0.0):
+    check(lib().clg_ks_hop(ctx.h, out.h, inp.h, gauge.h, a, x.h if x is not None else None, b))
+
+
+def ks_schur(ctx, out, inp, gauge, two_am):
+    check(lib().clg_ks_schur(ctx.h, out.h, inp.h, gauge.h, two_am))
 
 
 def axpy(ctx, y, a, x):

@@ -41,6 +41,7 @@ int wilson_hop_half(clg_context, clg_field_s*, const clg_field_s*, const clg_fie
                     const clg_field_s*, double, double);
 int wilson_schur_once(clg_context, clg_field_s*, const clg_field_s*, const clg_field_s*, double, bool, clg_field_s*);
 int ks_full(clg_context, clg_field_s*, const clg_field_s*, const clg_field_s*, double, bool, double, double);
+int ks_schur_once(clg_context, clg_field_s*, const clg_field_s*, const clg_field_s*, double, clg_field_s*);
 int gauge_convert(clg_context, clg_field_s* dst, const clg_field_s* src);
 }  // namespace clg
 
@@ -186,6 +187,18 @@ struct LinOp {
             }
             CLG_TRY(wilson_schur_once(ctx, teven, in, gauge, param, op == EFO_F_DDdagger, todd));
             return wilson_schur_once(ctx, out, teven, gauge, param, false, todd);
+        }
+        if (schur) {
+            // staggered: the parity block (2am)^2 - H H of D D^dagger, Hermitian (D = D^dagger here)
+            clg_field_s* other = nullptr;
+            CLG_TRY(scratch_field(ctx, in, CLG_KS, in->prec, in->subset == CLG_EVEN ? CLG_ODD : CLG_EVEN, 1, &other));
+            for (int i = 0; i < 4; ++i) other->bc[i] = in->bc[i];
+            if (op == EFO_F_D || op == EFO_F_Ddagger) return ks_schur_once(ctx, out, in, gauge, param, other);
+            clg_field_s* same = nullptr;
+            CLG_TRY(scratch_field(ctx, in, CLG_KS, in->prec, in->subset, 1, &same));
+            for (int i = 0; i < 4; ++i) same->bc[i] = in->bc[i];
+            CLG_TRY(ks_schur_once(ctx, same, in, gauge, param, other));
+            return ks_schur_once(ctx, out, same, gauge, param, other);
         }
         // staggered, full lattice
         if (op == EFO_F_D || op == EFO_F_Ddagger)
@@ -384,6 +397,9 @@ int validate_params(const clg_solver_params* p) {
     if (p->inner_prec == CLG_INNER_F16_VECTORS)
         CLG_CHECK_ARG(p->kind == CLG_WILSON && p->solver == CLG_SOLVER_GMRES && p->max_dim <= 16 && p->even_odd != 2,
                       "FP16-vector inner solves take Wilson GMRES with MaxDim <= 16 (no normal equations)");
+    // the staggered kernels read links in the spinor precision: no FP16 links under FP32 vectors
+    if (p->inner_prec == CLG_F16)
+        CLG_CHECK_ARG(p->kind == CLG_WILSON, "FP16-link inner solves are implemented for Wilson fermions");
     return CLG_OK;
 }
 
@@ -402,8 +418,40 @@ struct Solve1 {
     int outer = 0;
     double last_dev = 0;
 
+    // D D^dagger y = b for staggered fermions: D D^dagger = (2am)^2 - H^2 does not couple the
+    // parities, so each parity is one Schur-block solve (from a zero guess), each to a residual
+    // below accuracy |b| / sqrt(2)
+    int run_ks_dd(clg_field_s* y, const clg_field_s* b, double bnorm2, bool& conv) {
+        conv = true;
+        for (int par : {CLG_EVEN, CLG_ODD}) {
+            clg_field_s *yp = nullptr, *bp = nullptr;
+            CLG_TRY(clg_field_parity_view(y, par, &yp));
+            const int st = clg_field_parity_view(const_cast<clg_field_s*>(b), par, &bp);
+            if (st != CLG_OK) {
+                clg_field_destroy(yp);
+                return st;
+            }
+            bool c = false;
+            const int rs = run_system(yp, bp, true, bnorm2 / 2, true, c);
+            clg_field_destroy(yp);
+            clg_field_destroy(bp);
+            CLG_TRY(rs);
+            conv = conv && c;
+        }
+        return CLG_OK;
+    }
+
     // solve full-lattice A x = b with A = D or D^dagger; x holds the initial guess
     int run(clg_field_s* x, const clg_field_s* b, bool x_zero, double bnorm2, bool& conv) {
+        if (p->even_odd != 0 && kind == CLG_KS) {
+            // D x = b: x = D^dagger y with D D^dagger y = b; D^dagger x = b: x = D y (D and D^dagger
+            // commute for staggered fermions)
+            clg_field_s* y = nullptr;
+            CLG_TRY(scratch_field(ctx, b, CLG_KS, b->prec, CLG_FULL, 53, &y));
+            for (int i = 0; i < 4; ++i) y->bc[i] = b->bc[i];
+            CLG_TRY(run_ks_dd(y, b, bnorm2, conv));
+            return ks_full(ctx, x, y, gauge, param, !dag, 1.0, 0.0);
+        }
         const bool eo = p->even_odd != 0 && kind == CLG_WILSON;
         if (!eo) return run_system(x, b, x_zero, bnorm2, false, conv);
         // even-odd: bhat_e = b_e + kappa H_eo b_o ; Mhat x_e = bhat_e ; x_o = b_o + kappa H_oe x_e
@@ -440,7 +488,7 @@ struct Solve1 {
         A.gauge = gauge;
         A.param = param;
         A.op = dag ? EFO_F_Ddagger : EFO_F_D;
-        const bool normal = schur && p->even_odd == 2;
+        const bool normal = schur && p->even_odd == 2 && kind == CLG_WILSON;  // staggered blocks are HPD already
         const double bn = std::sqrt(bnorm2);  // |b| of the full system
         const double tol_abs = p->accuracy * bn;
         if (p->inner_prec == CLG_F64) {
@@ -617,7 +665,6 @@ int clg_solve(clg_context ctx, const clg_solver_params* p, clg_field x, clg_fiel
     if (op >= EFO_F_InverseD && op <= EFO_F_InverseDDdagger) base = op - EFO_F_InverseD;
     CLG_CHECK_ARG(base >= EFO_F_D && base <= EFO_F_DDdagger, "operator %d cannot be inverted", op);
     if (p->even_odd) {
-        CLG_CHECK_ARG(p->kind == CLG_WILSON, "even-odd solves are implemented for Wilson fermions");
         for (int i = 0; i < 4; ++i) CLG_CHECK_ARG((b->dims[i] & 1) == 0, "even-odd needs even extents");
     }
     (void)hipSetDevice(ctx->device);
@@ -646,7 +693,9 @@ int clg_solve(clg_context ctx, const clg_solver_params* p, clg_field x, clg_fiel
     // op decomposition: D, D^dagger: one solve; DD = D D: D^-1 twice; DD^dagger = D D^dagger:
     // (D^dagger)^-1 D^-1
     bool c1 = false, c2 = true;
-    if (base == EFO_F_D || base == EFO_F_Ddagger) {
+    if (p->kind == CLG_KS && p->even_odd && base == EFO_F_DDdagger) {
+        CLG_TRY(S.run_ks_dd(x, bb, bn2, c1));  // the parity blocks of D D^dagger directly
+    } else if (base == EFO_F_D || base == EFO_F_Ddagger) {
         S.dag = base == EFO_F_Ddagger;
         CLG_TRY(S.run(x, bb, false, bn2, c1));
     } else {

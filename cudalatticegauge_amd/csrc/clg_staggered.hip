@@ -34,9 +34,12 @@ struct KSArgs {
     int in_full, out_full;
     double two_am;   // mass term coefficient, applied to in(x)
     double hs;       // +1 for D, -1 for D^dagger
-    double cr, ci;   // overall coefficient
+    double cr, ci;   // overall coefficient (hop mode: cr = a)
     int pbase, nblk;
     int xcd, pinter;  // work decomposition (clgd::map_site)
+    const char* x;    // hop mode 2: out = a H in + two_am * x
+    const float* xn;
+    int mode;         // 0: D on full fields; 1: out = a H in; 2: out = a H in + b x (half fields)
 };
 
 template <int MU>
@@ -83,7 +86,9 @@ __device__ __forceinline__ void ks_mu(const KSArgs& a, const Site& s, int p, int
     if constexpr (PM == 2) pin3(acc);
 }
 
-template <typename R, int SP, int RECON, int PM>
+// MODE 0: out = c (2am in + hs H in) over both parities; MODE 1/2: the half-lattice hop
+// out_p = a H_{p,1-p} in (+ b x_p), where H = sum_mu eta_mu [U s+ psi(x+mu) - U^dag s- psi(x-mu)]
+template <typename R, int SP, int RECON, int PM, int MODE>
 __global__ __launch_bounds__(256) void k_ks(KSArgs a) {
     typedef KSIO<SP> SIO;
     typedef LinkIO<SP, RECON> LIO;
@@ -99,15 +104,29 @@ __global__ __launch_bounds__(256) void k_ks(KSArgs a) {
     ks_mu<3, PM, R, SIO, LIO>(a, s, p, h, acc);
     const unsigned po = a.out_full ? (unsigned)p : 0u;
     const unsigned eo = po * SIO::NPL * a.g.Vh + h, no = po * a.g.Vh + h;
-    const unsigned pi = a.in_full ? (unsigned)p : 0u;
-    C2<R> self[3], o[3];
-    SIO::load(a.in, a.inn, a.g.Vh, pi * SIO::NPL * a.g.Vh + h, pi * a.g.Vh + h, self);
-    const C2<R> cc = cmk<R>((R)a.cr, (R)a.ci);
-    const R m = (R)a.two_am, hs = (R)a.hs;
+    C2<R> o[3];
+    if constexpr (MODE == 0) {
+        const unsigned pi = a.in_full ? (unsigned)p : 0u;
+        C2<R> self[3];
+        SIO::load(a.in, a.inn, a.g.Vh, pi * SIO::NPL * a.g.Vh + h, pi * a.g.Vh + h, self);
+        const C2<R> cc = cmk<R>((R)a.cr, (R)a.ci);
+        const R m = (R)a.two_am, hs = (R)a.hs;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const C2<R> t = cmk<R>(m * self[c].x + hs * acc[c].x, m * self[c].y + hs * acc[c].y);
-        o[c] = cmul(cc, t);
+        for (int c = 0; c < 3; ++c) {
+            const C2<R> t = cmk<R>(m * self[c].x + hs * acc[c].x, m * self[c].y + hs * acc[c].y);
+            o[c] = cmul(cc, t);
+        }
+    } else {
+        const R ar = (R)a.cr;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o[c] = cmk<R>(ar * acc[c].x, ar * acc[c].y);
+        if constexpr (MODE == 2) {
+            C2<R> xv[3];
+            SIO::load(a.x, a.xn, a.g.Vh, eo, no, xv);
+            const R br = (R)a.two_am;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) { o[c].x += br * xv[c].x; o[c].y += br * xv[c].y; }
+        }
     }
     SIO::store(a.out, a.outn, a.g.Vh, eo, no, o);
 }
@@ -124,7 +143,9 @@ void ks_launch_pm(const KSArgs& a0, hipStream_t st, const KSCfg& c) {
     a.xcd = c.xcd;
     a.pinter = c.xcd > 1 ? 1 : 0;
     const unsigned grid = sched_grid(a.g.Vh, a.nblk, c.block, a.xcd, a.pinter);
-    k_ks<R, SP, RECON, PM><<<grid, c.block, 0, st>>>(a);
+    if (a.mode == 2) k_ks<R, SP, RECON, PM, 2><<<grid, c.block, 0, st>>>(a);
+    else if (a.mode == 1) k_ks<R, SP, RECON, PM, 1><<<grid, c.block, 0, st>>>(a);
+    else k_ks<R, SP, RECON, PM, 0><<<grid, c.block, 0, st>>>(a);
 }
 
 template <typename R, int SP, int RECON>
@@ -151,7 +172,7 @@ int ks_launch_cfg(const KSArgs& a, int prec, bool r12, hipStream_t st, const KSC
 // per-shape tuner: the first launch of a (precision, links, lattice) key times each candidate with
 // HIP events on the launch stream and keeps the fastest (every candidate writes the same output)
 struct KSKey {
-    int prec, r12, L0, L1, L2, L3;
+    int prec, r12, mode, L0, L1, L2, L3;
     bool operator<(const KSKey& o) const { return std::memcmp(this, &o, sizeof(KSKey)) < 0; }
 };
 std::map<KSKey, KSCfg>& ks_table() {
@@ -180,7 +201,7 @@ int ks_launch(const KSArgs& a, int prec, bool r12, hipStream_t st) {
     }
     KSKey k;
     std::memset(&k, 0, sizeof(k));
-    k.prec = prec; k.r12 = r12; k.L0 = a.g.L[0]; k.L1 = a.g.L[1]; k.L2 = a.g.L[2]; k.L3 = a.g.L[3];
+    k.prec = prec; k.r12 = r12; k.mode = a.mode; k.L0 = a.g.L[0]; k.L1 = a.g.L[1]; k.L2 = a.g.L[2]; k.L3 = a.g.L[3];
     bool need_tune = false;
     const std::string cpath = tune_cache_path();
     {
@@ -193,10 +214,11 @@ int ks_launch(const KSArgs& a, int prec, bool r12, hipStream_t st) {
                 const std::vector<long>& v = e.first;
                 const std::vector<long>& c = e.second;
                 const KSCfg kc{(int)c[0], c.size() > 1 ? (int)c[1] : -1, c.size() > 2 ? (int)c[2] : -1};
-                if (v.size() != 6 || c.size() != 3 || !ks_valid(kc)) continue;
+                if (v.size() != 7 || c.size() != 3 || !ks_valid(kc)) continue;
                 KSKey t;
                 std::memset(&t, 0, sizeof(t));
-                t.prec = (int)v[0]; t.r12 = (int)v[1]; t.L0 = (int)v[2]; t.L1 = (int)v[3]; t.L2 = (int)v[4]; t.L3 = (int)v[5];
+                t.prec = (int)v[0]; t.r12 = (int)v[1]; t.mode = (int)v[2];
+                t.L0 = (int)v[3]; t.L1 = (int)v[4]; t.L2 = (int)v[5]; t.L3 = (int)v[6];
                 ks_table()[t] = kc;
             }
             ks_loaded() = cpath;
@@ -207,7 +229,8 @@ int ks_launch(const KSArgs& a, int prec, bool r12, hipStream_t st) {
     }
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     (void)hipStreamIsCapturing(st, &cs);
-    if (need_tune && tuning_enabled() && cs == hipStreamCaptureStatusNone && (long)a.nblk * a.g.Vh >= 4096) {
+    const bool aliased = a.mode == 2 && a.x == a.out;  // repeated candidate launches would compound
+    if (need_tune && tuning_enabled() && !aliased && cs == hipStreamCaptureStatusNone && (long)a.nblk * a.g.Vh >= 4096) {
         const KSCfg cands[] = {{64, 0, 2}, {128, 0, 2}, {256, 0, 2}, {64, 8, 2}, {128, 8, 2}, {256, 8, 2},
                                {64, 8, 0}, {128, 8, 0}, {256, 8, 0}};
         hipEvent_t e0, e1;
@@ -228,12 +251,12 @@ int ks_launch(const KSArgs& a, int prec, bool r12, hipStream_t st) {
         (void)hipEventDestroy(e1);
         const char* verbose = std::getenv("CLG_TUNE_VERBOSE");
         if (verbose && verbose[0] == '1')
-            std::fprintf(stderr, "[clg tune] staggered D prec=%d recon=%d L=%dx%dx%dx%d -> block=%d xcd=%d pin=%d (%.1f us)\n",
-                         prec, r12 ? 12 : 18, a.g.L[0], a.g.L[1], a.g.L[2], a.g.L[3], cfg.block, cfg.xcd, cfg.pm,
+            std::fprintf(stderr, "[clg tune] staggered D prec=%d recon=%d mode=%d L=%dx%dx%dx%d -> block=%d xcd=%d pin=%d (%.1f us)\n",
+                         prec, r12 ? 12 : 18, a.mode, a.g.L[0], a.g.L[1], a.g.L[2], a.g.L[3], cfg.block, cfg.xcd, cfg.pm,
                          best * 1e3 / 3);
         std::lock_guard<std::mutex> lk(ks_mutex());
         ks_table()[k] = cfg;
-        tune_cache_append(cpath, "staggered_d", {k.prec, k.r12, k.L0, k.L1, k.L2, k.L3}, {cfg.block, cfg.xcd, cfg.pm});
+        tune_cache_append(cpath, "staggered_d", {k.prec, k.r12, k.mode, k.L0, k.L1, k.L2, k.L3}, {cfg.block, cfg.xcd, cfg.pm});
     }
     return ks_launch_cfg(a, prec, r12, st, cfg);
 }
@@ -253,8 +276,9 @@ int ks_tune_entry(int index, char* buf, int len) {
     auto it = ks_table().begin();
     std::advance(it, index);
     const KSKey& k = it->first;
-    std::snprintf(buf, (size_t)len, "staggered_d prec=%d recon=%d dims=%dx%dx%dx%d -> block=%d xcd=%d pin=%d", k.prec,
-                  k.r12 ? 12 : 18, k.L0, k.L1, k.L2, k.L3, it->second.block, it->second.xcd, it->second.pm);
+    std::snprintf(buf, (size_t)len, "staggered_d prec=%d recon=%d mode=%d dims=%dx%dx%dx%d -> block=%d xcd=%d pin=%d",
+                  k.prec, k.r12 ? 12 : 18, k.mode, k.L0, k.L1, k.L2, k.L3, it->second.block, it->second.xcd,
+                  it->second.pm);
     return CLG_OK;
 }
 
@@ -283,6 +307,38 @@ int ks_full(clg_context ctx, clg_field_s* out, const clg_field_s* in, const clg_
     a.pbase = 0;
     a.nblk = 2;
     return ks_launch(a, out->prec, gauge->kind == CLG_GAUGE12, ctx->stream);
+}
+
+// out_p = a H_{p,1-p} in + b x_p on half fields (x nullable)
+int ks_hop_half(clg_context ctx, clg_field_s* out, const clg_field_s* in, const clg_field_s* gauge, double av,
+                const clg_field_s* x, double bv) {
+    KSArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.g = make_geo(out);
+    a.in = (const char*)in->data;
+    a.inn = in->norm;
+    a.out = (char*)out->data;
+    a.outn = out->norm;
+    a.gauge = (const char*)gauge->data;
+    a.in_full = 0;
+    a.out_full = 0;
+    a.cr = av;
+    a.two_am = bv;
+    a.pbase = field_parity(out);
+    a.nblk = 1;
+    a.mode = x ? 2 : 1;
+    if (x) {
+        a.x = (const char*)x->data;
+        a.xn = x->norm;
+    }
+    return ks_launch(a, out->prec, gauge->kind == CLG_GAUGE12, ctx->stream);
+}
+
+// the parity-p block of D D^dagger = (2am)^2 - H^2: out = (2am)^2 in - H_{p,1-p} H_{1-p,p} in
+int ks_schur_once(clg_context ctx, clg_field_s* out, const clg_field_s* in, const clg_field_s* gauge, double two_am,
+                  clg_field_s* tmp_other) {
+    CLG_TRY(ks_hop_half(ctx, tmp_other, in, gauge, 1.0, nullptr, 0.0));
+    return ks_hop_half(ctx, out, tmp_other, gauge, -1.0, in, two_am * two_am);
 }
 
 }  // namespace clg
@@ -316,6 +372,40 @@ int clg_ks_apply(clg_context ctx, clg_field out, clg_field in, clg_field gauge, 
     CLG_TRY(scratch_field(ctx, in, CLG_KS, in->prec, CLG_FULL, 0, &tmp));
     CLG_TRY(ks_full(ctx, tmp, in, gauge, two_am, op == EFO_F_DDdagger, 1.0, 0.0));
     return ks_full(ctx, out, tmp, gauge, two_am, false, fr, fi);
+}
+
+int clg_ks_hop(clg_context ctx, clg_field out, clg_field in, clg_field gauge, double a, clg_field x, double b) {
+    CLG_CHECK_ARG(ctx && out && in && gauge, "null argument");
+    CLG_CHECK_ARG(out->kind == CLG_KS && in->kind == CLG_KS, "need staggered fields");
+    CLG_CHECK_ARG(out->subset != CLG_FULL && in->subset != CLG_FULL && out->subset != in->subset,
+                  "the hop maps one parity (half field) to the other");
+    CLG_CHECK_ARG(out->prec == in->prec && gauge->prec == in->prec, "precision mismatch");
+    CLG_CHECK_ARG(is_gauge(gauge->kind), "need a gauge field");
+    CLG_CHECK_ARG(out->data != in->data, "out and in must be distinct");
+    CLG_TRY(check_same_lattice(out, in));
+    CLG_TRY(check_same_lattice(out, gauge));
+    for (int i = 0; i < 4; ++i) CLG_CHECK_ARG((out->dims[i] & 1) == 0, "even-odd operators need even extents");
+    if (x) CLG_CHECK_ARG(x->kind == CLG_KS && x->subset == out->subset && x->prec == out->prec, "x must match out");
+    (void)hipSetDevice(ctx->device);
+    for (int i = 0; i < 4; ++i) out->bc[i] = in->bc[i];
+    return ks_hop_half(ctx, out, in, gauge, a, x, b);
+}
+
+int clg_ks_schur(clg_context ctx, clg_field out, clg_field in, clg_field gauge, double two_am) {
+    CLG_CHECK_ARG(ctx && out && in && gauge, "null argument");
+    CLG_CHECK_ARG(out->kind == CLG_KS && in->kind == CLG_KS, "need staggered fields");
+    CLG_CHECK_ARG(in->subset != CLG_FULL && out->subset == in->subset, "the Schur block acts on one parity");
+    CLG_CHECK_ARG(out->prec == in->prec && gauge->prec == in->prec, "precision mismatch");
+    CLG_CHECK_ARG(is_gauge(gauge->kind), "need a gauge field");
+    CLG_CHECK_ARG(out->data != in->data, "out and in must be distinct");
+    CLG_TRY(check_same_lattice(out, in));
+    CLG_TRY(check_same_lattice(out, gauge));
+    for (int i = 0; i < 4; ++i) CLG_CHECK_ARG((out->dims[i] & 1) == 0, "even-odd operators need even extents");
+    (void)hipSetDevice(ctx->device);
+    clg_field_s* tmp = nullptr;
+    CLG_TRY(scratch_field(ctx, in, CLG_KS, in->prec, in->subset == CLG_EVEN ? CLG_ODD : CLG_EVEN, 0, &tmp));
+    for (int i = 0; i < 4; ++i) { out->bc[i] = in->bc[i]; tmp->bc[i] = in->bc[i]; }
+    return ks_schur_once(ctx, out, in, gauge, two_am, tmp);
 }
 
 int clg_time_ks(clg_context ctx, clg_field out, clg_field in, clg_field gauge, double two_am, int reps, double* ms_avg) {

@@ -136,6 +136,16 @@ int clg_wilson_schur(clg_context ctx, clg_field out, clg_field in, clg_field gau
  * :34-117) */
 int clg_ks_apply(clg_context ctx, clg_field out, clg_field in, clg_field gauge, double two_am,
                  int op, int coeff_type, double coeff_re, double coeff_im);
+/* Staggered half-lattice hop: out = b*x + a*H_{p,1-p} in, where out (and x) live on parity p and in on
+ * the other parity; H = sum_mu eta_mu [U_mu(x) psi(x+mu) - U_mu(x-mu)^dag psi(x-mu)] is the hopping
+ * part of D = 2am + H (_kernelDFermionKS, CFieldFermionKSSU3.cu:34-117), anti-Hermitian, so
+ * D^dagger = 2am - H.  x may be NULL. */
+int clg_ks_hop(clg_context ctx, clg_field out, clg_field in, clg_field gauge, double a, clg_field x, double b);
+/* The parity-p block of D D^dagger = D^dagger D = (2am)^2 - H^2 for staggered fermions:
+ * out = (2am)^2 in - H_{p,1-p} H_{1-p,p} in, out and in on the same parity (EVEN or ODD); Hermitian
+ * positive definite.  The staggered form of even-odd preconditioning (the reference applies
+ * DD^dagger on the full lattice, CFieldFermionKSSU3.cu:1550-1575). */
+int clg_ks_schur(clg_context ctx, clg_field out, clg_field in, clg_field gauge, double two_am);
 
 /* ============================== BLAS ============================== */
 /* CField::Axpy (complex a), AxpyPlus/AxpyMinus (a = +-1) (CField.h:127-160) */
@@ -180,8 +190,12 @@ typedef struct {
     double accuracy;       /* YAML Accuracy (squared-norm units for STOP_REFERENCE) */
     int absolute_accuracy; /* YAML AbsoluteAccuracy */
     int stop_rule;         /* clg_stop_rule */
-    int even_odd;          /* 1: solve through the even-site Schur complement (Wilson) */
-    int inner_prec;        /* CLG_F64: plain solve; CLG_F32 / CLG_F16 / CLG_INNER_F16_VECTORS: defect correction */
+    int even_odd;          /* 1: even-odd preconditioning.  Wilson: solve through the even-site Schur
+                              complement (2: its normal equations).  Staggered: D D^dagger is solved
+                              as its two parity blocks (2am)^2 - H H, each from a zero guess; D and
+                              D^dagger solves become x = D^dagger y or D y with D D^dagger y = b */
+    int inner_prec;        /* CLG_F64: plain solve; CLG_F32 / CLG_F16 (Wilson) / CLG_INNER_F16_VECTORS
+                              (Wilson GMRES): defect correction */
     double inner_accuracy; /* relative residual target of each inner solve (e.g. 1e-5) */
     int max_outer;         /* defect-correction outer iterations (mixed precision) */
     int kind;              /* CLG_WILSON or CLG_KS */

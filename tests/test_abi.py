@@ -65,3 +65,5 @@ def test_tuner_and_graph_entry_points_check_arguments_without_a_device():
     assert L.clg_graph_destroy(None) == 0
     assert L.clg_multidot(None, 1, None, None, None) == 2
     assert L.clg_vector_multiply_matrix(None, 1, None, 0, None, None) == 2
+    assert L.clg_ks_hop(None, None, None, None, 1.0, None, 0.0) == 2
+    assert L.clg_ks_schur(None, None, None, None, 0.1) == 2

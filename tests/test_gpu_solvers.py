@@ -229,3 +229,71 @@ def test_ks_launch_configurations(ctx, monkeypatch):
     assert np.abs(fo.download() - ref).max() / np.abs(ref).max() < 1e-12
     assert any(e.startswith("staggered_d") and "dims=8x8x8x16" in e for e in clg.autotune_entries())
     clg.autotune_enable(-1)
+
+
+# ------------------------------------------------------------------ staggered even-odd
+KS_DIMS = (4, 4, 4, 8)
+
+
+def test_ks_even_odd_operators(ctx):
+    """The parity blocks of D D^dagger and the half-lattice hop against the oracle's full-lattice D:
+    D D^dagger of a one-parity field has no component on the other parity (the oracle side pins that),
+    and its own-parity part is the Schur block (2am)^2 - H H."""
+    dims, m = KS_DIMS, 0.07
+    V = orc.volume(dims)
+    U = orc.gauge_random(dims, 0.4, 3)
+    chi = orc.ks_gaussian(dims, 4).reshape(V, 6)
+    g = clg.Field(ctx, clg.GAUGE, clg.F64, dims)
+    g.upload(U)
+    sites = {par: orc.eo_sites(dims, par) for par in (0, 1)}
+    subset = {0: clg.EVEN, 1: clg.ODD}
+    for par in (0, 1):
+        only = np.zeros_like(chi)
+        only[sites[par]] = chi[sites[par]]
+        dd = orc.ks_op(dims, orc.BC_FERMION, m, U, only.ravel(), clg.EFO_F_DDdagger).reshape(V, 6)
+        assert np.abs(dd[sites[1 - par]]).max() < 1e-12 * np.abs(dd).max()
+        fin = clg.Field(ctx, clg.KS, clg.F64, dims, subset=subset[par], bc=orc.BC_FERMION)
+        fin.upload(chi[sites[par]].ravel())
+        fout = clg.Field(ctx, clg.KS, clg.F64, dims, subset=subset[par], bc=orc.BC_FERMION)
+        clg.ks_schur(ctx, fout, fin, g, m)
+        want = dd[sites[par]]
+        assert np.abs(fout.download().reshape(-1, 6) - want).max() < 1e-12 * np.abs(want).max()
+        # hop from this parity to the other: (D only)_(1-par) = H only, since only vanishes there
+        d1 = orc.ks_op(dims, orc.BC_FERMION, m, U, only.ravel(), clg.EFO_F_D).reshape(V, 6)[sites[1 - par]]
+        xo = clg.Field(ctx, clg.KS, clg.F64, dims, subset=subset[1 - par], bc=orc.BC_FERMION)
+        xo.upload(chi[sites[1 - par]].ravel())
+        hout = clg.Field(ctx, clg.KS, clg.F64, dims, subset=subset[1 - par], bc=orc.BC_FERMION)
+        clg.ks_hop(ctx, hout, fin, g)
+        assert np.abs(hout.download().reshape(-1, 6) - d1).max() < 1e-12 * np.abs(d1).max()
+        clg.ks_hop(ctx, hout, fin, g, a=-0.5, x=xo, b=0.3)
+        want2 = -0.5 * d1 + 0.3 * chi[sites[1 - par]]
+        assert np.abs(hout.download().reshape(-1, 6) - want2).max() < 1e-12 * np.abs(want2).max()
+    with pytest.raises(clg.CLGError):
+        clg.ks_schur(ctx, fout, fout, g, m)          # out aliases in
+
+
+@pytest.mark.parametrize("op", [clg.EFO_F_InverseDDdagger, clg.EFO_F_InverseD, clg.EFO_F_InverseDdagger])
+@pytest.mark.parametrize("inner", [clg.F64, clg.F32])
+def test_ks_even_odd_solves(ctx, op, inner):
+    dims, m = KS_DIMS, 0.1
+    U = orc.gauge_random(dims, 0.4, 1)
+    chi = orc.ks_gaussian(dims, 2)
+    g = clg.Field(ctx, clg.GAUGE, clg.F64, dims)
+    g.upload(U)
+    fb = clg.Field(ctx, clg.KS, clg.F64, dims, bc=orc.BC_FERMION)
+    fb.upload(chi)
+    fx = clg.Field(ctx, clg.KS, clg.F64, dims, bc=orc.BC_FERMION)
+    p = clg.solver_params(max_dim=20, restart=100, accuracy=1e-10, stop_rule=clg.STOP_TRUE_RESIDUAL, kind=clg.KS,
+                          even_odd=1, inner_prec=inner, inner_accuracy=1e-5)
+    st, rep = clg.solve(ctx, p, fx, fb, g, m, op)
+    assert st == clg.OK and rep.true_residual < 1e-10
+    base = op - clg.EFO_F_InverseD
+    r = chi - orc.ks_op(dims, orc.BC_FERMION, m, U, fx.download(), base)
+    assert np.linalg.norm(r) / np.linalg.norm(chi) < 1e-9
+
+
+def test_ks_rejects_fp16_link_inner(ctx):
+    g, fb, fx = fields(ctx, orc.gauge_random(DIMS0, 0.3, 1), orc.ks_gaussian(DIMS0, 2), kind=clg.KS)
+    p = clg.solver_params(max_dim=12, restart=10, accuracy=1e-8, kind=clg.KS, inner_prec=clg.F16)
+    with pytest.raises(clg.CLGError):
+        clg.solve(ctx, p, fx, fb, g, 0.1, clg.EFO_F_InverseDDdagger)
