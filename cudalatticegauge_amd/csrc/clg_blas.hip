@@ -256,6 +256,122 @@ __global__ __launch_bounds__(kRedBlock) void k_multiaxpy(long n, Ptrs<P> V, cons
     if constexpr (NORM) block_partial<1>(v, partial);
 }
 
+// Chunked forms for 17..kMaxChunkV vectors (long Arnoldi bases: multi-shift FOM, GMRES(m > 16),
+// GCRO-DR): one launch per pass instead of one per 16 vectors.
+constexpr int kMaxChunkV = 64;
+template <int P> struct PtrsN {
+    const typename Pk<P>::T* p[kMaxChunkV];
+};
+// multi-dot over chunk blockIdx.y (vectors 16c .. 16c+15, the last chunk partial); partials
+// [chunk][block][2*kMaxK + 1], the last slot |w|^2
+template <int P>
+__global__ __launch_bounds__(kRedBlock) void k_multidot_ch(long n, PtrsN<P> V, int nv, const typename Pk<P>::T* __restrict__ w,
+                                                            double* partial, const int* cond) {
+    constexpr int K = kMaxK, NC = Pk<P>::NC;
+    if (cond && *cond == 0) return;
+    const int k0 = (int)blockIdx.y * K, kk = min(K, nv - k0);
+    double v[2 * K + 1];
+#pragma unroll
+    for (int k = 0; k < 2 * K + 1; ++k) v[k] = 0;
+    GRID_STRIDE(i, n) {
+        typename Pk<P>::T raw[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (k < kk) raw[k] = V.p[k0 + k][i];
+        double y[2 * NC];
+        Pk<P>::ld(w, i, y);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (k < kk) {
+                double x[2 * NC];
+                Pk<P>::cvt(raw[k], x);
+                double re = 0.0, im = 0.0;
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    re += x[2 * c] * y[2 * c] + x[2 * c + 1] * y[2 * c + 1];
+                    im += x[2 * c] * y[2 * c + 1] - x[2 * c + 1] * y[2 * c];
+                }
+                v[2 * k] += re;
+                v[2 * k + 1] += im;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 2 * NC; ++c) v[2 * K] += y[c] * y[c];
+    }
+    block_partial<2 * K + 1>(v, partial + (size_t)blockIdx.y * gridDim.x * (2 * K + 1));
+}
+// w += s * sum_k c_k V_k over nv <= kMaxChunkV vectors in one pass (w read and written once);
+// NORM: partials of |w|^2 of the stored result
+template <int P, bool NORM>
+__global__ __launch_bounds__(kRedBlock) void k_multiaxpy_ch(long n, PtrsN<P> V, int nv, const double* __restrict__ cc,
+                                                             double s, typename Pk<P>::T* w, double* partial,
+                                                             const int* cond) {
+    constexpr int K = kMaxK, NC = Pk<P>::NC;
+    if (cond && *cond == 0) return;
+    double v[1] = {0};
+    GRID_STRIDE(i, n) {
+        double y[2 * NC];
+        Pk<P>::ld(w, i, y);
+        for (int k0 = 0; k0 < nv; k0 += K) {
+            typename Pk<P>::T raw[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (k0 + k < nv) raw[k] = V.p[k0 + k][i];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (k0 + k < nv) {
+                    const double cr = s * cc[2 * (k0 + k)], ci = s * cc[2 * (k0 + k) + 1];
+                    double x[2 * NC];
+                    Pk<P>::cvt(raw[k], x);
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        y[2 * c] += cr * x[2 * c] - ci * x[2 * c + 1];
+                        y[2 * c + 1] += cr * x[2 * c + 1] + ci * x[2 * c];
+                    }
+                }
+            }
+        }
+        Pk<P>::st(w, i, y);
+        if constexpr (NORM) {
+#pragma unroll
+            for (int c = 0; c < 2 * NC; ++c) v[0] += y[c] * y[c];
+        }
+    }
+    if constexpr (NORM) block_partial<1>(v, partial);
+}
+// finish of k_multidot_ch: one workgroup per chunk (blockIdx.y); chunk c writes its 2*kk sums to
+// out[2*kMaxK*c ..], chunk 0 also |w|^2 to wn (nullable)
+__global__ __launch_bounds__(kRedBlock) void k_red_finish_ch(const double* __restrict__ partial, int nblk, int nv,
+                                                             double* out, double* wn, const int* cond) {
+    constexpr int NR = 2 * kMaxK + 1;
+    if (cond && *cond == 0) return;
+    const int ch = (int)blockIdx.y, kk = min(kMaxK, nv - ch * kMaxK);
+    const double* pp = partial + (size_t)ch * nblk * NR;
+    __shared__ double sm[kRedBlock / 64][NR];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = 0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[r] += pp[(size_t)b * NR + r];
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = wave_sum(acc[r]);
+    if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) sm[wv][r] = acc[r];
+    }
+    __syncthreads();
+    if (threadIdx.x < NR) {
+        const int r = threadIdx.x;
+        double t = 0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += sm[q][r];
+        if (r < 2 * kk) out[2 * kMaxK * ch + r] = t;
+        else if (r == NR - 1 && ch == 0 && wn) wn[0] = t;
+    }
+}
+
 // Second stage of the BLAS reductions (one workgroup): total_r = sum over nblk partials of
 // partial[b * NR + r] in a fixed order; totals r < nout go to out[r], the one after them to
 // out2[0] (when given).  flag (nullable): *flag = (total_0 < eta * ref[0]).  cond (nullable): skip
@@ -679,6 +795,18 @@ template <int P>
 static int multidot_p(clg_context ctx, clg_field_s* const* V, int nv, const clg_field_s* w, double* out, double* wnorm,
                       const int* cond, long n) {
     const unsigned g = red_grid(ctx, n);
+    if (nv > kMaxK && nv <= kMaxChunkV) {
+        const int nch = (nv + kMaxK - 1) / kMaxK;
+        CLG_TRY(ensure_reduction_scratch(ctx, (size_t)kMaxRedGrid * (2 * kMaxK + 1) * nch));
+        PtrsN<P> Pn;
+        for (int k = 0; k < nv; ++k) Pn.p[k] = (const typename Pk<P>::T*)V[k]->data;
+        k_multidot_ch<P><<<dim3(g, nch), kRedBlock, 0, ctx->stream>>>(n, Pn, nv, (const typename Pk<P>::T*)w->data,
+                                                                      ctx->red_partial, cond);
+        CLG_HIP(hipGetLastError());
+        k_red_finish_ch<<<dim3(1, nch), kRedBlock, 0, ctx->stream>>>(ctx->red_partial, (int)g, nv, out, wnorm, cond);
+        CLG_HIP(hipGetLastError());
+        return CLG_OK;
+    }
     for (int k0 = 0; k0 < nv; k0 += kMaxK) {
         const int kk = std::min(kMaxK, nv - k0);
         Ptrs<P> Pp;
@@ -717,6 +845,21 @@ int blas_multidot_dev(clg_context ctx, clg_field_s* const* V, int nv, const clg_
 template <int P>
 static int multiaxpy_p(clg_context ctx, clg_field_s* w, clg_field_s* const* V, int nv, const double* c, double s,
                        double* nrm, const int* cond, const double* ref, double eta, int* flag, long n) {
+    if (nv > kMaxK && nv <= kMaxChunkV) {
+        PtrsN<P> Pn;
+        for (int k = 0; k < nv; ++k) Pn.p[k] = (const typename Pk<P>::T*)V[k]->data;
+        typename Pk<P>::T* wp = (typename Pk<P>::T*)w->data;
+        if (nrm) {
+            const unsigned g = red_grid(ctx, n);
+            k_multiaxpy_ch<P, true><<<g, kRedBlock, 0, ctx->stream>>>(n, Pn, nv, c, s, wp, ctx->red_partial, cond);
+            CLG_HIP(hipGetLastError());
+            return red_finish(ctx, g, 1, nrm, 1, nullptr, cond, ref, eta, flag);
+        }
+        const unsigned g = ew_grid(ctx, n);
+        k_multiaxpy_ch<P, false><<<g, kRedBlock, 0, ctx->stream>>>(n, Pn, nv, c, s, wp, nullptr, cond);
+        CLG_HIP(hipGetLastError());
+        return CLG_OK;
+    }
     for (int k0 = 0; k0 < nv; k0 += kMaxK) {
         const int kk = std::min(kMaxK, nv - k0);
         const bool want = k0 + kk >= nv && nrm != nullptr;

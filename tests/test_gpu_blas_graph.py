@@ -36,7 +36,9 @@ def fields(ctx, n, kind=clg.WILSON, prec=clg.F64, subset=clg.FULL, seed=1):
 
 
 @pytest.mark.parametrize("kind,prec,subset,n", [(clg.WILSON, clg.F64, clg.FULL, 5), (clg.WILSON, clg.F64, clg.EVEN, 20),
-                                                (clg.KS, clg.F64, clg.FULL, 3), (clg.WILSON, clg.F32, clg.ODD, 7)])
+                                                (clg.KS, clg.F64, clg.FULL, 3), (clg.WILSON, clg.F32, clg.ODD, 7),
+                                                (clg.KS, clg.F64, clg.FULL, 33), (clg.WILSON, clg.F32, clg.EVEN, 40),
+                                                (clg.KS, clg.F32, clg.FULL, 64), (clg.KS, clg.F64, clg.FULL, 70)])
 def test_multidot_matches_numpy(ctx, kind, prec, subset, n):
     V = fields(ctx, n, kind, prec, subset, seed=3)
     w = fields(ctx, 1, kind, prec, subset, seed=99)[0]
@@ -49,7 +51,7 @@ def test_multidot_matches_numpy(ctx, kind, prec, subset, n):
     assert abs(got[0] - clg.dot(ctx, V[0], w)) <= 1e-12 * abs(want[0]) + 1e-12
 
 
-@pytest.mark.parametrize("n", [1, 4, 17])
+@pytest.mark.parametrize("n", [1, 4, 17, 33, 61, 70])
 def test_multiaxpy_matches_numpy(ctx, n):
     V = fields(ctx, n, seed=5)
     w = fields(ctx, 1, seed=77)[0]
