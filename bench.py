@@ -33,6 +33,7 @@ BYTES_D12 = {8: 336 * 8, 4: 336 * 4}        # 12-real links
 BYTES_SCHUR = {8: 744 * 8, 4: 744 * 4}      # per even site
 FLOPS_D = 1368
 BYTES_KS = {8: 204 * 8, 4: 204 * 4}         # full staggered D: 8 neighbour vectors, 8 links, site in/out
+BYTES_KS_HOP = 198 * 8                      # staggered half hop per output site (FP64; + the x term)
 
 
 def log(msg):
@@ -149,6 +150,15 @@ def main():
         extra[tag + "_us"] = round(ms * 1e3, 2)
         extra[tag + "_GBs"] = round(BYTES_KS[8 if prec == clg.F64 else 4] * vk / (ms * 1e-3) / 1e9, 1)
         log(f"{tag}: {ms * 1e3:.1f} us  {extra[tag + '_GBs']:.0f} GB/s")
+        if prec == clg.F64:
+            # the even-parity Schur block (2am)^2 - H_eo H_oe: two half-lattice hops, 2 x 198 reals per even site
+            se, oe = s.parity(clg.EVEN), o.parity(clg.EVEN)
+            ms = clg.time_ks(ctx, oe, se, g, 0.02, args.steps, variant=1)
+            stag = tag.replace("_D", "_schur")
+            extra[stag + "_us"] = round(ms * 1e3, 2)
+            extra[stag + "_GBs"] = round(2 * BYTES_KS_HOP * (vk // 2) / (ms * 1e-3) / 1e9, 1)
+            log(f"{stag}: {ms * 1e3:.1f} us  {extra[stag + '_GBs']:.0f} GB/s")
+            del se, oe
         del g, s, o, g64, s64
 
     # ---------------- solves to |r|/|b| < 1e-10 (EO-preconditioned GMRES(12))

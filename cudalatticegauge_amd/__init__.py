@@ -144,7 +144,7 @@ def lib():
                                     C.POINTER(SolveReport)]),
             "clg_solver_params_default": (C.c_int, [C.POINTER(SolverParams)]),
             "clg_time_wilson": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, dp]),
-            "clg_time_ks": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_int, dp]),
+            "clg_time_ks": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, dp]),
             "clg_multishift_solve": (C.c_int, [vp, C.POINTER(SolverParams), C.c_int, C.c_int, C.POINTER(vp), dp,
                                                C.c_int, vp, vp, C.c_double, C.c_int, C.POINTER(SolveReport)]),
             "clg_rational_apply": (C.c_int, [vp, C.POINTER(SolverParams), C.c_int, C.c_int, vp, vp, vp, C.c_double,
@@ -490,9 +490,9 @@ def time_wilson(ctx, out, inp, gauge, kappa, variant, reps):
     return ms.value
 
 
-def time_ks(ctx, out, inp, gauge, two_am, reps):
+def time_ks(ctx, out, inp, gauge, two_am, reps, variant=0):
     ms = C.c_double()
-    check(lib().clg_time_ks(ctx.h, out.h, inp.h, gauge.h, two_am, reps, C.byref(ms)))
+    check(lib().clg_time_ks(ctx.h, out.h, inp.h, gauge.h, two_am, variant, reps, C.byref(ms)))
     return ms.value
 
 

@@ -408,10 +408,14 @@ int clg_ks_schur(clg_context ctx, clg_field out, clg_field in, clg_field gauge, 
     return ks_schur_once(ctx, out, in, gauge, two_am, tmp);
 }
 
-int clg_time_ks(clg_context ctx, clg_field out, clg_field in, clg_field gauge, double two_am, int reps, double* ms_avg) {
-    CLG_CHECK_ARG(ctx && ms_avg && reps > 0, "bad arguments");
+int clg_time_ks(clg_context ctx, clg_field out, clg_field in, clg_field gauge, double two_am, int variant, int reps,
+                double* ms_avg) {
+    CLG_CHECK_ARG(ctx && ms_avg && reps > 0 && (variant == 0 || variant == 1), "bad arguments");
     (void)hipSetDevice(ctx->device);
-    auto one = [&]() -> int { return clg_ks_apply(ctx, out, in, gauge, two_am, EFO_F_D, EOCT_None, 1.0, 0.0); };
+    auto one = [&]() -> int {
+        if (variant == 1) return clg_ks_schur(ctx, out, in, gauge, two_am);
+        return clg_ks_apply(ctx, out, in, gauge, two_am, EFO_F_D, EOCT_None, 1.0, 0.0);
+    };
     CLG_TRY(one());  // validates arguments, tunes, warms up
     CLG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     for (int r = 0; r < reps; ++r) CLG_TRY(one());

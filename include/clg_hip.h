@@ -469,10 +469,11 @@ int clg_graph_destroy(clg_graph g);
  * 1 half-lattice hop, 2 Schur operator.  No reference counterpart (bench support). */
 int clg_time_wilson(clg_context ctx, clg_field out, clg_field in, clg_field gauge, double kappa,
                     int variant, int reps, double* ms_avg);
-/* The same for the full-lattice staggered D (clg_ks_apply, EFO_F_D) with mass term two_am.
- * No reference counterpart (bench support). */
-int clg_time_ks(clg_context ctx, clg_field out, clg_field in, clg_field gauge, double two_am, int reps,
-                double* ms_avg);
+/* The same for the staggered operator with mass term two_am: variant 0 the full-lattice D
+ * (clg_ks_apply, EFO_F_D), 1 the parity Schur block (clg_ks_schur, half fields).  No reference
+ * counterpart (bench support). */
+int clg_time_ks(clg_context ctx, clg_field out, clg_field in, clg_field gauge, double two_am, int variant,
+                int reps, double* ms_avg);
 
 #ifdef __cplusplus
 }
