@@ -161,6 +161,9 @@ def main():
             del se, oe
         del g, s, o, g64, s64
 
+    # ---------------- HMC fermion force kernels (_kernelDWilsonForceSU3, _kernelDFermionKSForce)
+    extra.update(force_kernels(ctx, clg, np, args.steps))
+
     # ---------------- solves to |r|/|b| < 1e-10 (EO-preconditioned GMRES(12))
     if not args.no_solve:
         x = clg.Field(ctx, clg.WILSON, clg.F64, DIMS, bc=(1, 1, 1, -1))
@@ -249,6 +252,62 @@ def main():
 
 
 HMC_DIMS = (24, 24, 24, 8)
+
+
+def force_kernels(ctx, clg, np, steps):
+    """Kernel time of the fermion force kernels on resident inputs (host clock around `steps`
+    back-to-back launches, device synchronised): the Wilson force at 32^3x16 and the staggered force
+    with 12 poles at 24^3x8.  Algorithmic bytes per site, FP64: Wilson X, Y at x and at x+mu (4 mu), the
+    link and a read-modify-write of the force link: 2*192 + 4*(2*192 + 144 + 2*144) = 3648;
+    staggered per mu the link, the force read-modify-write and 4 colour vectors per pole:
+    4*(144 + 288 + 12*4*48) = 10944."""
+    out = {}
+    reps = max(10, steps // 4)
+    g = clg.Field(ctx, clg.GAUGE, clg.F64, DIMS)
+    g.gauge_random(0.5, 1)
+    F = clg.Field(ctx, clg.GAUGE, clg.F64, DIMS)
+    F.zero()
+    X = clg.Field(ctx, clg.WILSON, clg.F64, DIMS, bc=(1, 1, 1, -1))
+    X.gaussian(5)
+    Y = clg.Field(ctx, clg.WILSON, clg.F64, DIMS, bc=(1, 1, 1, -1))
+    Y.gaussian(6)
+    clg.wilson_force_kernel(ctx, F, g, X, Y, KAPPA)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        clg.wilson_force_kernel(ctx, F, g, X, Y, KAPPA)
+    ctx.sync()
+    us = (time.perf_counter() - t0) * 1e6 / reps
+    V = int(np.prod(DIMS))
+    out["wilson_force_kernel_us"] = round(us, 1)
+    out["wilson_force_kernel_GBs"] = round(3648 * V / (us * 1e-6) / 1e9, 1)
+    log(f"Wilson force kernel 32x32x32x16: {us:.1f} us  {out['wilson_force_kernel_GBs']:.0f} GB/s")
+    del g, F, X, Y
+    g = clg.Field(ctx, clg.GAUGE, clg.F64, HMC_DIMS)
+    g.gauge_random(0.4, 1)
+    F = clg.Field(ctx, clg.GAUGE, clg.F64, HMC_DIMS)
+    F.zero()
+    phis, ds = [], []
+    for r in range(12):
+        a = clg.Field(ctx, clg.KS, clg.F64, HMC_DIMS, bc=(1, 1, 1, -1))
+        a.gaussian(10 + r)
+        b = clg.Field(ctx, clg.KS, clg.F64, HMC_DIMS, bc=(1, 1, 1, -1))
+        b.gaussian(40 + r)
+        phis.append(a)
+        ds.append(b)
+    num = np.linspace(0.1, 1.0, 12)
+    clg.ks_force_kernel(ctx, F, g, phis, ds, num)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        clg.ks_force_kernel(ctx, F, g, phis, ds, num)
+    ctx.sync()
+    us = (time.perf_counter() - t0) * 1e6 / reps
+    V = int(np.prod(HMC_DIMS))
+    out["ks_force_kernel_12poles_us"] = round(us, 1)
+    out["ks_force_kernel_12poles_GBs"] = round(10944 * V / (us * 1e-6) / 1e9, 1)
+    log(f"staggered force kernel 24x24x24x8, 12 poles: {us:.1f} us  {out['ks_force_kernel_12poles_GBs']:.0f} GB/s")
+    return out
 
 
 def fit_rational(power, shifts, lo, hi):

@@ -61,10 +61,16 @@ __device__ __forceinline__ void add_link(char* F, int Vh, unsigned e, const cx* 
     LinkIO<CLG_F64, 18>::store(F, Vh, e, cur);
 }
 
+// Launch shape of the force kernels: 128-thread workgroups (the register budget that fits their
+// per-site state without spilling), XCD-contiguous slabs with the parities interleaved block by
+// block, so the x+mu neighbour reads of one parity meet the site reads of the other in one L2.
+constexpr int kForceBlock = 128;
+
 // Wilson force kernel; X = (DD^dagger)^-1 phi ("left"), Y = D^dagger X ("right"), FULL FP64 fields
-__global__ void k_wilson_force(Geo g, const char* X, const char* Y, const char* U, char* F, double kappa) {
+__global__ __launch_bounds__(kForceBlock) void k_wilson_force(Geo g, const char* X, const char* Y, const char* U, char* F,
+                                                              double kappa) {
     int p, h;
-    if (!map_site(g.Vh, 2, 0, 0, 0, p, h)) return;
+    if (!map_site(g.Vh, 2, 0, 8, 1, p, h)) return;
     const Site s = site_of(g, p, h);
     const int Vh = g.Vh;
     cx xl[12], xr[12];
@@ -109,9 +115,9 @@ struct PolePtrs {
     const char* d[32];
     double a[32];
 };
-__global__ void k_ks_force(Geo g, PolePtrs pp, int npole, const char* U, char* F) {
+__global__ __launch_bounds__(kForceBlock) void k_ks_force(Geo g, PolePtrs pp, int npole, const char* U, char* F) {
     int p, h;
-    if (!map_site(g.Vh, 2, 0, 0, 0, p, h)) return;
+    if (!map_site(g.Vh, 2, 0, 8, 1, p, h)) return;
     const Site s = site_of(g, p, h);
     const int Vh = g.Vh;
     for (int mu = 0; mu < 4; ++mu) {
@@ -176,7 +182,7 @@ int clg_wilson_force_kernel(clg_context ctx, clg_field force, clg_field gauge, c
                   "need FULL FP64 Wilson fields");
     Geo g = make_geo(inv_dddag);
     (void)hipSetDevice(ctx->device);
-    k_wilson_force<<<grid_for(2L * g.Vh, 128), 128, 0, ctx->stream>>>(g, (const char*)inv_dddag->data,
+    k_wilson_force<<<sched_grid(g.Vh, 2, kForceBlock, 8, 1), kForceBlock, 0, ctx->stream>>>(g, (const char*)inv_dddag->data,
                                                                        (const char*)inv_d->data,
                                                                        (const char*)gauge->data, (char*)force->data, kappa);
     CLG_HIP(hipGetLastError());
@@ -201,7 +207,7 @@ int clg_ks_force_kernel(clg_context ctx, clg_field force, clg_field gauge, const
     }
     Geo g = make_geo(phis[0]);
     (void)hipSetDevice(ctx->device);
-    k_ks_force<<<grid_for(2L * g.Vh, 128), 128, 0, ctx->stream>>>(g, pp, npole, (const char*)gauge->data,
+    k_ks_force<<<sched_grid(g.Vh, 2, kForceBlock, 8, 1), kForceBlock, 0, ctx->stream>>>(g, pp, npole, (const char*)gauge->data,
                                                                    (char*)force->data);
     CLG_HIP(hipGetLastError());
     return CLG_OK;
