@@ -1,0 +1,141 @@
+"""Size-independent checks at the benchmark sizes, where the oracle is too slow to run: the tuned
+launches there (LDS-staged planes, XCD slabs, several sites per thread) are the ones the benchmark
+measures.
+
+- Free field: with unit links a plane wave e^{ipx} u is an eigenvector; Wilson
+  D(p) = 1 - kappa sum_mu (2 cos p_mu - 2i gamma_mu sin p_mu) on the spinor u (D^dagger with +2i),
+  staggered D D^dagger = (2am)^2 + 4 sum_mu sin^2 p_mu, with p_t = (2n+1) pi / L_t for the
+  antiperiodic time direction.  These closed forms fix every projector, phase, sign and boundary.
+- Random links: full-lattice D against its even-odd composition from two half-lattice hops (other
+  kernels, other launch shapes), and gamma5-hermiticity <chi, D psi> = <D^dagger chi, psi>.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import cudalatticegauge_amd as clg
+import orc
+
+pytestmark = pytest.mark.gpu
+
+KAPPA = 0.1355
+C1 = (32, 32, 32, 16)       # Wilson Dslash benchmark lattice
+C4 = (24, 24, 24, 8)        # staggered RHMC lattice
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = clg.Context(0)
+    yield c
+    c.close()
+
+
+def momenta(dims, n):
+    return [2 * np.pi * n[i] / dims[i] + (np.pi / dims[i] if orc.BC_FERMION[i] < 0 else 0) for i in range(4)]
+
+
+def phase(dims, p):
+    X = np.array(list(itertools.product(*[range(d) for d in dims])), dtype=np.float64)
+    return np.exp(1j * (X @ np.asarray(p)))
+
+
+def unit_gauge(ctx, dims, prec=clg.F64, kind=clg.GAUGE):
+    g = clg.Field(ctx, kind, prec, dims)
+    g.gauge_unit()
+    return g
+
+
+@pytest.mark.parametrize("prec,tol", [(clg.F64, 1e-12), (clg.F32, 2e-6)])
+def test_wilson_free_field_at_32x32x32x16(ctx, prec, tol):
+    dims = C1
+    p = momenta(dims, [3, 1, 5, 2])
+    rng = np.random.default_rng(7)
+    u = rng.normal(size=12) + 1j * rng.normal(size=12)
+    ph = phase(dims, p)
+    psi = (ph[:, None] * u[None, :])
+    g = unit_gauge(ctx, dims, prec, clg.GAUGE12 if prec == clg.F32 else clg.GAUGE)
+    fin = clg.Field(ctx, clg.WILSON, prec, dims, bc=orc.BC_FERMION)
+    fin.upload(orc.from_complex(psi.ravel()))
+    gam = [orc.gamma(i) for i in range(4)]
+    for dag in (False, True):
+        M = np.eye(4, dtype=complex)
+        for mu in range(4):
+            M -= KAPPA * (2 * np.cos(p[mu]) * np.eye(4) + (2j if dag else -2j) * np.sin(p[mu]) * gam[mu])
+        want = np.kron(M, np.eye(3)) @ u
+        out = clg.Field(ctx, clg.WILSON, prec, dims)
+        clg.wilson_apply(ctx, out, fin, g, KAPPA, clg.EFO_F_Ddagger if dag else clg.EFO_F_D)
+        got = orc.as_complex(out.download()).reshape(-1, 12) / ph[:, None]
+        assert np.abs(got - want[None, :]).max() < tol * np.abs(want).max(), dag
+
+
+def test_wilson_full_d_equals_even_odd_composition_at_32x32x32x16(ctx):
+    dims = C1
+    g = clg.Field(ctx, clg.GAUGE, clg.F64, dims)
+    g.gauge_random(0.5, 1)
+    psi = clg.Field(ctx, clg.WILSON, clg.F64, dims, bc=orc.BC_FERMION)
+    psi.gaussian(2)
+    chi = clg.Field(ctx, clg.WILSON, clg.F64, dims, bc=orc.BC_FERMION)
+    chi.gaussian(3)
+    for dag in (False, True):
+        full = clg.Field(ctx, clg.WILSON, clg.F64, dims)
+        clg.wilson_apply(ctx, full, psi, g, KAPPA, clg.EFO_F_Ddagger if dag else clg.EFO_F_D)
+        comp = clg.Field(ctx, clg.WILSON, clg.F64, dims)
+        for out_sub in (clg.EVEN, clg.ODD):
+            in_sub = clg.ODD if out_sub == clg.EVEN else clg.EVEN
+            clg.wilson_hop(ctx, comp.parity(out_sub), psi.parity(in_sub), g, dagger=dag, a=-KAPPA,
+                           x=psi.parity(out_sub), b=1.0)
+        a, b = full.download(), comp.download()
+        assert np.abs(a - b).max() < 1e-13 * np.abs(a).max(), dag
+    # gamma5-hermiticity: <chi, D psi> = <D^dagger chi, psi>
+    dpsi = clg.Field(ctx, clg.WILSON, clg.F64, dims)
+    clg.wilson_apply(ctx, dpsi, psi, g, KAPPA, clg.EFO_F_D)
+    ddchi = clg.Field(ctx, clg.WILSON, clg.F64, dims)
+    clg.wilson_apply(ctx, ddchi, chi, g, KAPPA, clg.EFO_F_Ddagger)
+    lhs, rhs = clg.dot(ctx, chi, dpsi), clg.dot(ctx, ddchi, psi)
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+@pytest.mark.parametrize("dims", [C4, C1])
+def test_staggered_free_field_and_parity_blocks(ctx, dims):
+    m = 0.05
+    p = momenta(dims, [2, 5, 1, 3])
+    rng = np.random.default_rng(11)
+    c = rng.normal(size=3) + 1j * rng.normal(size=3)
+    ph = phase(dims, p)
+    chi = ph[:, None] * c[None, :]
+    lam = m * m + 4 * sum(np.sin(q) ** 2 for q in p)
+    g = unit_gauge(ctx, dims)
+    fin = clg.Field(ctx, clg.KS, clg.F64, dims, bc=orc.BC_FERMION)
+    fin.upload(orc.from_complex(chi.ravel()))
+    out = clg.Field(ctx, clg.KS, clg.F64, dims, bc=orc.BC_FERMION)
+    clg.ks_apply(ctx, out, fin, g, m, clg.EFO_F_DDdagger)
+    got = orc.as_complex(out.download()).reshape(-1, 3)
+    assert np.abs(got - lam * chi).max() < 1e-12 * lam * np.abs(chi).max()
+    # the parity blocks of D D^dagger act on the parity parts of the same plane wave
+    for sub in (clg.EVEN, clg.ODD):
+        blk = clg.Field(ctx, clg.KS, clg.F64, dims, subset=sub, bc=orc.BC_FERMION)
+        clg.ks_schur(ctx, blk, fin.parity(sub), g, m)
+        ref = orc.as_complex(fin.parity(sub).download())
+        assert np.abs(orc.as_complex(blk.download()) - lam * ref).max() < 1e-12 * lam * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("inner,acc", [(clg.F64, 0.0), (clg.F32, 1e-4), (clg.INNER_F16_VECTORS, 1e-2)])
+def test_wilson_solves_at_32x32x32x16(ctx, inner, acc):
+    """EO-GMRES(12) to 1e-10 at the benchmark lattice; the residual is recomputed here with the full-lattice
+    D (not the solver's even-odd operators)."""
+    dims = C1
+    g = clg.Field(ctx, clg.GAUGE, clg.F64, dims)
+    g.gauge_random(0.5, 1)
+    b = clg.Field(ctx, clg.WILSON, clg.F64, dims, bc=orc.BC_FERMION)
+    b.gaussian(2)
+    x = clg.Field(ctx, clg.WILSON, clg.F64, dims, bc=orc.BC_FERMION)
+    p = clg.solver_params(max_dim=12, restart=500, accuracy=1e-10, stop_rule=clg.STOP_TRUE_RESIDUAL, even_odd=1,
+                          inner_prec=inner, inner_accuracy=acc, max_outer=60)
+    st, rep = clg.solve(ctx, p, x, b, g, KAPPA, clg.EFO_F_InverseD)
+    assert st == clg.OK
+    r = clg.Field(ctx, clg.WILSON, clg.F64, dims)
+    clg.wilson_apply(ctx, r, x, g, KAPPA, clg.EFO_F_D)
+    clg.axpy(ctx, r, -1.0, b)                      # r = D x - b
+    res = np.sqrt(clg.norm2(ctx, r) / clg.norm2(ctx, b))
+    assert res < 1e-10 and abs(res - rep.true_residual) < 1e-3 * rep.true_residual + 1e-14
