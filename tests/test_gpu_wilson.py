@@ -240,3 +240,27 @@ def test_lds_staged_launches(ctx, dims):
                 assert rel(out.download().reshape(-1, 24), ref_eo[orc.eo_sites(dims, par)]) < 1e-12, (pm, par)
     finally:
         clg.autotune_force(0)
+
+
+@pytest.mark.parametrize("dims", [(2, 4, 6, 8), (6, 6, 6, 6), (4, 8, 4, 12), (2, 2, 16, 16), (4, 2, 32, 16),
+                                  (8, 4, 8, 10)])
+def test_operators_on_lattice_shapes(ctx, dims):
+    """Tuned launches (whatever the tuner picks per shape: staged planes where L2*L3/2 is 64, 128 or 256,
+    XCD slabs, sites per thread) against the oracle, Wilson D / D^dagger and staggered D / D^dagger."""
+    U = orc.gauge_random(dims, 0.4, 21)
+    psi = orc.wilson_gaussian(dims, 22)
+    chi = orc.ks_gaussian(dims, 23)
+    g = upload(ctx, clg.GAUGE, clg.F64, dims, U)
+    fw = upload(ctx, clg.WILSON, clg.F64, dims, psi)
+    fk = upload(ctx, clg.KS, clg.F64, dims, chi)
+    clg.autotune_enable(1)
+    try:
+        for op in (clg.EFO_F_D, clg.EFO_F_Ddagger):
+            out = clg.Field(ctx, clg.WILSON, clg.F64, dims)
+            clg.wilson_apply(ctx, out, fw, g, KAPPA, op)
+            assert rel(out.download(), orc.wilson_op(dims, orc.BC_FERMION, KAPPA, U, psi, op)) < 1e-12, op
+            outk = clg.Field(ctx, clg.KS, clg.F64, dims)
+            clg.ks_apply(ctx, outk, fk, g, 0.07, op)
+            assert rel(outk.download(), orc.ks_op(dims, orc.BC_FERMION, 0.07, U, chi, op)) < 1e-12, op
+    finally:
+        clg.autotune_enable(-1)
