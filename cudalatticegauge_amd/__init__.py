@@ -223,6 +223,7 @@ class Context:
         self.h = h
         self._fields = weakref.WeakSet()
         self._graphs = weakref.WeakSet()
+        self._solvers = weakref.WeakSet()
 
     def sync(self):
         check(lib().clg_synchronize(self.h))
@@ -237,6 +238,8 @@ class Context:
         if self.h:
             for g in list(self._graphs):
                 g.close()
+            for sv in list(self._solvers):
+                sv.close()
             fields = list(self._fields)
             for f in fields:
                 if f._parent is not None:
@@ -699,6 +702,7 @@ class Gcrodr:
         self.params = p
         self.h = C.c_void_p()
         check(lib().clg_gcrodr_create(ctx.h, C.byref(p), C.byref(self.h)))
+        ctx._solvers.add(self)
 
     def solve(self, x, b, gauge, kappa_or_mass, op=EFO_F_D, phase=PHASE_ONCE, start=None):
         rep = SolveReport()
@@ -711,9 +715,10 @@ class Gcrodr:
         check(lib().clg_gcrodr_reset(self.h))
 
     def close(self):
-        if self.h:
+        # after Context.close the library has destroyed the solver with its context
+        if self.h and self.ctx.h:
             lib().clg_gcrodr_destroy(self.h)
-            self.h = C.c_void_p()
+        self.h = C.c_void_p()
 
     def __del__(self):
         try:

@@ -444,20 +444,27 @@ int clg_gcrodr_params_default(clg_gcrodr_params* p) {
     return CLG_OK;
 }
 
+static void gcrodr_free(void* obj) {
+    clg_gcrodr_s* s = static_cast<clg_gcrodr_s*>(obj);
+    for (clg_field_s* f : s->U) clg_field_destroy(f);
+    delete s;
+}
+
 int clg_gcrodr_create(clg_context ctx, const clg_gcrodr_params* params, clg_gcrodr* out) {
     CLG_CHECK_ARG(ctx && out, "null argument");
     CLG_TRY(check_params(params));
     clg_gcrodr_s* s = new clg_gcrodr_s;
     s->ctx = ctx;
     s->p = *params;
+    ctx_adopt(ctx, s, gcrodr_free);
     *out = s;
     return CLG_OK;
 }
 
 int clg_gcrodr_destroy(clg_gcrodr s) {
     if (!s) return CLG_OK;
-    for (clg_field_s* f : s->U) clg_field_destroy(f);
-    delete s;
+    ctx_release(s->ctx, s);
+    gcrodr_free(s);
     return CLG_OK;
 }
 

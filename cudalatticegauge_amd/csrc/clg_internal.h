@@ -18,6 +18,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/clg_hip.h"
@@ -82,6 +83,9 @@ struct clg_context_s {
     // scratch fields keyed by shape (owned)
     std::vector<clg_field_s*> scratch;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // objects made on this context (GCRO-DR solvers, graphs) with their destroy functions:
+    // clg_finalize destroys those the caller has not
+    std::vector<std::pair<void*, void (*)(void*)>> owned;
 };
 
 struct clg_field_s {
@@ -132,6 +136,14 @@ inline float* norm_ptr(const clg_field_s* f, int p) {
     return f->norm;
 }
 inline int field_parity(const clg_field_s* f) { return f->subset == CLG_ODD ? 1 : 0; }
+inline void ctx_adopt(clg_context c, void* obj, void (*destroy)(void*)) { c->owned.emplace_back(obj, destroy); }
+inline void ctx_release(clg_context c, void* obj) {
+    for (auto it = c->owned.begin(); it != c->owned.end(); ++it)
+        if (it->first == obj) {
+            c->owned.erase(it);
+            return;
+        }
+}
 
 // reductions (clg_blas.hip)
 int ensure_reduction_scratch(clg_context ctx, size_t doubles);

@@ -493,6 +493,10 @@ int clg_finalize(clg_context c) {
     if (!c) return CLG_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    // solvers and graphs the caller left: newest first, while the context is still whole
+    std::vector<std::pair<void*, void (*)(void*)>> owned;
+    owned.swap(c->owned);
+    for (auto it = owned.rbegin(); it != owned.rend(); ++it) it->second(it->first);
     for (clg_field_s* f : c->scratch) clg_field_destroy(f);
     c->scratch.clear();
     if (c->red_partial) (void)hipFree(c->red_partial);
@@ -527,6 +531,8 @@ int clg_graph_begin(clg_context c) {
     return CLG_OK;
 }
 
+static void graph_free(void* obj);
+
 int clg_graph_end(clg_context c, clg_graph* out) {
     CLG_CHECK_ARG(c && out, "null argument");
     CLG_CHECK_ARG(stream_capturing(c), "the context records no graph");
@@ -554,6 +560,7 @@ int clg_graph_end(clg_context c, clg_graph* out) {
     r->graph = g;
     r->exec = ex;
     r->nodes = (int)nodes;
+    ctx_adopt(c, r, graph_free);
     *out = r;
     return CLG_OK;
 }
@@ -568,13 +575,19 @@ int clg_graph_launch(clg_graph g) {
 
 int clg_graph_nodes(clg_graph g) { return g ? g->nodes : -1; }
 
-int clg_graph_destroy(clg_graph g) {
-    if (!g) return CLG_OK;
+static void graph_free(void* obj) {
+    clg_graph_s* g = static_cast<clg_graph_s*>(obj);
     (void)hipSetDevice(g->ctx->device);
     (void)hipStreamSynchronize(g->ctx->stream);
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     delete g;
+}
+
+int clg_graph_destroy(clg_graph g) {
+    if (!g) return CLG_OK;
+    ctx_release(g->ctx, g);
+    graph_free(g);
     return CLG_OK;
 }
 

@@ -245,6 +245,8 @@ typedef struct {
 /* defaults of CSolverGCRODR.cpp:67-118 (MDim 10, KDim MDim-2, SVD deflation) */
 int clg_gcrodr_params_default(clg_gcrodr_params* p);
 int clg_gcrodr_create(clg_context ctx, const clg_gcrodr_params* params, clg_gcrodr* out);
+/* destroy a solver before clg_finalize of its context; clg_finalize destroys any solver (and graph)
+ * still alive on the context, after which their handles are invalid */
 int clg_gcrodr_destroy(clg_gcrodr solver);
 /* forget the recycled subspace */
 int clg_gcrodr_reset(clg_gcrodr solver);

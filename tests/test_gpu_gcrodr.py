@@ -83,3 +83,36 @@ def test_gcrodr_bad_params(ctx):
         clg.Gcrodr(ctx, m_dim=4)
     with pytest.raises(clg.CLGError):
         clg.Gcrodr(ctx, m_dim=10, k_dim=9)
+
+
+def test_context_closed_before_its_solver_and_graph():
+    """Closing a context first destroys its GCRO-DR solvers and graphs (they hold device fields of the
+    context); closing the Python objects afterwards, or finalizing a context that still owns a solver
+    through the C ABI, must not touch freed memory."""
+    c = clg.Context(0)
+    U = orc.gauge_random(DIMS, 0.5, 1)
+    g = up(c, clg.GAUGE, U)
+    b = up(c, clg.WILSON, orc.wilson_gaussian(DIMS, 5))
+    x = clg.Field(c, clg.WILSON, clg.F64, DIMS, bc=BC)
+    s = clg.Gcrodr(c, m_dim=8, k_dim=4, restart=50, accuracy=1e-16)
+    st, rep = s.solve(x, b, g, 0.12, clg.EFO_F_D, phase=clg.PHASE_START)   # leaves a recycled subspace
+    assert st == clg.OK
+    clg.Graph.begin(c)
+    clg.scale(c, x, 0.5)
+    gr = clg.Graph.end(c)
+    gr.launch()
+    c.sync()
+    c.close()
+    s.close()
+    gr.close()
+    del s, gr
+    # C ABI: clg_finalize with a live solver destroys it
+    L = clg.lib()
+    import ctypes as C
+    h = C.c_void_p()
+    assert L.clg_init(0, C.byref(h)) == 0
+    p = clg.GcrodrParams()
+    assert L.clg_gcrodr_params_default(C.byref(p)) == 0
+    sv = C.c_void_p()
+    assert L.clg_gcrodr_create(h, C.byref(p), C.byref(sv)) == 0
+    assert L.clg_finalize(h) == 0
