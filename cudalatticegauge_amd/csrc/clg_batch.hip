@@ -827,15 +827,18 @@ __global__ __launch_bounds__(kRedBlock) void k_bh_multidot(const BTab* T, Act ac
     }
     block_partial<2 * K + 1>(acc, partial + (size_t)blockIdx.y * gridDim.x * (2 * K + 1));
 }
-// w' = w - sum_k h_k V_k (stored) with the partials of V_k^H w' and |w'|^2; h at ws[s*stride + offh]
+// w' = w - sum_k h_k V_k, stored to vector osel (w itself or the next basis vector), with the
+// partials of V_k^H w' and |w'|^2; h at ws[s*stride + offh]
 template <int K>
-__global__ __launch_bounds__(kRedBlock) void k_bh_gs_project(const BTab* T, Act act, int wsel, const double* ws,
+__global__ __launch_bounds__(kRedBlock) void k_bh_gs_project(const BTab* T, Act act, int wsel, int osel, const double* ws,
                                                               int stride, int offh, double* partial) {
     const int s = act.idx[blockIdx.y];
     const unsigned n = T->vh[s];
     const double* h = ws + (size_t)s * stride + offh;
-    uint4* wc = hvec(T, s, wsel);
-    float* wn = hnrm(T, s, wsel);
+    const uint4* wc = hvec(T, s, wsel);
+    const float* wn = hnrm(T, s, wsel);
+    uint4* oc = hvec(T, s, osel);
+    float* on = hnrm(T, s, osel);
     float hr[K], hi[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) { hr[k] = (float)h[2 * k]; hi[k] = (float)h[2 * k + 1]; }
@@ -883,7 +886,7 @@ __global__ __launch_bounds__(kRedBlock) void k_bh_gs_project(const BTab* T, Act 
 #pragma unroll
             for (int q = 0; q < 8; ++q) ww += yc[q] * yc[q];
         }
-        h16_store_site(wc, wn, n, i, y);
+        h16_store_site(oc, on, n, i, y);
         acc[2 * K] += ww;
     }
     block_partial<2 * K + 1>(acc, partial + (size_t)blockIdx.y * gridDim.x * (2 * K + 1));
@@ -950,24 +953,28 @@ __global__ __launch_bounds__(kRedBlock) void k_bgs_finish(Act act, const double*
         gs[3] = flag ? 1.0 : 0.0;
     }
 }
-// V[s][dsel] = (w' - flag sum_k c2_k V_k) / |w''|, w' = V[s][wsel]
+// in place on V[s][dsel], which holds w': (w' - flag sum_k c2_k V_k) / |w''|.  Without the second
+// pass only the per-site scales change: the stored halves of w'/|w'| are those of w'.
 template <int K>
-__global__ __launch_bounds__(256) void k_bh_gs_finalize(const BTab* T, Act act, int dsel, int wsel, const double* ws, int stride, int offgs) {
+__global__ __launch_bounds__(256) void k_bh_gs_finalize(const BTab* T, Act act, int dsel, const double* ws, int stride,
+                                                        int offgs) {
     const int s = act.idx[blockIdx.y];
     const unsigned n = T->vh[s];
     const double* gs = ws + (size_t)s * stride + offgs;
     const bool flag = gs[3] != 0.0;
     const float inv = (float)gs[2];
+    uint4* dc = hvec(T, s, dsel);
+    float* dn = hnrm(T, s, dsel);
+    if (!flag) {
+        BSITE_LOOP(i, n) dn[i] *= inv;
+        return;
+    }
     float cr[K], ci[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) { cr[k] = (float)gs[4 + 2 * k]; ci[k] = (float)gs[4 + 2 * k + 1]; }
-    const uint4* wc = hvec(T, s, wsel);
-    const float* wn = hnrm(T, s, wsel);
-    uint4* dc = hvec(T, s, dsel);
-    float* dn = hnrm(T, s, dsel);
     BSITE_LOOP(i, n) {
         float y[24];
-        h16_load_site(wc, wn, n, i, y);
+        h16_load_site(dc, dn, n, i, y);
         if (flag) {
 #pragma unroll
             for (int ch = 0; ch < 3; ++ch) {
@@ -1348,11 +1355,12 @@ struct BatchSolver {
     k_bred_finish<<<dim3(1, A.n), kRedBlock, 0, ctx->stream>>>(A, ctx->red_partial, (int)rg, 2 * KK + 1,         \
                                                                ws + offh, 2 * KK, ws + L.offGS, nullptr, nullptr, \
                                                                0.0, nullptr, stride);                            \
-    k_bh_gs_project<KK><<<rgrid, kRedBlock, 0, ctx->stream>>>(dtab, A, m, ws, stride, offh, ctx->red_partial);   \
+    k_bh_gs_project<KK><<<rgrid, kRedBlock, 0, ctx->stream>>>(dtab, A, m, j < m - 1 ? j + 1 : m, ws, stride, offh, \
+                                                              ctx->red_partial);                                 \
     k_bgs_finish<KK><<<dim3(1, A.n), kRedBlock, 0, ctx->stream>>>(A, ctx->red_partial, (int)rg, ws, stride, offh, \
                                                                   (int)L.offGS);                                 \
     if (j < m - 1)                                                                                               \
-        k_bh_gs_finalize<KK><<<egrid, 256, 0, ctx->stream>>>(dtab, A, j + 1, m, ws, stride, (int)L.offGS);
+        k_bh_gs_finalize<KK><<<egrid, 256, 0, ctx->stream>>>(dtab, A, j + 1, ws, stride, (int)L.offGS);
                 switch (nv) {
                     case 1: CALL(1) break; case 2: CALL(2) break; case 3: CALL(3) break; case 4: CALL(4) break;
                     case 5: CALL(5) break; case 6: CALL(6) break; case 7: CALL(7) break; case 8: CALL(8) break;
