@@ -102,6 +102,9 @@ __global__ __launch_bounds__(256) void k_wilson_hop(HopArgs a) {
     const char* st = nullptr;
     const float* stn = nullptr;
     int hbase = 0;
+    const unsigned po = a.out_full ? (unsigned)p : 0u;
+    const unsigned eo = po * SIO::NPL * a.g.Vh + h, no = po * a.g.Vh + h;
+    C2<R> xv[12];
     if constexpr (STAGED) {
         extern __shared__ uint4 stage[];
         const int B = (int)blockDim.x;
@@ -109,6 +112,10 @@ __global__ __launch_bounds__(256) void k_wilson_hop(HopArgs a) {
         // padded blocks (xcd remap, spt) are whole: every lane of a workgroup agrees on valid,
         // and blocks past the end stay past it
         if (!valid) break;
+        // a staged workgroup's own-parity sites have usually left L2 by the end of its hops (its
+        // other reads all go to the stage or to other planes): read the x term first, in flight
+        // together with the stage loads
+        if constexpr (HASX) SIO::load(a.x, a.xn, a.g.Vh, eo, no, xv);
         hbase = h - (int)threadIdx.x;
         const unsigned pe = a.in_full ? (unsigned)(1 - p) : 0u;
 #pragma unroll
@@ -121,12 +128,6 @@ __global__ __launch_bounds__(256) void k_wilson_hop(HopArgs a) {
         if (!valid) continue;
     }
     const Site s = site_of(a.g, p, h);
-    const unsigned po = a.out_full ? (unsigned)p : 0u;
-    const unsigned eo = po * SIO::NPL * a.g.Vh + h, no = po * a.g.Vh + h;
-    // a staged workgroup's own-parity sites have usually left L2 by the end of its hops (its other
-    // reads all go to the stage or to other planes): read the x term first
-    C2<R> xv[12];
-    if constexpr (HASX && STAGED) SIO::load(a.x, a.xn, a.g.Vh, eo, no, xv);
     C2<R> acc[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[k] = cmk<R>(0, 0);
