@@ -377,7 +377,6 @@ __global__ __launch_bounds__(kRedBlock) void k_red_finish_ch(const double* __res
 // out2[0] (when given).  flag (nullable): *flag = (total_0 < eta * ref[0]).  cond (nullable): skip
 // when *cond == 0, as the first stage did.  Every load of a thread is independent (block-major
 // sweep with NR accumulators), so they all stay in flight.
-constexpr int kMaxRedNR = 2 * kMaxK + 1;
 template <int NR>
 __global__ __launch_bounds__(kRedBlock) void k_red_finish(const double* __restrict__ partial, int nblk, double* out,
                                                           int nout, double* out2, const int* cond, const double* ref,
