@@ -64,7 +64,8 @@ struct BHopArgs {
     double br;
 };
 
-template <int MU, bool DAG, int NR, typename R, class SIO, class LIO>
+// NT: links with the non-temporal cache policy (a half-lattice hop reads every link once)
+template <int MU, bool DAG, int NR, bool NT, typename R, class SIO, class LIO>
 __device__ __forceinline__ void bhop_mu(const BHopArgs& a, const BSeg& sg, const Site& s, int p, int h,
                                         C2<R> (&acc)[NR][12]) {
     const int Vh = sg.g.Vh;
@@ -72,7 +73,7 @@ __device__ __forceinline__ void bhop_mu(const BHopArgs& a, const BSeg& sg, const
     {
         const Nbr n = fwd_nbr<MU>(sg.g, s, p);
         const unsigned pe = a.in_full ? (unsigned)n.p : 0u;
-        LIO::load(sg.gauge, Vh, (unsigned)(p * 4 + MU) * LIO::NPL * Vh + h, u);
+        LIO::template load<NT>(sg.gauge, Vh, (unsigned)(p * 4 + MU) * LIO::NPL * Vh + h, u);
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
             if (r < sg.nr) {
@@ -90,7 +91,7 @@ __device__ __forceinline__ void bhop_mu(const BHopArgs& a, const BSeg& sg, const
     {
         const Nbr n = bwd_nbr<MU>(sg.g, s, p);
         const unsigned pe = a.in_full ? (unsigned)n.p : 0u;
-        LIO::load(sg.gauge, Vh, (unsigned)(n.p * 4 + MU) * LIO::NPL * Vh + n.h, u);
+        LIO::template load<NT>(sg.gauge, Vh, (unsigned)(n.p * 4 + MU) * LIO::NPL * Vh + n.h, u);
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
             if (r < sg.nr) {
@@ -107,7 +108,7 @@ __device__ __forceinline__ void bhop_mu(const BHopArgs& a, const BSeg& sg, const
     for (int r = 0; r < NR; ++r) pin_acc(acc[r]);
 }
 
-template <typename R, int SP, int GP, int RECON, bool DAG, bool HASX, int NR>
+template <typename R, int SP, int GP, int RECON, bool DAG, bool HASX, int NR, bool NT>
 __global__ __launch_bounds__(256) void k_wilson_hop_batch(BHopArgs a) {
     typedef WilsonIO<SP> SIO;
     typedef LinkIO<GP, RECON> LIO;
@@ -130,10 +131,10 @@ __global__ __launch_bounds__(256) void k_wilson_hop_batch(BHopArgs a) {
     for (int r = 0; r < NR; ++r)
 #pragma unroll
         for (int k = 0; k < 12; ++k) acc[r][k] = cmk<R>(0, 0);
-    bhop_mu<0, DAG, NR, R, SIO, LIO>(a, sg, s, p, h, acc);
-    bhop_mu<1, DAG, NR, R, SIO, LIO>(a, sg, s, p, h, acc);
-    bhop_mu<2, DAG, NR, R, SIO, LIO>(a, sg, s, p, h, acc);
-    bhop_mu<3, DAG, NR, R, SIO, LIO>(a, sg, s, p, h, acc);
+    bhop_mu<0, DAG, NR, NT, R, SIO, LIO>(a, sg, s, p, h, acc);
+    bhop_mu<1, DAG, NR, NT, R, SIO, LIO>(a, sg, s, p, h, acc);
+    bhop_mu<2, DAG, NR, NT, R, SIO, LIO>(a, sg, s, p, h, acc);
+    bhop_mu<3, DAG, NR, NT, R, SIO, LIO>(a, sg, s, p, h, acc);
     const unsigned po = a.out_full ? (unsigned)p : 0u;
     const unsigned eo = po * SIO::NPL * Vh + h, no = po * Vh + h;
     const R ca = (R)sg.ar;
@@ -158,29 +159,35 @@ __global__ __launch_bounds__(256) void k_wilson_hop_batch(BHopArgs a) {
 
 struct BCfg {
     int nr, block, xcd;
+    int nt = 0;  // 1: non-temporal link loads
 };
 
-template <typename R, int SP, int GP, int RECON, bool DAG, bool HASX>
+template <typename R, int SP, int GP, int RECON, bool DAG, bool HASX, bool NT>
 int launch_nr(const BHopArgs& a, unsigned grid, int block, int nr, hipStream_t st) {
     if (nr == 4) {
         if constexpr (SP == CLG_F64) return CLG_UNSUPPORTED;
-        else k_wilson_hop_batch<R, SP, GP, RECON, DAG, HASX, 4><<<grid, block, 0, st>>>(a);
+        else k_wilson_hop_batch<R, SP, GP, RECON, DAG, HASX, 4, NT><<<grid, block, 0, st>>>(a);
     } else if (nr == 2) {
-        k_wilson_hop_batch<R, SP, GP, RECON, DAG, HASX, 2><<<grid, block, 0, st>>>(a);
+        k_wilson_hop_batch<R, SP, GP, RECON, DAG, HASX, 2, NT><<<grid, block, 0, st>>>(a);
     } else {
-        k_wilson_hop_batch<R, SP, GP, RECON, DAG, HASX, 1><<<grid, block, 0, st>>>(a);
+        k_wilson_hop_batch<R, SP, GP, RECON, DAG, HASX, 1, NT><<<grid, block, 0, st>>>(a);
     }
     CLG_HIP(hipGetLastError());
     return CLG_OK;
 }
-template <typename R, int SP, int GP, int RECON>
-int launch_dx(const BHopArgs& a, bool dag, bool hasx, unsigned grid, int block, int nr, hipStream_t st) {
+template <typename R, int SP, int GP, int RECON, bool NT>
+int launch_dxn(const BHopArgs& a, bool dag, bool hasx, unsigned grid, int block, int nr, hipStream_t st) {
     if (dag) {
-        return hasx ? launch_nr<R, SP, GP, RECON, true, true>(a, grid, block, nr, st)
-                    : launch_nr<R, SP, GP, RECON, true, false>(a, grid, block, nr, st);
+        return hasx ? launch_nr<R, SP, GP, RECON, true, true, NT>(a, grid, block, nr, st)
+                    : launch_nr<R, SP, GP, RECON, true, false, NT>(a, grid, block, nr, st);
     }
-    return hasx ? launch_nr<R, SP, GP, RECON, false, true>(a, grid, block, nr, st)
-                : launch_nr<R, SP, GP, RECON, false, false>(a, grid, block, nr, st);
+    return hasx ? launch_nr<R, SP, GP, RECON, false, true, NT>(a, grid, block, nr, st)
+                : launch_nr<R, SP, GP, RECON, false, false, NT>(a, grid, block, nr, st);
+}
+template <typename R, int SP, int GP, int RECON>
+int launch_dx(const BHopArgs& a, bool dag, bool hasx, unsigned grid, int block, int nr, int nt, hipStream_t st) {
+    return nt ? launch_dxn<R, SP, GP, RECON, true>(a, dag, hasx, grid, block, nr, st)
+              : launch_dxn<R, SP, GP, RECON, false>(a, dag, hasx, grid, block, nr, st);
 }
 
 // One batched hop launch description: systems in launch order with their fields.
@@ -258,11 +265,11 @@ int launch_batch_cfg(const HopBatch& B, bool dag, bool hasx, double br, int pbas
     const clg_field_s* f = B.in[0];
     const clg_field_s* gf = B.gauge[0];
     const int recon = gf->kind == CLG_GAUGE12 ? 12 : 18;
-    if (f->prec == CLG_F64) return launch_dx<double, CLG_F64, CLG_F64, 18>(a, dag, hasx, grid, c.block, c.nr, st);
-    if (f->prec == CLG_F16) return launch_dx<float, CLG_F16, CLG_F16, 12>(a, dag, hasx, grid, c.block, c.nr, st);
-    if (gf->prec == CLG_F16) return launch_dx<float, CLG_F32, CLG_F16, 12>(a, dag, hasx, grid, c.block, c.nr, st);
+    if (f->prec == CLG_F64) return launch_dx<double, CLG_F64, CLG_F64, 18>(a, dag, hasx, grid, c.block, c.nr, c.nt, st);
+    if (f->prec == CLG_F16) return launch_dx<float, CLG_F16, CLG_F16, 12>(a, dag, hasx, grid, c.block, c.nr, c.nt, st);
+    if (gf->prec == CLG_F16) return launch_dx<float, CLG_F32, CLG_F16, 12>(a, dag, hasx, grid, c.block, c.nr, c.nt, st);
     (void)recon;
-    return launch_dx<float, CLG_F32, CLG_F32, 18>(a, dag, hasx, grid, c.block, c.nr, st);
+    return launch_dx<float, CLG_F32, CLG_F32, 18>(a, dag, hasx, grid, c.block, c.nr, c.nt, st);
 }
 
 // tuner: key = precision combination, dagger, x-term, parities and the batch shape
@@ -358,10 +365,11 @@ int batch_hop(clg_context ctx, const HopBatch& B, bool dag, double br, int tile)
             tune_cache_read(cpath, "wilson_hop_batch", ent);
             for (const auto& e : ent) {
                 const std::vector<long>& c = e.second;
-                if (e.first.size() < 13 || (e.first.size() - 9) % 4 || c.size() != 3) continue;
+                if (e.first.size() < 13 || (e.first.size() - 9) % 4 || c.size() != 4) continue;
                 const bool okc = (c[0] == 1 || c[0] == 2 || (c[0] == 4 && e.first[0] != CLG_F64)) &&
-                                 (c[1] == 64 || c[1] == 128 || c[1] == 256) && (c[2] == 0 || c[2] == 8);
-                if (okc) btune()[e.first] = BCfg{(int)c[0], (int)c[1], (int)c[2]};
+                                 (c[1] == 64 || c[1] == 128 || c[1] == 256) && (c[2] == 0 || c[2] == 8) &&
+                                 (c[3] == 0 || c[3] == 1);
+                if (okc) btune()[e.first] = BCfg{(int)c[0], (int)c[1], (int)c[2], (int)c[3]};
             }
             btune_loaded() = cpath;
             it = btune().find(key);
@@ -377,7 +385,8 @@ int batch_hop(clg_context ctx, const HopBatch& B, bool dag, double br, int tile)
         for (int nr : {1, 2, 4}) {
             if (nr == 4 && f64) continue;
             for (int blk : {64, 128, 256})
-                for (int xcd : {0, 8}) cands.push_back(BCfg{nr, blk, xcd});
+                for (int xcd : {0, 8})
+                    for (int nt : {0, 1}) cands.push_back(BCfg{nr, blk, xcd, nt});
         }
         hipEvent_t e0, e1;
         CLG_HIP(hipEventCreate(&e0));
@@ -397,11 +406,11 @@ int batch_hop(clg_context ctx, const HopBatch& B, bool dag, double br, int tile)
         (void)hipEventDestroy(e1);
         const char* verbose = std::getenv("CLG_TUNE_VERBOSE");
         if (verbose && verbose[0] == '1')
-            std::fprintf(stderr, "[clg tune] batched wilson hop: %d systems prec=%d dag=%d x=%d nblk=%d -> tile=%d block=%d xcd=%d (%.1f us)\n",
-                         B.n, in0->prec, (int)dag, (int)hasx, nblk, cfg.nr, cfg.block, cfg.xcd, best * 1e3 / 3);
+            std::fprintf(stderr, "[clg tune] batched wilson hop: %d systems prec=%d dag=%d x=%d nblk=%d -> tile=%d block=%d xcd=%d nt=%d (%.1f us)\n",
+                         B.n, in0->prec, (int)dag, (int)hasx, nblk, cfg.nr, cfg.block, cfg.xcd, cfg.nt, best * 1e3 / 3);
         std::lock_guard<std::mutex> lk(btune_mutex());
         btune()[key] = cfg;
-        tune_cache_append(cpath, "wilson_hop_batch", key, {cfg.nr, cfg.block, cfg.xcd});
+        tune_cache_append(cpath, "wilson_hop_batch", key, {cfg.nr, cfg.block, cfg.xcd, cfg.nt});
     }
     return launch_batch_cfg(B, dag, hasx, br, pbase, nblk, cfg, st);
 }
@@ -1497,8 +1506,9 @@ int batch_tune_entry(int index, char* buf, int len) {
     }
     std::snprintf(buf, (size_t)len,
                   "wilson_hop_batch prec=%ld gauge_prec=%ld dagger=%ld x=%ld parities=%ld dims=%s -> tile=%d block=%d "
-                  "xcd=%d",
-                  k[0], k[1], k[3], k[4], k[5], shapes.c_str(), it->second.nr, it->second.block, it->second.xcd);
+                  "xcd=%d nt=%d",
+                  k[0], k[1], k[3], k[4], k[5], shapes.c_str(), it->second.nr, it->second.block, it->second.xcd,
+                  it->second.nt);
     return CLG_OK;
 }
 
