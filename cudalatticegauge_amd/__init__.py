@@ -36,7 +36,8 @@ EXPORTED = [
     "clg_init", "clg_finalize", "clg_synchronize", "clg_status_string", "clg_last_error",
     "clg_device_info", "clg_field_create", "clg_field_destroy", "clg_field_set_bc",
     "clg_field_upload", "clg_field_download", "clg_field_copy", "clg_field_zero",
-    "clg_field_parity_view", "clg_field_host_doubles", "clg_field_bytes", "clg_gauge_random",
+    "clg_field_parity_view", "clg_field_host_doubles", "clg_field_bytes", "clg_file_read", "clg_file_write",
+    "clg_md5", "clg_field_load_file", "clg_field_save_file", "clg_gauge_random",
     "clg_gauge_unit", "clg_field_gaussian", "clg_field_point_source", "clg_wilson_apply",
     "clg_wilson_hop", "clg_wilson_schur", "clg_ks_apply", "clg_ks_hop", "clg_ks_schur", "clg_axpy", "clg_scale", "clg_dot",
     "clg_norm2", "clg_solve", "clg_solver_params_default", "clg_time_wilson", "clg_time_ks",
@@ -126,6 +127,11 @@ def lib():
             "clg_field_parity_view": (C.c_int, [vp, C.c_int, C.POINTER(vp)]),
             "clg_field_host_doubles": (C.c_long, [vp]),
             "clg_field_bytes": (C.c_long, [vp]),
+            "clg_file_read": (C.c_int, [C.c_char_p, C.c_int, C.c_int, ip, dp]),
+            "clg_file_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, ip, dp, C.c_char_p]),
+            "clg_md5": (C.c_int, [C.c_void_p, C.c_ulong, C.c_char_p]),
+            "clg_field_load_file": (C.c_int, [vp, C.c_char_p, C.c_int]),
+            "clg_field_save_file": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_char_p]),
             "clg_gauge_random": (C.c_int, [vp, C.c_double, C.c_uint]),
             "clg_gauge_unit": (C.c_int, [vp]),
             "clg_field_gaussian": (C.c_int, [vp, C.c_uint]),
@@ -203,6 +209,39 @@ def lib():
             fn.argtypes = args
         _lib = L
     return _lib
+
+
+# EFieldFileType (CField.h:26-35)
+(FILE_BRIDGEPP_TXT, FILE_BRIDGEPP_BIN, FILE_CLG_BIN, FILE_CLG_BIN_COMPRESSED, FILE_CLG_BIN_FLOAT,
+ FILE_CLG_BIN_DOUBLE) = range(6)
+
+
+def file_read(path, file_type, kind, dims):
+    """Host-only read of a full-lattice field file into the host AoS order (no device needed)."""
+    n = int(np.prod(dims)) * {WILSON: 24, KS: 6, GAUGE: 72, GAUGE12: 72}[kind]
+    a = np.empty(n, dtype=np.float64)
+    check(lib().clg_file_read(os.fsencode(path), file_type, kind, _i4(dims), a.ctypes.data_as(C.POINTER(C.c_double))))
+    return a
+
+
+def file_write(path, file_type, kind, dims, host):
+    """Host-only write of the host AoS order in a CLG binary format; returns the MD5 string."""
+    a = np.ascontiguousarray(host, dtype=np.float64)
+    n = int(np.prod(dims)) * {WILSON: 24, KS: 6, GAUGE: 72, GAUGE12: 72}[kind]
+    if a.size != n:
+        raise CLGError(f"file_write expects {n} doubles, got {a.size}")
+    out = C.create_string_buffer(33)
+    check(lib().clg_file_write(os.fsencode(path), file_type, kind, _i4(dims), a.ctypes.data_as(C.POINTER(C.c_double)),
+                               out))
+    return out.value.decode()
+
+
+def md5(data):
+    """CLGMD5Hash: upper-case hex MD5 of a bytes-like object."""
+    b = bytes(data)
+    out = C.create_string_buffer(33)
+    check(lib().clg_md5(b, len(b), out))
+    return out.value.decode()
 
 
 def check(status, allow=(OK,)):
@@ -298,6 +337,18 @@ class Field:
         a = np.empty(self.host_doubles(), dtype=np.float64)
         check(lib().clg_field_download(self.h, a.ctypes.data_as(C.POINTER(C.c_double))))
         return a
+
+    def load_file(self, path, file_type=None):
+        """CField::InitialFieldWithFile (CFieldGaugeSU3.cu:1080-1185); Bridge++ gauge files are
+        re-unitarised after loading."""
+        check(lib().clg_field_load_file(self.h, os.fsencode(path), FILE_CLG_BIN if file_type is None else file_type))
+
+    def save_file(self, path, file_type=None):
+        """CField::SaveToFile (CField.cpp:32-72); returns the MD5 string of the bytes written."""
+        out = C.create_string_buffer(33)
+        check(lib().clg_field_save_file(self.h, os.fsencode(path), FILE_CLG_BIN if file_type is None else file_type,
+                                        out))
+        return out.value.decode()
 
     def copy_from(self, src):
         check(lib().clg_field_copy(self.h, src.h))

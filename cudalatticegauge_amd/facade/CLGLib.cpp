@@ -187,6 +187,27 @@ void CFieldGaugeSU3::InitialField(EFieldInitialType t, unsigned seed, Real eps) 
     else if (t == EFIT_Zero) check(clg_field_zero(m_h), "gauge zero");
     else check(clg_gauge_random(m_h, eps, seed), "gauge random");
 }
+bool appFileTypeFromString(const std::string& name, EFieldFileType& out) {
+    static const std::pair<const char*, EFieldFileType> names[] = {
+        {"EFFT_BridgePPTXT", EFFT_BridgePPTXT}, {"EFFT_BridgePPBin", EFFT_BridgePPBin},
+        {"EFFT_CLGBin", EFFT_CLGBin},           {"EFFT_CLGBinCompressed", EFFT_CLGBinCompressed},
+        {"EFFT_CLGBinFloat", EFFT_CLGBinFloat}, {"EFFT_CLGBinDouble", EFFT_CLGBinDouble}};
+    for (const auto& n : names)
+        if (name == n.first) {
+            out = n.second;
+            return true;
+        }
+    return false;
+}
+void CField::InitialFieldWithFile(const std::string& fileName, EFieldFileType type) {
+    check(clg_field_load_file(m_h, fileName.c_str(), (int)type), "InitialFieldWithFile");
+}
+std::string CField::SaveToFile(const std::string& fileName, EFieldFileType type) const {
+    char md5[33] = {0};
+    check(clg_field_save_file(m_h, fileName.c_str(), (int)type, md5), "SaveToFile");
+    return md5;
+}
+
 DOUBLE CFieldGaugeSU3::CalculatePlaqutteEnergy(DOUBLE betaOverN) const {
     double e = 0;
     check(clg_gauge_plaquette_energy(m_pOwner->m_ctx, m_h, 3.0 * betaOverN, &e), "plaquette");
@@ -628,7 +649,17 @@ CLatticeData* appInitialCLG(const CParameters& sec, int device) {
             CParameters g;
             std::string init = "EFIT_Identity";
             if (sec.FetchParameterValue("Gauge", g)) g.FetchStringValue("FieldInitialType", init);
-            L->m_pGaugeField->InitialField(init == "EFIT_Random" ? EFIT_Random : EFIT_Identity, L->m_uiSeed, 3.0);
+            if (init == "EFIT_ReadFromFile") {
+                // GaugeFileType / GaugeFileName (CLatticeData / CLGLibManager gauge section)
+                std::string ftype = "EFFT_CLGBin", fname;
+                g.FetchStringValue("GaugeFileType", ftype);
+                EFieldFileType t;
+                if (!g.FetchStringValue("GaugeFileName", fname) || !appFileTypeFromString(ftype, t))
+                    throw std::runtime_error("Gauge: EFIT_ReadFromFile needs GaugeFileName and a known GaugeFileType");
+                L->m_pGaugeField->InitialFieldWithFile(fname, t);
+            } else {
+                L->m_pGaugeField->InitialField(init == "EFIT_Random" ? EFIT_Random : EFIT_Identity, L->m_uiSeed, 3.0);
+            }
         }
         // fermion fields
         INT nf = 1;

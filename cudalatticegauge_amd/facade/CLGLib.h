@@ -32,8 +32,20 @@ typedef clg_field_operator EFieldOperator;
 typedef clg_coeff_type EOperatorCoefficientType;
 // ESolverPhase (CCommonData.h:168-172)
 enum ESolverPhase { ESP_StartTrajectory = 0, ESP_InTrajectory = 1, ESP_EndTrajectory = 2, ESP_Once = 3 };
-// EFieldInitialType subset (CFieldGaugeSU3.cu:32-82, CFieldFermionWilsonSquareSU3.cu:463-500)
-enum EFieldInitialType { EFIT_Zero, EFIT_Identity, EFIT_Random, EFIT_RandomGaussian };
+// EFieldInitialType subset (CFieldGaugeSU3.cu:32-82, CFieldFermionWilsonSquareSU3.cu:463-500,
+// CLatticeData.h:18-33)
+enum EFieldInitialType { EFIT_Zero, EFIT_Identity, EFIT_Random, EFIT_RandomGaussian, EFIT_ReadFromFile };
+// EFieldFileType, same names and numbering (CField.h:26-35)
+enum EFieldFileType {
+    EFFT_BridgePPTXT = CLG_FILE_BRIDGEPP_TXT,
+    EFFT_BridgePPBin = CLG_FILE_BRIDGEPP_BIN,
+    EFFT_CLGBin = CLG_FILE_CLG_BIN,
+    EFFT_CLGBinCompressed = CLG_FILE_CLG_BIN_COMPRESSED,
+    EFFT_CLGBinFloat = CLG_FILE_CLG_BIN_FLOAT,
+    EFFT_CLGBinDouble = CLG_FILE_CLG_BIN_DOUBLE
+};
+// "EFFT_BridgePPTXT" -> EFFT_BridgePPTXT, ... (the YAML key GaugeFileType); false when unknown
+bool appFileTypeFromString(const std::string& name, EFieldFileType& out);
 
 // ---------------------------------------------------------------- parameters (YAML tree)
 class CParameters {
@@ -85,6 +97,10 @@ public:
     virtual CField* GetCopy() const = 0;
     std::vector<double> Download() const;
     void Upload(const std::vector<double>& host);
+    // CField::InitialFieldWithFile / SaveToFile (CField.h:97, 154; CField.cpp:32-72); SaveToFile
+    // returns the MD5 string of the bytes written
+    void InitialFieldWithFile(const std::string& fileName, EFieldFileType type = EFFT_CLGBin);
+    std::string SaveToFile(const std::string& fileName, EFieldFileType type = EFFT_CLGBin) const;
 
     int m_byFieldId = 0;
 

@@ -6,6 +6,8 @@
 //                   trajectories, Metropolis measured ones; average plaquette, acceptance, H_diff
 //          solver   TestSolver (TestSolver.cpp:12-121): |D^-1 D phi - phi|^2 and the other three
 //                   combinations for D and D^dagger, and DD^dagger
+//          fileio   TestFileIO (TestConfigurationFileIO.cpp:12-37): average plaquette of the gauge
+//                   field the section loads (EFIT_ReadFromFile), against ExpectedRes
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -82,6 +84,16 @@ static int run_solver(CParameters& sec) {
     return 0;
 }
 
+static int run_fileio(CParameters& sec) {
+    CLatticeData* L = appGetLattice();
+    Real expected = 0.625129946974942;
+    sec.FetchValueReal("ExpectedRes", expected);
+    const double V = (double)L->m_dims[0] * L->m_dims[1] * L->m_dims[2] * L->m_dims[3];
+    const double plaq = 1.0 - L->m_pGaugeField->CalculatePlaqutteEnergy(1.0 / 3.0) / (6.0 * V);
+    std::printf("{\"test\": \"fileio\", \"plaquette\": %.15f, \"expected\": %.15f}\n", plaq, expected);
+    return 0;
+}
+
 static void dump(const CParameters& p) {
     std::printf("{");
     bool first = true;
@@ -104,7 +116,7 @@ static void dump(const CParameters& p) {
 
 int main(int argc, char** argv) {
     if (argc < 4) {
-        std::fprintf(stderr, "usage: %s <yaml> <section> <update|solver>\n", argv[0]);
+        std::fprintf(stderr, "usage: %s <yaml> <section> <update|solver|fileio|dump>\n", argv[0]);
         return 2;
     }
     CParameters all, sec;
@@ -123,6 +135,7 @@ int main(int argc, char** argv) {
         const std::string t = argv[3];
         if (t == "update") rc = run_update(sec);
         else if (t == "solver") rc = run_solver(sec);
+        else if (t == "fileio") rc = run_fileio(sec);
         else std::fprintf(stderr, "unknown test %s\n", argv[3]);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "failed: %s\n", e.what());

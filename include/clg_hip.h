@@ -99,6 +99,36 @@ long clg_field_host_doubles(clg_field f);
 /* device bytes held by the field */
 long clg_field_bytes(clg_field f);
 
+/* ---- configuration files: EFieldFileType in the reference's order (CField.h:26-35) ---- */
+typedef enum {
+    CLG_FILE_BRIDGEPP_TXT = 0,       /* Bridge++ text: one real a line, x fastest, 18 per link */
+    CLG_FILE_BRIDGEPP_BIN = 1,       /* the same values as big-endian doubles */
+    CLG_FILE_CLG_BIN = 2,            /* the host AoS order above in Real = double (a double build's
+                                      * EFFT_CLGBin) */
+    CLG_FILE_CLG_BIN_COMPRESSED = 3, /* SU(3) logarithms: not supported (CLG_UNSUPPORTED) */
+    CLG_FILE_CLG_BIN_FLOAT = 4,      /* the host AoS order in float */
+    CLG_FILE_CLG_BIN_DOUBLE = 5      /* the host AoS order in double */
+} clg_file_type;
+/* Host only (no device needed).  Reads a full-lattice field of `kind` (CLG_GAUGE[12], CLG_WILSON,
+ * CLG_KS) on `dims` into the host AoS order.  Bridge++ files are gauge only
+ * (CFieldGaugeSU3::InitialFieldWithFile, CFieldGaugeSU3.cu:1080-1185); fermion fields read
+ * CLG_FILE_CLG_BIN only (CFieldFermionWilsonSquareSU3.cu:602-613).  A file whose size (or
+ * number of text values) does not match the lattice is CLG_BAD_ARG. */
+int clg_file_read(const char* path, int file_type, int kind, const int dims[4], double* host);
+/* Host only.  Writes the host AoS order in one of the CLG binary formats (CField::SaveToFile,
+ * CField.cpp:32-72); md5_out (NULL or 33 chars) receives the upper-case hex MD5 of the bytes
+ * written, as SaveToFile returns it (CLGMD5.h:566-590). */
+int clg_file_write(const char* path, int file_type, int kind, const int dims[4], const double* host,
+                   char md5_out[33]);
+/* CLGMD5Hash (CLGMD5.h:566-590): MD5 of n bytes, upper-case hex.  The length word is the bit
+ * count modulo 2^32 (CLGMD5.h:376), which equals standard MD5 below 512 MiB. */
+int clg_md5(const void* data, unsigned long n, char md5_out[33]);
+/* CField::InitialFieldWithFile on a FULL field: read + upload; Bridge++ gauge files are then
+ * re-unitarised (SetByArray -> ElementNormalize, CFieldGaugeSU3.cu:1262-1285).  Blocking. */
+int clg_field_load_file(clg_field f, const char* path, int file_type);
+/* CField::SaveToFile on a FULL field: download + clg_file_write.  Blocking. */
+int clg_field_save_file(clg_field f, const char* path, int file_type, char md5_out[33]);
+
 /* ============================== synthetic inputs ============================== */
 /* U = reunitarise(exp(i eps H)), H traceless Hermitian with Gaussian entries from a
  * counter-based Philox4x32-10 keyed by (seed, link); replaces

@@ -10,6 +10,7 @@ SECTIONS = {
     "TestSuit_Solver.yaml": ["TestSolverBiCGStab", "TestSolverGCRODR"],
     "TestSuit_FermionUpdatorKS.yaml": ["TestFermionUpdatorKS", "TestFermionUpdatorKSNestedForceGradient"],
     "TestSuit_FermionUpdator.yaml": ["TestFermionUpdatorNestedForceGradient"],
+    "TestSuit_FileIO.yaml": ["TestFileIOBridgePPText", "TestFileIOBridgePPBin"],
 }
 
 

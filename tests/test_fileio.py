@@ -1,0 +1,109 @@
+"""Configuration files and the MD5 string, host side (no device): the reference's own configuration
+fixtures (Bin/Debug/bridgepp_conf_text.dat, bridgepp_conf_binary.dat, testGaugeDouble.con_,
+testGaugeSingle.con_, copied unchanged under tests/golden/configs) and its expected values
+(TestConfigurationFileIO.cpp:12-37, 119-157, TestSuit_FileIO.yaml).
+
+- Bridge++ text and binary hold one 4^3x8 configuration: both readers must give the same links,
+  and its average plaquette is 0.625129946974942 (TestFileIO).  A wrong site order (x fastest in
+  the file, t fastest in the host order) or a wrong byte order changes that value.
+- The CLG double and float files hold one 8^4 SU(3) configuration: sum |U|^2 = 49152 =
+  3 x 4 links x 8^4 (TestFileDOUBLE, TestFileFloat), every link unitary, the two precisions agree.
+- MD5: RFC 1321's test suite and Python's hashlib on the bytes written.
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+import cudalatticegauge_amd as clg
+import orc
+
+HERE = os.path.join(os.path.dirname(__file__), "golden", "configs")
+BRIDGE_DIMS = (4, 4, 4, 8)
+CLG_DIMS = (8, 8, 8, 8)
+
+
+def links(U):
+    z = U[0::2] + 1j * U[1::2]
+    return z.reshape(-1, 3, 3)
+
+
+def test_md5_rfc1321_suite_and_hashlib():
+    rfc = {
+        b"": "D41D8CD98F00B204E9800998ECF8427E",
+        b"a": "0CC175B9C0F1B6A831C399E269772661",
+        b"abc": "900150983CD24FB0D6963F7D28E17F72",
+        b"message digest": "F96B697D7CB7938D525A2F31AAF161D0",
+        b"abcdefghijklmnopqrstuvwxyz": "C3FCD3D76192E4007DFB496CCA67E13B",
+        b"12345678901234567890123456789012345678901234567890123456789012345678901234567890":
+            "57EDF4A22BE3C955AC49DA2E2107B67A",
+    }
+    for msg, want in rfc.items():
+        assert clg.md5(msg) == want, msg
+    rng = np.random.default_rng(5)
+    # every tail length around the one- and two-block padding boundaries, and a longer buffer
+    for n in list(range(0, 140)) + [1 << 20]:
+        b = rng.integers(0, 256, size=n, dtype=np.uint8).tobytes()
+        assert clg.md5(b) == hashlib.md5(b).hexdigest().upper(), n
+
+
+def test_bridgepp_text_and_binary_give_the_reference_plaquette():
+    txt = clg.file_read(os.path.join(HERE, "bridgepp_conf_text.dat"), clg.FILE_BRIDGEPP_TXT, clg.GAUGE, BRIDGE_DIMS)
+    binr = clg.file_read(os.path.join(HERE, "bridgepp_conf_binary.dat"), clg.FILE_BRIDGEPP_BIN, clg.GAUGE, BRIDGE_DIMS)
+    assert np.abs(txt - binr).max() < 1e-14
+    V = int(np.prod(BRIDGE_DIMS))
+    for U in (txt, binr):
+        u = links(U)
+        assert np.abs(u @ u.conj().transpose(0, 2, 1) - np.eye(3)).max() < 1e-12
+        plaq = 1.0 - orc.plaquette_sum(BRIDGE_DIMS, U) / 3.0 / (6 * V)
+        assert abs(plaq - 0.625129946974942) < 1e-12, plaq
+    # the same values taken in the host's own order (no reordering) do not give that plaquette
+    raw = np.fromfile(os.path.join(HERE, "bridgepp_conf_binary.dat"), dtype=">f8").astype(np.float64)
+    assert abs(1.0 - orc.plaquette_sum(BRIDGE_DIMS, raw) / 3.0 / (6 * V) - 0.625129946974942) > 1e-3
+
+
+def test_clg_double_and_float_configurations():
+    d = clg.file_read(os.path.join(HERE, "testGaugeDouble.con_"), clg.FILE_CLG_BIN_DOUBLE, clg.GAUGE, CLG_DIMS)
+    s = clg.file_read(os.path.join(HERE, "testGaugeSingle.con_"), clg.FILE_CLG_BIN_FLOAT, clg.GAUGE, CLG_DIMS)
+    assert abs(np.dot(d, d) - 49152.0) < 1e-6 and abs(np.dot(s, s) - 49152.0) < 1e-2
+    ud = links(d)
+    assert np.abs(ud @ ud.conj().transpose(0, 2, 1) - np.eye(3)).max() < 1e-12
+    assert np.abs(d - s).max() < 1e-6
+    # EFFT_CLGBin of a double build is the double layout
+    assert np.array_equal(d, clg.file_read(os.path.join(HERE, "testGaugeDouble.con_"), clg.FILE_CLG_BIN, clg.GAUGE,
+                                           CLG_DIMS))
+
+
+@pytest.mark.parametrize("kind,per_site", [(clg.GAUGE, 72), (clg.WILSON, 24), (clg.KS, 6)])
+def test_clg_binary_round_trip_and_md5(tmp_path, kind, per_site):
+    dims = (4, 2, 6, 4)
+    host = np.random.default_rng(3).normal(size=int(np.prod(dims)) * per_site)
+    for ftype, dt in ((clg.FILE_CLG_BIN, "<f8"), (clg.FILE_CLG_BIN_DOUBLE, "<f8"), (clg.FILE_CLG_BIN_FLOAT, "<f4")):
+        p = str(tmp_path / f"f{ftype}.con")
+        md5 = clg.file_write(p, ftype, kind, dims, host)
+        raw = open(p, "rb").read()
+        assert md5 == hashlib.md5(raw).hexdigest().upper()
+        assert np.array_equal(np.frombuffer(raw, dtype=dt), host.astype(dt))
+        if kind == clg.GAUGE or ftype == clg.FILE_CLG_BIN:
+            back = clg.file_read(p, ftype, kind, dims)
+            assert np.array_equal(back, host.astype(dt).astype(np.float64))
+
+
+def test_file_errors(tmp_path):
+    p = str(tmp_path / "g.con")
+    clg.file_write(p, clg.FILE_CLG_BIN, clg.GAUGE, (2, 2, 2, 2), np.zeros(16 * 72))
+    with pytest.raises(clg.CLGError, match="bytes"):       # size does not match the lattice
+        clg.file_read(p, clg.FILE_CLG_BIN, clg.GAUGE, (2, 2, 2, 4))
+    with pytest.raises(clg.CLGError, match="values"):      # text file with too few values
+        clg.file_read(os.path.join(HERE, "bridgepp_conf_text.dat"), clg.FILE_BRIDGEPP_TXT, clg.GAUGE, (4, 4, 4, 16))
+    with pytest.raises(clg.CLGError, match="values"):      # and with more values than the lattice holds
+        clg.file_read(os.path.join(HERE, "bridgepp_conf_text.dat"), clg.FILE_BRIDGEPP_TXT, clg.GAUGE, (4, 4, 4, 4))
+    with pytest.raises(clg.CLGError, match="fermion"):
+        clg.file_read(p, clg.FILE_BRIDGEPP_BIN, clg.WILSON, (2, 2, 2, 2))
+    with pytest.raises(clg.CLGError):
+        clg.file_read(p, clg.FILE_CLG_BIN_COMPRESSED, clg.GAUGE, (2, 2, 2, 2))
+    with pytest.raises(clg.CLGError):
+        clg.file_write(p, clg.FILE_BRIDGEPP_BIN, clg.GAUGE, (2, 2, 2, 2), np.zeros(16 * 72))
+    with pytest.raises(clg.CLGError):
+        clg.file_read(str(tmp_path / "missing.con"), clg.FILE_CLG_BIN, clg.GAUGE, (2, 2, 2, 2))

@@ -13,9 +13,9 @@ RUNNER = os.path.join(ROOT, "cudalatticegauge_amd", "clg_test_runner")
 YAML = os.path.join(ROOT, "tests", "golden", "clgtest_sections.yaml")
 
 
-def run(section, test):
+def run(section, test, cwd=None):
     print(f"running {section} {test}", flush=True)
-    out = subprocess.run([RUNNER, YAML, section, test], capture_output=True, timeout=240)
+    out = subprocess.run([RUNNER, YAML, section, test], capture_output=True, timeout=240, cwd=cwd)
     assert out.returncode == 0, out.stderr.decode()[-2000:]
     res = json.loads(out.stdout.decode().strip().splitlines()[-1])
     print(res)
@@ -38,3 +38,14 @@ def test_updator_sections(section):
     assert abs(r["plaquette_avg"] - r["expected"]) < 0.15, r
     assert r["accepted"] >= r["expect_accept"], r
     assert r["hdiff"] < 0.12 and r["last_hdiff"] < 0.1, r
+
+
+@pytest.mark.parametrize("section", ["TestFileIOBridgePPText", "TestFileIOBridgePPBin"])
+def test_fileio_sections(section, tmp_path):
+    # TestFileIO (TestConfigurationFileIO.cpp:12-37): the loaded configuration's average plaquette
+    # within 1e-6 of ExpectedRes.  The sections name ../Debug/<file>, as the reference's Bin/Release
+    # working directory sees them; the fixtures stand in for that folder.
+    (tmp_path / "Debug").symlink_to(os.path.join(ROOT, "tests", "golden", "configs"))
+    (tmp_path / "Release").mkdir()
+    r = run(section, "fileio", cwd=str(tmp_path / "Release"))
+    assert abs(r["plaquette"] - r["expected"]) < 1e-6, r
