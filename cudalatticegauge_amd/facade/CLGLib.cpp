@@ -664,11 +664,18 @@ CLatticeData* appInitialCLG(const CParameters& sec, int device) {
         // fermion fields
         INT nf = 1;
         sec.FetchValueINT("FermionFieldCount", nf);
+        // even-parity fields (CFieldFermionWilsonSU3DEven) named by a full field's EvenFieldId: a
+        // solver registered for one solves its full field's system even-odd preconditioned
+        // (CFieldFermionWilsonSquareSU3.cu InverseD with m_byEvenFieldId)
+        std::map<int, int> even_parent;
         for_each_numbered(sec, "FermionField", nf, [&](const CParameters& f, int) {
             std::string name;
             f.FetchStringValue("FieldName", name);
             INT id = 2;
             f.FetchValueINT("FieldId", id);
+            INT even_id = -1;
+            if (f.FetchValueINT("EvenFieldId", even_id)) even_parent[even_id] = id;
+            if (name == "CFieldFermionWilsonSU3DEven") return;
             int bc[4];
             read_bc(f, bc);
             std::unique_ptr<CFieldFermion> field;
@@ -716,6 +723,13 @@ CLatticeData* appInitialCLG(const CParameters& sec, int device) {
                 else if (name == "CSLASolverGCRODR") sv.reset(new CSLASolverGCRODR);
                 else throw std::runtime_error("unsupported solver " + name);
                 sv->Configurate(s);
+                const auto ep = even_parent.find(id);
+                if (ep != even_parent.end()) {
+                    if (name == "CSLASolverGCRODR")
+                        throw std::runtime_error("even-odd GCRO-DR through an even field is not supported");
+                    sv->m_params.even_odd = 1;
+                    id = ep->second;
+                }
                 L->m_pFermionSolvers[id] = std::move(sv);
             }
         }

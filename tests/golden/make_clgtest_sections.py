@@ -7,7 +7,9 @@ import os
 import sys
 
 SECTIONS = {
-    "TestSuit_Solver.yaml": ["TestSolverBiCGStab", "TestSolverGCRODR"],
+    "TestSuit_Solver.yaml": ["TestSolverBiCGStab", "TestSolverGCRODR", "TestSolverGMRESLowMode",
+                             "TestSolverGCRODRLowMode", "TestSolverBiCGStabLowMode"],
+    "TestSuit_EvenOdd.yaml": ["TestEOSolverBiCGStab", "TestEOSolverGMRES"],
     "TestSuit_FermionUpdatorKS.yaml": ["TestFermionUpdatorKS", "TestFermionUpdatorKSNestedForceGradient"],
     "TestSuit_FermionUpdator.yaml": ["TestFermionUpdatorNestedForceGradient"],
     "TestSuit_FileIO.yaml": ["TestFileIOBridgePPText", "TestFileIOBridgePPBin"],
