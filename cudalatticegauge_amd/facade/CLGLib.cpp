@@ -285,6 +285,18 @@ void CFieldFermionKSSU3::PrepareForHMC(const CFieldGaugeSU3* gauge, unsigned see
                              m_rMC.data(), (int)(m_rMC.size() - 1) / 2, seed, &rep),
           "KS PrepareForHMC");
 }
+void CFieldFermion::RationalApproximation(EFieldOperator op, const CFieldGaugeSU3* gauge,
+                                         const CRatinalApproximation* r) {
+    CMultiShiftSolver* ms = appGetMultiShiftSolver(m_byFieldId);
+    if (!ms || !r || r->m_c.size() < 3 || r->m_c.size() % 2 == 0)
+        throw std::runtime_error("RationalApproximation needs an MSSolver and [c, a_1..a_n, b_1..b_n]");
+    clg_solve_report rep{};
+    clg_solver_params p = ms->m_params;
+    p.kind = dynamic_cast<const CFieldFermionKSSU3*>(this) ? CLG_KS : CLG_WILSON;
+    check(clg_rational_apply(m_pOwner->m_ctx, &p, ms->m_kind, ms->m_prec, m_h, m_h, gauge->Handle(), m_fParam,
+                             (int)op, r->m_c.data(), r->Degree(), &rep),
+          "RationalApproximation");
+}
 void CFieldFermionKSSU3::D_MC(const CFieldGaugeSU3* gauge) {
     CMultiShiftSolver* ms = appGetMultiShiftSolver(m_byFieldId);
     clg_solve_report rep{};

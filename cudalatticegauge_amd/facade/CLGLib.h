@@ -79,6 +79,14 @@ public:
 class CLatticeData;
 
 // ---------------------------------------------------------------- fields
+// CRatinalApproximation (CRationalApproximation.h:38-117): coefficients [c, a_1..a_n, b_1..b_n]
+class CRatinalApproximation {
+public:
+    explicit CRatinalApproximation(const std::vector<Real>& coeffs) : m_c(coeffs) {}
+    int Degree() const { return (int)(m_c.size() - 1) / 2; }
+    std::vector<Real> m_c;
+};
+
 class CField {
 public:
     virtual ~CField();
@@ -139,6 +147,9 @@ public:
     // EFS_Point source at (site coordinates, spin, colour) (CFieldFermionWilsonSquareSU3.cu:1110-1160)
     void InitialAsSource(const int coord[4], int spin, int colour);
     virtual void PrepareForHMC(const CFieldGaugeSU3* gauge, unsigned seed) = 0;
+    // CFieldFermion::RationalApproximation (CField.cpp:114-148): this <- c this + sum_k a_k (op + b_k)^-1 this,
+    // through the multi-shift solver registered for this field id
+    void RationalApproximation(EFieldOperator op, const CFieldGaugeSU3* gauge, const class CRatinalApproximation* r);
     Real Param() const { return m_fParam; }
 
 protected:

@@ -6,6 +6,8 @@
 //                   trajectories, Metropolis measured ones; average plaquette, acceptance, H_diff
 //          solver   TestSolver (TestSolver.cpp:12-121): |D^-1 D phi - phi|^2 and the other three
 //                   combinations for D and D^dagger, and DD^dagger
+//          mssolver_ks  TestMultiShiftSolverKS (TestMultiShiftSolver.cpp:109-200): rational powers
+//                   D^{+-1/2}, D^{+-1/4} and of DD^dagger through the section's multi-shift solver
 //          fileio   TestFileIO (TestConfigurationFileIO.cpp:12-37): average plaquette of the gauge
 //                   field the section loads (EFIT_ReadFromFile), against ExpectedRes
 #include <chrono>
@@ -84,6 +86,37 @@ static int run_solver(CParameters& sec) {
     return 0;
 }
 
+static int run_mssolver_ks(CParameters& sec) {
+    // TestMultiShiftSolverKS (TestMultiShiftSolver.cpp:109-200): |R^-1/n R^1/n phi - phi|^2 for n = 2, 4
+    // with op = D and DD^dagger, the rational coefficients taken from the section
+    CLatticeData* L = appGetLattice();
+    std::vector<Real> c2, m2, c4, m4;
+    sec.FetchValueArrayReal("OneOver2", c2);
+    sec.FetchValueArrayReal("MinusOneOver2", m2);
+    sec.FetchValueArrayReal("OneOver4", c4);
+    sec.FetchValueArrayReal("MinusOneOver4", m4);
+    Real expected = 1e-5;
+    sec.FetchValueReal("ExpectedErr", expected);
+    CFieldFermion* phi = dynamic_cast<CFieldFermion*>(L->GetFieldById(2));
+    const CFieldGaugeSU3* g = L->m_pGaugeField.get();
+    if (!phi) return 2;
+    const CRatinalApproximation r2(c2), r_2(m2), r4(c4), r_4(m4);
+    double err[4];
+    int k = 0;
+    for (EFieldOperator op : {EFO_F_D, EFO_F_DDdagger})
+        for (const auto* pr : {&r2, &r4}) {
+            std::unique_ptr<CFieldFermion> t(dynamic_cast<CFieldFermion*>(phi->GetCopy()));
+            t->RationalApproximation(op, g, pr);
+            t->RationalApproximation(op, g, pr == &r2 ? &r_2 : &r_4);
+            t->AxpyMinus(phi);
+            err[k++] = std::abs(t->Dot(t.get()));
+        }
+    std::printf("{\"test\": \"mssolver_ks\", \"errors\": [%.6e, %.6e, %.6e, %.6e], \"expected\": %.3e, "
+                "\"phi2\": %.6f}\n",
+                err[0], err[1], err[2], err[3], expected, std::abs(phi->Dot(phi)));
+    return 0;
+}
+
 static int run_fileio(CParameters& sec) {
     CLatticeData* L = appGetLattice();
     Real expected = 0.625129946974942;
@@ -116,7 +149,7 @@ static void dump(const CParameters& p) {
 
 int main(int argc, char** argv) {
     if (argc < 4) {
-        std::fprintf(stderr, "usage: %s <yaml> <section> <update|solver|fileio|dump>\n", argv[0]);
+        std::fprintf(stderr, "usage: %s <yaml> <section> <update|solver|mssolver_ks|fileio|dump>\n", argv[0]);
         return 2;
     }
     CParameters all, sec;
@@ -136,6 +169,7 @@ int main(int argc, char** argv) {
         if (t == "update") rc = run_update(sec);
         else if (t == "solver") rc = run_solver(sec);
         else if (t == "fileio") rc = run_fileio(sec);
+        else if (t == "mssolver_ks") rc = run_mssolver_ks(sec);
         else std::fprintf(stderr, "unknown test %s\n", argv[3]);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "failed: %s\n", e.what());

@@ -8,7 +8,8 @@ import sys
 
 SECTIONS = {
     "TestSuit_Solver.yaml": ["TestSolverBiCGStab", "TestSolverGCRODR", "TestSolverGMRESLowMode",
-                             "TestSolverGCRODRLowMode", "TestSolverBiCGStabLowMode"],
+                             "TestSolverGCRODRLowMode", "TestSolverBiCGStabLowMode", "TestMSKSSolverFOM",
+                             "TestMSKSSolverBiCGStab", "TestMSKSSolverNested"],
     "TestSuit_EvenOdd.yaml": ["TestEOSolverBiCGStab", "TestEOSolverGMRES"],
     "TestSuit_FermionUpdatorKS.yaml": ["TestFermionUpdatorKS", "TestFermionUpdatorKSNestedForceGradient"],
     "TestSuit_FermionUpdator.yaml": ["TestFermionUpdatorNestedForceGradient"],

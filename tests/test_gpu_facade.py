@@ -53,3 +53,11 @@ def test_fileio_sections(section, tmp_path):
     (tmp_path / "Release").mkdir()
     r = run(section, "fileio", cwd=str(tmp_path / "Release"))
     assert abs(r["plaquette"] - r["expected"]) < 1e-6, r
+
+
+@pytest.mark.parametrize("section", ["TestMSKSSolverFOM", "TestMSKSSolverBiCGStab", "TestMSKSSolverNested"])
+def test_multishift_ks_sections(section):
+    # TestMultiShiftSolverKS (TestMultiShiftSolver.cpp:109-200): every |R^-1/n R^1/n phi - phi|^2
+    # below ExpectedErr (default 1e-5)
+    r = run(section, "mssolver_ks")
+    assert all(e < r["expected"] for e in r["errors"]), r
