@@ -1,5 +1,5 @@
 """CLGTest sections of the reference's test suite replayed through the C++ facade (CLGLib class
-names over the HIP library), judged by the reference tests' own criteria and expected values."""
+names over the HIP library), checked by the reference tests' own criteria and expected values."""
 import json
 import os
 import subprocess
