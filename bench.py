@@ -6,14 +6,20 @@ Workload: 32x32x32x16 lattice, kappa = 0.1355, synthetic gauge U = reunitarise(e
 
 A step is one application of the full-lattice Wilson D (the reference operator
 _kernelDFermionWilsonSquareSU3) in FP64 to one right-hand side already resident in HBM.
-`value` is the algorithmic Dslash bandwidth: 384 reals x 8 bytes per site (8 neighbour
-spinors, 8 links, the site spinor in and out; no cache reuse counted) times the volume,
-divided by the measured time per step.  The same line carries FP32 / 12-real-link Dslash
-rates, the even-odd Schur operator, GMRES(12) solve times to |r|/|b| < 1e-10 in plain FP64
-and in FP32/FP16-inner defect correction, the CPU oracle's Dslash on the host cores, and one
-staggered RHMC trajectory at 24^3x8 (2+1 flavours, 12-pole rational functions on log-spaced
-shifts in [1e-4, 10], nested force gradient 8 outer x 6 nested steps, beta 5.5) run once with
-FP32 multi-shift force solves and once in plain FP64, with Delta H of both.
+`value` is the compulsory-byte Dslash bandwidth: the bytes no schedule can avoid moving between
+HBM and the chip for one D (the site spinor in, 4 links per site, the spinor out: 120 reals x 8
+bytes = 960 B per site) times the volume, divided by the measured time per step.  `roofline`
+prices the same bytes against the kernel's HIP-event time and the 8 TB/s HBM peak, so its `frac`
+cannot exceed what the hardware allows; the no-reuse count (384 reals per site: every neighbour
+spinor and link counted at each use) is kept in `extra` only.  The same line carries FP32 /
+12-real-link Dslash rates, the even-odd Schur operator, the staggered operator and the force
+kernels (all on compulsory bytes), GMRES(12) solves at 32^3x16 (plain FP64 to 1e-10, plain FP32 to
+1e-6, FP32/FP16-inner defect correction to 1e-10) with their iteration counts, the 32^3x32
+kappa 0.137 mixed solves with their error against the CPU oracle's FP64 solution, the batched
+12+4-system solve, the CPU oracle's Dslash on the host cores, and one staggered RHMC trajectory
+at 24^3x8 (2+1 flavours, 12-pole rational functions on log-spaced shifts in [1e-4, 10], nested
+force gradient 8 outer x 6 nested steps, beta 5.5) run once with FP32 multi-shift force solves
+and once in plain FP64, with Delta H of both.
 """
 import argparse
 import json
@@ -28,12 +34,27 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 DIMS = (32, 32, 32, 16)
 KAPPA = 0.1355
 HBM_PEAK_GBS = 8000.0
-BYTES_D = {8: 384 * 8, 4: 384 * 4}          # full Wilson D, 18-real links
-BYTES_D12 = {8: 336 * 8, 4: 336 * 4}        # 12-real links
-BYTES_SCHUR = {8: 744 * 8, 4: 744 * 4}      # per even site
+# Compulsory HBM bytes per unit (s = bytes per real): what one launch must read and write at least,
+# with every reused neighbour spinor and link served on chip.
+#   Wilson full D, per site: spinor in 24 + links 4 x 18 + spinor out 24 reals = 120 s
+#   (12-real links: 24 + 4 x 12 + 24 = 96 s)
+#   Wilson Schur 1 - kappa^2 H_eo H_oe, per even site: two half-lattice hops, each reading the
+#   other parity's spinor and every link once (8 x 18 reals per output site): (24 + 144 + 24) +
+#   (24 + 144 + 24 x-term + 24) = 408 s
+#   staggered full D, per site: 6 + 4 x 18 + 6 = 84 s; Schur block per even site: two half hops
+#   (6 + 144 + 6) + (6 + 144 + 6 + 6) = 318 s
+#   Wilson force kernel, per site: X 24 + Y 24 + links 72 + force read and write 2 x 72 = 264 s
+#   staggered force kernel, n poles, per site: links 72 + force 2 x 72 + n x (6 + 6) = (216 + 12 n) s
+COMP_D = {8: 120 * 8, 4: 120 * 4}
+COMP_D12 = {8: 96 * 8, 4: 96 * 4}
+COMP_SCHUR = {8: 408 * 8, 4: 408 * 4}
+COMP_KS = {8: 84 * 8, 4: 84 * 4}
+COMP_KS_SCHUR = 318 * 8
+COMP_WFORCE = 264 * 8
+COMP_KSFORCE12 = (216 + 12 * 12) * 8
+# no-reuse count (every neighbour spinor and link counted at each use), reported beside it
+NOREUSE_D = {8: 384 * 8, 4: 384 * 4}
 FLOPS_D = 1368
-BYTES_KS = {8: 204 * 8, 4: 204 * 4}         # full staggered D: 8 neighbour vectors, 8 links, site in/out
-BYTES_KS_HOP = 198 * 8                      # staggered half hop per output site (FP64; + the x term)
 
 
 def log(msg):
@@ -94,8 +115,8 @@ def main():
     ctx.sync()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
-    gbs = BYTES_D[8] * V / (ms_per_step * 1e-3) / 1e9
-    log(f"FP64 D: {ms_per_step * 1e3:.1f} us/step  {gbs:.0f} GB/s (algorithmic)")
+    gbs = COMP_D[8] * V / (ms_per_step * 1e-3) / 1e9
+    log(f"FP64 D: {ms_per_step * 1e3:.1f} us/step  {gbs:.0f} GB/s (compulsory bytes)")
     if args.profile:
         nbytes = out64.nbytes()
         for _ in range(5):
@@ -108,20 +129,22 @@ def main():
     # kernel-only time with HIP events on the library stream (same launches)
     k64 = clg.time_wilson(ctx, out64, src64, gauge64, KAPPA, 0, args.steps)
     extra = {"fp64_D_kernel_us": round(k64 * 1e3, 2),
-             "fp64_D_GBs": round(BYTES_D[8] * V / (k64 * 1e-3) / 1e9, 1),
-             "fp64_D_Gflops": round(FLOPS_D * V / (k64 * 1e-3) / 1e9, 1)}
+             "fp64_D_GBs": round(COMP_D[8] * V / (k64 * 1e-3) / 1e9, 1),
+             "fp64_D_noreuse_GBs": round(NOREUSE_D[8] * V / (k64 * 1e-3) / 1e9, 1),
+             "fp64_D_Gflops": round(FLOPS_D * V / (k64 * 1e-3) / 1e9, 1),
+             "fp64_D_tuned_launch": clg.autotune_entries()}
 
     # ---------------- other precisions / layouts
-    for name, prec, gk, nbytes in (("fp32_D", clg.F32, clg.GAUGE, BYTES_D[4]),
-                                   ("fp32_D_recon12", clg.F32, clg.GAUGE12, BYTES_D12[4]),
-                                   ("fp64_D_recon12", clg.F64, clg.GAUGE12, BYTES_D12[8])):
+    for name, prec, gk, nbytes in (("fp32_D", clg.F32, clg.GAUGE, COMP_D[4]),
+                                   ("fp32_D_recon12", clg.F32, clg.GAUGE12, COMP_D12[4]),
+                                   ("fp64_D_recon12", clg.F64, clg.GAUGE12, COMP_D12[8])):
         g, s, o = make_inputs(prec, gk)
         ms = clg.time_wilson(ctx, o, s, g, KAPPA, 0, args.steps)
         extra[name + "_kernel_us"] = round(ms * 1e3, 2)
         extra[name + "_GBs"] = round(nbytes * V / (ms * 1e-3) / 1e9, 1)
         log(f"{name}: {ms * 1e3:.1f} us  {extra[name + '_GBs']:.0f} GB/s")
         del g, s, o
-    for name, prec, nb in (("fp64_schur", clg.F64, BYTES_SCHUR[8]), ("fp32_schur", clg.F32, BYTES_SCHUR[4])):
+    for name, prec, nb in (("fp64_schur", clg.F64, COMP_SCHUR[8]), ("fp32_schur", clg.F32, COMP_SCHUR[4])):
         g, s, o = make_inputs(prec, clg.GAUGE)
         e_in, e_out = s.parity(clg.EVEN), o.parity(clg.EVEN)
         ms = clg.time_wilson(ctx, e_out, e_in, g, KAPPA, 2, args.steps)
@@ -148,15 +171,15 @@ def main():
         ms = clg.time_ks(ctx, o, s, g, 0.02, args.steps)
         vk = int(np.prod(dims))
         extra[tag + "_us"] = round(ms * 1e3, 2)
-        extra[tag + "_GBs"] = round(BYTES_KS[8 if prec == clg.F64 else 4] * vk / (ms * 1e-3) / 1e9, 1)
+        extra[tag + "_GBs"] = round(COMP_KS[8 if prec == clg.F64 else 4] * vk / (ms * 1e-3) / 1e9, 1)
         log(f"{tag}: {ms * 1e3:.1f} us  {extra[tag + '_GBs']:.0f} GB/s")
         if prec == clg.F64:
-            # the even-parity Schur block (2am)^2 - H_eo H_oe: two half-lattice hops, 2 x 198 reals per even site
+            # the even-parity Schur block (2am)^2 - H_eo H_oe: two half-lattice hops
             se, oe = s.parity(clg.EVEN), o.parity(clg.EVEN)
             ms = clg.time_ks(ctx, oe, se, g, 0.02, args.steps, variant=1)
             stag = tag.replace("_D", "_schur")
             extra[stag + "_us"] = round(ms * 1e3, 2)
-            extra[stag + "_GBs"] = round(2 * BYTES_KS_HOP * (vk // 2) / (ms * 1e-3) / 1e9, 1)
+            extra[stag + "_GBs"] = round(COMP_KS_SCHUR * (vk // 2) / (ms * 1e-3) / 1e9, 1)
             log(f"{stag}: {ms * 1e3:.1f} us  {extra[stag + '_GBs']:.0f} GB/s")
             del se, oe
         del g, s, o, g64, s64
@@ -183,10 +206,35 @@ def main():
             results[name] = rep
             extra[f"solve_{name}_s"] = round(rep.seconds, 4)
             extra[f"solve_{name}_cycles"] = rep.cycles
+            extra[f"solve_{name}_iterations"] = rep.operator_applies
             extra[f"solve_{name}_outer"] = rep.outer
             extra[f"solve_{name}_true_res"] = float(f"{rep.true_residual:.3e}")
-            log(f"solve {name}: {rep.seconds * 1e3:.1f} ms, cycles {rep.cycles}, outer {rep.outer}, "
-                f"res {rep.true_residual:.2e}, status {st}")
+            log(f"solve {name}: {rep.seconds * 1e3:.1f} ms, cycles {rep.cycles}, operator applications "
+                f"{rep.operator_applies}, outer {rep.outer}, res {rep.true_residual:.2e}, status {st}")
+        # plain single-precision solve (FP32 fields, FP32 links, FP64 reductions and Givens) to 1e-6
+        g32, s32, x32 = make_inputs(clg.F32, clg.GAUGE)
+        p32 = clg.solver_params(max_dim=12, restart=500, accuracy=1e-6, stop_rule=clg.STOP_TRUE_RESIDUAL, even_odd=1)
+        clg.solve(ctx, p32, x32, s32, g32, KAPPA, clg.EFO_F_InverseD)  # warm
+        best = None
+        for _ in range(2):
+            st, rep = clg.solve(ctx, p32, x32, s32, g32, KAPPA, clg.EFO_F_InverseD)
+            if best is None or rep.seconds < best[1].seconds:
+                best = (st, rep)
+        st, rep = best
+        # the FP32 solution's residual in the FP64 operator
+        x64c = clg.Field(ctx, clg.WILSON, clg.F64, DIMS, bc=(1, 1, 1, -1))
+        x64c.copy_from(x32)
+        r64 = clg.Field(ctx, clg.WILSON, clg.F64, DIMS, bc=(1, 1, 1, -1))
+        clg.wilson_apply(ctx, r64, x64c, gauge64, KAPPA, clg.EFO_F_D)
+        clg.axpy(ctx, r64, -1.0, src64)
+        res64 = float(np.sqrt(clg.norm2(ctx, r64) / clg.norm2(ctx, src64)))
+        extra.update({"solve_fp32_s": round(rep.seconds, 4), "solve_fp32_cycles": rep.cycles,
+                      "solve_fp32_iterations": rep.operator_applies,
+                      "solve_fp32_true_res": float(f"{rep.true_residual:.3e}"),
+                      "solve_fp32_res_in_fp64_operator": float(f"{res64:.3e}")})
+        log(f"solve fp32 (plain, to 1e-6): {rep.seconds * 1e3:.1f} ms, cycles {rep.cycles}, operator applications "
+            f"{rep.operator_applies}, res {rep.true_residual:.2e} (FP64 operator: {res64:.2e}), status {st}")
+        del g32, s32, x32, x64c, r64
         extra["solve_speedup_mixed_fp32_vs_fp64"] = round(results["fp64"].seconds / results["mixed_fp32"].seconds, 3)
         extra["solve_speedup_mixed_fp16link_vs_fp64"] = round(
             results["fp64"].seconds / results["mixed_fp16link"].seconds, 3)
@@ -215,19 +263,25 @@ def main():
         t_one = orc.time_wilson_d(DIMS, 0, 1, nth)
         reps = max(1, min(50, int(10.0 / max(t_one, 1e-3))))
         t = orc.time_wilson_d(DIMS, 0, reps, nth)
-        cpu_gbs = BYTES_D[8] * V * reps / t / 1e9
+        cpu_gbs = COMP_D[8] * V * reps / t / 1e9
         cpu = {"value": round(cpu_gbs, 2), "unit": "GB/s", "cores": nth, "kind": "port",
                "sample": f"{reps} FP64 Wilson D applications at 32x32x32x16 on {nth} OpenMP threads "
                          f"({t:.1f} s), oracle restatement of _kernelDFermionWilsonSquareSU3"}
         log(f"CPU oracle D: {cpu_gbs:.1f} GB/s on {nth} threads")
 
+    # HBM bytes per launch from the committed rocprofv3 counter runs of this kernel (FETCH_SIZE and
+    # WRITE_SIZE, each calibrated by the counters' reading for an in-place scale of known size): a
+    # profile's figure, not a counter read in this run; the launch it was taken on is named beside it
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "wilson_dslash_fp64_32x32x32x16_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        traffic = tj.get("hbm_bytes_per_launch")
+        extra["traffic_source"] = f"profiles/{os.path.basename(tpath)}: {tj.get('kernel')}"
+        extra["traffic_over_compulsory"] = round(traffic / (COMP_D[8] * V), 3) if traffic else None
 
-    achieved = BYTES_D[8] * V / (k64 * 1e-3) / 1e9
+    achieved = COMP_D[8] * V / (k64 * 1e-3) / 1e9
     line = {
         "metric": "Dslash GB/s (% HBM peak); 32\u00b3\u00d716 solve time to 1e-10: mixed/batched vs FP64",
         "value": round(gbs, 1),
@@ -296,10 +350,7 @@ def gcrodr_sequence(ctx, clg, np, gauge):
 def force_kernels(ctx, clg, np, steps):
     """Kernel time of the fermion force kernels on resident inputs (host clock around `steps`
     back-to-back launches, device synchronised): the Wilson force at 32^3x16 and the staggered force
-    with 12 poles at 24^3x8.  Algorithmic bytes per site, FP64: Wilson X, Y at x and at x+mu (4 mu), the
-    link and a read-modify-write of the force link: 2*192 + 4*(2*192 + 144 + 2*144) = 3648;
-    staggered per mu the link, the force read-modify-write and 4 colour vectors per pole:
-    4*(144 + 288 + 12*4*48) = 10944."""
+    with 12 poles at 24^3x8, priced on compulsory bytes (COMP_WFORCE, COMP_KSFORCE12)."""
     out = {}
     reps = max(10, steps // 4)
     g = clg.Field(ctx, clg.GAUGE, clg.F64, DIMS)
@@ -319,7 +370,7 @@ def force_kernels(ctx, clg, np, steps):
     us = (time.perf_counter() - t0) * 1e6 / reps
     V = int(np.prod(DIMS))
     out["wilson_force_kernel_us"] = round(us, 1)
-    out["wilson_force_kernel_GBs"] = round(3648 * V / (us * 1e-6) / 1e9, 1)
+    out["wilson_force_kernel_GBs"] = round(COMP_WFORCE * V / (us * 1e-6) / 1e9, 1)
     log(f"Wilson force kernel 32x32x32x16: {us:.1f} us  {out['wilson_force_kernel_GBs']:.0f} GB/s")
     del g, F, X, Y
     g = clg.Field(ctx, clg.GAUGE, clg.F64, HMC_DIMS)
@@ -344,7 +395,7 @@ def force_kernels(ctx, clg, np, steps):
     us = (time.perf_counter() - t0) * 1e6 / reps
     V = int(np.prod(HMC_DIMS))
     out["ks_force_kernel_12poles_us"] = round(us, 1)
-    out["ks_force_kernel_12poles_GBs"] = round(10944 * V / (us * 1e-6) / 1e9, 1)
+    out["ks_force_kernel_12poles_GBs"] = round(COMP_KSFORCE12 * V / (us * 1e-6) / 1e9, 1)
     log(f"staggered force kernel 24x24x24x8, 12 poles: {us:.1f} us  {out['ks_force_kernel_12poles_GBs']:.0f} GB/s")
     return out
 

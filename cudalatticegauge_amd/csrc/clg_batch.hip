@@ -315,6 +315,17 @@ int check_batch(const HopBatch& B) {
         }
         CLG_CHECK_ARG(in->data != out->data, "slot %d: out must not alias in", i);
     }
+    // every workgroup reads in[j] (neighbours) and x[j] while others write out[i]: an output may
+    // share storage with nothing but its own slot's x (read and written by the same thread)
+    for (int i = 0; i < B.n; ++i)
+        for (int j = 0; j < B.n; ++j) {
+            CLG_CHECK_ARG(!fields_overlap(B.out[i], B.in[j]), "out of slot %d overlaps in of slot %d", i, j);
+            if (j != i) {
+                CLG_CHECK_ARG(!fields_overlap(B.out[i], B.out[j]), "outs of slots %d and %d overlap", i, j);
+                CLG_CHECK_ARG(!B.x[j] || !fields_overlap(B.out[i], B.x[j]), "out of slot %d overlaps x of slot %d",
+                              i, j);
+            }
+        }
     return CLG_OK;
 }
 
@@ -332,9 +343,11 @@ int batch_hop(clg_context ctx, const HopBatch& B, bool dag, double br, int tile)
         if (in0->subset != CLG_FULL)
             CLG_CHECK_ARG(in0->subset != out0->subset, "a half-lattice hop maps one parity to the other");
     }
+    // an in-place x term (x_i sharing out_i's storage) accumulates on every repeated tuning launch
     bool aliased = false;
     for (int i = 0; i < B.n; ++i)
-        if (hasx && B.x[i]->data == B.out[i]->data) aliased = true;
+        for (int j = 0; j < B.n; ++j)
+            if (hasx && fields_overlap(B.x[i], B.out[j])) aliased = true;
     hipStream_t st = ctx->stream;
     const bool f64 = in0->prec == CLG_F64;
     if (tile > 0) {
@@ -1562,8 +1575,11 @@ int clg_solve_batch(clg_context ctx, const clg_solver_params* params, int n, clg
         CLG_TRY(check_same_lattice(b[i], gauge[i]));
         CLG_CHECK_ARG(x[i] != b[i], "system %d: x must not alias b", i);
         for (int d = 0; d < 4; ++d) CLG_CHECK_ARG(x[i]->bc[d] == b[i]->bc[d], "system %d: x and b bc differ", i);
-        for (int k = 0; k < i; ++k) CLG_CHECK_ARG(x[k] != x[i], "systems %d and %d share x", k, i);
+        for (int k = 0; k < i; ++k) CLG_CHECK_ARG(!fields_overlap(x[k], x[i]), "systems %d and %d share x", k, i);
     }
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < n; ++k)
+            CLG_CHECK_ARG(!fields_overlap(x[i], b[k]), "x of system %d overlaps b of system %d", i, k);
     (void)hipSetDevice(ctx->device);
     const auto t0 = std::chrono::steady_clock::now();
     BatchSolver S;

@@ -564,7 +564,14 @@ int clg_gcrodr_solve(clg_gcrodr s, clg_field x, clg_field b, clg_field gauge, do
         clg_field_s* bhat = nullptr;
         CLG_TRY(scratch_field(ctx, b, CLG_WILSON, CLG_F64, CLG_EVEN, 30203, &bhat));
         CLG_TRY(wilson_hop_half(ctx, bhat, bo, gauge, dag, kappa_or_mass, 0.0, be, 1.0, 0.0));
-        CLG_TRY(g.run(xe, bhat, se, phase, blen, conv));
+        // relative accuracy is measured on the even-site right-hand side the Schur solve is given
+        // (the reference's even-field solve: CFieldFermionWilsonSquareSU3.cu:1289-1292)
+        double blen_e = blen;
+        if (!p->absolute_accuracy) {
+            CLG_TRY(blas_norm2_dev(ctx, bhat, g.dws + g.offS));
+            CLG_TRY(fetch_doubles(ctx, g.dws + g.offS, 1, &blen_e));
+        }
+        CLG_TRY(g.run(xe, bhat, se, phase, blen_e, conv));
         CLG_TRY(wilson_hop_half(ctx, xo, xe, gauge, dag, kappa_or_mass, 0.0, bo, 1.0, 0.0));
     }
     // true residual of the requested full system

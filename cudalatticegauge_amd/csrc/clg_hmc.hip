@@ -709,25 +709,39 @@ int clg_hmc_trajectory(clg_context ctx, clg_field gauge, const clg_hmc_params* h
     CLG_TRY(scratch_field(ctx, gauge, CLG_GAUGE, CLG_F64, CLG_FULL, 13, &mom));
     CLG_TRY(scratch_field(ctx, gauge, CLG_GAUGE, CLG_F64, CLG_FULL, 14, &U0));
     CLG_TRY(clg_field_copy(U0, gauge));
+    // from here on, any failure puts the caller's gauge field back before returning its status
+    // (the rejected-trajectory rule, applied to errors as well)
+    auto restore = [&](int status) -> int {
+        (void)clg_field_copy(gauge, U0);
+        (void)hipStreamSynchronize(ctx->stream);
+        return status;
+    };
     const uint64_t s0 = mix64(seed);
     for (int i = 0; i < nactions; ++i) {
         clg_action& a = actions[i];
         const unsigned si = (unsigned)mix64(s0 + 1 + (uint64_t)i);
         if (a.kind == CLG_ACTION_FERMION_WILSON_NF2) {
-            CLG_TRY(clg_wilson_prepare_hmc(ctx, a.phi, gauge, a.kappa_or_mass, si));
+            const int st = clg_wilson_prepare_hmc(ctx, a.phi, gauge, a.kappa_or_mass, si);
+            if (st != CLG_OK) return restore(st);
         } else if (a.kind == CLG_ACTION_FERMION_KS) {
             CLG_CHECK_ARG(a.mc && a.mc_degree >= 1 && a.mc_degree <= 32, "action %d: MC rational needs 1..32 poles", i);
             clg_solve_report r{};
             const int st = clg_ks_prepare_hmc(ctx, &a.solver, a.ms_kind, a.ms_prec, a.phi, gauge, a.kappa_or_mass, a.mc,
                                               a.mc_degree, si, &r);
-            if (st != CLG_OK && st != CLG_NOT_CONVERGED) return st;
+            if (st != CLG_OK && st != CLG_NOT_CONVERGED) return restore(st);
             add_report(rep, r);
         }
     }
-    CLG_TRY(clg_gauge_momentum_random(mom, (unsigned)s0, (unsigned)(s0 >> 32)));
+    {
+        int st = clg_gauge_momentum_random(mom, (unsigned)s0, (unsigned)(s0 >> 32));
+        if (st != CLG_OK) return restore(st);
+    }
     Integrator it;
-    CLG_TRY(make_integrator(ctx, gauge, mom, hp, actions, nactions, rep, it));
-    CLG_TRY(evolve(it, rep));
+    {
+        int st = make_integrator(ctx, gauge, mom, hp, actions, nactions, rep, it);
+        if (st == CLG_OK) st = evolve(it, rep);
+        if (st != CLG_OK) return restore(st);
+    }
     const double dh = rep->h_diff;
     int accepted = 1;
     if (hp->metropolis) {

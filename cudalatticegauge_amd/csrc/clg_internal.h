@@ -136,6 +136,21 @@ inline float* norm_ptr(const clg_field_s* f, int p) {
     return f->norm;
 }
 inline int field_parity(const clg_field_s* f) { return f->subset == CLG_ODD ? 1 : 0; }
+// true when the device storage of a and b overlaps (data bytes or FP16 site scales); parity views
+// of one FULL field overlap it, and the two parity views of one field do not overlap each other
+inline size_t data_bytes(const clg_field_s* f) {
+    return f->block_bytes * (size_t)(is_gauge(f->kind) ? 8 : (f->subset == CLG_FULL ? 2 : 1));
+}
+inline bool fields_overlap(const clg_field_s* a, const clg_field_s* b) {
+    if (!a || !b) return false;
+    const char *a0 = (const char*)a->data, *b0 = (const char*)b->data;
+    if (a0 < b0 + data_bytes(b) && b0 < a0 + data_bytes(a)) return true;
+    if (a->norm && b->norm) {
+        const size_t na = (size_t)a->nsites, nb = (size_t)b->nsites;
+        if (a->norm < b->norm + nb && b->norm < a->norm + na) return true;
+    }
+    return false;
+}
 inline void ctx_adopt(clg_context c, void* obj, void (*destroy)(void*)) { c->owned.emplace_back(obj, destroy); }
 inline void ctx_release(clg_context c, void* obj) {
     for (auto it = c->owned.begin(); it != c->owned.end(); ++it)

@@ -465,16 +465,12 @@ const char* clg_status_string(int s) {
 
 const char* clg_last_error(void) { return clg::last_error(); }
 
-int clg_init(int device, clg_context* out) {
-    CLG_CHECK_ARG(out, "null output handle");
-    int n = 0;
-    CLG_HIP(hipGetDeviceCount(&n));
-    CLG_CHECK_ARG(device >= 0 && device < n, "device %d not present (%d devices)", device, n);
-    CLG_HIP(hipSetDevice(device));
-    clg_context c = new clg_context_s();
-    c->device = device;
+}  // extern "C"
+
+namespace {
+int init_context(clg_context c) {
     hipDeviceProp_t prop;
-    CLG_HIP(hipGetDeviceProperties(&prop, device));
+    CLG_HIP(hipGetDeviceProperties(&prop, c->device));
     c->num_cu = prop.multiProcessorCount;
     std::snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);
     CLG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -485,6 +481,26 @@ int clg_init(int device, clg_context* out) {
     CLG_HIP(hipEventCreate(&c->ev0));
     CLG_HIP(hipEventCreate(&c->ev1));
     CLG_HIP(hipStreamSynchronize(c->stream));
+    return CLG_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int clg_init(int device, clg_context* out) {
+    CLG_CHECK_ARG(out, "null output handle");
+    int n = 0;
+    CLG_HIP(hipGetDeviceCount(&n));
+    CLG_CHECK_ARG(device >= 0 && device < n, "device %d not present (%d devices)", device, n);
+    CLG_HIP(hipSetDevice(device));
+    clg_context c = new clg_context_s();
+    c->device = device;
+    // a failure part way releases whatever was made so far (clg_finalize skips what is still null)
+    const int st = init_context(c);
+    if (st != CLG_OK) {
+        clg_finalize(c);
+        return st;
+    }
     *out = c;
     return CLG_OK;
 }
@@ -492,7 +508,7 @@ int clg_init(int device, clg_context* out) {
 int clg_finalize(clg_context c) {
     if (!c) return CLG_OK;
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     // solvers and graphs the caller left: newest first, while the context is still whole
     std::vector<std::pair<void*, void (*)(void*)>> owned;
     owned.swap(c->owned);
@@ -506,7 +522,7 @@ int clg_finalize(clg_context c) {
     if (c->ws) (void)hipFree(c->ws);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
-    (void)hipStreamDestroy(c->stream);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return CLG_OK;
 }

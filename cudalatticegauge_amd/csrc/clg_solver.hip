@@ -472,15 +472,25 @@ struct Solve1 {
         } else if (!x_zero) {
             // normal-equation variant: same guess
         }
+        // the reference's relative stop rule measures the right-hand side the even-site solver is
+        // given (fBLength = |bhat_e|^2, CFieldFermionWilsonSquareSU3.cu:1289-1292 with
+        // CSolverGMRES.cpp:122 / CSolverBiCGstab.cpp:422-429); the true-residual rule keeps |b|
+        double ref_blen2 = -1.0;
+        if (p->stop_rule == CLG_STOP_REFERENCE && !p->absolute_accuracy && p->inner_prec == CLG_F64) {
+            double* d = ctx->red_result + 9;
+            CLG_TRY(blas_norm2_dev(ctx, bhat, d));
+            CLG_TRY(fetch_doubles(ctx, d, 1, &ref_blen2));
+        }
         bool c = false;
-        CLG_TRY(run_system(xe, bhat, x_zero, bnorm2, true, c));
+        CLG_TRY(run_system(xe, bhat, x_zero, bnorm2, true, c, ref_blen2));
         CLG_TRY(wilson_hop_half(ctx, xo, xe, gauge, dag, param, 0.0, bo, 1.0, 0.0));
         conv = c;
         return CLG_OK;
     }
 
     // A x = b on one lattice subset, precision strategy applied here
-    int run_system(clg_field_s* x, const clg_field_s* b, bool x_zero, double bnorm2, bool schur, bool& conv) {
+    int run_system(clg_field_s* x, const clg_field_s* b, bool x_zero, double bnorm2, bool schur, bool& conv,
+                   double ref_blen2 = -1.0) {
         LinOp A;
         A.ctx = ctx;
         A.kind = kind;
@@ -493,7 +503,7 @@ struct Solve1 {
         const double tol_abs = p->accuracy * bn;
         if (p->inner_prec == CLG_F64) {
             if (normal) return run_normal(A, x, b, x_zero, bn, conv);
-            return run_plain(A, x, b, x_zero, bnorm2, tol_abs, conv);
+            return run_plain(A, x, b, x_zero, ref_blen2 >= 0 ? ref_blen2 : bnorm2, tol_abs, conv);
         }
         return run_mixed(A, x, b, x_zero, bn, conv);
     }
@@ -656,8 +666,14 @@ int clg_solve(clg_context ctx, const clg_solver_params* p, clg_field x, clg_fiel
     CLG_TRY(validate_params(p));
     CLG_CHECK_ARG(b->kind == p->kind && x->kind == p->kind, "field kind does not match params.kind");
     CLG_CHECK_ARG(b->subset == CLG_FULL && x->subset == CLG_FULL, "solve takes FULL fields");
-    CLG_CHECK_ARG(b->prec == CLG_F64 && x->prec == CLG_F64, "solve takes FP64 fields (inner_prec selects the rest)");
-    CLG_CHECK_ARG(gauge->prec == CLG_F64, "solve takes the FP64 gauge field");
+    // FP64 fields (inner_prec selects defect correction), or a plain FP32 solve: FP32 fields and an
+    // FP32 gauge field with everything in FP32 arithmetic except the FP64 reductions, Hessenberg and
+    // Givens step (the reference's single-precision build, CSolverGMRES.cpp with Real = float)
+    CLG_CHECK_ARG(x->prec == b->prec && (b->prec == CLG_F64 || b->prec == CLG_F32),
+                  "solve takes FP64 fields (inner_prec selects the rest) or FP32 fields");
+    CLG_CHECK_ARG(gauge->prec == b->prec, "the gauge field must have the precision of the fermion fields");
+    CLG_CHECK_ARG(b->prec == CLG_F64 || p->inner_prec == CLG_F64,
+                  "an FP32 solve runs in FP32 throughout (inner_prec must be CLG_F64, no defect correction)");
     CLG_TRY(check_same_lattice(x, b));
     CLG_TRY(check_same_lattice(x, gauge));
     if (start) CLG_TRY(check_same_lattice(x, start));
@@ -676,7 +692,7 @@ int clg_solve(clg_context ctx, const clg_solver_params* p, clg_field x, clg_fiel
     const clg_field_s* bb = b;
     if (x == b || x->data == b->data) {
         clg_field_s* bc = nullptr;
-        CLG_TRY(scratch_field(ctx, b, b->kind, CLG_F64, CLG_FULL, 50, &bc));
+        CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, CLG_FULL, 50, &bc));
         CLG_TRY(clg_field_copy(bc, b));
         bb = bc;
     }
@@ -700,7 +716,7 @@ int clg_solve(clg_context ctx, const clg_solver_params* p, clg_field x, clg_fiel
         CLG_TRY(S.run(x, bb, false, bn2, c1));
     } else {
         clg_field_s* mid = nullptr;
-        CLG_TRY(scratch_field(ctx, b, b->kind, CLG_F64, CLG_FULL, 51, &mid));
+        CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, CLG_FULL, 51, &mid));
         CLG_TRY(clg_field_copy(mid, const_cast<clg_field_s*>(bb)));
         S.dag = false;
         CLG_TRY(S.run(mid, bb, false, bn2, c1));
@@ -713,10 +729,10 @@ int clg_solve(clg_context ctx, const clg_solver_params* p, clg_field x, clg_fiel
     }
     conv = c1 && c2;
 
-    // true residual of the requested system in FP64
+    // true residual of the requested system (FP64; FP32 arithmetic with FP64 sums for an FP32 solve)
     {
         clg_field_s* r = nullptr;
-        CLG_TRY(scratch_field(ctx, b, b->kind, CLG_F64, CLG_FULL, 52, &r));
+        CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, CLG_FULL, 52, &r));
         LinOp A;
         A.ctx = ctx;
         A.kind = p->kind;

@@ -202,14 +202,16 @@ int staged_block(const Geo& g) {
     const int b = g.L[2] * g.L[3] / 2;
     return (b == 64 || b == 128 || b == 256) ? b : 0;
 }
-// a staged configuration that does not fit the lattice runs as its unstaged pin mode
-HopCfg fit_cfg(const HopArgs& a, HopCfg c) {
-    if (c.pm >= 5 && (c.block != staged_block(a.g) || c.block == 0)) c.pm -= 4;
-    return c;
-}
-
 int launch_cfg(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool hasx, hipStream_t st, HopCfg c) {
-    c = fit_cfg(a, c);
+    // a staged launch needs one workgroup per (x,y) plane of one parity; the tuner only proposes
+    // fitting ones, so a misfit comes from a forced configuration and is refused, not replaced
+    if (c.pm >= 5) {
+        const int sb = staged_block(a.g);
+        CLG_CHECK_ARG(sb != 0 && c.block == sb,
+                      "LDS-staged pin mode %d needs block = L2*L3/2 in {64,128,256} (lattice %dx%dx%dx%d gives %d), "
+                      "asked for block %d",
+                      c.pm, a.g.L[0], a.g.L[1], a.g.L[2], a.g.L[3], a.g.L[2] * a.g.L[3] / 2, c.block);
+    }
     if (prec == CLG_F64) {
         return recon == 12 ? launch_t<double, CLG_F64, CLG_F64, 12>(a, dag, hasx, st, c)
                            : launch_t<double, CLG_F64, CLG_F64, 18>(a, dag, hasx, st, c);

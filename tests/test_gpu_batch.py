@@ -140,3 +140,41 @@ def test_solve_batch_dagger_and_errors(ctx, gauges):
                         [gauges[d]], KAPPA)
     with pytest.raises(clg.CLGError):
         clg.solve_batch(ctx, p, [x], [x], [gauges[d]], KAPPA)
+
+
+def test_batch_rejects_cross_slot_aliasing(ctx, gauges):
+    """Every workgroup of a batched hop reads the inputs and x terms of all slots while others write the
+    outputs: storage shared across slots (an output that is another slot's input, output or x, or a
+    parity view of one) is refused before any launch, and so are overlapping x or x/b fields of a
+    batched solve.  An output sharing only its own slot's x stays allowed (same site, same thread)."""
+    dims = (4, 4, 4, 4)
+    g = gauges[dims]
+    a, b, c = (field(ctx, dims, clg.F64, seed=s) for s in (1, 2, 3))
+    o1, o2 = field(ctx, dims, clg.F64), field(ctx, dims, clg.F64)
+    bad = [
+        dict(outs=[o1, a], ins=[a, b]),              # out of slot 1 is the input of slot 0
+        dict(outs=[o1, o1], ins=[a, b]),             # two slots write one field
+        dict(outs=[o1, o2], ins=[a, b], xs=[o2, b]),  # x of slot 0 is the output of slot 1
+    ]
+    for kw in bad:
+        with pytest.raises(clg.CLGError):
+            clg.wilson_hop_batch(ctx, kw["outs"], kw["ins"], [g, g], -KAPPA, xs=kw.get("xs"), b=1.0 if "xs" in kw
+                                 else 0.0)
+    # half-lattice hops: an output that is a parity view of another slot's full input overlaps it
+    oe = clg.Field(ctx, clg.WILSON, clg.F64, dims, subset=clg.EVEN, bc=BC)
+    with pytest.raises(clg.CLGError):
+        clg.wilson_hop_batch(ctx, [oe, c.parity(clg.EVEN)], [a.parity(clg.ODD), b.parity(clg.ODD)], [g, g], -KAPPA)
+    # allowed: out of each slot is its own x
+    want = []
+    for o, i in ((o1, a), (o2, b)):
+        o.copy_from(c)
+        t = clg.Field(ctx, clg.WILSON, clg.F64, dims, bc=BC)
+        clg.wilson_hop_batch(ctx, [t], [i], [g], -KAPPA, xs=[c], b=1.0)
+        want.append(t.download())
+    clg.wilson_hop_batch(ctx, [o1, o2], [a, b], [g, g], -KAPPA, xs=[o1, o2], b=1.0)
+    assert rel(o1.download(), want[0]) < 1e-15 and rel(o2.download(), want[1]) < 1e-15
+    # batched solve: x shared between systems, or x of one system is b of another
+    p = clg.solver_params(max_dim=8, restart=20, accuracy=1e-8, stop_rule=clg.STOP_TRUE_RESIDUAL, even_odd=1)
+    for xs, bs in (([o1, o1], [a, b]), ([o1, o2], [a, o1])):
+        with pytest.raises(clg.CLGError):
+            clg.solve_batch(ctx, p, xs, bs, [g, g], KAPPA)

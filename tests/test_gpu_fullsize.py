@@ -1,6 +1,10 @@
-"""Size-independent checks at the benchmark sizes, where the oracle is too slow to run: the tuned
-launches there (LDS-staged planes, XCD slabs, several sites per thread) are the ones the benchmark
-measures.
+"""Checks at the benchmark sizes.  The tuned launches there (LDS-staged planes, XCD slabs, several
+sites per thread) are the ones the benchmark measures.
+
+- Oracle: the benchmark's own inputs (32^3x16, eps 0.5 gauge seed 1, Gaussian source seed 2, made on
+  the GPU and downloaded) go through the oracle's Wilson D and D^dagger (one application takes tens
+  of milliseconds on the host cores); the GPU's D and D^dagger must match to 1e-12 (FP64) and 1e-6
+  (FP32), with the launch the benchmark times forced and with whatever the tuner picks.
 
 - Free field: with unit links a plane wave e^{ipx} u is an eigenvector; Wilson
   D(p) = 1 - kappa sum_mu (2 cos p_mu - 2i gamma_mu sin p_mu) on the spinor u (D^dagger with +2i),
@@ -44,6 +48,66 @@ def unit_gauge(ctx, dims, prec=clg.F64, kind=clg.GAUGE):
     g = clg.Field(ctx, kind, prec, dims)
     g.gauge_unit()
     return g
+
+
+@pytest.fixture(scope="module")
+def c1_inputs(ctx):
+    """The benchmark's gauge field and source, generated on the GPU, and the oracle's D and D^dagger of
+    them."""
+    g = clg.Field(ctx, clg.GAUGE, clg.F64, C1)
+    g.gauge_random(0.5, 1)
+    psi = clg.Field(ctx, clg.WILSON, clg.F64, C1, bc=orc.BC_FERMION)
+    psi.gaussian(2)
+    U, x = g.download(), psi.download()
+    refs = {op: orc.wilson_op(C1, orc.BC_FERMION, KAPPA, U, x, op) for op in (orc.D, orc.DDAG)}
+    return g, psi, U, x, refs
+
+
+def maxrel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+# the benchmark's FP64 launch (LDS-staged (x,y) planes of 256 sites, XCD slabs with the parity
+# interleave, one plane per workgroup) and the other schedules the tuner may pick at this lattice
+C1_LAUNCHES = [(6, 256, 8, 1), (5, 256, 8, 1), (8, 256, 8, 1), (2, 256, 8, 1), (4, 128, 8, 2), (1, 64, 0, 1)]
+
+
+@pytest.mark.parametrize("launch", C1_LAUNCHES + [None], ids=[str(c) for c in C1_LAUNCHES] + ["tuned"])
+def test_wilson_d_against_oracle_at_32x32x32x16(ctx, c1_inputs, launch):
+    g, psi, U, x, refs = c1_inputs
+    if launch is None:
+        clg.autotune_enable(1)
+    else:
+        clg.autotune_force(*launch)
+    try:
+        for op in (orc.D, orc.DDAG):
+            out = clg.Field(ctx, clg.WILSON, clg.F64, C1)
+            clg.wilson_apply(ctx, out, psi, g, KAPPA, op)
+            err = maxrel(out.download(), refs[op])
+            print(f"launch {launch} op {op}: max rel err {err:.2e}")
+            assert err < 1e-12, (launch, op)
+        if launch is None:
+            print(clg.autotune_entries())
+    finally:
+        clg.autotune_force(0)
+        clg.autotune_enable(-1)
+
+
+@pytest.mark.parametrize("prec,kind,tol", [(clg.F32, clg.GAUGE, 1e-6), (clg.F32, clg.GAUGE12, 1e-6),
+                                           (clg.F64, clg.GAUGE12, 1e-12)])
+def test_wilson_d_other_formats_against_oracle_at_32x32x32x16(ctx, c1_inputs, prec, kind, tol):
+    """FP32 with 18- and 12-real links and FP64 with 12-real links (tuned launches) on the benchmark inputs."""
+    g64, psi64, U, x, refs = c1_inputs
+    g = clg.Field(ctx, kind, prec, C1)
+    g.copy_from(g64)
+    psi = clg.Field(ctx, clg.WILSON, prec, C1, bc=orc.BC_FERMION)
+    psi.copy_from(psi64)
+    for op in (orc.D, orc.DDAG):
+        out = clg.Field(ctx, clg.WILSON, prec, C1)
+        clg.wilson_apply(ctx, out, psi, g, KAPPA, op)
+        err = maxrel(out.download(), refs[op])
+        print(f"prec {prec} kind {kind} op {op}: max rel err {err:.2e}")
+        assert err < tol, (prec, kind, op)
 
 
 @pytest.mark.parametrize("prec,tol", [(clg.F64, 1e-12), (clg.F32, 2e-6)])

@@ -204,7 +204,7 @@ def test_precision_conversion(ctx, c0):
 def test_lds_staged_launches(ctx, dims):
     """Pin modes 5..8 stage the z and t neighbours in LDS (one workgroup per (x,y) plane of one
     parity, block = L2*L3/2: 64 and 128 here), also with several planes per workgroup in turn
-    (sites per thread); a staged mode whose block does not fit runs unstaged."""
+    (sites per thread); a forced staged mode whose block does not fit the lattice is refused."""
     plane = dims[2] * dims[3] // 2
     U = orc.gauge_random(dims, 0.3, 11)
     psi = orc.wilson_gaussian(dims, 12)
@@ -224,8 +224,8 @@ def test_lds_staged_launches(ctx, dims):
     ref_eo = refs[False].reshape(-1, 24)
     try:
         for pm, blk, xcd, spt in ((1, plane, 8, 1), (5, plane, 8, 1), (6, plane, 8, 1), (6, plane, 0, 1),
-                                  (5, plane, 1, 1), (7, plane, 8, 1), (8, plane, 0, 1), (5, 256, 8, 1),
-                                  (8, 256, 8, 1), (6, plane, 8, 2), (8, plane, 8, 4), (5, plane, 0, 4)):
+                                  (5, plane, 1, 1), (7, plane, 8, 1), (8, plane, 0, 1),
+                                  (6, plane, 8, 2), (8, plane, 8, 4), (5, plane, 0, 4)):
             clg.autotune_force(pm, blk, xcd, spt)
             for f, g, tol in cases:
                 for dag in (False, True):
@@ -238,6 +238,17 @@ def test_lds_staged_launches(ctx, dims):
                 out = clg.Field(ctx, clg.WILSON, clg.F64, dims, subset=out_sub, bc=orc.BC_FERMION)
                 clg.wilson_hop(ctx, out, f64.parity(in_sub), g64, dagger=False, a=-KAPPA, x=f64.parity(out_sub), b=1.0)
                 assert rel(out.download().reshape(-1, 24), ref_eo[orc.eo_sites(dims, par)]) < 1e-12, (pm, par)
+        # a staged mode forced with a block other than the plane size fails loudly, for the full
+        # operator and for the half-lattice hop, and leaves the output untouched
+        for pm in (5, 8):
+            clg.autotune_force(pm, 256, 8, 1)
+            out = clg.Field(ctx, clg.WILSON, clg.F64, dims)
+            with pytest.raises(clg.CLGError):
+                clg.wilson_apply(ctx, out, f64, g64, KAPPA, clg.EFO_F_D)
+            assert not np.any(out.download())
+            oute = clg.Field(ctx, clg.WILSON, clg.F64, dims, subset=clg.EVEN, bc=orc.BC_FERMION)
+            with pytest.raises(clg.CLGError):
+                clg.wilson_hop(ctx, oute, f64.parity(clg.ODD), g64, dagger=False, a=-KAPPA)
     finally:
         clg.autotune_force(0)
 
