@@ -50,7 +50,7 @@ EXPORTED = [
     "clg_gcrodr_destroy", "clg_gcrodr_reset", "clg_gcrodr_solve", "clg_multidot", "clg_multiaxpy",
     "clg_vector_multiply_matrix", "clg_autotune_enable", "clg_autotune_force", "clg_autotune_reset",
     "clg_autotune_count", "clg_autotune_entry", "clg_graph_begin", "clg_graph_end", "clg_graph_launch",
-    "clg_graph_nodes", "clg_graph_destroy",
+    "clg_graph_nodes", "clg_graph_destroy", "clg_gauge_transform_random", "clg_gauge_transform_apply",
 ]
 
 
@@ -170,6 +170,8 @@ def lib():
             "clg_gauge_axpy": (C.c_int, [vp, vp, C.c_double, vp]),
             "clg_gauge_exp_mult": (C.c_int, [vp, vp, vp, C.c_double, C.c_int]),
             "clg_gauge_normalize": (C.c_int, [vp, vp]),
+            "clg_gauge_transform_random": (C.c_int, [vp, C.c_double, C.c_uint]),
+            "clg_gauge_transform_apply": (C.c_int, [vp, vp]),
             "clg_gauge_plaquette_energy": (C.c_int, [vp, vp, C.c_double, dp]),
             "clg_gauge_force": (C.c_int, [vp, vp, vp, C.c_double]),
             "clg_wilson_prepare_hmc": (C.c_int, [vp, vp, vp, C.c_double, C.c_uint]),
@@ -645,6 +647,16 @@ def exp_mult(ctx, P, gauge, a, exp_precision=0):
 
 def gauge_normalize(ctx, gauge):
     check(lib().clg_gauge_normalize(ctx.h, gauge.h))
+
+
+def gauge_transform_random(gt, eps, seed):
+    """g(x) = reunitarise(exp(i eps H(x))) per site, held in the mu = 0 links of the FP64 gauge field gt"""
+    check(lib().clg_gauge_transform_random(gt.h, eps, seed))
+
+
+def gauge_transform_apply(gt, target):
+    """U_mu(x) -> g(x) U_mu(x) g(x+mu)^dagger for a gauge field, psi(x) -> g(x) psi(x) for a fermion field"""
+    check(lib().clg_gauge_transform_apply(gt.h, target.h))
 
 
 def plaquette_energy(ctx, gauge, beta):

@@ -328,6 +328,31 @@ __device__ __forceinline__ bool block_site(unsigned lb, int Vh, int nblk, int pb
     p = nblk == 2 ? blk : pbase;
     return true;
 }
+// Tiled plane schedule for launches of one workgroup per (x,y) plane (blockDim = L2*L3/2 sites of one
+// parity; nblk = 2 interleaves the parities plane by plane): the L0 x L1 grid of planes is cut into
+// 8 tiles, 8/TY along x and TY along y, and XCD k (hardware workgroups k, k+8, ...) sweeps tile k,
+// x-major (y fastest) or y-major.  A plane's x+-1 and y+-1 neighbour planes then come up within
+// 2*L1/TY (x-major) workgroups of it in the same XCD's L2, instead of a whole x-slab later.  Needs
+// L0 % (8/TY) == 0, L1 % TY == 0 and a grid of exactly nblk*L0*L1 workgroups.
+__device__ __forceinline__ unsigned tile_block(const clg::Geo& g, int nblk, int TY, int ymajor) {
+    const unsigned wg = blockIdx.x;
+    const unsigned k = wg & 7u, j = wg >> 3;
+    const int TX = 8 / TY;
+    const int tx = g.L[0] / TX, ty = g.L[1] / TY;
+    const int ix = (int)k / TY, iy = (int)k % TY;
+    const unsigned par = nblk == 2 ? (j & 1u) : 0u;
+    const int q = (int)(nblk == 2 ? (j >> 1) : j);
+    int xl, yl;
+    if (ymajor) {
+        yl = q / tx;
+        xl = q - yl * tx;
+    } else {
+        xl = q / ty;
+        yl = q - xl * ty;
+    }
+    const unsigned P = (unsigned)((ix * tx + xl) * g.L[1] + iy * ty + yl);
+    return nblk == 2 ? 2u * P + par : P;
+}
 __device__ __forceinline__ bool map_site(int Vh, int nblk, int pbase, int xcd, int pinter, int& p, int& h) {
     return block_site(logical_wg(xcd), Vh, nblk, pbase, pinter, p, h);
 }
@@ -765,7 +790,52 @@ __device__ __forceinline__ void gauss2(unsigned idx, unsigned draw, unsigned str
     g[0] = r * c;
     g[1] = r * s;
 }
-enum { STREAM_GAUGE = 0, STREAM_WILSON = 1, STREAM_KS = 2, STREAM_MOMENTUM = 3 };
+enum { STREAM_GAUGE = 0, STREAM_WILSON = 1, STREAM_KS = 2, STREAM_MOMENTUM = 3, STREAM_GAUGE_TRANSFORM = 4 };
+
+// reunitarise(exp(i eps H)) for H traceless Hermitian with Gaussian entries drawn from stream `stream`
+// at counter `idx`, by Taylor scaling-and-squaring (the same recipe as the oracle's generator)
+__device__ inline void random_su3(unsigned idx, double eps, unsigned seed, unsigned stream, cd* out) {
+    double g[10];
+    for (int d = 0; d < 5; ++d) gauss2(idx, (unsigned)d, stream, seed, g + 2 * d);
+    const double tr = (g[0] + g[1] + g[2]) / 3.0;
+    const double is2 = 0.70710678118654752440084436210485;
+    cd H[9];
+    H[0] = cd{g[0] - tr, 0.0};
+    H[4] = cd{g[1] - tr, 0.0};
+    H[8] = cd{g[2] - tr, 0.0};
+    H[1] = cd{g[3] * is2, g[4] * is2};
+    H[2] = cd{g[5] * is2, g[6] * is2};
+    H[5] = cd{g[7] * is2, g[8] * is2};
+    H[3] = cconj(H[1]);
+    H[6] = cconj(H[2]);
+    H[7] = cconj(H[5]);
+    cd A[9];
+    double nrm = 0;
+    for (int i = 0; i < 9; ++i) {
+        A[i] = cd{-eps * H[i].y, eps * H[i].x};
+        nrm += A[i].x * A[i].x + A[i].y * A[i].y;
+    }
+    nrm = sqrt(nrm);
+    int s = 0;
+    while (nrm > 0.5) { nrm *= 0.5; ++s; }
+    const double f = ldexp(1.0, -s);
+    for (int i = 0; i < 9; ++i) A[i] = cscale(A[i], f);
+    cd T[9], S[9], tmp[9];
+    for (int i = 0; i < 9; ++i) T[i] = cd{0.0, 0.0};
+    T[0] = T[4] = T[8] = cd{1.0, 0.0};
+    for (int i = 0; i < 9; ++i) S[i] = T[i];
+    for (int k = 1; k <= 20; ++k) {
+        su3_mm(T, A, tmp);
+        for (int i = 0; i < 9; ++i) { T[i] = cscale(tmp[i], 1.0 / k); S[i] = cadd(S[i], T[i]); }
+    }
+    for (int k = 0; k < s; ++k) {
+        su3_mm(S, S, tmp);
+        for (int i = 0; i < 9; ++i) S[i] = tmp[i];
+    }
+    su3_reunit(S);
+    for (int i = 0; i < 9; ++i) out[i] = S[i];
+}
+
 
 // ------------------------------------------------------------------ grid reductions
 constexpr int kRedBlock = 256;

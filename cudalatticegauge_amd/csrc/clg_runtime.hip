@@ -67,47 +67,9 @@ __global__ void k_zero4(unsigned* p, long n4) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x) p[i] = 0u;
 }
 
-// reunitarise(exp(i eps H)) with Taylor scaling-and-squaring (same recipe as the oracle)
+// reunitarise(exp(i eps H)) of the Gaussian draws of this link (clgd::random_su3)
 __device__ void random_link(unsigned link, double eps, unsigned seed, cd* out) {
-    double g[10];
-    for (int d = 0; d < 5; ++d) gauss2(link, (unsigned)d, STREAM_GAUGE, seed, g + 2 * d);
-    const double tr = (g[0] + g[1] + g[2]) / 3.0;
-    const double is2 = 0.70710678118654752440084436210485;
-    cd H[9];
-    H[0] = cd{g[0] - tr, 0.0};
-    H[4] = cd{g[1] - tr, 0.0};
-    H[8] = cd{g[2] - tr, 0.0};
-    H[1] = cd{g[3] * is2, g[4] * is2};
-    H[2] = cd{g[5] * is2, g[6] * is2};
-    H[5] = cd{g[7] * is2, g[8] * is2};
-    H[3] = cconj(H[1]);
-    H[6] = cconj(H[2]);
-    H[7] = cconj(H[5]);
-    cd A[9];
-    double nrm = 0;
-    for (int i = 0; i < 9; ++i) {
-        A[i] = cd{-eps * H[i].y, eps * H[i].x};
-        nrm += A[i].x * A[i].x + A[i].y * A[i].y;
-    }
-    nrm = sqrt(nrm);
-    int s = 0;
-    while (nrm > 0.5) { nrm *= 0.5; ++s; }
-    const double f = ldexp(1.0, -s);
-    for (int i = 0; i < 9; ++i) A[i] = cscale(A[i], f);
-    cd T[9], S[9], tmp[9];
-    for (int i = 0; i < 9; ++i) T[i] = cd{0.0, 0.0};
-    T[0] = T[4] = T[8] = cd{1.0, 0.0};
-    for (int i = 0; i < 9; ++i) S[i] = T[i];
-    for (int k = 1; k <= 20; ++k) {
-        su3_mm(T, A, tmp);
-        for (int i = 0; i < 9; ++i) { T[i] = cscale(tmp[i], 1.0 / k); S[i] = cadd(S[i], T[i]); }
-    }
-    for (int k = 0; k < s; ++k) {
-        su3_mm(S, S, tmp);
-        for (int i = 0; i < 9; ++i) S[i] = tmp[i];
-    }
-    su3_reunit(S);
-    for (int i = 0; i < 9; ++i) out[i] = S[i];
+    random_su3(link, eps, seed, STREAM_GAUGE, out);
 }
 
 // thread -> (parity, h) over the blocks a field stores

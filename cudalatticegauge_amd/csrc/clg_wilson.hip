@@ -42,6 +42,7 @@ struct HopArgs {
     int pbase, nblk;     // output parities handled
     int xcd, pinter;     // work decomposition (clgd::map_site)
     int spt;             // logical blocks (sites per thread) per workgroup
+    int tile_y, tile_ymaj;  // tiled plane schedule (clgd::tile_block) when tile_y > 0
 };
 
 // Neighbour spinors come from the input field, or (STAGED, z and t only) from the copy of this
@@ -95,7 +96,7 @@ __global__ __launch_bounds__(256) void k_wilson_hop(HopArgs a) {
     typedef LinkIO<GP, RECON> LIO;
     constexpr bool STAGED = PM >= 5;
     constexpr int PMH = STAGED ? PM - 4 : PM;
-    const unsigned lw = logical_wg(a.xcd);
+    const unsigned lw = a.tile_y > 0 ? tile_block(a.g, a.nblk, a.tile_y, a.tile_ymaj) : logical_wg(a.xcd);
     for (int vs = 0; vs < a.spt; ++vs) {
     int p, h;
     const bool valid = block_site(lw * (unsigned)a.spt + (unsigned)vs, a.g.Vh, a.nblk, a.pbase, a.pinter, p, h);
@@ -160,17 +161,32 @@ struct HopCfg {
                 // 5..8 as 1..4 with the z and t neighbours staged in LDS (block = L2*L3/2)
     int block;  // threads per workgroup
     int xcd;    // 0: hardware order; 8: XCD-contiguous slabs, for full D with the two parities
-                // interleaved block by block; 1: XCD-contiguous slabs, parities one after the other
+                // interleaved block by block; 1: XCD-contiguous slabs, parities one after the other;
+                // 10*TY + m (TY in 2, 4, 8; m = 0 x-major, 1 y-major): tiled plane schedule, staged
+                // launches only (clgd::tile_block), parities interleaved
     int spt = 1;  // sites per thread: consecutive logical blocks per workgroup (1, 2 or 4)
 };
+inline bool tiled_xcd(int xcd) { return xcd >= 20 && xcd <= 81 && (xcd / 10 == 2 || xcd / 10 == 4 || xcd / 10 == 8) && xcd % 10 <= 1; }
 
 template <typename R, int SP, int GP, int RECON, int PM>
 void launch_pm(const HopArgs& a0, bool dag, bool hasx, hipStream_t st, int block, int xcd, int spt) {
     HopArgs a = a0;
-    a.xcd = xcd ? 8 : 0;
-    a.pinter = xcd == 8 ? 1 : 0;
-    a.spt = spt;
-    const unsigned grid = sched_grid(a.g.Vh, a.nblk, block, a.xcd, a.pinter, spt);
+    unsigned grid;
+    if (tiled_xcd(xcd)) {
+        a.xcd = 0;
+        a.pinter = 1;
+        a.spt = 1;
+        a.tile_y = xcd / 10;
+        a.tile_ymaj = xcd % 10;
+        grid = (unsigned)(a.nblk * a.g.L[0] * a.g.L[1]);
+    } else {
+        a.xcd = xcd ? 8 : 0;
+        a.pinter = xcd == 8 ? 1 : 0;
+        a.spt = spt;
+        a.tile_y = 0;
+        a.tile_ymaj = 0;
+        grid = sched_grid(a.g.Vh, a.nblk, block, a.xcd, a.pinter, spt);
+    }
     // staged launches: NPL 16-byte chunks per site of the plane, plus the FP16 site scales
     const size_t shm = PM >= 5 ? (size_t)block * (16 * WilsonIO<SP>::NPL + (WilsonIO<SP>::SCALED ? 4 : 0)) : 0;
     if (dag) {
@@ -211,6 +227,13 @@ int launch_cfg(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool 
                       "LDS-staged pin mode %d needs block = L2*L3/2 in {64,128,256} (lattice %dx%dx%dx%d gives %d), "
                       "asked for block %d",
                       c.pm, a.g.L[0], a.g.L[1], a.g.L[2], a.g.L[3], a.g.L[2] * a.g.L[3] / 2, c.block);
+    }
+    if (tiled_xcd(c.xcd)) {
+        const int TY = c.xcd / 10, TX = 8 / TY;
+        CLG_CHECK_ARG(c.pm >= 5 && c.spt == 1 && a.g.L[0] % TX == 0 && a.g.L[1] % TY == 0,
+                      "tiled schedule %d needs an LDS-staged pin mode, one site per thread, L0 %% %d == 0 and "
+                      "L1 %% %d == 0 (lattice %dx%dx%dx%d)",
+                      c.xcd, TX, TY, a.g.L[0], a.g.L[1], a.g.L[2], a.g.L[3]);
     }
     if (prec == CLG_F64) {
         return recon == 12 ? launch_t<double, CLG_F64, CLG_F64, 12>(a, dag, hasx, st, c)
@@ -253,19 +276,19 @@ std::vector<long> key_vec(const TuneKey& k) {
     return {k.prec, k.gprec, k.recon, k.dag, k.hasx, k.nblk, k.L0, k.L1, k.L2, k.L3};
 }
 bool valid_cfg(int pm, int blk, int xcd, int spt) {
-    return pm >= 1 && pm <= 8 && (blk == 64 || blk == 128 || blk == 256) && (xcd == 0 || xcd == 1 || xcd == 8) &&
-           (spt == 1 || spt == 2 || spt == 4);
+    return pm >= 1 && pm <= 8 && (blk == 64 || blk == 128 || blk == 256) &&
+           (xcd == 0 || xcd == 1 || xcd == 8 || tiled_xcd(xcd)) && (spt == 1 || spt == 2 || spt == 4);
 }
 // process-wide overrides set through clg_autotune_enable / clg_autotune_force
 std::atomic<int> g_tune_mode{-1};  // -1: CLG_TUNE environment (default on), 0 off, 1 on
-std::atomic<int> g_forced{0};      // packed pin | block << 4 | xcd << 16 | spt << 20, 0 = none
+std::atomic<int> g_forced{0};      // packed pin | block << 4 | xcd << 16 | spt << 24, 0 = none
 // CLG_TUNE_FORCE="pin,block,xcd[,spt]" (or clg_autotune_force) fixes the launch configuration of
 // every hop launch (used to profile one configuration under counters, where event timings are
 // not representative)
 bool forced_cfg(HopCfg& c) {
     const int f = g_forced.load();
     if (f) {
-        c = HopCfg{f & 15, (f >> 4) & 4095, (f >> 16) & 15, f >> 20};
+        c = HopCfg{f & 15, (f >> 4) & 4095, (f >> 16) & 255, f >> 24};
         return true;
     }
     const char* e = std::getenv("CLG_TUNE_FORCE");
@@ -330,6 +353,11 @@ int launch_hop(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool 
         if (const int sb = staged_block(a.g)) {
             for (int pm = 5; pm <= 8; ++pm) all.push_back(HopCfg{pm, sb, 8});
             for (int pm : {6, 8}) all.push_back(HopCfg{pm, sb, 8, 2});
+            // tiled plane schedules where the plane grid divides
+            for (int TY : {2, 4, 8})
+                for (int m : {0, 1})
+                    if (a.g.L[0] % (8 / TY) == 0 && a.g.L[1] % TY == 0)
+                        for (int pm = 5; pm <= 8; ++pm) all.push_back(HopCfg{pm, sb, 10 * TY + m});
         }
         for (const HopCfg& c : all) {
             if (c.xcd == 1 && a.nblk != 2) continue;  // one parity: the same launch as xcd 8
@@ -564,7 +592,7 @@ int clg_autotune_force(int pin_mode, int block, int xcd, int sites_per_thread) {
     if (sites_per_thread == 0) sites_per_thread = 1;
     CLG_CHECK_ARG(valid_cfg(pin_mode, block, xcd, sites_per_thread),
                   "pin_mode 1..8, block 64|128|256, xcd 0|1|8, sites_per_thread 1|2|4");
-    g_forced.store(pin_mode | block << 4 | xcd << 16 | sites_per_thread << 20);
+    g_forced.store(pin_mode | block << 4 | xcd << 16 | sites_per_thread << 24);
     return CLG_OK;
 }
 

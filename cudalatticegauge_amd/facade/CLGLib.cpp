@@ -1,6 +1,7 @@
 // CLGLib.cpp -- implementation of the CLGLib-named facade over libclg_hip.so (see CLGLib.h).
 #include "CLGLib.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -160,19 +161,19 @@ void CField::Zero() { check(clg_field_zero(m_h), "Zero"); }
 void CField::CopyTo(CField* dst) const { check(clg_field_copy(dst->m_h, m_h), "CopyTo"); }
 void CField::Axpy(const CLGComplex& a, const CField* x) {
     if (m_kind == CLG_GAUGE) {
-        check(clg_gauge_axpy(m_pOwner->m_ctx, m_h, a.real(), x->m_h), "Axpy");
+        check(clg_gauge_axpy(m_pOwner->m_ctx, m_h, a.x, x->m_h), "Axpy");
         return;
     }
-    check(clg_axpy(m_pOwner->m_ctx, m_h, a.real(), a.imag(), x->m_h), "Axpy");
+    check(clg_axpy(m_pOwner->m_ctx, m_h, a.x, a.y, x->m_h), "Axpy");
 }
-void CField::Axpy(Real a, const CField* x) { Axpy(CLGComplex(a, 0.0), x); }
+void CField::Axpy(Real a, const CField* x) { Axpy(_make_cuComplex(a, 0.0), x); }
 void CField::AxpyPlus(const CField* x) { Axpy(1.0, x); }
 void CField::AxpyMinus(const CField* x) { Axpy(-1.0, x); }
-void CField::ScalarMultply(const CLGComplex& a) { check(clg_scale(m_pOwner->m_ctx, m_h, a.real(), a.imag()), "Scale"); }
+void CField::ScalarMultply(const CLGComplex& a) { check(clg_scale(m_pOwner->m_ctx, m_h, a.x, a.y), "Scale"); }
 CLGComplex CField::Dot(const CField* other) const {
     double z[2];
     check(clg_dot(m_pOwner->m_ctx, m_h, other->m_h, z), "Dot");
-    return CLGComplex(z[0], z[1]);
+    return _make_cuComplex(z[0], z[1]);
 }
 std::vector<double> CField::Download() const {
     std::vector<double> v((size_t)clg_field_host_doubles(m_h));
@@ -181,7 +182,7 @@ std::vector<double> CField::Download() const {
 }
 void CField::Upload(const std::vector<double>& host) { check(clg_field_upload(m_h, host.data()), "upload"); }
 
-CFieldGaugeSU3::CFieldGaugeSU3(CLatticeData* owner) : CField(owner, CLG_GAUGE) {}
+CFieldGaugeSU3::CFieldGaugeSU3(CLatticeData* owner) : CFieldGauge(owner, CLG_GAUGE) {}
 void CFieldGaugeSU3::InitialField(EFieldInitialType t, unsigned seed, Real eps) {
     if (t == EFIT_Identity) check(clg_gauge_unit(m_h), "gauge unit");
     else if (t == EFIT_Zero) check(clg_field_zero(m_h), "gauge zero");
@@ -224,40 +225,276 @@ CFieldFermion::CFieldFermion(CLatticeData* owner, int kind, Real param, const in
     for (int i = 0; i < 4; ++i) m_bc[i] = bc[i];
     check(clg_field_set_bc(m_h, bc), "set bc");
 }
-void CFieldFermion::ApplyOperator(EFieldOperator op, const CFieldGaugeSU3* gauge, int ct, Real re, Real im) {
-    if (op >= EFO_F_InverseD) {
-        Inverse(op, gauge);
-        return;
-    }
+const CFieldGaugeSU3* CFieldFermion::GetDefaultGauge(INT gaugeNum, const CFieldGauge* const* gaugeFields) const {
+    for (INT i = 0; i < gaugeNum; ++i)
+        if (gaugeFields[i] && gaugeFields[i]->m_byFieldId == (int)m_byGaugeFieldId)
+            return dynamic_cast<const CFieldGaugeSU3*>(gaugeFields[i]);
+    return nullptr;
+}
+namespace {
+const CFieldGaugeSU3* need_gauge(const CFieldFermion* f, INT gaugeNum, const CFieldGauge* const* gaugeFields) {
+    const CFieldGaugeSU3* g = f->GetDefaultGauge(gaugeNum, gaugeFields);
+    if (!g) throw std::runtime_error("no SU(3) gauge field with the fermion's gauge field id among the gauge fields");
+    return g;
+}
+}  // namespace
+// this <- coeff * op(this), op one of D, D^dagger, DD, DD^dagger, with the given mass / hopping
+void CFieldFermion::Apply(EFieldOperator op, const CFieldGaugeSU3* gauge, Real param, EOperatorCoefficientType ct,
+                          Real re, Real im) {
     std::unique_ptr<CField> tmp(GetCopy());
     if (Kind() == CLG_WILSON)
-        check(clg_wilson_apply(m_pOwner->m_ctx, m_h, tmp->Handle(), gauge->Handle(), m_fParam, op, ct, re, im), "D");
+        check(clg_wilson_apply(m_pOwner->m_ctx, m_h, tmp->Handle(), gauge->Handle(), param, op, ct, re, im), "D");
     else
-        check(clg_ks_apply(m_pOwner->m_ctx, m_h, tmp->Handle(), gauge->Handle(), m_fParam, op, ct, re, im), "D");
+        check(clg_ks_apply(m_pOwner->m_ctx, m_h, tmp->Handle(), gauge->Handle(), param, op, ct, re, im), "D");
 }
-void CFieldFermion::D(const CFieldGaugeSU3* g, int t, Real re, Real im) { ApplyOperator(EFO_F_D, g, t, re, im); }
-void CFieldFermion::Ddagger(const CFieldGaugeSU3* g, int t, Real re, Real im) {
-    ApplyOperator(EFO_F_Ddagger, g, t, re, im);
+void CFieldFermion::D(INT gn, INT bn, const CFieldGauge* const* gf, const CFieldBoson* const*,
+                      EOperatorCoefficientType t, Real re, Real im) {
+    Apply(EFO_F_D, need_gauge(this, gn, gf), m_fParam, t, re, im);
+    (void)bn;
 }
-void CFieldFermion::DD(const CFieldGaugeSU3* g, int t, Real re, Real im) { ApplyOperator(EFO_F_DD, g, t, re, im); }
-void CFieldFermion::DDdagger(const CFieldGaugeSU3* g, int t, Real re, Real im) {
-    ApplyOperator(EFO_F_DDdagger, g, t, re, im);
+void CFieldFermion::Ddagger(INT gn, INT, const CFieldGauge* const* gf, const CFieldBoson* const*,
+                            EOperatorCoefficientType t, Real re, Real im) {
+    Apply(EFO_F_Ddagger, need_gauge(this, gn, gf), m_fParam, t, re, im);
 }
-UBOOL CFieldFermion::Inverse(EFieldOperator op, const CFieldGaugeSU3* gauge) {
+void CFieldFermion::DD(INT gn, INT, const CFieldGauge* const* gf, const CFieldBoson* const*, EOperatorCoefficientType t,
+                       Real re, Real im) {
+    Apply(EFO_F_DD, need_gauge(this, gn, gf), m_fParam, t, re, im);
+}
+void CFieldFermion::DDdagger(INT gn, INT, const CFieldGauge* const* gf, const CFieldBoson* const*,
+                             EOperatorCoefficientType t, Real re, Real im) {
+    Apply(EFO_F_DDdagger, need_gauge(this, gn, gf), m_fParam, t, re, im);
+}
+void CFieldFermion::DWithMass(INT gn, INT, const CFieldGauge* const* gf, const CFieldBoson* const*, Real m,
+                              EOperatorCoefficientType t, Real re, Real im) {
+    Apply(EFO_F_D, need_gauge(this, gn, gf), m, t, re, im);
+}
+void CFieldFermion::DdaggerWithMass(INT gn, INT, const CFieldGauge* const* gf, const CFieldBoson* const*, Real m,
+                                    EOperatorCoefficientType t, Real re, Real im) {
+    Apply(EFO_F_Ddagger, need_gauge(this, gn, gf), m, t, re, im);
+}
+void CFieldFermion::DDWithMass(INT gn, INT, const CFieldGauge* const* gf, const CFieldBoson* const*, Real m,
+                               EOperatorCoefficientType t, Real re, Real im) {
+    Apply(EFO_F_DD, need_gauge(this, gn, gf), m, t, re, im);
+}
+void CFieldFermion::DDdaggerWithMass(INT gn, INT, const CFieldGauge* const* gf, const CFieldBoson* const*, Real m,
+                                     EOperatorCoefficientType t, Real re, Real im) {
+    Apply(EFO_F_DDdagger, need_gauge(this, gn, gf), m, t, re, im);
+}
+UBOOL CFieldFermion::ApplyOperator(EFieldOperator op, INT gn, INT bn, const CFieldGauge* const* gf,
+                                   const CFieldBoson* const* bf, EOperatorCoefficientType t, Real re, Real im,
+                                   void* other) {
+    switch ((int)op) {
+        case EFO_F_D: D(gn, bn, gf, bf, t, re, im); return true;
+        case EFO_F_Ddagger: Ddagger(gn, bn, gf, bf, t, re, im); return true;
+        case EFO_F_DD: DD(gn, bn, gf, bf, t, re, im); return true;
+        case EFO_F_DDdagger: DDdagger(gn, bn, gf, bf, t, re, im); return true;
+        case EFO_F_InverseD:
+        case EFO_F_InverseDdagger:
+        case EFO_F_InverseDD:
+        case EFO_F_InverseDDdagger:
+            if (t != EOCT_None) {
+                std::fprintf(stderr, "[CLGLib] ApplyOperator: an inverse with a coefficient is not implemented\n");
+                return false;
+            }
+            return Inverse(op, gn, bn, gf, bf);
+        case EFO_F_RationalD:
+            if (!other) {
+                std::fprintf(stderr, "[CLGLib] ApplyOperator: EFO_F_RationalD needs a CRatinalApproximation\n");
+                return false;
+            }
+            return RationalApproximation(EFO_F_D, gn, bn, gf, bf, static_cast<const CRatinalApproximation*>(other));
+        case EFO_F_D_WithMass:
+        case EFO_F_Ddagger_WithMass:
+        case EFO_F_DD_WithMass:
+        case EFO_F_DDdagger_WithMass: {
+            if (!other) {
+                std::fprintf(stderr, "[CLGLib] ApplyOperator: a *_WithMass operator needs the mass\n");
+                return false;
+            }
+            const Real m = *static_cast<const Real*>(other);
+            const EFieldOperator base = (EFieldOperator)((int)op - (int)EFO_F_D_WithMass);
+            Apply(base, need_gauge(this, gn, gf), m, t, re, im);
+            return true;
+        }
+        default:
+            std::fprintf(stderr, "[CLGLib] ApplyOperator: operator %d is not implemented\n", (int)op);
+            return false;
+    }
+}
+UBOOL CFieldFermion::Inverse(EFieldOperator op, INT gn, INT bn, const CFieldGauge* const* gf,
+                             const CFieldBoson* const* bf) {
     CSLASolver* s = appGetFermionSolver(m_byFieldId);
     if (!s) {
         std::fprintf(stderr, "[CLGLib] no solver for field %d\n", m_byFieldId);
         return false;
     }
-    return s->Solve(this, this, gauge, op);
+    return s->Solve(this, this, gn, bn, gf, bf, op);
 }
-UBOOL CFieldFermion::InverseD(const CFieldGaugeSU3* g) { return Inverse(EFO_F_InverseD, g); }
-UBOOL CFieldFermion::InverseDdagger(const CFieldGaugeSU3* g) { return Inverse(EFO_F_InverseDdagger, g); }
-UBOOL CFieldFermion::InverseDD(const CFieldGaugeSU3* g) { return Inverse(EFO_F_InverseDD, g); }
-UBOOL CFieldFermion::InverseDDdagger(const CFieldGaugeSU3* g) { return Inverse(EFO_F_InverseDDdagger, g); }
+UBOOL CFieldFermion::InverseD(INT gn, INT bn, const CFieldGauge* const* gf, const CFieldBoson* const* bf) {
+    return Inverse(EFO_F_InverseD, gn, bn, gf, bf);
+}
+UBOOL CFieldFermion::InverseDdagger(INT gn, INT bn, const CFieldGauge* const* gf, const CFieldBoson* const* bf) {
+    return Inverse(EFO_F_InverseDdagger, gn, bn, gf, bf);
+}
+UBOOL CFieldFermion::InverseDD(INT gn, INT bn, const CFieldGauge* const* gf, const CFieldBoson* const* bf) {
+    return Inverse(EFO_F_InverseDD, gn, bn, gf, bf);
+}
+UBOOL CFieldFermion::InverseDDdagger(INT gn, INT bn, const CFieldGauge* const* gf, const CFieldBoson* const* bf) {
+    return Inverse(EFO_F_InverseDDdagger, gn, bn, gf, bf);
+}
+void CFieldFermion::PrepareForHMC(INT gn, INT, const CFieldGauge* const* gf, const CFieldBoson* const*) {
+    // the heat-bath draw is seeded per lattice and field id
+    PrepareForHMCS(need_gauge(this, gn, gf), m_pOwner->m_uiSeed * 31u + (unsigned)m_byFieldId + 7u * m_uiHeatBath++);
+}
+UBOOL CFieldFermion::CalculateForce(INT gn, INT bn, const CFieldGauge* const* gf, const CFieldBoson* const*,
+                                    CFieldGauge* const* gforce, CFieldBoson* const*, ESolverPhase) const {
+    const CFieldGaugeSU3* g = need_gauge(this, gn, gf);
+    INT idx = -1;
+    for (INT i = 0; i < gn; ++i)
+        if (gf[i] == g) idx = i;
+    CFieldGaugeSU3* F = idx >= 0 ? dynamic_cast<CFieldGaugeSU3*>(gforce[idx]) : nullptr;
+    if (!F) throw std::runtime_error("CalculateForce: no SU(3) force field for the fermion's gauge field");
+    clg_solve_report rep{};
+    int st;
+    if (Kind() == CLG_WILSON) {
+        CSLASolver* s = appGetFermionSolver(m_byFieldId);
+        if (!s) throw std::runtime_error("CalculateForce: no solver for the field");
+        clg_solver_params p = s->m_params;
+        p.kind = CLG_WILSON;
+        st = clg_wilson_force(m_pOwner->m_ctx, &p, F->Handle(), m_h, g->Handle(), m_fParam, nullptr, nullptr, &rep);
+    } else {
+        const CFieldFermionKSSU3* ks = static_cast<const CFieldFermionKSSU3*>(this);
+        CMultiShiftSolver* ms = appGetMultiShiftSolver(m_byFieldId);
+        if (!ms || ks->m_rMD.size() < 3) throw std::runtime_error("CalculateForce: the KS field needs an MSSolver and MD");
+        clg_solver_params p = ms->m_params;
+        p.kind = CLG_KS;
+        st = clg_ks_force(m_pOwner->m_ctx, &p, ms->m_kind, ms->m_prec, F->Handle(), m_h, g->Handle(), m_fParam,
+                          ks->m_rMD.data(), (int)(ks->m_rMD.size() - 1) / 2, &rep);
+    }
+    check(st, "CalculateForce");
+    (void)bn;
+    return st == CLG_OK;
+}
 void CFieldFermion::InitialField(EFieldInitialType t, unsigned seed) {
     if (t == EFIT_RandomGaussian) check(clg_field_gaussian(m_h, seed), "gaussian");
     else check(clg_field_zero(m_h), "zero");
+}
+void CFieldFermion::InitialField(EFieldInitialType t) { InitialField(t, m_pOwner->NextSeed()); }
+
+namespace {
+// columns of the dense operator: apply(out, in) to every point source of the field's lattice,
+// downloaded as complex columns of n = nsites * ncomp entries
+template <class Apply>
+std::vector<std::vector<CLGComplex>> dense_columns(clg_field in, clg_field out, const int dims[4], int nspin,
+                                                   Apply apply) {
+    const long V = (long)dims[0] * dims[1] * dims[2] * dims[3];
+    const int nc = 3 * nspin;
+    const long n = V * nc;
+    std::vector<std::vector<CLGComplex>> cols((size_t)n);
+    std::vector<double> host((size_t)(2 * n));
+    for (long site = 0; site < V; ++site) {
+        int c[4];
+        long r = site;
+        for (int d = 3; d >= 0; --d) {
+            c[d] = (int)(r % dims[d]);
+            r /= dims[d];
+        }
+        for (int k = 0; k < nc; ++k) {
+            check(clg_field_zero(in), "zero");
+            check(clg_field_point_source(in, c, nspin == 4 ? k / 3 : 0, k % 3), "point source");
+            apply(out, in);
+            check(clg_field_download(out, host.data()), "download");
+            std::vector<CLGComplex>& col = cols[(size_t)(site * nc + k)];
+            col.resize((size_t)n);
+            for (long i = 0; i < n; ++i) col[(size_t)i] = _make_cuComplex(host[(size_t)(2 * i)], host[(size_t)(2 * i + 1)]);
+        }
+    }
+    return cols;
+}
+double cabs(const CLGComplex& z) { return std::sqrt(z.x * z.x + z.y * z.y); }
+}  // namespace
+
+UINT CFieldFermionWilsonSquareSU3::TestGamma5Hermitian(const CFieldGauge* pGauge, UBOOL bTestGamma5) {
+    const CFieldGaugeSU3* g = dynamic_cast<const CFieldGaugeSU3*>(pGauge);
+    if (!g) throw std::runtime_error("TestGamma5Hermitian needs an SU(3) gauge field");
+    clg_context ctx = m_pOwner->m_ctx;
+    const int* dims = m_pOwner->m_dims;
+    clg_field in = nullptr, out = nullptr;
+    check(clg_field_create(ctx, CLG_WILSON, CLG_F64, CLG_FULL, dims, &in), "field");
+    check(clg_field_create(ctx, CLG_WILSON, CLG_F64, CLG_FULL, dims, &out), "field");
+    clg_field_set_bc(in, m_bc);
+    auto applier = [&](int op) {
+        return [&, op](clg_field o, clg_field i) {
+            check(clg_wilson_apply(ctx, o, i, g->Handle(), m_fParam, op, EOCT_None, 1.0, 0.0), "D");
+        };
+    };
+    const auto Dm = dense_columns(in, out, dims, 4, applier(EFO_F_D));
+    const long n = (long)Dm.size();
+    double big = 0, worst = 0;
+    for (const auto& col : Dm)
+        for (const auto& z : col) big = std::max(big, cabs(z));
+    if (bTestGamma5) {
+        // the implemented D^dagger against the conjugate transpose of D
+        const auto Dd = dense_columns(in, out, dims, 4, applier(EFO_F_Ddagger));
+        for (long j = 0; j < n; ++j)
+            for (long i = 0; i < n; ++i) {
+                const CLGComplex a = Dd[(size_t)j][(size_t)i], b = Dm[(size_t)i][(size_t)j];
+                worst = std::max(worst, std::hypot(a.x - b.x, a.y + b.y));
+            }
+    } else {
+        // gamma5 D Hermitian: g5_i D_ij = conj(g5_j D_ji), g5 = +1 on spins 0,1 and -1 on spins 2,3
+        auto g5 = [](long idx) { return (idx % 12) < 6 ? 1.0 : -1.0; };
+        for (long j = 0; j < n; ++j)
+            for (long i = 0; i < n; ++i) {
+                const CLGComplex a = Dm[(size_t)j][(size_t)i], b = Dm[(size_t)i][(size_t)j];
+                const double si = g5(i), sj = g5(j);
+                worst = std::max(worst, std::hypot(si * a.x - sj * b.x, si * a.y + sj * b.y));
+            }
+    }
+    clg_field_destroy(in);
+    clg_field_destroy(out);
+    m_fLastTestError = big > 0 ? worst / big : worst;
+    return m_fLastTestError > 1e-12 ? 1u : 0u;
+}
+
+UINT CFieldFermionKSSU3::TestAntiHermitian(INT gn, INT, const CFieldGauge* const* gf, const CFieldBoson* const*) {
+    const CFieldGaugeSU3* g = need_gauge(this, gn, gf);
+    clg_context ctx = m_pOwner->m_ctx;
+    const int* dims = m_pOwner->m_dims;
+    clg_field in = nullptr, out = nullptr;
+    check(clg_field_create(ctx, CLG_KS, CLG_F64, CLG_FULL, dims, &in), "field");
+    check(clg_field_create(ctx, CLG_KS, CLG_F64, CLG_FULL, dims, &out), "field");
+    clg_field_set_bc(in, m_bc);
+    const auto D0 = dense_columns(in, out, dims, 1, [&](clg_field o, clg_field i) {
+        check(clg_ks_apply(ctx, o, i, g->Handle(), 0.0, EFO_F_D, EOCT_None, 1.0, 0.0), "D");
+    });
+    const long n = (long)D0.size();
+    double big = 0, worst = 0;
+    for (const auto& col : D0)
+        for (const auto& z : col) big = std::max(big, cabs(z));
+    for (long j = 0; j < n; ++j)
+        for (long i = 0; i < n; ++i) {
+            const CLGComplex a = D0[(size_t)j][(size_t)i], b = D0[(size_t)i][(size_t)j];
+            worst = std::max(worst, std::hypot(a.x + b.x, a.y - b.y));  // D_ij + conj(D_ji)
+        }
+    clg_field_destroy(in);
+    clg_field_destroy(out);
+    m_fLastTestError = big > 0 ? worst / big : worst;
+    return m_fLastTestError > 1e-12 ? 1u : 0u;
+}
+
+CGaugeFixingRandom::CGaugeFixingRandom(CLatticeData* owner, Real eps) {
+    check(clg_field_create(owner->m_ctx, CLG_GAUGE, CLG_F64, CLG_FULL, owner->m_dims, &m_gt), "field");
+    check(clg_gauge_transform_random(m_gt, eps, owner->NextSeed()), "gauge transform");
+}
+CGaugeFixingRandom::~CGaugeFixingRandom() {
+    if (m_gt) clg_field_destroy(m_gt);
+}
+void CGaugeFixingRandom::GaugeFixing(CFieldGauge* pGauge) {
+    check(clg_gauge_transform_apply(m_gt, pGauge->Handle()), "GaugeFixing");
+}
+void CGaugeFixingRandom::AlsoFixingFermion(CFieldFermion* pFermion) {
+    check(clg_gauge_transform_apply(m_gt, pFermion->Handle()), "AlsoFixingFermion");
 }
 void CFieldFermion::InitialAsSource(const int coord[4], int spin, int colour) {
     check(clg_field_point_source(m_h, coord, spin, colour), "point source");
@@ -265,7 +502,7 @@ void CFieldFermion::InitialAsSource(const int coord[4], int spin, int colour) {
 
 CFieldFermionWilsonSquareSU3::CFieldFermionWilsonSquareSU3(CLatticeData* owner, Real kappa, const int bc[4])
     : CFieldFermion(owner, CLG_WILSON, kappa, bc) {}
-void CFieldFermionWilsonSquareSU3::PrepareForHMC(const CFieldGaugeSU3* gauge, unsigned seed) {
+void CFieldFermionWilsonSquareSU3::PrepareForHMCS(const CFieldGaugeSU3* gauge, unsigned seed) {
     check(clg_wilson_prepare_hmc(m_pOwner->m_ctx, m_h, gauge->Handle(), m_fParam, seed), "PrepareForHMC");
 }
 CField* CFieldFermionWilsonSquareSU3::GetCopy() const {
@@ -277,7 +514,7 @@ CField* CFieldFermionWilsonSquareSU3::GetCopy() const {
 
 CFieldFermionKSSU3::CFieldFermionKSSU3(CLatticeData* owner, Real mass, const int bc[4])
     : CFieldFermion(owner, CLG_KS, mass, bc) {}
-void CFieldFermionKSSU3::PrepareForHMC(const CFieldGaugeSU3* gauge, unsigned seed) {
+void CFieldFermionKSSU3::PrepareForHMCS(const CFieldGaugeSU3* gauge, unsigned seed) {
     CMultiShiftSolver* ms = appGetMultiShiftSolver(m_byFieldId);
     if (!ms || m_rMC.size() < 3) throw std::runtime_error("KS PrepareForHMC needs an MSSolver and MC");
     clg_solve_report rep{};
@@ -285,31 +522,29 @@ void CFieldFermionKSSU3::PrepareForHMC(const CFieldGaugeSU3* gauge, unsigned see
                              m_rMC.data(), (int)(m_rMC.size() - 1) / 2, seed, &rep),
           "KS PrepareForHMC");
 }
-void CFieldFermion::RationalApproximation(EFieldOperator op, const CFieldGaugeSU3* gauge,
-                                         const CRatinalApproximation* r) {
+UBOOL CFieldFermion::RationalApproximation(EFieldOperator op, INT gn, INT, const CFieldGauge* const* gf,
+                                           const CFieldBoson* const*, const CRatinalApproximation* r) {
+    const CFieldGaugeSU3* gauge = need_gauge(this, gn, gf);
     CMultiShiftSolver* ms = appGetMultiShiftSolver(m_byFieldId);
-    if (!ms || !r || r->m_c.size() < 3 || r->m_c.size() % 2 == 0)
-        throw std::runtime_error("RationalApproximation needs an MSSolver and [c, a_1..a_n, b_1..b_n]");
+    if (!ms || !r || r->m_c.size() < 3 || r->m_c.size() % 2 == 0) {
+        std::fprintf(stderr, "[CLGLib] RationalApproximation needs an MSSolver and [c, a_1..a_n, b_1..b_n]\n");
+        return false;
+    }
     clg_solve_report rep{};
     clg_solver_params p = ms->m_params;
-    p.kind = dynamic_cast<const CFieldFermionKSSU3*>(this) ? CLG_KS : CLG_WILSON;
-    check(clg_rational_apply(m_pOwner->m_ctx, &p, ms->m_kind, ms->m_prec, m_h, m_h, gauge->Handle(), m_fParam,
-                             (int)op, r->m_c.data(), r->Degree(), &rep),
-          "RationalApproximation");
+    p.kind = Kind();
+    const int st = clg_rational_apply(m_pOwner->m_ctx, &p, ms->m_kind, ms->m_prec, m_h, m_h, gauge->Handle(), m_fParam,
+                                      (int)op, r->m_c.data(), r->Degree(), &rep);
+    check(st, "RationalApproximation");
+    return st == CLG_OK;
 }
-void CFieldFermionKSSU3::D_MC(const CFieldGaugeSU3* gauge) {
-    CMultiShiftSolver* ms = appGetMultiShiftSolver(m_byFieldId);
-    clg_solve_report rep{};
-    check(clg_rational_apply(m_pOwner->m_ctx, &ms->m_params, ms->m_kind, ms->m_prec, m_h, m_h, gauge->Handle(),
-                             m_fParam, EFO_F_DDdagger, m_rMC.data(), (int)(m_rMC.size() - 1) / 2, &rep),
-          "D_MC");
+void CFieldFermionKSSU3::D_MC(INT gn, INT bn, const CFieldGauge* const* gf, const CFieldBoson* const* bf) {
+    const CRatinalApproximation r(m_rMC);
+    RationalApproximation(EFO_F_DDdagger, gn, bn, gf, bf, &r);
 }
-void CFieldFermionKSSU3::D_MD(const CFieldGaugeSU3* gauge) {
-    CMultiShiftSolver* ms = appGetMultiShiftSolver(m_byFieldId);
-    clg_solve_report rep{};
-    check(clg_rational_apply(m_pOwner->m_ctx, &ms->m_params, ms->m_kind, ms->m_prec, m_h, m_h, gauge->Handle(),
-                             m_fParam, EFO_F_DDdagger, m_rMD.data(), (int)(m_rMD.size() - 1) / 2, &rep),
-          "D_MD");
+void CFieldFermionKSSU3::D_MD(INT gn, INT bn, const CFieldGauge* const* gf, const CFieldBoson* const* bf) {
+    const CRatinalApproximation r(m_rMD);
+    RationalApproximation(EFO_F_DDdagger, gn, bn, gf, bf, &r);
 }
 CField* CFieldFermionKSSU3::GetCopy() const {
     CFieldFermionKSSU3* c = new CFieldFermionKSSU3(m_pOwner, m_fParam, m_bc);
@@ -337,10 +572,11 @@ void CSLASolver::Configurate(const CParameters& p) {
     if (p.FetchStringValue("InnerPrecision", s)) m_params.inner_prec = s == "FP32" ? CLG_F32 : s == "FP16" ? CLG_F16 : CLG_F64;
     if (p.FetchValueReal("InnerAccuracy", fv)) m_params.inner_accuracy = fv;
 }
-UBOOL CSLASolver::Solve(CField* x, const CField* b, const CFieldGaugeSU3* gauge, EFieldOperator op, ESolverPhase,
-                        const CField* start) {
+UBOOL CSLASolver::Solve(CField* x, const CField* b, INT gn, INT, const CFieldGauge* const* gf,
+                        const CFieldBoson* const*, EFieldOperator op, ESolverPhase, const CField* start) {
     const CFieldFermion* fb = dynamic_cast<const CFieldFermion*>(b);
     if (!fb) throw std::runtime_error("solver needs a fermion field");
+    const CFieldGaugeSU3* gauge = need_gauge(fb, gn, gf);
     clg_solver_params p = m_params;
     p.kind = dynamic_cast<const CFieldFermionKSSU3*>(b) ? CLG_KS : CLG_WILSON;
     if (p.kind == CLG_KS) p.even_odd = 0;
@@ -378,10 +614,11 @@ void CSLASolverGCRODR::Configurate(const CParameters& p) {
     if (p.FetchStringValue("DeflateSpace", s))
         m_gparams.deflate = s == "EEDT_REV" ? CLG_DEFLATE_REV : s == "EEDT_HEV" ? CLG_DEFLATE_HEV : CLG_DEFLATE_SVD;
 }
-UBOOL CSLASolverGCRODR::Solve(CField* x, const CField* b, const CFieldGaugeSU3* gauge, EFieldOperator op,
-                              ESolverPhase phase, const CField* start) {
+UBOOL CSLASolverGCRODR::Solve(CField* x, const CField* b, INT gn, INT, const CFieldGauge* const* gf,
+                              const CFieldBoson* const*, EFieldOperator op, ESolverPhase phase, const CField* start) {
     const CFieldFermion* fb = dynamic_cast<const CFieldFermion*>(b);
     if (!fb) throw std::runtime_error("solver needs a fermion field");
+    const CFieldGaugeSU3* gauge = need_gauge(fb, gn, gf);
     const int kind = dynamic_cast<const CFieldFermionKSSU3*>(b) ? CLG_KS : CLG_WILSON;
     if (!m_solver || kind != m_gparams.kind || m_ctx != x->Lattice()->m_ctx) {
         if (m_solver) clg_gcrodr_destroy(m_solver);
@@ -426,21 +663,24 @@ void CMultiShiftSolver::Configurate(const CParameters& p) {
         m_params.absolute_accuracy = g.m_params.absolute_accuracy;
     }
 }
-UBOOL CMultiShiftSolver::Solve(std::vector<CField*>& X, const std::vector<CLGComplex>& cn, const CField* b,
-                               const CFieldGaugeSU3* gauge, EFieldOperator op) {
+UBOOL CMultiShiftSolver::Solve(TArray<CField*>& X, const TArray<CLGComplex>& cn, const CField* b, INT gn, INT,
+                               const CFieldGauge* const* gf, const CFieldBoson* const*, EFieldOperator op,
+                               ESolverPhase, const CField*) {
     const CFieldFermion* fb = dynamic_cast<const CFieldFermion*>(b);
     if (!fb) throw std::runtime_error("solver needs a fermion field");
+    if (X.Num() != cn.Num()) throw std::runtime_error("one solution field per shift");
+    const CFieldGaugeSU3* gauge = need_gauge(fb, gn, gf);
     clg_solver_params p = m_params;
     p.kind = dynamic_cast<const CFieldFermionKSSU3*>(b) ? CLG_KS : CLG_WILSON;
     std::vector<clg_field> xs;
     std::vector<double> sh;
-    for (size_t i = 0; i < X.size(); ++i) {
+    for (INT i = 0; i < X.Num(); ++i) {
         xs.push_back(X[i]->Handle());
-        sh.push_back(cn[i].real());
-        sh.push_back(cn[i].imag());
+        sh.push_back(cn[i].x);
+        sh.push_back(cn[i].y);
     }
     clg_solve_report rep{};
-    const int st = clg_multishift_solve(b->Lattice()->m_ctx, &p, m_kind, m_prec, xs.data(), sh.data(), (int)X.size(),
+    const int st = clg_multishift_solve(b->Lattice()->m_ctx, &p, m_kind, m_prec, xs.data(), sh.data(), X.Num(),
                                         b->Handle(), gauge->Handle(), fb->Param(), op, &rep);
     check(st, "MultiShift Solve");
     return st == CLG_OK;
@@ -460,11 +700,12 @@ clg_action CActionGaugePlaquette::Describe() const {
     a.beta = 3.0 * m_fBetaOverN;
     return a;
 }
-DOUBLE CActionGaugePlaquette::Energy(const CFieldGaugeSU3* gauge) {
+DOUBLE CActionGaugePlaquette::EnergySingleField(const CFieldGaugeSU3* gauge) const {
     return gauge->CalculatePlaqutteEnergy(m_fBetaOverN);
 }
-void CActionGaugePlaquette::CalculateForce(const CFieldGaugeSU3* gauge, CFieldGaugeSU3* force) {
+UBOOL CActionGaugePlaquette::CalculateForceOnGaugeSingleField(const CFieldGaugeSU3* gauge, CFieldGaugeSU3* force) const {
     check(clg_gauge_force(m_pOwner->m_ctx, force->Handle(), gauge->Handle(), 3.0 * m_fBetaOverN), "gauge force");
+    return true;
 }
 
 void CActionFermionWilsonNf2::Initialize(CLatticeData* owner, const CParameters& p) {
@@ -486,19 +727,21 @@ clg_action CActionFermionWilsonNf2::Describe() const {
     a.solver.kind = CLG_WILSON;
     return a;
 }
-DOUBLE CActionFermionWilsonNf2::Energy(const CFieldGaugeSU3* gauge) {
+DOUBLE CActionFermionWilsonNf2::EnergySingleField(const CFieldGaugeSU3* gauge) const {
     clg_action a = Describe();
     double e = 0;
     clg_solve_report rep{};
     check(clg_wilson_pf_energy(m_pOwner->m_ctx, &a.solver, a.phi, gauge->Handle(), a.kappa_or_mass, &e, &rep), "energy");
     return e;
 }
-void CActionFermionWilsonNf2::CalculateForce(const CFieldGaugeSU3* gauge, CFieldGaugeSU3* force) {
+UBOOL CActionFermionWilsonNf2::CalculateForceOnGaugeSingleField(const CFieldGaugeSU3* gauge,
+                                                                CFieldGaugeSU3* force) const {
     clg_action a = Describe();
     clg_solve_report rep{};
-    check(clg_wilson_force(m_pOwner->m_ctx, &a.solver, force->Handle(), a.phi, gauge->Handle(), a.kappa_or_mass,
-                           nullptr, nullptr, &rep),
-          "Wilson force");
+    const int st = clg_wilson_force(m_pOwner->m_ctx, &a.solver, force->Handle(), a.phi, gauge->Handle(),
+                                    a.kappa_or_mass, nullptr, nullptr, &rep);
+    check(st, "Wilson force");
+    return st == CLG_OK;
 }
 
 void CActionFermionKS::Initialize(CLatticeData* owner, const CParameters& p) {
@@ -526,7 +769,7 @@ clg_action CActionFermionKS::Describe() const {
     a.ms_prec = ms->m_prec;
     return a;
 }
-DOUBLE CActionFermionKS::Energy(const CFieldGaugeSU3* gauge) {
+DOUBLE CActionFermionKS::EnergySingleField(const CFieldGaugeSU3* gauge) const {
     clg_action a = Describe();
     double e = 0;
     clg_solve_report rep{};
@@ -535,12 +778,33 @@ DOUBLE CActionFermionKS::Energy(const CFieldGaugeSU3* gauge) {
           "KS energy");
     return e;
 }
-void CActionFermionKS::CalculateForce(const CFieldGaugeSU3* gauge, CFieldGaugeSU3* force) {
+UBOOL CActionFermionKS::CalculateForceOnGaugeSingleField(const CFieldGaugeSU3* gauge, CFieldGaugeSU3* force) const {
     clg_action a = Describe();
     clg_solve_report rep{};
-    check(clg_ks_force(m_pOwner->m_ctx, &a.solver, a.ms_kind, a.ms_prec, force->Handle(), a.phi, gauge->Handle(),
-                       a.kappa_or_mass, a.md, a.md_degree, &rep),
-          "KS force");
+    const int st = clg_ks_force(m_pOwner->m_ctx, &a.solver, a.ms_kind, a.ms_prec, force->Handle(), a.phi,
+                                gauge->Handle(), a.kappa_or_mass, a.md, a.md_degree, &rep);
+    check(st, "KS force");
+    return st == CLG_OK;
+}
+
+// CAction::Energy / CalculateForce (CAction.h:42-58): the action's gauge field is the SU(3) field of id 1
+namespace {
+INT gauge_index(INT gn, const CFieldGauge* const* gf) {
+    for (INT i = 0; i < gn; ++i)
+        if (gf[i] && gf[i]->m_byFieldId == 1 && dynamic_cast<const CFieldGaugeSU3*>(gf[i])) return i;
+    throw std::runtime_error("no SU(3) gauge field of id 1 among the gauge fields");
+}
+}  // namespace
+DOUBLE CAction::Energy(UBOOL, INT gn, INT, const CFieldGauge* const* gf, const CFieldBoson* const*,
+                       const CFieldGauge* const*) {
+    return EnergySingleField(static_cast<const CFieldGaugeSU3*>(gf[gauge_index(gn, gf)]));
+}
+UBOOL CAction::CalculateForce(INT gn, INT, const CFieldGauge* const* gf, const CFieldBoson* const*,
+                              CFieldGauge* const* gforce, CFieldBoson* const*, CFieldGauge* const*, ESolverPhase) const {
+    const INT i = gauge_index(gn, gf);
+    CFieldGaugeSU3* F = dynamic_cast<CFieldGaugeSU3*>(gforce[i]);
+    if (!F) throw std::runtime_error("CalculateForce: the force field must be an SU(3) gauge field");
+    return CalculateForceOnGaugeSingleField(static_cast<const CFieldGaugeSU3*>(gf[i]), F);
 }
 
 // ================================================================ measurement, updator
@@ -586,18 +850,28 @@ UINT CHMC::Update(UINT iSteps, UBOOL bMeasure) {
         // records H_diff only in that mode (CHMC.cpp:100-140)
         clg_hmc_params hp = m_params;
         if (m_bTestHDiff) hp.metropolis = 0;
-        check(clg_hmc_trajectory(m_pOwner->m_ctx, m_pOwner->m_pGaugeField->Handle(), &hp, acts.data(), (int)acts.size(),
+        check(clg_hmc_trajectory(m_pOwner->m_ctx, m_pOwner->GetGaugeSU3()->Handle(), &hp, acts.data(), (int)acts.size(),
                                  m_uiSeed * 1000003ull + (++m_uiTrajectory), &rep),
               "HMC trajectory");
         m_fLastHDiff = rep.h_diff;
-        if (m_bTestHDiff) m_lstHDiff.push_back(rep.h_diff);
+        m_lstSeconds.push_back(rep.seconds);
+        if (m_bTestHDiff) {
+            m_lstHDiff.push_back(rep.h_diff);
+            m_lstH.push_back(rep.h_before);
+        }
         const bool accepted = rep.accepted != 0;
         if (accepted) ++m_iAcceptedConfigurationCount;
         std::fprintf(stderr, " HMC: step = %u, H_diff = %f (%f - %f) %s\n", i + 1, rep.h_diff, rep.h_before, rep.h_after,
                      accepted ? "accepted" : "rejected");
-        if (bMeasure && accepted && m_pOwner->m_pMeasure) m_pOwner->m_pMeasure->OnConfigurationAccepted(m_pOwner->m_pGaugeField.get());
+        if (bMeasure && accepted && m_pOwner->m_pMeasure) m_pOwner->m_pMeasure->OnConfigurationAccepted(m_pOwner->GetGaugeSU3());
     }
     return m_iAcceptedConfigurationCount;
+}
+Real CHMC::GetHValue() const {
+    if (m_lstH.empty()) return 0.0;
+    double s = 0;
+    for (double h : m_lstH) s += h;
+    return s / (double)m_lstH.size();
 }
 Real CHMC::GetHDiff() const {
     if (m_lstHDiff.empty()) return 0.0;
@@ -613,14 +887,27 @@ CLatticeData::~CLatticeData() {
     m_pActionList.clear();
     m_pMSSolvers.clear();
     m_pFermionSolvers.clear();
+    m_pPool.clear();
     m_pFermionFields.clear();
-    m_pGaugeField.reset();
+    m_pGaugeField.RemoveAll();
+    m_pOwnedGauge.reset();
     if (m_ctx) clg_finalize(m_ctx);
 }
 CField* CLatticeData::GetFieldById(int id) const {
-    if (id == 1) return m_pGaugeField.get();
+    if (id == 1) return m_pOwnedGauge.get();
     auto it = m_pFermionFields.find(id);
     return it == m_pFermionFields.end() ? nullptr : it->second.get();
+}
+CField* CLatticeData::GetPooledFieldById(int id) {
+    const CField* f = GetFieldById(id);
+    if (!f) return nullptr;
+    m_pPool.emplace_back(f->GetCopy());
+    return m_pPool.back().get();
+}
+CAction* CLatticeData::GetActionById(int id) const {
+    for (const auto& a : m_pActionList)
+        if (a->m_byActionId == id) return a.get();
+    return nullptr;
 }
 
 namespace {
@@ -655,8 +942,9 @@ CLatticeData* appInitialCLG(const CParameters& sec, int device) {
         check(clg_init(device, &L->m_ctx), "clg_init");
         g_lattice = L.get();
         // gauge field (id 1)
-        L->m_pGaugeField.reset(new CFieldGaugeSU3(L.get()));
-        L->m_pGaugeField->m_byFieldId = 1;
+        L->m_pOwnedGauge.reset(new CFieldGaugeSU3(L.get()));
+        L->m_pOwnedGauge->m_byFieldId = 1;
+        L->m_pGaugeField.AddItem(L->m_pOwnedGauge.get());
         {
             CParameters g;
             std::string init = "EFIT_Identity";
@@ -668,9 +956,9 @@ CLatticeData* appInitialCLG(const CParameters& sec, int device) {
                 EFieldFileType t;
                 if (!g.FetchStringValue("GaugeFileName", fname) || !appFileTypeFromString(ftype, t))
                     throw std::runtime_error("Gauge: EFIT_ReadFromFile needs GaugeFileName and a known GaugeFileType");
-                L->m_pGaugeField->InitialFieldWithFile(fname, t);
+                L->m_pOwnedGauge->InitialFieldWithFile(fname, t);
             } else {
-                L->m_pGaugeField->InitialField(init == "EFIT_Random" ? EFIT_Random : EFIT_Identity, L->m_uiSeed, 3.0);
+                L->m_pOwnedGauge->InitialField(init == "EFIT_Random" ? EFIT_Random : EFIT_Identity, L->m_uiSeed, 3.0);
             }
         }
         // fermion fields

@@ -61,7 +61,12 @@ typedef enum {
     EFO_F_InverseD = 4,
     EFO_F_InverseDdagger = 5,
     EFO_F_InverseDD = 6,
-    EFO_F_InverseDDdagger = 7
+    EFO_F_InverseDDdagger = 7,
+    EFO_F_RationalD = 8,          /* the C++ facade's ApplyOperator only (takes a rational approximation) */
+    EFO_F_D_WithMass = 9,         /* the facade's ApplyOperator only: D with another mass / hopping */
+    EFO_F_Ddagger_WithMass = 10,
+    EFO_F_DD_WithMass = 11,
+    EFO_F_DDdagger_WithMass = 12
 } clg_field_operator;
 
 /* EOperatorCoefficientType (CField.h:62-68), same numbering */
@@ -356,6 +361,14 @@ int clg_gauge_axpy(clg_context ctx, clg_field y, double a, clg_field x);
 int clg_gauge_exp_mult(clg_context ctx, clg_field momentum, clg_field gauge, double a, int exp_precision);
 /* CFieldGaugeSU3::ElementNormalize (CFieldGaugeSU3.cu:518-528, SU3::Norm) */
 int clg_gauge_normalize(clg_context ctx, clg_field gauge);
+/* CGaugeFixingRandom (CGaugeFixingRandom.cu; used by CLGTest's TestGaugeInvarience,
+ * TestCommon.cpp:206-239): a random local gauge transformation g(x) = reunitarise(exp(i eps H(x))),
+ * one SU(3) per site from its own Philox stream, held in the mu = 0 links of the FP64 gauge field gt */
+int clg_gauge_transform_random(clg_field gt, double eps, unsigned seed);
+/* GaugeFixing / AlsoFixingFermion of that transformation: a gauge field (any format) takes
+ * U_mu(x) -> g(x) U_mu(x) g(x+mu)^dagger, a Wilson or staggered field (any subset, precision)
+ * psi(x) -> g(x) psi(x) */
+int clg_gauge_transform_apply(clg_field gt, clg_field target);
 /* CActionGaugePlaquette::Energy (CActionGaugePlaquette.cpp:14-107, CFieldGaugeSU3.cu:308-371):
  * beta/3 sum_{x, mu<nu} (3 - ReTr U_mu nu(x)), FP64 */
 int clg_gauge_plaquette_energy(clg_context ctx, clg_field gauge, double beta, double* out);
