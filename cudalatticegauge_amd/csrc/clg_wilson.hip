@@ -369,11 +369,15 @@ int launch_hop(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool 
             float ms = 0.f;
             CLG_HIP(hipEventElapsedTime(&ms, e0, e1));
             if (ms < best) { best = ms; cfg = c; }
+            const char* v2 = std::getenv("CLG_TUNE_VERBOSE");
+            if (v2 && v2[0] == '2')  // every candidate
+                std::fprintf(stderr, "[clg tune]   candidate pin=%d block=%d xcd=%d spt=%d: %.1f us\n", c.pm, c.block,
+                             c.xcd, c.spt, ms * 1e3 / 3);
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         const char* verbose = std::getenv("CLG_TUNE_VERBOSE");
-        if (verbose && verbose[0] == '1')
+        if (verbose && (verbose[0] == '1' || verbose[0] == '2'))
             std::fprintf(stderr, "[clg tune] wilson hop prec=%d gprec=%d recon=%d dag=%d x=%d nblk=%d L=%dx%dx%dx%d -> pin=%d block=%d xcd=%d spt=%d (%.1f us)\n",
                          prec, gprec, recon, (int)dag, (int)hasx, a.nblk, a.g.L[0], a.g.L[1], a.g.L[2], a.g.L[3],
                          cfg.pm, cfg.block, cfg.xcd, cfg.spt, best * 1e3 / 3);

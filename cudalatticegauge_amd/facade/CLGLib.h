@@ -26,6 +26,14 @@ typedef unsigned char BYTE;
 typedef double Real;
 typedef double DOUBLE;
 typedef bool UBOOL;
+// the literal helpers CLGLib code is written with: F(v) a Real literal (CLGFloat.h:40, the
+// double-precision build) and _T(s) a narrow string (PlatformDefine.h:134)
+#ifndef F
+#define F(v) (v)
+#endif
+#ifndef _T
+#define _T(s) s
+#endif
 
 // CLGComplex / cuDoubleComplex: two doubles .x (real) and .y (imaginary), as the reference's type
 // (CLGLib_Private.h, cuComplex.h), with its constructor and modulus helpers

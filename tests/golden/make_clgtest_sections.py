@@ -12,7 +12,9 @@ SECTIONS = {
                              "TestMSKSSolverBiCGStab", "TestMSKSSolverNested"],
     "TestSuit_EvenOdd.yaml": ["TestEOSolverBiCGStab", "TestEOSolverGMRES"],
     "TestSuit_FermionUpdatorKS.yaml": ["TestFermionUpdatorKS", "TestFermionUpdatorKSNestedForceGradient"],
-    "TestSuit_FermionUpdator.yaml": ["TestFermionUpdatorNestedForceGradient"],
+    "TestSuit_FermionUpdator.yaml": ["TestFermionUpdatorNestedForceGradient", "TestFermionUpdatorLargeScale",
+                                     "TestFermionUpdatorLargeScaleFloat"],
+    "TestSuit_Common.yaml": ["TestOperators", "TestGamm5Hermiticity", "TestAnitiHermiticity", "TestGaugeInvarience"],
     "TestSuit_FileIO.yaml": ["TestFileIOBridgePPText", "TestFileIOBridgePPBin"],
 }
 

@@ -67,3 +67,24 @@ def test_tuner_and_graph_entry_points_check_arguments_without_a_device():
     assert L.clg_vector_multiply_matrix(None, 1, None, 0, None, None) == 2
     assert L.clg_ks_hop(None, None, None, None, 1.0, None, 0.0) == 2
     assert L.clg_ks_schur(None, None, None, None, 0.1) == 2
+
+
+def test_integration_snippet_compiles(tmp_path):
+    """The CLGTest-style body in INTEGRATION.md section 1 compiles and links against the facade."""
+    import re
+    import shutil
+    import subprocess
+    txt = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    m = re.search(r"## 1\. CLGLib-side adapter.*?```cpp\n(.*?)```", txt, re.S)
+    assert m, "no C++ block in INTEGRATION.md section 1"
+    src = tmp_path / "snippet.cpp"
+    src.write_text(m.group(1))
+    pkg = os.path.join(ROOT, "cudalatticegauge_amd")
+    if not os.path.exists(os.path.join(pkg, "libclg_facade.so")):
+        pytest.skip("libclg_facade.so is not built")
+    gxx = shutil.which("g++")
+    assert gxx
+    out = subprocess.run([gxx, "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(pkg, "facade"), "-I",
+                          os.path.join(ROOT, "include"), str(src), "-o", str(tmp_path / "snippet"), "-L", pkg,
+                          "-lclg_facade", "-lclg_hip", "-Wl,-rpath," + pkg], capture_output=True, text=True)
+    assert out.returncode == 0, out.stderr[-3000:]
