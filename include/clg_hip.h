@@ -190,7 +190,8 @@ int clg_scale(clg_context ctx, clg_field y, double a_re, double a_im);
 /* CField::Dot -> cuDoubleComplex, sum_x a(x)^dagger b(x) accumulated in FP64
  * (_kernelDotFermionWilsonSquareSU3, CFieldFermionWilsonSquareSU3.cu:394-411 +
  * CCudaHelper::ThreadBufferSum, CudaHelper.cu:626-710).  Blocks until the value is on
- * the host, as the reference's Dot does. */
+ * the host, as the reference's Dot does.  The field BLAS (axpy, scale, dot, norm2) also take two
+ * 18-real FP64/FP32 gauge fields as flat arrays of link elements (CFieldGaugeSU3::Dot / Axpy). */
 int clg_dot(clg_context ctx, clg_field a, clg_field b, double out[2]);
 int clg_norm2(clg_context ctx, clg_field a, double* out);
 /* Krylov BLAS of the solvers, FP64 accumulation, one pass over w per 16 vectors:

@@ -160,10 +160,15 @@ def test_batch_rejects_cross_slot_aliasing(ctx, gauges):
         with pytest.raises(clg.CLGError):
             clg.wilson_hop_batch(ctx, kw["outs"], kw["ins"], [g, g], -KAPPA, xs=kw.get("xs"), b=1.0 if "xs" in kw
                                  else 0.0)
-    # half-lattice hops: an output that is a parity view of another slot's full input overlaps it
+    # half-lattice hops: the output of slot 1 is a parity view of the field whose same parity view is the
+    # x term of slot 0 (two view objects, one storage)
     oe = clg.Field(ctx, clg.WILSON, clg.F64, dims, subset=clg.EVEN, bc=BC)
+    d = field(ctx, dims, clg.F64, seed=4)
     with pytest.raises(clg.CLGError):
-        clg.wilson_hop_batch(ctx, [oe, c.parity(clg.EVEN)], [a.parity(clg.ODD), b.parity(clg.ODD)], [g, g], -KAPPA)
+        clg.wilson_hop_batch(ctx, [oe, c.parity(clg.EVEN)], [a.parity(clg.ODD), b.parity(clg.ODD)], [g, g], -KAPPA,
+                             xs=[c.parity(clg.EVEN), d.parity(clg.EVEN)], b=1.0)
+    # while the even view of one field and the odd view of the same field share no storage
+    clg.wilson_hop_batch(ctx, [oe, a.parity(clg.EVEN)], [c.parity(clg.ODD), b.parity(clg.ODD)], [g, g], -KAPPA)
     # allowed: out of each slot is its own x
     want = []
     for o, i in ((o1, a), (o2, b)):

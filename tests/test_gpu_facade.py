@@ -4,6 +4,8 @@ import json
 import os
 import subprocess
 
+import numpy as np
+
 import pytest
 
 pytestmark = pytest.mark.gpu
@@ -64,12 +66,20 @@ def test_multishift_ks_sections(section):
 
 
 def test_operators_section():
-    # TestOperators (TestCommon.cpp:13-88): the Gaussian-source norm estimators f.f / 12V within 0.01
-    # of 1 (cross terms of 0), and <f2, D f1> = <D^dagger f2, f1> within 0.02 (real) / 0.01 (imaginary)
+    # TestOperators (TestCommon.cpp:13-88): the Gaussian-source norm estimators f.f / 12V near 1 (cross
+    # terms near 0), and <f2, D f1> = <D^dagger f2, f1> within 0.02 (real) / 0.01 (imaginary).
+    # The reference bounds the estimators by 0.01, but on this 4x4x4x8 lattice (n = 6144 complex
+    # components of unit variance) their own standard deviation is 1/sqrt(n) = 0.0128 for f.f / n and
+    # sqrt(1/2n) = 0.009 for each part of f1.f2 / n, so 0.01 passes or fails with the random stream.
+    # Here they are held to 5 standard deviations; the generator's variance itself is checked on 16^4 to
+    # 5 standard deviations there, 0.0056 (tests/test_oracle_core.py; the device generator equals the
+    # oracle's bit for bit, tests/test_gpu_wilson.py).
     r = run("TestOperators", "operators")
-    assert abs(1 - r["f1f1"][0]) <= 0.01 and abs(r["f1f1"][1]) <= 0.01, r
-    assert abs(1 - r["f2f2"][0]) <= 0.01 and abs(r["f2f2"][1]) <= 0.01, r
-    assert max(abs(v) for v in r["f1f2"] + r["f2f1"]) <= 0.01, r
+    n = r["n"]
+    s_nn, s_x = 1.0 / np.sqrt(n), np.sqrt(0.5 / n)
+    for key in ("f1f1", "f2f2"):
+        assert abs(1 - r[key][0]) <= 5 * s_nn and r[key][1] == 0.0, r
+    assert max(abs(v) for v in r["f1f2"] + r["f2f1"]) <= 5 * s_x, r
     a, b = r["f2_Df1"], r["Ddag_f2_f1"]
     assert abs(a[0] - b[0]) <= 0.02 and abs(a[1] - b[1]) <= 0.01, r
     # the identity itself holds to FP64 rounding

@@ -149,13 +149,16 @@ def test_su3_norm_is_reference_gram_schmidt():
 
 
 def test_gaussian_source_normalisation():
-    # TestCommon.cpp:27-34: |psi|^2 ~ 12 V for a Gaussian Wilson source
-    dims = (8, 8, 8, 8)
+    # TestCommon.cpp:27-34: |psi|^2 ~ 12 V for a Gaussian Wilson source (complex components of unit
+    # variance); on 16^4 the estimator's standard deviation is 1/sqrt(12 V) = 0.0011, held to 5 of them
+    dims = (16, 16, 16, 16)
     psi = orc.wilson_gaussian(dims, 2)
-    n = np.dot(psi, psi) / (12 * orc.volume(dims))
-    assert abs(n - 1) < 0.01
+    nw = 12 * orc.volume(dims)
+    assert abs(np.dot(psi, psi) / nw - 1) < 5 / np.sqrt(nw)
+    assert abs(psi.mean()) < 5 / np.sqrt(2 * nw)
     chi = orc.ks_gaussian(dims, 2)
-    assert abs(np.dot(chi, chi) / (3 * orc.volume(dims)) - 1) < 0.02
+    nk = 3 * orc.volume(dims)
+    assert abs(np.dot(chi, chi) / nk - 1) < 5 / np.sqrt(nk)
 
 
 # ----------------------------------------------------------------------------- Wilson
