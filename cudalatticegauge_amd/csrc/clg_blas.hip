@@ -644,6 +644,7 @@ int ensure_reduction_scratch(clg_context ctx, size_t doubles) {
 int reduction_ready(clg_context ctx) { return ensure_reduction_scratch(ctx, (size_t)kMaxRedGrid * (2 * kMaxK + 1)); }
 
 long n_complex(const clg_field_s* f) {
+    if (f->kind == CLG_GAUGE) return 8 * 9 * f->Vh;  // 18-real links: 8 (parity, mu) blocks of 9 complex planes
     const long nblk = f->subset == CLG_FULL ? 2 : 1;
     return nblk * f->Vh * (f->kind == CLG_WILSON ? 12 : 3);
 }
@@ -657,8 +658,10 @@ static int pk_of(const clg_field_s* f, long& npk) {
     return PK_F2;
 }
 
+// fermion fields, and 18-real gauge fields (a flat array of link elements, as CFieldGaugeSU3::Dot /
+// Axpy / ScalarMultply see them)
 static int check_flat(const clg_field_s* a) {
-    CLG_CHECK_ARG(a && is_fermion(a->kind), "need a fermion field");
+    CLG_CHECK_ARG(a && (is_fermion(a->kind) || a->kind == CLG_GAUGE), "need a fermion field or an 18-real gauge field");
     CLG_CHECK_ARG(a->prec != CLG_F16, "BLAS on FP16-storage fields is not supported; convert first");
     return CLG_OK;
 }
