@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 using namespace clg;
 using namespace clgd;
@@ -629,6 +630,32 @@ int h16_scale_inv_sqrt(clg_context, clg_field_s*, const clg_field_s*, const doub
 int h16_multiaxpy(clg_context, clg_field_s*, clg_field_s* const*, int, const double*, double);
 int h16_gs_step(clg_context, clg_field_s* const*, int, clg_field_s*, double*, int, bool, clg_field_s*, double*);
 
+// ---- first-stage buffer of clg_multidot_noalloc: a file-scope device array, so that entry point
+// allocates nothing.  Each device holds its own copy for the life of the process; host code reaches it
+// only through the address hipGetSymbolAddress returns on that device (kept per device below).
+constexpr size_t kNoAllocPartials = (size_t)kMaxRedGrid * (2 * kMaxK + 1) * (kMaxChunkV / kMaxK);  // 2,162,688 B
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[kMaxDevices] = {};
+
+// called by clg_init with `device` current: looks the array's address up once per device
+int noalloc_scratch_init(int device) {
+    CLG_CHECK_ARG(device >= 0 && device < kMaxDevices, "device %d beyond the %d devices the library tracks", device,
+                  kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return CLG_OK;
+    void* ptr = nullptr;
+    CLG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_noalloc_ptr[device] = static_cast<double*>(ptr);
+    return CLG_OK;
+}
+static double* noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    return g_noalloc_ptr[device];
+}
+
 int ensure_reduction_scratch(clg_context ctx, size_t doubles) {
     if (ctx->red_partial_cap >= doubles) return CLG_OK;
     if (ctx->red_partial) {
@@ -736,12 +763,12 @@ int blas_conj(clg_context ctx, clg_field_s* dst, const clg_field_s* src) {
 }
 
 // second stage of a reduction whose first stage wrote nblk * nr partials to ctx->red_partial
-static int red_finish(clg_context ctx, unsigned nblk, int nr, double* out, int nout, double* out2, const int* cond,
-                      const double* ref, double eta, int* flag) {
-#define FIN(N)                                                                                                  \
-    case N:                                                                                                     \
-        k_red_finish<N><<<1, kRedBlock, 0, ctx->stream>>>(ctx->red_partial, (int)nblk, out, nout, out2, cond, ref, \
-                                                          eta, flag);                                            \
+// second stage of a reduction whose first stage wrote nblk x nr partials to `partial`
+static int red_finish_at(clg_context ctx, const double* partial, unsigned nblk, int nr, double* out, int nout,
+                         double* out2, const int* cond, const double* ref, double eta, int* flag) {
+#define FIN(N)                                                                                                    \
+    case N:                                                                                                       \
+        k_red_finish<N><<<1, kRedBlock, 0, ctx->stream>>>(partial, (int)nblk, out, nout, out2, cond, ref, eta, flag); \
         break;
     switch (nr) {
         FIN(1) FIN(2) FIN(3) FIN(5) FIN(7) FIN(9) FIN(11) FIN(13) FIN(15) FIN(17) FIN(19) FIN(21) FIN(23) FIN(25)
@@ -751,6 +778,10 @@ static int red_finish(clg_context ctx, unsigned nblk, int nr, double* out, int n
 #undef FIN
     CLG_HIP(hipGetLastError());
     return CLG_OK;
+}
+static int red_finish(clg_context ctx, unsigned nblk, int nr, double* out, int nout, double* out2, const int* cond,
+                      const double* ref, double eta, int* flag) {
+    return red_finish_at(ctx, ctx->red_partial, nblk, nr, out, nout, out2, cond, ref, eta, flag);
 }
 
 int blas_dot_dev(clg_context ctx, const clg_field_s* a, const clg_field_s* b, double* out) {
@@ -793,22 +824,28 @@ int blas_norm2_dev(clg_context ctx, const clg_field_s* a, double* out) {
         case 15: CALL(15); break; default: CALL(16); break;       \
     }
 
+// partial: the first-stage buffer, kMaxRedGrid x (2 kMaxK + 1) doubles per 16-vector chunk (nullptr: the
+// context's, grown here as needed)
 template <int P>
 static int multidot_p(clg_context ctx, clg_field_s* const* V, int nv, const clg_field_s* w, double* out, double* wnorm,
-                      const int* cond, long n) {
+                      const int* cond, long n, double* partial) {
     const unsigned g = red_grid(ctx, n);
     if (nv > kMaxK && nv <= kMaxChunkV) {
         const int nch = (nv + kMaxK - 1) / kMaxK;
-        CLG_TRY(ensure_reduction_scratch(ctx, (size_t)kMaxRedGrid * (2 * kMaxK + 1) * nch));
+        if (!partial) {
+            CLG_TRY(ensure_reduction_scratch(ctx, (size_t)kMaxRedGrid * (2 * kMaxK + 1) * nch));
+            partial = ctx->red_partial;
+        }
         PtrsN<P> Pn;
         for (int k = 0; k < nv; ++k) Pn.p[k] = (const typename Pk<P>::T*)V[k]->data;
         k_multidot_ch<P><<<dim3(g, nch), kRedBlock, 0, ctx->stream>>>(n, Pn, nv, (const typename Pk<P>::T*)w->data,
-                                                                      ctx->red_partial, cond);
+                                                                      partial, cond);
         CLG_HIP(hipGetLastError());
-        k_red_finish_ch<<<dim3(1, nch), kRedBlock, 0, ctx->stream>>>(ctx->red_partial, (int)g, nv, out, wnorm, cond);
+        k_red_finish_ch<<<dim3(1, nch), kRedBlock, 0, ctx->stream>>>(partial, (int)g, nv, out, wnorm, cond);
         CLG_HIP(hipGetLastError());
         return CLG_OK;
     }
+    if (!partial) partial = ctx->red_partial;
     for (int k0 = 0; k0 < nv; k0 += kMaxK) {
         const int kk = std::min(kMaxK, nv - k0);
         Ptrs<P> Pp;
@@ -816,16 +853,16 @@ static int multidot_p(clg_context ctx, clg_field_s* const* V, int nv, const clg_
         const typename Pk<P>::T* wp = (const typename Pk<P>::T*)w->data;
         double* wn = k0 == 0 ? wnorm : nullptr;
         double* o = out + 2 * k0;
-#define CALL(KK)                                                                                    \
-    if (wn)                                                                                         \
-        k_multidot<P, KK, true><<<g, kRedBlock, 0, ctx->stream>>>(n, Pp, wp, ctx->red_partial, cond); \
-    else                                                                                            \
-        k_multidot<P, KK, false><<<g, kRedBlock, 0, ctx->stream>>>(n, Pp, wp, ctx->red_partial, cond);
+#define CALL(KK)                                                                            \
+    if (wn)                                                                                 \
+        k_multidot<P, KK, true><<<g, kRedBlock, 0, ctx->stream>>>(n, Pp, wp, partial, cond); \
+    else                                                                                    \
+        k_multidot<P, KK, false><<<g, kRedBlock, 0, ctx->stream>>>(n, Pp, wp, partial, cond);
         SWITCH_K(kk, CALL)
 #undef CALL
         CLG_HIP(hipGetLastError());
         // the first stage writes 2K + 1 partials per block (the last is |w|^2, kept only when asked)
-        CLG_TRY(red_finish(ctx, g, 2 * kk + 1, o, 2 * kk, wn, cond, nullptr, 0.0, nullptr));
+        CLG_TRY(red_finish_at(ctx, partial, g, 2 * kk + 1, o, 2 * kk, wn, cond, nullptr, 0.0, nullptr));
     }
     return CLG_OK;
 }
@@ -839,9 +876,9 @@ int blas_multidot_dev(clg_context ctx, clg_field_s* const* V, int nv, const clg_
     CLG_TRY(ensure_reduction_scratch(ctx, (size_t)kMaxRedGrid * (2 * kMaxK + 1)));
     long n;
     const int pk = pk_of(w, n);
-    if (pk == PK_D) return multidot_p<PK_D>(ctx, V, nv, w, out, wnorm, cond, n);
-    if (pk == PK_F4) return multidot_p<PK_F4>(ctx, V, nv, w, out, wnorm, cond, n);
-    return multidot_p<PK_F2>(ctx, V, nv, w, out, wnorm, cond, n);
+    if (pk == PK_D) return multidot_p<PK_D>(ctx, V, nv, w, out, wnorm, cond, n, nullptr);
+    if (pk == PK_F4) return multidot_p<PK_F4>(ctx, V, nv, w, out, wnorm, cond, n, nullptr);
+    return multidot_p<PK_F2>(ctx, V, nv, w, out, wnorm, cond, n, nullptr);
 }
 
 template <int P>
@@ -1098,6 +1135,27 @@ int clg_vector_multiply_matrix(clg_context ctx, int nres, clg_field* res, int nl
     CLG_TRY(blas_vmm_dev(ctx, in.data(), nin, in.data(), nres, m));
     CLG_HIP(hipStreamSynchronize(ctx->stream));
     return CLG_OK;
+}
+
+int clg_multidot_noalloc(clg_context ctx, int n, clg_field* V, clg_field w, double* out) {
+    CLG_CHECK_ARG(ctx && V && w && out, "null argument");
+    CLG_CHECK_ARG(n >= 1 && n <= kMaxChunkV, "n must be in [1, %d]: the static partial-sum buffer holds %d vectors",
+                  kMaxChunkV, kMaxChunkV);
+    for (int k = 0; k < n; ++k) {
+        CLG_CHECK_ARG(V[k], "null field V[%d]", k);
+        CLG_TRY(check_pair(V[k], w));
+    }
+    CLG_CHECK_ARG(w->ctx->device == ctx->device, "fields live on device %d, the context on %d", w->ctx->device,
+                  ctx->device);
+    (void)hipSetDevice(ctx->device);
+    double* part = noalloc_scratch(ctx->device);
+    CLG_CHECK_ARG(part, "no static partial-sum buffer on device %d (clg_init was not called there)", ctx->device);
+    long np;
+    const int pk = pk_of(w, np);
+    // at most kMaxRedGrid workgroups x (2 kMaxK + 1) partials per chunk of 16 vectors, 4 chunks: the array's size
+    if (pk == PK_D) return multidot_p<PK_D>(ctx, V, n, w, out, nullptr, nullptr, np, part);
+    if (pk == PK_F4) return multidot_p<PK_F4>(ctx, V, n, w, out, nullptr, nullptr, np, part);
+    return multidot_p<PK_F2>(ctx, V, n, w, out, nullptr, nullptr, np, part);
 }
 
 }  // extern "C"

@@ -164,6 +164,8 @@ inline void ctx_release(clg_context c, void* obj) {
 int ensure_reduction_scratch(clg_context ctx, size_t doubles);
 // allocates the partial-sum buffer every reduction kernel uses (its final size)
 int reduction_ready(clg_context ctx);
+// looks up, once per device, the device address of clg_multidot_noalloc's static partial-sum array
+int noalloc_scratch_init(int device);
 // scratch field of the same shape as `like` (kind/prec/subset/dims), index slot
 int scratch_field(clg_context ctx, const clg_field_s* like, int kind, int prec, int subset,
                   int slot, clg_field_s** out);

@@ -442,6 +442,7 @@ int init_context(clg_context c) {
     CLG_HIP(hipHostMalloc((void**)&c->host_result, 4096 * sizeof(double), hipHostMallocDefault));
     CLG_HIP(hipEventCreate(&c->ev0));
     CLG_HIP(hipEventCreate(&c->ev1));
+    CLG_TRY(noalloc_scratch_init(c->device));
     CLG_HIP(hipStreamSynchronize(c->stream));
     return CLG_OK;
 }

@@ -201,6 +201,13 @@ int clg_multidot(clg_context ctx, int n, clg_field* V, clg_field w, double* out)
 /* w += sum_k c_k V_k, c_k = coeffs[2k] + i coeffs[2k+1], n <= 992 -- the solution update the
  * reference runs as n CField::Axpy calls (CSolverGMRES.cpp:222-229).  No V_k may be w. */
 int clg_multiaxpy(clg_context ctx, clg_field w, int n, clg_field* V, const double* coeffs);
+/* clg_multidot that allocates nothing.  No counterpart in the reference: for callers that cannot
+ * call hipMalloc (code under HIP graph capture, serving loops, harnesses that forbid allocation).
+ * out (DEVICE memory, 2n doubles): out[2k..2k+1] = <V_k, w>, k < n <= 64, written on the context's
+ * stream without blocking (the same kernels and sums as clg_multidot).  The partial sums live in
+ * a static 2,162,688-byte device array, one per device, whose address clg_init looks up; every call
+ * on a device shares it, so calls that overlap on one device (other streams or threads) race. */
+int clg_multidot_noalloc(clg_context ctx, int n, clg_field* V, clg_field w, double* out);
 /* CFieldMatrixOperation::VectorMultiplyMatrix (CField.h:372-380; _kernelMatrixMultiply,
  * CFieldFermionWilsonSquareSU3.cu:1533-1560): with W = (res_0..res_{nres-1}, left_0..left_{nleft-1}),
  * res_i <- sum_j W_j M[j*nres + i], M a host (nres+nleft) x nres complex matrix (interleaved
