@@ -8,6 +8,7 @@
 // so the solvers never wait on the host for a dot product (the reference's Dot does ~20
 // launches and a blocking copy, CudaHelper.cu:626-710).
 #include "clg_internal.h"
+#include "clg_packets.h"
 
 #include <algorithm>
 #include <cstring>
@@ -23,49 +24,6 @@ const int kMaxK = 16;  // vectors per multi-dot / multi-axpy launch
 // the context's 4096-double device result buffer: results from 0, host-supplied coefficients of
 // the BLAS entry points from 2112 (1984 doubles)
 const int kHostArgs = 2112;
-
-// ---- 16-byte packets: NC complex numbers per lane and load, values handled as doubles
-enum { PK_D = 0, PK_F4 = 1, PK_F2 = 2 };
-template <int P> struct Pk;
-template <> struct Pk<PK_D> {
-    typedef double2 T;
-    static constexpr int NC = 1;
-    __device__ static void cvt(const T& a, double* v) { v[0] = a.x; v[1] = a.y; }
-    __device__ static void ld(const T* p, long i, double* v) {
-        const double2 a = p[i];
-        v[0] = a.x; v[1] = a.y;
-    }
-    __device__ static void st(T* p, long i, double* v) { p[i] = make_double2(v[0], v[1]); }
-};
-template <> struct Pk<PK_F4> {
-    typedef float4 T;
-    static constexpr int NC = 2;
-    __device__ static void cvt(const T& a, double* v) { v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; }
-    __device__ static void ld(const T* p, long i, double* v) {
-        const float4 a = p[i];
-        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-    }
-    // stores the FP32-rounded value and leaves it in v (so norms see what was stored)
-    __device__ static void st(T* p, long i, double* v) {
-        const float4 a = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
-        p[i] = a;
-        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-    }
-};
-template <> struct Pk<PK_F2> {
-    typedef float2 T;
-    static constexpr int NC = 1;
-    __device__ static void cvt(const T& a, double* v) { v[0] = a.x; v[1] = a.y; }
-    __device__ static void ld(const T* p, long i, double* v) {
-        const float2 a = p[i];
-        v[0] = a.x; v[1] = a.y;
-    }
-    __device__ static void st(T* p, long i, double* v) {
-        const float2 a = make_float2((float)v[0], (float)v[1]);
-        p[i] = a;
-        v[0] = a.x; v[1] = a.y;
-    }
-};
 
 #define GRID_STRIDE(i, n) for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (long)gridDim.x * blockDim.x)
 // four independent packets per lane per trip, so every wave keeps four loads in flight
@@ -684,6 +642,8 @@ static int pk_of(const clg_field_s* f, long& npk) {
     npk = n;
     return PK_F2;
 }
+// the same for other files' kernels
+int pk_kind(const clg_field_s* f, long& npk) { return pk_of(f, npk); }
 
 // fermion fields, and 18-real gauge fields (a flat array of link elements, as CFieldGaugeSU3::Dot /
 // Axpy / ScalarMultply see them)

@@ -17,6 +17,8 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <functional>
+#include <map>
 #include <string>
 #include <utility>
 #include <vector>
@@ -75,7 +77,7 @@ struct clg_context_s {
     double* red_partial = nullptr;
     size_t red_partial_cap = 0;  // doubles
     unsigned* red_counter = nullptr;
-    double* red_result = nullptr;  // 256 doubles of device-resident results
+    double* red_result = nullptr;  // 4096 doubles of device-resident results
     double* host_result = nullptr;  // pinned
     // solver workspace (Hessenberg, Givens, scalars), grow-only
     double* ws = nullptr;
@@ -86,6 +88,8 @@ struct clg_context_s {
     // objects made on this context (GCRO-DR solvers, graphs) with their destroy functions:
     // clg_finalize destroys those the caller has not
     std::vector<std::pair<void*, void (*)(void*)>> owned;
+    // recorded solver cycles (clg::cycle_graph_run): key -> instantiated graph
+    std::map<std::vector<unsigned long long>, hipGraphExec_t> cycle_graphs;
 };
 
 struct clg_field_s {
@@ -166,11 +170,25 @@ int ensure_reduction_scratch(clg_context ctx, size_t doubles);
 int reduction_ready(clg_context ctx);
 // looks up, once per device, the device address of clg_multidot_noalloc's static partial-sum array
 int noalloc_scratch_init(int device);
+// device scratch (doubles) one Arnoldi orthogonalisation step (arn_step, clg_arnoldi.hip) needs
+constexpr int kArnScratch = 264;
 // scratch field of the same shape as `like` (kind/prec/subset/dims), index slot
 int scratch_field(clg_context ctx, const clg_field_s* like, int kind, int prec, int subset,
                   int slot, clg_field_s** out);
 // true while the context stream records a graph (blocking calls are refused then)
 bool stream_capturing(clg_context ctx);
+// Runs the device work of one solver cycle as a recorded graph: the first call with a key records
+// what `record` launches on the context stream (it must neither block nor allocate) and instantiates
+// it; every call (the first included) then replays it.  The key names every pointer and scalar the
+// recorded launches take.  CLG_GRAPHS=0 runs `record` directly instead.
+int cycle_graph_run(clg_context ctx, const std::vector<unsigned long long>& key, const std::function<int()>& record);
+// the bits of a double, for graph keys
+inline unsigned long long key_bits(double v) {
+    unsigned long long u;
+    static_assert(sizeof(u) == sizeof(v), "64-bit double");
+    __builtin_memcpy(&u, &v, sizeof(u));
+    return u;
+}
 
 // launch autotuner (clg_wilson.hip, clg_batch.hip): process-wide switch and tuned-shape tables
 bool tuning_enabled();
@@ -380,6 +398,7 @@ template <> struct NtVec<double2> { typedef double V __attribute__((ext_vector_t
 template <> struct NtVec<float2> { typedef float V __attribute__((ext_vector_type(2))); };
 template <> struct NtVec<float4> { typedef float V __attribute__((ext_vector_type(4))); };
 template <> struct NtVec<uint2> { typedef unsigned V __attribute__((ext_vector_type(2))); };
+template <> struct NtVec<uint4> { typedef unsigned V __attribute__((ext_vector_type(4))); };
 template <> struct NtVec<unsigned> { typedef unsigned V; };
 template <typename T, bool NT>
 __device__ __forceinline__ T ldq(const char* base, int Vh, int k, unsigned e) {
@@ -393,6 +412,21 @@ __device__ __forceinline__ T ldq(const char* base, int Vh, int k, unsigned e) {
         return out;
     } else {
         return ldp<T>(base, Vh, k, e);
+    }
+}
+
+// store form of ldq: with NT the line is written with the non-temporal policy (streamed towards memory
+// and marked for early eviction), for outputs no later read of the same launch wants in the cache
+template <typename T, bool NT>
+__device__ __forceinline__ void stq(char* base, int Vh, int k, unsigned e, T v) {
+    if constexpr (NT) {
+        typedef typename NtVec<T>::V V;
+        V* p = reinterpret_cast<V*>(base + (size_t)k * (size_t)Vh * sizeof(T) + (unsigned)(e * (unsigned)sizeof(T)));
+        V x;
+        __builtin_memcpy(&x, &v, sizeof(T));
+        __builtin_nontemporal_store(x, p);
+    } else {
+        stp<T>(base, Vh, k, e, v);
     }
 }
 
@@ -461,10 +495,10 @@ template <> struct WilsonIO<CLG_F64> {
             v[k] = C2<R>{(R)a.x, (R)a.y};
         }
     }
-    template <typename R>
+    template <typename R, bool NT = false>
     __device__ __forceinline__ static void store(char* b, float*, int Vh, unsigned e, unsigned, const C2<R>* v) {
 #pragma unroll
-        for (int k = 0; k < 12; ++k) stp<double2>(b, Vh, k, e, make_double2((double)v[k].x, (double)v[k].y));
+        for (int k = 0; k < 12; ++k) stq<double2, NT>(b, Vh, k, e, make_double2((double)v[k].x, (double)v[k].y));
     }
 };
 
@@ -480,12 +514,12 @@ template <> struct WilsonIO<CLG_F32> {
             v[2 * k + 1] = C2<R>{(R)a.z, (R)a.w};
         }
     }
-    template <typename R>
+    template <typename R, bool NT = false>
     __device__ __forceinline__ static void store(char* b, float*, int Vh, unsigned e, unsigned, const C2<R>* v) {
 #pragma unroll
         for (int k = 0; k < 6; ++k)
-            stp<float4>(b, Vh, k, e,
-                        make_float4((float)v[2 * k].x, (float)v[2 * k].y, (float)v[2 * k + 1].x, (float)v[2 * k + 1].y));
+            stq<float4, NT>(b, Vh, k, e,
+                            make_float4((float)v[2 * k].x, (float)v[2 * k].y, (float)v[2 * k + 1].x, (float)v[2 * k + 1].y));
     }
 };
 
@@ -521,7 +555,7 @@ template <> struct WilsonIO<CLG_F16> {
             f = unpack_h2(a.w); v[4 * k + 3] = C2<R>{(R)(f.x * s), (R)(f.y * s)};
         }
     }
-    template <typename R>
+    template <typename R, bool NT = false>
     __device__ __forceinline__ static void store(char* b, float* nb, int Vh, unsigned e, unsigned ne, const C2<R>* v) {
         float m = 0.f;
 #pragma unroll
@@ -535,7 +569,7 @@ template <> struct WilsonIO<CLG_F16> {
             a.y = pack_h2((float)v[4 * k + 1].x * inv, (float)v[4 * k + 1].y * inv);
             a.z = pack_h2((float)v[4 * k + 2].x * inv, (float)v[4 * k + 2].y * inv);
             a.w = pack_h2((float)v[4 * k + 3].x * inv, (float)v[4 * k + 3].y * inv);
-            stp<uint4>(b, Vh, k, e, a);
+            stq<uint4, NT>(b, Vh, k, e, a);
         }
     }
 };
@@ -579,7 +613,7 @@ template <> struct KSIO<CLG_F16> {
             v[k] = C2<R>{(R)(f.x * s), (R)(f.y * s)};
         }
     }
-    template <typename R>
+    template <typename R, bool NT = false>
     __device__ __forceinline__ static void store(char* b, float* nb, int Vh, unsigned e, unsigned ne, const C2<R>* v) {
         float m = 0.f;
 #pragma unroll

@@ -18,6 +18,7 @@
 #include "clg_smallla.h"
 
 #include <algorithm>
+#include <cstdint>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -41,6 +42,7 @@ long n_complex(const clg_field_s*);
 int linop_apply(clg_context ctx, int kind, int op, const clg_field_s* gauge, double param, clg_field_s* out,
                 const clg_field_s* in);
 int gauge_convert(clg_context, clg_field_s*, const clg_field_s*);
+int arn_step(clg_context, clg_field_s* const*, int, clg_field_s*, double*, int, bool, clg_field_s*, double*);
 }  // namespace clg
 
 namespace {
@@ -159,18 +161,6 @@ __global__ void k_msfom_update(long n, int ns, int m, ShiftPtrs sp, VPtrs vp, co
         }
     }
 }
-// Hessenberg column j: second Gram-Schmidt pass and the sub-diagonal |w|
-__global__ void k_ms_hess(double* H, int ld, int j, const double* c2, const double* wn2, const int* reorth) {
-    const int i = threadIdx.x;
-    if (i <= j && *reorth) {
-        H[2 * (j * ld + i)] += c2[2 * i];
-        H[2 * (j * ld + i) + 1] += c2[2 * i + 1];
-    }
-    if (i == 0) {
-        H[2 * (j * ld + j + 1)] = sqrt(wn2[0]);
-        H[2 * (j * ld + j + 1) + 1] = 0.0;
-    }
-}
 // per-shift FOM solve (CMultiShiftFOM RotateHSolveY, CMultiShiftFOM.cpp:272-330): workgroup k (one
 // wave) copies H (m x m, column-major, ld = m+1) into LDS, adds c_k on the diagonal, applies the m-1
 // Givens rotations to g = beta0_k e1 (each rotation spread over the columns it touches), back-
@@ -282,12 +272,11 @@ int msfom(clg_context ctx, OpRef& A, std::vector<clg_field_s*>& X, const std::ve
     clg_field_s* w = V[m];
     const int ld = m + 1;
     const size_t oH = 0, oY = oH + 2 * (size_t)ld * m, oB = oY + 2 * (size_t)m * ns, oE = oB + 2 * (size_t)ns,
-                 oSh = oE + ns, oS = oSh + 2 * (size_t)ns, oC = oS + 16, tot = oC + 2 * (size_t)m;
+                 oSh = oE + ns, oS = oSh + 2 * (size_t)ns, oA = oS + 16, tot = oA + kArnScratch;
     double* ws = nullptr;
     CLG_TRY(workspace(ctx, tot, &ws));
     double* H = ws + oH;
     double* sc = ws + oS;
-    int* reorth = reinterpret_cast<int*>(ws + oS + 8);
     std::vector<double> hb(2 * ns), hsh(2 * ns);
     for (int k = 0; k < ns; ++k) { hsh[2 * k] = shifts[k].x; hsh[2 * k + 1] = shifts[k].y; }
     for (auto x : X) CLG_TRY(clg_field_zero(x));
@@ -300,17 +289,33 @@ int msfom(clg_context ctx, OpRef& A, std::vector<clg_field_s*>& X, const std::ve
     double bn2;
     CLG_TRY(fetch_doubles(ctx, sc, 1, &bn2));
     for (int k = 0; k < ns; ++k) { hb[2 * k] = std::sqrt(bn2); hb[2 * k + 1] = 0.0; }
-    for (int cyc = 0; cyc < restart; ++cyc) {
-        ++cycles;
+    // the m Arnoldi steps of a cycle: the same launches on the same storage every cycle (v_m is copied
+    // into V[0] between cycles), so after the first cycle, which tunes the launches and grows the
+    // scratch, they replay as one recorded graph
+    auto arnoldi = [&]() -> int {
         for (int j = 0; j < m; ++j) {
             CLG_TRY(A.apply(ctx, w, V[j]));
-            double* Hj = H + 2 * (size_t)j * ld;
-            CLG_TRY(blas_multidot_dev(ctx, V.data(), j + 1, w, Hj, sc + 4, nullptr));
-            CLG_TRY(blas_multiaxpy_dev(ctx, w, V.data(), j + 1, Hj, -1.0, sc + 1, nullptr, sc + 4, 0.5, reorth));
-            CLG_TRY(blas_multidot_dev(ctx, V.data(), j + 1, w, ws + oC, nullptr, reorth));
-            CLG_TRY(blas_multiaxpy_dev(ctx, w, V.data(), j + 1, ws + oC, -1.0, sc + 1, reorth, nullptr, 0.0, nullptr));
-            k_ms_hess<<<1, 64, 0, ctx->stream>>>(H, ld, j, ws + oC, sc + 1, reorth);
-            CLG_TRY(blas_scale_inv_sqrt(ctx, j < m - 1 ? V[j + 1] : vnext, w, sc + 1));
+            CLG_TRY(arn_step(ctx, V.data(), j + 1, w, H + 2 * (size_t)j * ld, j, false, j < m - 1 ? V[j + 1] : vnext,
+                             ws + oA));
+        }
+        return CLG_OK;
+    };
+    std::vector<unsigned long long> key = {0x4d53464fULL, (unsigned long long)A.kind, (unsigned long long)A.op,
+                                           key_bits(A.param), (unsigned long long)(uintptr_t)A.gauge->data,
+                                           (unsigned long long)(uintptr_t)vnext->data, (unsigned long long)(uintptr_t)ws,
+                                           (unsigned long long)(uintptr_t)ctx->red_partial, (unsigned long long)m,
+                                           (unsigned long long)b->kind, (unsigned long long)b->prec,
+                                           (unsigned long long)b->subset, (unsigned long long)b->Vh};
+    for (int d = 0; d < 4; ++d) key.push_back((unsigned long long)(long long)b->bc[d]);
+    for (int k = 0; k <= m; ++k) key.push_back((unsigned long long)(uintptr_t)V[k]->data);
+    for (int cyc = 0; cyc < restart; ++cyc) {
+        ++cycles;
+        if (cyc == 0) {
+            CLG_TRY(arnoldi());
+        } else {
+            const int a0 = A.applies;
+            CLG_TRY(cycle_graph_run(ctx, key, arnoldi));
+            A.applies = a0 + m;
         }
         std::vector<int> idx;
         for (int k = 0; k < ns; ++k)
@@ -336,9 +341,11 @@ int msfom(clg_context ctx, OpRef& A, std::vector<clg_field_s*>& X, const std::ve
         if (b->prec == CLG_F64) k_msfom_update<CLG_F64><<<g, 256, shm, ctx->stream>>>(np, na, m, sp, vp, ws + oY);
         else k_msfom_update<CLG_F32><<<g, 256, shm, ctx->stream>>>(np, na, m, sp, vp, ws + oY);
         CLG_HIP(hipGetLastError());
-        std::vector<double> e(na), bnew(2 * na);
-        CLG_TRY(fetch_doubles(ctx, ws + oE, na, e.data()));
-        CLG_TRY(fetch_doubles(ctx, ws + oB, 2 * na, bnew.data()));
+        // beta0 and the errors in one read: [oB, oB + 2 na) and [oE, oE + na) of the region [oB, oE + ns)
+        std::vector<double> be(3 * (size_t)ns);
+        CLG_TRY(fetch_doubles(ctx, ws + oB, 3 * ns, be.data()));
+        const double* bnew = be.data();
+        const double* e = be.data() + (oE - oB);
         maxerr = 0;
         for (int q = 0; q < na; ++q) {
             const int k = idx[q];
@@ -347,8 +354,8 @@ int msfom(clg_context ctx, OpRef& A, std::vector<clg_field_s*>& X, const std::ve
             maxerr = std::max(maxerr, e[q]);
             if (e[q] < accuracy * blen) active[k] = 0;
         }
-        std::swap(V[0], vnext);  // v_m seeds the next cycle
         if (maxerr < accuracy * blen) return CLG_OK;
+        CLG_TRY(clg_field_copy(V[0], vnext));  // v_m seeds the next cycle
     }
     return CLG_NOT_CONVERGED;
 }
@@ -372,13 +379,12 @@ int msgmres(clg_context ctx, OpRef& A, std::vector<clg_field_s*>& X, const std::
     clg_field_s *xb = nullptr, *w = V[m];
     CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 2100, &xb));
     const int ld = m + 1;
-    const size_t oH = 0, oY = oH + 2 * (size_t)ld * m, oS = oY + 2 * (size_t)m * (ns + 1), oC = oS + 16,
-                 tot = oC + 2 * (size_t)(m + 1);
+    const size_t oH = 0, oY = oH + 2 * (size_t)ld * m, oS = oY + 2 * (size_t)m * (ns + 1), oA = oS + 16,
+                 tot = oA + kArnScratch;
     double* ws = nullptr;
     CLG_TRY(workspace(ctx, tot, &ws));
     double* H = ws + oH;
     double* sc = ws + oS;
-    int* reorth = reinterpret_cast<int*>(ws + oS + 8);
     CLG_TRY(clg_field_zero(xb));
     for (auto x : X) CLG_TRY(clg_field_zero(x));
     std::vector<zc> beta0(ns, zc(1.0, 0.0));
@@ -399,14 +405,8 @@ int msgmres(clg_context ctx, OpRef& A, std::vector<clg_field_s*>& X, const std::
         CLG_TRY(blas_scale_inv_sqrt(ctx, V[0], V[0], sc));
         for (int j = 0; j < m; ++j) {
             CLG_TRY(A.apply(ctx, w, V[j]));
-            double* Hj = H + 2 * (size_t)j * ld;
-            CLG_TRY(blas_multidot_dev(ctx, V.data(), j + 1, w, Hj, sc + 4, nullptr));
-            CLG_TRY(blas_multiaxpy_dev(ctx, w, V.data(), j + 1, Hj, -1.0, sc + 1, nullptr, sc + 4, 0.5, reorth));
-            CLG_TRY(blas_multidot_dev(ctx, V.data(), j + 1, w, ws + oC, nullptr, reorth));
-            CLG_TRY(blas_multiaxpy_dev(ctx, w, V.data(), j + 1, ws + oC, -1.0, sc + 1, reorth, nullptr, 0.0, nullptr));
-            k_ms_hess<<<1, 64, 0, ctx->stream>>>(H, ld, j, ws + oC, sc + 1, reorth);
-            CLG_HIP(hipGetLastError());
-            if (j < m - 1) CLG_TRY(blas_scale_inv_sqrt(ctx, V[j + 1], w, sc + 1));
+            CLG_TRY(arn_step(ctx, V.data(), j + 1, w, H + 2 * (size_t)j * ld, j, false, j < m - 1 ? V[j + 1] : nullptr,
+                             ws + oA));
         }
         std::vector<double> hh(2 * (size_t)ld * m);
         CLG_TRY(fetch_doubles(ctx, H, (int)hh.size(), hh.data()));

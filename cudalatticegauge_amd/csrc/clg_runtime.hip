@@ -331,6 +331,46 @@ bool stream_capturing(clg_context ctx) {
     return cs != hipStreamCaptureStatusNone;
 }
 
+static bool cycle_graphs_enabled() {
+    const char* e = std::getenv("CLG_GRAPHS");
+    return !(e && e[0] == '0');
+}
+
+int cycle_graph_run(clg_context ctx, const std::vector<unsigned long long>& key, const std::function<int()>& record) {
+    // inside a caller's own recording the work is simply recorded there
+    if (!cycle_graphs_enabled() || stream_capturing(ctx)) return record();
+    auto it = ctx->cycle_graphs.find(key);
+    if (it == ctx->cycle_graphs.end()) {
+        constexpr size_t kMaxCycleGraphs = 32;
+        if (ctx->cycle_graphs.size() >= kMaxCycleGraphs) {
+            CLG_HIP(hipStreamSynchronize(ctx->stream));
+            for (auto& kv : ctx->cycle_graphs) (void)hipGraphExecDestroy(kv.second);
+            ctx->cycle_graphs.clear();
+        }
+        CLG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+        const int st = record();
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (st != CLG_OK || e != hipSuccess || !g) {
+            (void)hipGetLastError();
+            if (g) (void)hipGraphDestroy(g);
+            if (st != CLG_OK) return st;
+            set_error("recording a solver cycle failed: %s", hipGetErrorString(e));
+            return CLG_HIP_ERROR;
+        }
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            set_error("hipGraphInstantiate (solver cycle): %s", hipGetErrorString(ei));
+            return CLG_HIP_ERROR;
+        }
+        it = ctx->cycle_graphs.emplace(key, ex).first;
+    }
+    CLG_HIP(hipGraphLaunch(it->second, ctx->stream));
+    return CLG_OK;
+}
+
 // ---- persistent tuning cache
 std::string tune_cache_path() {
     const char* e = std::getenv("CLG_TUNE_CACHE");
@@ -476,6 +516,8 @@ int clg_finalize(clg_context c) {
     std::vector<std::pair<void*, void (*)(void*)>> owned;
     owned.swap(c->owned);
     for (auto it = owned.rbegin(); it != owned.rend(); ++it) it->second(it->first);
+    for (auto& kv : c->cycle_graphs) (void)hipGraphExecDestroy(kv.second);
+    c->cycle_graphs.clear();
     for (clg_field_s* f : c->scratch) clg_field_destroy(f);
     c->scratch.clear();
     if (c->red_partial) (void)hipFree(c->red_partial);

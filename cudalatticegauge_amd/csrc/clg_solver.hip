@@ -26,6 +26,7 @@ namespace clg {
 int blas_axpby(clg_context, clg_field_s*, double, double, const clg_field_s*, double, double);
 int blas_axpy_dev(clg_context, clg_field_s*, const double*, double, const clg_field_s*);
 int blas_scale_inv_sqrt(clg_context, clg_field_s*, const clg_field_s*, const double*);
+int arn_step(clg_context, clg_field_s* const*, int, clg_field_s*, double*, int, bool, clg_field_s*, double*);
 int blas_dot_dev(clg_context, const clg_field_s*, const clg_field_s*, double*);
 int blas_norm2_dev(clg_context, const clg_field_s*, double*);
 int blas_multidot_dev(clg_context, clg_field_s* const*, int, const clg_field_s*, double*, double*, const int*);
@@ -48,32 +49,6 @@ int gauge_convert(clg_context, clg_field_s* dst, const clg_field_s* src);
 namespace {
 
 // ------------------------------------------------------------------ small device kernels
-// Hessenberg column j (stored column-major, ld = m+1): add the second Gram-Schmidt pass
-// and set the sub-diagonal to |w|.  shifted: the Arnoldi step ran on K = 1 - A, whose
-// Krylov space is A's; then column j of A's Hessenberg is e_j - (column j of K's).
-__global__ void k_hess_update(double* H, int ld, int j, const double* c2, const double* wn2, const int* reorth,
-                              int shifted) {
-    const int i = threadIdx.x;
-    if (i <= j) {
-        double re = H[2 * (j * ld + i)], im = H[2 * (j * ld + i) + 1];
-        if (*reorth) {
-            re += c2[2 * i];
-            im += c2[2 * i + 1];
-        }
-        if (shifted) {
-            re = (i == j ? 1.0 : 0.0) - re;
-            im = -im;
-        }
-        H[2 * (j * ld + i)] = re;
-        H[2 * (j * ld + i) + 1] = im;
-    }
-    if (i == 0) {
-        const double s = sqrt(wn2[0]);
-        H[2 * (j * ld + j + 1)] = shifted ? -s : s;
-        H[2 * (j * ld + j + 1) + 1] = 0.0;
-    }
-}
-
 // Givens rotations of the (m+1) x m Hessenberg matrix with g = beta e1 and back
 // substitution (RotateH / SolveY, CSolverGMRES.cpp:262-358), FP64, one thread.
 // out: y[0..m), dev[0] = |g_m|^2, dev[1] = |g_m|
@@ -246,13 +221,11 @@ int gmres(clg_context ctx, LinOp& A, clg_field_s* x, const clg_field_s* b, int m
     const size_t offH = 0, offC = offH + 2 * (size_t)ld * m, offG = offC + 2 * (size_t)m,
                  offY = offG + 2 * (size_t)(m + 1), offS = offY + 2 * (size_t)m, offGS = offS + 16;
     double* wsp = nullptr;
-    CLG_TRY(workspace(ctx, offGS + 40, &wsp));
+    CLG_TRY(workspace(ctx, offGS + kArnScratch, &wsp));
     double* H = wsp + offH;
-    double* c2 = wsp + offC;
     double* g = wsp + offG;
     double* y = wsp + offY;
     double* sc = wsp + offS;  // sc[0] |r|^2, sc[1] |w|^2, sc[2..3] deviation, sc[4] |w|^2 before GS
-    int* reorth = reinterpret_cast<int*>(wsp + offS + 8);
     const bool shifted = A.shiftable();
     out = GmresOut();
     for (int cyc = 0; cyc < max_cycles; ++cyc) {
@@ -284,12 +257,7 @@ int gmres(clg_context ctx, LinOp& A, clg_field_s* x, const clg_field_s* b, int m
                                      wsp + offGS));
                 continue;
             }
-            CLG_TRY(blas_multidot_dev(ctx, V.data(), j + 1, w, Hj, sc + 4, nullptr));
-            CLG_TRY(blas_multiaxpy_dev(ctx, w, V.data(), j + 1, Hj, -1.0, sc + 1, nullptr, sc + 4, 0.5, reorth));
-            CLG_TRY(blas_multidot_dev(ctx, V.data(), j + 1, w, c2, nullptr, reorth));
-            CLG_TRY(blas_multiaxpy_dev(ctx, w, V.data(), j + 1, c2, -1.0, sc + 1, reorth, nullptr, 0.0, nullptr));
-            k_hess_update<<<1, 64, 0, ctx->stream>>>(H, ld, j, c2, sc + 1, reorth, shifted ? 1 : 0);
-            if (j < m - 1) CLG_TRY(blas_scale_inv_sqrt(ctx, V[j + 1], w, sc + 1));
+            CLG_TRY(arn_step(ctx, V.data(), j + 1, w, Hj, j, shifted, j < m - 1 ? V[j + 1] : nullptr, wsp + offGS));
         }
         k_gmres_lsq<<<1, 64, 0, ctx->stream>>>(H, ld, m, sc, g, y, sc + 2);
         CLG_HIP(hipGetLastError());

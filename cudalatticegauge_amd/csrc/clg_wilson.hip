@@ -43,6 +43,7 @@ struct HopArgs {
     int xcd, pinter;     // work decomposition (clgd::map_site)
     int spt;             // logical blocks (sites per thread) per workgroup
     int tile_y, tile_ymaj;  // tiled plane schedule (clgd::tile_block) when tile_y > 0
+    int nt_out;             // output written with the non-temporal policy
 };
 
 // Neighbour spinors come from the input field, or (STAGED, z and t only) from the copy of this
@@ -150,7 +151,8 @@ __global__ __launch_bounds__(256) void k_wilson_hop(HopArgs a) {
 #pragma unroll
         for (int k = 0; k < 12; ++k) o[k] = cmul(ca, acc[k]);
     }
-    SIO::store(a.out, a.outn, a.g.Vh, eo, no, o);
+    if (a.nt_out) SIO::template store<R, true>(a.out, a.outn, a.g.Vh, eo, no, o);
+    else SIO::store(a.out, a.outn, a.g.Vh, eo, no, o);
     if constexpr (STAGED) __syncthreads();  // the next block rewrites the stage
     }
 }
@@ -165,12 +167,15 @@ struct HopCfg {
                 // 10*TY + m (TY in 2, 4, 8; m = 0 x-major, 1 y-major): tiled plane schedule, staged
                 // launches only (clgd::tile_block), parities interleaved
     int spt = 1;  // sites per thread: consecutive logical blocks per workgroup (1, 2 or 4)
+    int nt = 0;   // 1: the output spinor is stored with the non-temporal policy (it is not read again by
+                  // the launch, so it need not take L2 space from the neighbour spinors and links)
 };
 inline bool tiled_xcd(int xcd) { return xcd >= 20 && xcd <= 81 && (xcd / 10 == 2 || xcd / 10 == 4 || xcd / 10 == 8) && xcd % 10 <= 1; }
 
 template <typename R, int SP, int GP, int RECON, int PM>
-void launch_pm(const HopArgs& a0, bool dag, bool hasx, hipStream_t st, int block, int xcd, int spt) {
+void launch_pm(const HopArgs& a0, bool dag, bool hasx, hipStream_t st, int block, int xcd, int spt, int nt) {
     HopArgs a = a0;
+    a.nt_out = nt;
     unsigned grid;
     if (tiled_xcd(xcd)) {
         a.xcd = 0;
@@ -200,14 +205,14 @@ void launch_pm(const HopArgs& a0, bool dag, bool hasx, hipStream_t st, int block
 
 template <typename R, int SP, int GP, int RECON>
 int launch_t(const HopArgs& a, bool dag, bool hasx, hipStream_t st, HopCfg c) {
-    if (c.pm == 8) launch_pm<R, SP, GP, RECON, 8>(a, dag, hasx, st, c.block, c.xcd, c.spt);
-    else if (c.pm == 7) launch_pm<R, SP, GP, RECON, 7>(a, dag, hasx, st, c.block, c.xcd, c.spt);
-    else if (c.pm == 6) launch_pm<R, SP, GP, RECON, 6>(a, dag, hasx, st, c.block, c.xcd, c.spt);
-    else if (c.pm == 5) launch_pm<R, SP, GP, RECON, 5>(a, dag, hasx, st, c.block, c.xcd, c.spt);
-    else if (c.pm == 4) launch_pm<R, SP, GP, RECON, 4>(a, dag, hasx, st, c.block, c.xcd, c.spt);
-    else if (c.pm == 3) launch_pm<R, SP, GP, RECON, 3>(a, dag, hasx, st, c.block, c.xcd, c.spt);
-    else if (c.pm == 2) launch_pm<R, SP, GP, RECON, 2>(a, dag, hasx, st, c.block, c.xcd, c.spt);
-    else launch_pm<R, SP, GP, RECON, 1>(a, dag, hasx, st, c.block, c.xcd, c.spt);
+    if (c.pm == 8) launch_pm<R, SP, GP, RECON, 8>(a, dag, hasx, st, c.block, c.xcd, c.spt, c.nt);
+    else if (c.pm == 7) launch_pm<R, SP, GP, RECON, 7>(a, dag, hasx, st, c.block, c.xcd, c.spt, c.nt);
+    else if (c.pm == 6) launch_pm<R, SP, GP, RECON, 6>(a, dag, hasx, st, c.block, c.xcd, c.spt, c.nt);
+    else if (c.pm == 5) launch_pm<R, SP, GP, RECON, 5>(a, dag, hasx, st, c.block, c.xcd, c.spt, c.nt);
+    else if (c.pm == 4) launch_pm<R, SP, GP, RECON, 4>(a, dag, hasx, st, c.block, c.xcd, c.spt, c.nt);
+    else if (c.pm == 3) launch_pm<R, SP, GP, RECON, 3>(a, dag, hasx, st, c.block, c.xcd, c.spt, c.nt);
+    else if (c.pm == 2) launch_pm<R, SP, GP, RECON, 2>(a, dag, hasx, st, c.block, c.xcd, c.spt, c.nt);
+    else launch_pm<R, SP, GP, RECON, 1>(a, dag, hasx, st, c.block, c.xcd, c.spt, c.nt);
     CLG_HIP(hipGetLastError());
     return CLG_OK;
 }
@@ -281,22 +286,22 @@ bool valid_cfg(int pm, int blk, int xcd, int spt) {
 }
 // process-wide overrides set through clg_autotune_enable / clg_autotune_force
 std::atomic<int> g_tune_mode{-1};  // -1: CLG_TUNE environment (default on), 0 off, 1 on
-std::atomic<int> g_forced{0};      // packed pin | block << 4 | xcd << 16 | spt << 24, 0 = none
-// CLG_TUNE_FORCE="pin,block,xcd[,spt]" (or clg_autotune_force) fixes the launch configuration of
+std::atomic<int> g_forced{0};      // packed pin | block << 4 | xcd << 16 | spt << 24 | nt << 28, 0 = none
+// CLG_TUNE_FORCE="pin,block,xcd[,spt[,nt]]" (or clg_autotune_force) fixes the launch configuration of
 // every hop launch (used to profile one configuration under counters, where event timings are
 // not representative)
 bool forced_cfg(HopCfg& c) {
     const int f = g_forced.load();
     if (f) {
-        c = HopCfg{f & 15, (f >> 4) & 4095, (f >> 16) & 255, f >> 24};
+        c = HopCfg{f & 15, (f >> 4) & 4095, (f >> 16) & 255, (f >> 24) & 7, (f >> 28) & 1};
         return true;
     }
     const char* e = std::getenv("CLG_TUNE_FORCE");
     if (!e) return false;
-    int pm = 0, blk = 0, xcd = 0, spt = 1;
-    const int got = std::sscanf(e, "%d,%d,%d,%d", &pm, &blk, &xcd, &spt);
-    if (got < 3 || !valid_cfg(pm, blk, xcd, spt)) return false;
-    c = HopCfg{pm, blk, xcd, spt};
+    int pm = 0, blk = 0, xcd = 0, spt = 1, nt = 0;
+    const int got = std::sscanf(e, "%d,%d,%d,%d,%d", &pm, &blk, &xcd, &spt, &nt);
+    if (got < 3 || !valid_cfg(pm, blk, xcd, spt) || (nt != 0 && nt != 1)) return false;
+    c = HopCfg{pm, blk, xcd, spt, nt};
     return true;
 }
 
@@ -319,12 +324,14 @@ int launch_hop(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool 
             for (const auto& e : ent) {
                 const std::vector<long>& v = e.first;
                 const std::vector<long>& c = e.second;
-                if (v.size() != 10 || c.size() != 4 || !valid_cfg((int)c[0], (int)c[1], (int)c[2], (int)c[3])) continue;
+                if (v.size() != 10 || (c.size() != 4 && c.size() != 5) ||
+                    !valid_cfg((int)c[0], (int)c[1], (int)c[2], (int)c[3]))
+                    continue;
                 TuneKey t;
                 std::memset(&t, 0, sizeof(t));
                 t.prec = (int)v[0]; t.gprec = (int)v[1]; t.recon = (int)v[2]; t.dag = (int)v[3]; t.hasx = (int)v[4];
                 t.nblk = (int)v[5]; t.L0 = (int)v[6]; t.L1 = (int)v[7]; t.L2 = (int)v[8]; t.L3 = (int)v[9];
-                tune_table()[t] = HopCfg{(int)c[0], (int)c[1], (int)c[2], (int)c[3]};
+                tune_table()[t] = HopCfg{(int)c[0], (int)c[1], (int)c[2], (int)c[3], c.size() == 5 ? (int)(c[4] != 0) : 0};
             }
             tune_loaded() = cpath;
             it = tune_table().find(k);
@@ -357,7 +364,10 @@ int launch_hop(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool 
             for (int TY : {2, 4, 8})
                 for (int m : {0, 1})
                     if (a.g.L[0] % (8 / TY) == 0 && a.g.L[1] % TY == 0)
-                        for (int pm = 5; pm <= 8; ++pm) all.push_back(HopCfg{pm, sb, 10 * TY + m});
+                        for (int pm = 5; pm <= 8; ++pm) {
+                            all.push_back(HopCfg{pm, sb, 10 * TY + m});
+                            if (m == 0 && TY >= 4) all.push_back(HopCfg{pm, sb, 10 * TY + m, 1, 1});
+                        }
         }
         for (const HopCfg& c : all) {
             if (c.xcd == 1 && a.nblk != 2) continue;  // one parity: the same launch as xcd 8
@@ -371,19 +381,19 @@ int launch_hop(const HopArgs& a, int prec, int gprec, int recon, bool dag, bool 
             if (ms < best) { best = ms; cfg = c; }
             const char* v2 = std::getenv("CLG_TUNE_VERBOSE");
             if (v2 && v2[0] == '2')  // every candidate
-                std::fprintf(stderr, "[clg tune]   candidate pin=%d block=%d xcd=%d spt=%d: %.1f us\n", c.pm, c.block,
-                             c.xcd, c.spt, ms * 1e3 / 3);
+                std::fprintf(stderr, "[clg tune]   candidate pin=%d block=%d xcd=%d spt=%d nt=%d: %.1f us\n", c.pm,
+                             c.block, c.xcd, c.spt, c.nt, ms * 1e3 / 3);
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         const char* verbose = std::getenv("CLG_TUNE_VERBOSE");
         if (verbose && (verbose[0] == '1' || verbose[0] == '2'))
-            std::fprintf(stderr, "[clg tune] wilson hop prec=%d gprec=%d recon=%d dag=%d x=%d nblk=%d L=%dx%dx%dx%d -> pin=%d block=%d xcd=%d spt=%d (%.1f us)\n",
+            std::fprintf(stderr, "[clg tune] wilson hop prec=%d gprec=%d recon=%d dag=%d x=%d nblk=%d L=%dx%dx%dx%d -> pin=%d block=%d xcd=%d spt=%d nt=%d (%.1f us)\n",
                          prec, gprec, recon, (int)dag, (int)hasx, a.nblk, a.g.L[0], a.g.L[1], a.g.L[2], a.g.L[3],
-                         cfg.pm, cfg.block, cfg.xcd, cfg.spt, best * 1e3 / 3);
+                         cfg.pm, cfg.block, cfg.xcd, cfg.spt, cfg.nt, best * 1e3 / 3);
         std::lock_guard<std::mutex> lk(tune_mutex());
         tune_table()[k] = cfg;
-        tune_cache_append(cpath, "wilson_hop", key_vec(k), {cfg.pm, cfg.block, cfg.xcd, cfg.spt});
+        tune_cache_append(cpath, "wilson_hop", key_vec(k), {cfg.pm, cfg.block, cfg.xcd, cfg.spt, cfg.nt});
     }
     return launch_cfg(a, prec, gprec, recon, dag, hasx, st, cfg);
 }
@@ -595,7 +605,8 @@ int clg_autotune_force(int pin_mode, int block, int xcd, int sites_per_thread) {
     }
     if (sites_per_thread == 0) sites_per_thread = 1;
     CLG_CHECK_ARG(valid_cfg(pin_mode, block, xcd, sites_per_thread),
-                  "pin_mode 1..8, block 64|128|256, xcd 0|1|8, sites_per_thread 1|2|4");
+                  "pin_mode 1..8, block 64|128|256, xcd 0|1|8 or a tiled plane schedule 20|21|40|41|80|81, "
+                  "sites_per_thread 1|2|4");
     g_forced.store(pin_mode | block << 4 | xcd << 16 | sites_per_thread << 24);
     return CLG_OK;
 }
@@ -631,9 +642,9 @@ int clg_autotune_entry(int index, char* buf, int len) {
             const HopCfg& c = it->second;
             std::snprintf(buf, (size_t)len,
                           "wilson_hop prec=%d gauge_prec=%d recon=%d dagger=%d x=%d parities=%d dims=%dx%dx%dx%d "
-                          "-> pin=%d block=%d xcd=%d spt=%d",
+                          "-> pin=%d block=%d xcd=%d spt=%d nt=%d",
                           k.prec, k.gprec, k.recon, k.dag, k.hasx, k.nblk, k.L0, k.L1, k.L2, k.L3, c.pm, c.block, c.xcd,
-                          c.spt);
+                          c.spt, c.nt);
             return CLG_OK;
         }
         index -= (int)tune_table().size();
