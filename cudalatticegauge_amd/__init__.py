@@ -47,7 +47,7 @@ EXPORTED = [
     "clg_gauge_normalize", "clg_gauge_plaquette_energy", "clg_gauge_force", "clg_wilson_prepare_hmc",
     "clg_ks_prepare_hmc", "clg_hmc_params_default", "clg_hmc_evolve", "clg_hmc_trajectory",
     "clg_wilson_hop_batch", "clg_solve_batch", "clg_gcrodr_params_default", "clg_gcrodr_create",
-    "clg_gcrodr_destroy", "clg_gcrodr_reset", "clg_gcrodr_solve", "clg_multidot", "clg_multidot_noalloc", "clg_multiaxpy",
+    "clg_gcrodr_destroy", "clg_gcrodr_reset", "clg_gcrodr_solve", "clg_multidot", "clg_multidot_noalloc", "clg_multiaxpy", "clg_gram_matrix",
     "clg_vector_multiply_matrix", "clg_autotune_enable", "clg_autotune_force", "clg_autotune_reset",
     "clg_autotune_count", "clg_autotune_entry", "clg_graph_begin", "clg_graph_end", "clg_graph_launch",
     "clg_graph_nodes", "clg_graph_destroy", "clg_gauge_transform_random", "clg_gauge_transform_apply",
@@ -193,6 +193,7 @@ def lib():
             "clg_gcrodr_solve": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, vp, C.POINTER(SolveReport)]),
             "clg_multidot": (C.c_int, [vp, C.c_int, C.POINTER(vp), vp, dp]),
             "clg_multidot_noalloc": (C.c_int, [vp, C.c_int, C.POINTER(vp), vp, vp]),
+            "clg_gram_matrix": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.c_int, C.POINTER(vp), dp]),
             "clg_multiaxpy": (C.c_int, [vp, vp, C.c_int, C.POINTER(vp), dp]),
             "clg_vector_multiply_matrix": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.c_int, C.POINTER(vp), dp]),
             "clg_autotune_enable": (C.c_int, [C.c_int]),
@@ -450,6 +451,14 @@ def multidot(ctx, V, w):
     out = (C.c_double * (2 * len(V)))()
     check(lib().clg_multidot(ctx.h, len(V), _handles(V), w.h, out))
     return np.array(out[0::2]) + 1j * np.array(out[1::2])
+
+
+def gram_matrix(ctx, A, B):
+    """[[<A_i, B_j>]] for FP64 fields (matrix cores, FP64 sums)."""
+    out = (C.c_double * (2 * len(A) * len(B)))()
+    check(lib().clg_gram_matrix(ctx.h, len(A), _handles(A), len(B), _handles(B), out))
+    z = np.array(out[0::2]) + 1j * np.array(out[1::2])
+    return z.reshape(len(A), len(B))
 
 
 def multiaxpy(ctx, w, V, coeffs):

@@ -18,7 +18,7 @@
 // (the reference's _kernelMatrixMultiply plus FieldSolveY's k^2 axpys, CFieldFermionWilsonSquareSU3.cu:1525-1606).
 // The small dense problems (least squares, QR, eigenproblems; m <= 31) run in FP64 on the host.
 #include "clg_internal.h"
-#include "clg_smallla.h"
+#include "clg_smallla_dev.h"
 
 #include <chrono>
 #include <cmath>
@@ -27,8 +27,6 @@
 
 using namespace clg;
 using namespace clgd;
-using clg::sla::cx;
-using clg::sla::Mat;
 
 namespace clg {
 int blas_axpby(clg_context, clg_field_s*, double, double, const clg_field_s*, double, double);
@@ -44,6 +42,9 @@ int linop_apply(clg_context, int, int, const clg_field_s*, double, clg_field_s*,
 int wilson_hop_half(clg_context, clg_field_s*, const clg_field_s*, const clg_field_s*, bool, double, double,
                     const clg_field_s*, double, double);
 int wilson_schur_once(clg_context, clg_field_s*, const clg_field_s*, const clg_field_s*, double, bool, clg_field_s*);
+int arn_step(clg_context, clg_field_s* const*, int, clg_field_s*, double*, int, bool, clg_field_s*, double*);
+int blas_gram_dev(clg_context, clg_field_s* const*, int, clg_field_s* const*, int, double*, int);
+int blas_scale_inv_sqrt(clg_context, clg_field_s*, const clg_field_s*, const double*);
 }  // namespace clg
 
 struct clg_gcrodr_s {
@@ -59,24 +60,157 @@ namespace {
 
 constexpr int kMaxDimDR = 31;  // MDim limit of the reference (_kMaxDimDR, CSolverGCRODR.h)
 
-// column j of the device G (column-major, ld rows): rows 0..nr-1 = c1 (+ c2 when the second pass
-// ran), row j+1 = |w|
-__global__ void k_gcol(double* G, int ld, int j, int nr, const double* c1, const double* c2, const int* flag,
-                       const double* wn2) {
-    const int i = threadIdx.x;
-    if (i < nr) {
-        double re = c1[2 * i], im = c1[2 * i + 1];
-        if (*flag) {
-            re += c2[2 * i];
-            im += c2[2 * i + 1];
+// ------------------------------------------------------------------ the small problems of a cycle, on the device
+// One workgroup solves a cycle's dense problems on the (m+1) x m matrix G the Arnoldi steps left in
+// device memory (column-major, ld m+1): the least squares for y, the residual update coefficients, the
+// deflation eigenvectors P (FindPk1 / FindPk2, CSolverGCRODR.cpp:541-700, CLinearAlgebraHelper.h:
+// 150-237) and the QR of G P with the matrices GenerateCU multiplies the basis by.  Only |r| goes back
+// to the host, for the stop test.
+struct CycleArgs {
+    const double* G;   // (m+1) x m, column-major
+    int m, k, mode;    // mode 0: after the first GMRES cycle (H = G, rhs beta e_0); 1: GCRO-DR cycle
+    int deflate;       // CLG_DEFLATE_REV / HEV / SVD
+    const double* nrm;  // mode 1: |U_i|^2 before the normalisation (the diagonal block of G is 1/|U_i|)
+    const double* r2;   // mode 0: beta^2; mode 1: |r|^2 (rhs = sqrt(.) e_k)
+    const double* UU;   // k x k, row-major ld 32 (mode 1 SVD and REV)
+    const double* WU;   // (m+1) x k (mode 1 HEV)
+    const double* UW;   // k x (m+1) (mode 1 REV)
+    double* y;          // m coefficients of the solution update
+    double* cw;         // m+1 coefficients of the residual update: Wr += sum_i cw_i W_i
+    double* MU;         // m x k row-major: the matrix U is recomputed with (P R^-1)
+    double* MC;         // (m+1) x k row-major: the matrix C is recomputed with (Q)
+    double* status;     // 0 on success, 1 if the deflation eigenproblem failed
+};
+
+__device__ __forceinline__ clgd::sld::cz ldz(const double* p, size_t e) { return clgd::sld::cz{p[2 * e], p[2 * e + 1]}; }
+__device__ __forceinline__ void stz(double* p, size_t e, clgd::sld::cz v) {
+    p[2 * e] = v.x;
+    p[2 * e + 1] = v.y;
+}
+
+__global__ __launch_bounds__(256) void k_gcro_cycle(CycleArgs a) {
+    using namespace clgd::sld;
+    __shared__ cz mats[9][kLD * kLD];
+    __shared__ cz vy[kLD], vr[kLD];
+    __shared__ double red[4];
+    cz *Gs = mats[0], *A = mats[1], *B = mats[2], *P = mats[3];
+    cz *S0 = mats[4], *S1 = mats[5], *S2 = mats[6], *S3 = mats[7], *S4 = mats[8];
+    const int m = a.m, k = a.k, t = threadIdx.x, T = blockDim.x;
+    for (int e = t; e < (m + 1) * m; e += T) {
+        const int i = e / m, j = e % m;
+        Gs[i * kLD + j] = ldz(a.G, (size_t)j * (m + 1) + i);
+    }
+    __syncthreads();
+    if (a.mode == 1) {
+        // the recycled block: G(i, j < k) = 0 but G(i, i) = 1 / |U_i| (NormUkAndSetD)
+        for (int e = t; e < k * k; e += T) {
+            const int i = e / k, j = e % k;
+            Gs[i * kLD + j] = i == j ? cz{1.0 / sqrt(a.nrm[i]), 0.0} : zz();
         }
-        G[2 * (j * ld + i)] = re;
-        G[2 * (j * ld + i) + 1] = im;
+        __syncthreads();
     }
-    if (i == 0) {
-        G[2 * (j * ld + j + 1)] = sqrt(wn2[0]);
-        G[2 * (j * ld + j + 1) + 1] = 0.0;
+    // least squares min |rhs - G y|, rhs = beta e_0 or |r| e_k
+    const int ir = a.mode == 0 ? 0 : k;
+    const double beta = sqrt(a.r2[0]);
+    copy(S0, Gs, m + 1, m);
+    for (int i = t; i <= m; i += T) vr[i] = i == ir ? cz{beta, 0.0} : zz();
+    __syncthreads();
+    hessenberg_lsq(S0, vr, vy, m, red);
+    for (int j = t; j < m; j += T) stz(a.y, j, vy[j]);
+    // residual update: mode 0 Wr = -(G y - beta e_0) over W; mode 1 Wr = |r| v_0 - (G y) over W
+    for (int i = t; i <= m; i += T) {
+        cz s = zz();
+        for (int j = 0; j < m; ++j) cfma(s, Gs[i * kLD + j], vy[j]);
+        if (a.mode == 0 && i == 0) s = csub(s, cz{beta, 0.0});
+        stz(a.cw, i, cz{-s.x, -s.y});
     }
+    __syncthreads();
+    // deflation vectors P (m x k)
+    bool ok = true;
+    if (a.mode == 0) {
+        if (a.deflate == CLG_DEFLATE_REV) {
+            copy(A, Gs, m, m);
+            ok = select_eigvecs(A, nullptr, false, m, k, true, P, S0, S1, S2, S3, S4, red);
+        } else if (a.deflate == CLG_DEFLATE_HEV) {
+            copy(A, Gs, m, m);
+            matmul(B, Gs, Gs, m, m + 1, m, true);  // H^H H
+            ok = select_eigvecs(A, B, false, m, k, false, P, S0, S1, S2, S3, S4, red);
+        } else {
+            matmul(A, Gs, Gs, m, m + 1, m, true);
+            ok = select_eigvecs(A, nullptr, true, m, k, true, P, S0, S1, S2, S3, S4, red);
+        }
+    } else if (a.deflate == CLG_DEFLATE_SVD) {
+        // G^H G p = lam B p, B = the Gram matrix of [U, V] (identity but for the U U^H block)
+        matmul(A, Gs, Gs, m, m + 1, m, true);
+        for (int e = t; e < m * m; e += T) {
+            const int i = e / m, j = e % m;
+            B[i * kLD + j] = (i < k && j < k && i != j) ? ldz(a.UU, (size_t)i * kLD + j) : (i == j ? one() : zz());
+        }
+        __syncthreads();
+        ok = select_eigvecs(A, B, true, m, k, true, P, S0, S1, S2, S3, S4, red);
+    } else if (a.deflate == CLG_DEFLATE_HEV) {
+        // ALeft ((m+1) x m): columns j < k from W^H U, identity elsewhere; G^H AL p = lam G^H G p
+        for (int e = t; e < (m + 1) * m; e += T) {
+            const int i = e / m, j = e % m;
+            S0[i * kLD + j] = j < k ? ldz(a.WU, (size_t)i * kLD + j) : (i == j ? one() : zz());
+        }
+        __syncthreads();
+        matmul(A, Gs, S0, m, m + 1, m, true);
+        matmul(B, Gs, Gs, m, m + 1, m, true);
+        ok = select_eigvecs(A, B, false, m, k, false, P, S0, S1, S2, S3, S4, red);
+    } else {
+        // REV: ALeft (m x (m+1)) rows i < k from U^H W, identity elsewhere; B the Gram matrix of [U, V]
+        for (int e = t; e < m * (m + 1); e += T) {
+            const int i = e / (m + 1), j = e % (m + 1);
+            S0[i * kLD + j] = i < k ? ldz(a.UW, (size_t)i * kLD + j) : (i == j ? one() : zz());
+        }
+        __syncthreads();
+        for (int e = t; e < m * m; e += T) {
+            const int i = e / m, j = e % m;
+            cz v;
+            if (i < k && j < k) v = i == j ? one() : ldz(a.UU, (size_t)i * kLD + j);
+            else if (i < k) v = S0[i * kLD + j];
+            else if (j < k) v = cconj(S0[j * kLD + i]);
+            else v = i == j ? one() : zz();
+            B[i * kLD + j] = v;
+        }
+        __syncthreads();
+        matmul(A, S0, Gs, m, m + 1, m, false);
+        ok = select_eigvecs(A, B, false, m, k, true, P, S0, S1, S2, S3, S4, red);
+    }
+    if (!ok) {
+        if (t == 0) a.status[0] = 1.0;
+        return;
+    }
+    // GenerateCU: G P = Q R; U <- basis P R^-1, C <- W Q
+    matmul(A, Gs, P, m + 1, m, k, false);
+    thin_qr(A, B, S0, m + 1, k, red);
+    upper_inverse(S0, S1, k);
+    matmul(S2, P, S1, m, k, k, false);
+    for (int e = t; e < m * k; e += T) stz(a.MU, e, S2[(e / k) * kLD + e % k]);
+    for (int e = t; e < (m + 1) * k; e += T) stz(a.MC, e, B[(e / k) * kLD + e % k]);
+    if (t == 0) a.status[0] = 0.0;
+}
+
+// GenerateCUFirstTime's R: R(j, i) = the two Gram-Schmidt passes' projections, R(i, i) = |C_i'|; the
+// matrix U is multiplied by (R^-1, row-major k x k)
+__global__ __launch_bounds__(64) void k_gcro_rinv(int k, const double* proj, const double* nrm, double* Rinv) {
+    using namespace clgd::sld;
+    __shared__ cz R[kLD * kLD], X[kLD * kLD];
+    for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
+        const int j = e / k, i = e % k;  // R(j, i)
+        cz v = zz();
+        if (j < i) v = cadd(ldz(proj, ((size_t)0 * kLD + i) * kLD + j), ldz(proj, ((size_t)1 * kLD + i) * kLD + j));
+        else if (j == i) v = cz{sqrt(nrm[i]), 0.0};
+        R[j * kLD + i] = v;
+    }
+    __syncthreads();
+    upper_inverse(R, X, k);
+    for (int e = threadIdx.x; e < k * k; e += blockDim.x) stz(Rinv, e, X[(e / k) * kLD + e % k]);
+}
+
+__global__ void k_fill0(double* p, int n) {
+    for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < n; i += blockDim.x * gridDim.x) p[i] = 0.0;
 }
 
 struct Gcro {
@@ -91,11 +225,33 @@ struct Gcro {
     clg_field_s* todd = nullptr;
     std::vector<clg_field_s*> C, V;  // C[0..k), V[0..m-k]
     clg_field_s *X = nullptr, *Wr = nullptr, *tmp = nullptr;
-    double* dws = nullptr;  // device scratch: G, coefficient vectors, scalars
-    size_t offG = 0, offC1 = 0, offC2 = 0, offS = 0, offM = 0;
+    double* dws = nullptr;  // device scratch: G, coefficients, Gram matrices, scalars
+    // offsets (doubles) into dws
+    size_t offG = 0, offY = 0, offCW = 0, offMU = 0, offMC = 0, offUU = 0, offWU = 0, offUW = 0, offNrm = 0,
+           offR2 = 0, offSt = 0, offC1 = 0, offProj = 0, offMX = 0, offS = 0, offA = 0, end = 0;
     int applies = 0, cycles = 0;
     double dev = 0;
 
+    void layout() {
+        constexpr size_t M32 = 2 * 32 * 32;  // one 32 x 32 complex matrix
+        offG = 0;
+        offY = offG + 2 * (size_t)(m + 1) * m;
+        offCW = offY + 2 * 32;
+        offMU = offCW + 2 * 33;
+        offMC = offMU + M32;
+        offUU = offMC + M32;
+        offWU = offUU + M32;
+        offUW = offWU + M32;
+        offNrm = offUW + M32;
+        offR2 = offNrm + 32;
+        offSt = offR2 + 4;
+        offC1 = offSt + 4;
+        offProj = offC1 + 2 * 32;
+        offMX = offProj + 2 * M32;
+        offS = offMX + M32;
+        offA = offS + 16;
+        end = offA + kArnScratch;
+    }
     clg_field_s* W(int j) { return j < k ? C[j] : V[j - k]; }           // GetW
     clg_field_s* Vv(int j) { return j < k ? S->U[j] : V[j - k]; }       // GetV
 
@@ -104,6 +260,7 @@ struct Gcro {
         if (schur) return wilson_schur_once(ctx, out, in, gauge, param, op == EFO_F_Ddagger, todd);
         return linop_apply(ctx, p->kind, op, gauge, param, out, in);
     }
+    // |f| on the host (the stop tests)
     int norm(const clg_field_s* f, double& out) {
         CLG_TRY(blas_norm2_dev(ctx, f, dws + offS));
         double v;
@@ -111,180 +268,93 @@ struct Gcro {
         out = std::sqrt(v);
         return CLG_OK;
     }
-    int scale(clg_field_s* f, double a) { return blas_axpby(ctx, f, a, 0.0, f, 0.0, 0.0); }
-    int dot(const clg_field_s* a, const clg_field_s* b, cx& out) {
-        CLG_TRY(blas_dot_dev(ctx, a, b, dws + offS));
-        double z[2];
-        CLG_TRY(fetch_doubles(ctx, dws + offS, 2, z));
-        out = cx(z[0], z[1]);
-        return CLG_OK;
-    }
-    int upload(const std::vector<cx>& v, double* dst) {
-        CLG_HIP(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(cx), hipMemcpyHostToDevice, ctx->stream));
-        CLG_HIP(hipStreamSynchronize(ctx->stream));  // the host vector is a temporary
-        return CLG_OK;
-    }
-    // y += s * sum_j c_j vecs_j with host coefficients
-    int axpys(clg_field_s* y, const std::vector<clg_field_s*>& vecs, const std::vector<cx>& c, double s) {
-        if (vecs.empty()) return CLG_OK;
-        CLG_TRY(upload(c, dws + offC1));
-        return blas_multiaxpy_dev(ctx, y, vecs.data(), (int)vecs.size(), dws + offC1, s, nullptr, nullptr, nullptr, 0.0,
-                                  nullptr);
-    }
-    // out_i = sum_j in_j M(j, i), in place allowed when out are the first inputs
-    int vmm(const std::vector<clg_field_s*>& in, const std::vector<clg_field_s*>& out, const Mat& M) {
-        CLG_CHECK_ARG(M.r == (int)in.size() && M.c == (int)out.size(), "bad VMM shape");
-        std::vector<cx> mv(M.a.begin(), M.a.end());
-        CLG_TRY(upload(mv, dws + offM));
-        return blas_vmm_dev(ctx, in.data(), (int)in.size(), out.data(), (int)out.size(), dws + offM);
-    }
-    // w <- w - sum_j W_j (W_j^H w) over W_0..W_{nw-1} with a conditional second pass; column j of G
-    int orthogonalise(clg_field_s* w, int nw, int j) {
-        std::vector<clg_field_s*> list(nw);
-        for (int i = 0; i < nw; ++i) list[i] = W(i);
-        double* c1 = dws + offC1;
-        double* c2 = dws + offC2;
-        double* sc = dws + offS;  // sc[1] |w'|^2, sc[4] |w|^2
-        int* flag = reinterpret_cast<int*>(sc + 8);
-        CLG_TRY(blas_multidot_dev(ctx, list.data(), nw, w, c1, sc + 4, nullptr));
-        CLG_TRY(blas_multiaxpy_dev(ctx, w, list.data(), nw, c1, -1.0, sc + 1, nullptr, sc + 4, 0.5, flag));
-        CLG_TRY(blas_multidot_dev(ctx, list.data(), nw, w, c2, nullptr, flag));
-        CLG_TRY(blas_multiaxpy_dev(ctx, w, list.data(), nw, c2, -1.0, sc + 1, flag, nullptr, 0.0, nullptr));
-        k_gcol<<<1, 64, 0, ctx->stream>>>(dws + offG, m + 1, j, nw, c1, c2, flag, sc + 1);
+    int fill0(double* d, int n) {
+        k_fill0<<<std::max(1, std::min(64, (n + 255) / 256)), 256, 0, ctx->stream>>>(d, n);
         CLG_HIP(hipGetLastError());
         return CLG_OK;
     }
-    int fetch_G(Mat& G) {
-        std::vector<double> g(2 * (size_t)(m + 1) * m);
-        CLG_TRY(fetch_doubles(ctx, dws + offG, (int)g.size(), g.data()));
-        G = Mat(m + 1, m);
-        for (int j = 0; j < m; ++j)
-            for (int i = 0; i <= m; ++i) G(i, j) = cx(g[2 * (j * (m + 1) + i)], g[2 * (j * (m + 1) + i) + 1]);
-        return CLG_OK;
-    }
-    int zero_G() {
-        std::vector<cx> z((size_t)(m + 1) * m, cx(0.0, 0.0));
-        return upload(z, dws + offG);
+    // w <- (w - sum_i W_i (W_i^H w)) / |.| over W_0..W_{nw-1}, with the conditional second pass, in
+    // place (it is the next basis vector); column j of G on the device
+    int orthogonalise(clg_field_s* w, int nw, int j) {
+        std::vector<clg_field_s*> list(nw);
+        for (int i = 0; i < nw; ++i) list[i] = W(i);
+        return arn_step(ctx, list.data(), nw, w, dws + offG + 2 * (size_t)j * (m + 1), j, false, nullptr, dws + offA);
     }
     // r = b - A x
     int residual(clg_field_s* r, const clg_field_s* x, const clg_field_s* b) {
         CLG_TRY(apply(r, x));
         return blas_axpby(ctx, r, 1.0, 0.0, b, -1.0, 0.0);
     }
-    // Gram matrix entries  list_i^H w_j  (rows i, columns j)
-    int gram(const std::vector<clg_field_s*>& rows, const std::vector<clg_field_s*>& cols, Mat& out) {
-        out = Mat((int)rows.size(), (int)cols.size());
-        std::vector<clg_field_s*> r = rows;
-        for (size_t j = 0; j < cols.size(); ++j) {
-            CLG_TRY(blas_multidot_dev(ctx, r.data(), (int)r.size(), cols[j], dws + offC1, nullptr, nullptr));
-            std::vector<double> v(2 * r.size());
-            CLG_TRY(fetch_doubles(ctx, dws + offC1, (int)v.size(), v.data()));
-            for (size_t i = 0; i < r.size(); ++i) out((int)i, (int)j) = cx(v[2 * i], v[2 * i + 1]);
+    // the cycle's dense problems (see k_gcro_cycle)
+    int small(int mode) {
+        if (mode == 1) {
+            std::vector<clg_field_s*> Us(S->U.begin(), S->U.end());
+            if (p->deflate != CLG_DEFLATE_HEV) CLG_TRY(blas_gram_dev(ctx, Us.data(), k, Us.data(), k, dws + offUU, 32));
+            std::vector<clg_field_s*> Ws;
+            for (int j = 0; j <= m; ++j) Ws.push_back(W(j));
+            if (p->deflate == CLG_DEFLATE_HEV) CLG_TRY(blas_gram_dev(ctx, Ws.data(), m + 1, Us.data(), k, dws + offWU, 32));
+            if (p->deflate == CLG_DEFLATE_REV) CLG_TRY(blas_gram_dev(ctx, Us.data(), k, Ws.data(), m + 1, dws + offUW, 32));
         }
+        CycleArgs a;
+        a.G = dws + offG;
+        a.m = m;
+        a.k = k;
+        a.mode = mode;
+        a.deflate = p->deflate;
+        a.nrm = dws + offNrm;
+        a.r2 = dws + offR2;
+        a.UU = dws + offUU;
+        a.WU = dws + offWU;
+        a.UW = dws + offUW;
+        a.y = dws + offY;
+        a.cw = dws + offCW;
+        a.MU = dws + offMU;
+        a.MC = dws + offMC;
+        a.status = dws + offSt;
+        k_gcro_cycle<<<1, 256, 0, ctx->stream>>>(a);
+        CLG_HIP(hipGetLastError());
         return CLG_OK;
     }
-
-    // ---- deflation vectors (FindPk1 after the first GMRES cycle, FindPk2 after GCRO-DR cycles)
-    int find_pk1(const Mat& H, Mat& P) {
-        Mat Hm(m, m);
-        for (int i = 0; i < m; ++i)
-            for (int j = 0; j < m; ++j) Hm(i, j) = H(i, j);
-        bool ok = false;
-        if (p->deflate == CLG_DEFLATE_REV) {
-            ok = sla::select_eigvecs(Hm, nullptr, false, k, true, P);
-        } else if (p->deflate == CLG_DEFLATE_HEV) {
-            const Mat B = sla::mul(sla::adj(H), H);
-            ok = sla::select_eigvecs(Hm, &B, false, k, false, P);
-        } else {
-            ok = sla::select_eigvecs(sla::mul(sla::adj(H), H), nullptr, true, k, true, P);
-        }
-        CLG_CHECK_ARG(ok, "GCRODR: deflation eigenproblem failed");
+    // |Wr| and the status of the cycle's eigenproblem in one read
+    int fetch_dev() {
+        CLG_TRY(blas_norm2_dev(ctx, Wr, dws + offSt + 1));
+        double v[2];
+        CLG_TRY(fetch_doubles(ctx, dws + offSt, 2, v));
+        CLG_CHECK_ARG(v[0] == 0.0, "GCRODR: deflation eigenproblem failed");
+        dev = std::sqrt(v[1]);
         return CLG_OK;
     }
-    int find_pk2(const Mat& G, Mat& P) {
-        std::vector<clg_field_s*> Us(S->U.begin(), S->U.end());
-        bool ok = false;
-        if (p->deflate == CLG_DEFLATE_SVD) {
-            Mat UU;
-            CLG_TRY(gram(Us, Us, UU));
-            Mat B = Mat::eye(m);
-            for (int i = 0; i < k; ++i)
-                for (int j = 0; j < k; ++j)
-                    if (i != j) B(i, j) = UU(i, j);
-            ok = sla::select_eigvecs(sla::mul(sla::adj(G), G), &B, true, k, true, P);
-        } else if (p->deflate == CLG_DEFLATE_HEV) {
-            // ALeft ((m+1) x m): rows of C and V against the columns U, identity elsewhere
-            std::vector<clg_field_s*> rows;
-            for (int i = 0; i <= m; ++i) rows.push_back(W(i));
-            Mat WU;
-            CLG_TRY(gram(rows, Us, WU));
-            Mat AL(m + 1, m);
-            for (int i = 0; i <= m; ++i)
-                for (int j = 0; j < m; ++j) AL(i, j) = j < k ? WU(i, j) : (i == j ? cx(1.0, 0.0) : cx(0.0, 0.0));
-            const Mat A = sla::mul(sla::adj(G), AL), B = sla::mul(sla::adj(G), G);
-            ok = sla::select_eigvecs(A, &B, false, k, false, P);
-        } else {
-            // REV: ALeft (m x (m+1)) = [U^H C, U^H V; 0 I]; B = Gram of [U, V_0..V_{m-k-1}]
-            std::vector<clg_field_s*> cols;
-            for (int j = 0; j <= m; ++j) cols.push_back(W(j));
-            Mat UW, UU;
-            CLG_TRY(gram(Us, cols, UW));
-            CLG_TRY(gram(Us, Us, UU));
-            Mat AL(m, m + 1);
-            for (int i = 0; i < m; ++i)
-                for (int j = 0; j <= m; ++j) AL(i, j) = i < k ? UW(i, j) : (i == j ? cx(1.0, 0.0) : cx(0.0, 0.0));
-            Mat B(m, m);
-            for (int i = 0; i < m; ++i)
-                for (int j = 0; j < m; ++j) {
-                    if (i < k && j < k) B(i, j) = i == j ? cx(1.0, 0.0) : UU(i, j);
-                    else if (i < k) B(i, j) = AL(i, j);
-                    else if (j < k) B(i, j) = std::conj(AL(j, i));
-                    else B(i, j) = i == j ? cx(1.0, 0.0) : cx(0.0, 0.0);
-                }
-            ok = sla::select_eigvecs(sla::mul(AL, G), &B, false, k, true, P);
-        }
-        CLG_CHECK_ARG(ok, "GCRODR: deflation eigenproblem failed");
-        return CLG_OK;
-    }
-    // GenerateCU (CSolverGCRODR.cpp:541-560): G P = Q R; U <- [first m of the basis] P R^-1;
-    // C <- [C, V] Q when update_c
-    int generate_cu(const Mat& G, const Mat& P, bool update_c, bool just_after_gmres) {
-        Mat Q, R;
-        sla::thin_qr(sla::mul(G, P), Q, R);
-        const Mat PR = sla::mul(P, sla::upper_inverse(R));
+    // GenerateCU (CSolverGCRODR.cpp:541-560) with the device matrices: U <- basis (P R^-1); C <- W Q
+    int generate_cu(bool update_c, bool just_after_gmres) {
         std::vector<clg_field_s*> in, outU(S->U.begin(), S->U.end());
         for (int j = 0; j < m; ++j) in.push_back(just_after_gmres ? W(j) : Vv(j));
-        CLG_TRY(vmm(in, outU, PR));
+        CLG_TRY(blas_vmm_dev(ctx, in.data(), m, outU.data(), k, dws + offMU));
         if (update_c) {
             std::vector<clg_field_s*> inC, outC(C.begin(), C.end());
             for (int j = 0; j <= m; ++j) inC.push_back(W(j));
-            CLG_TRY(vmm(inC, outC, Q));
+            CLG_TRY(blas_vmm_dev(ctx, inC.data(), m + 1, outC.data(), k, dws + offMC));
         }
         S->have_u = true;
         return CLG_OK;
     }
-    // GenerateCUFirstTime: C = A U orthonormalised (thin QR by Gram-Schmidt), U <- U R^-1, r = b - A x
+    // GenerateCUFirstTime: C = A U orthonormalised (Gram-Schmidt with a second pass, the projections
+    // kept on the device), U <- U R^-1, r = b - A x
     int generate_cu_first(clg_field_s* x, clg_field_s* r, const clg_field_s* b) {
         for (int i = 0; i < k; ++i) CLG_TRY(apply(C[i], S->U[i]));
-        Mat R(k, k);
         for (int i = 0; i < k; ++i) {
-            for (int pass = 0; pass < 2; ++pass) {
-                if (i == 0) break;
+            for (int pass = 0; pass < 2 && i > 0; ++pass) {
+                double* pr = dws + offProj + 2 * ((size_t)pass * 32 + i) * 32;
                 std::vector<clg_field_s*> prev(C.begin(), C.begin() + i);
-                CLG_TRY(blas_multidot_dev(ctx, prev.data(), i, C[i], dws + offC1, nullptr, nullptr));
-                std::vector<double> v(2 * i);
-                CLG_TRY(fetch_doubles(ctx, dws + offC1, 2 * i, v.data()));
-                for (int j = 0; j < i; ++j) R(j, i) += cx(v[2 * j], v[2 * j + 1]);
-                CLG_TRY(blas_multiaxpy_dev(ctx, C[i], prev.data(), i, dws + offC1, -1.0, nullptr, nullptr, nullptr, 0.0,
-                                           nullptr));
+                CLG_TRY(blas_multidot_dev(ctx, prev.data(), i, C[i], pr, nullptr, nullptr));
+                CLG_TRY(blas_multiaxpy_dev(ctx, C[i], prev.data(), i, pr, -1.0, nullptr, nullptr, nullptr, 0.0, nullptr));
             }
-            double n;
-            CLG_TRY(norm(C[i], n));
-            R(i, i) = n;
-            CLG_TRY(scale(C[i], 1.0 / n));
+            CLG_TRY(blas_norm2_dev(ctx, C[i], dws + offNrm + i));
+            CLG_TRY(blas_scale_inv_sqrt(ctx, C[i], C[i], dws + offNrm + i));
         }
+        k_gcro_rinv<<<1, 64, 0, ctx->stream>>>(k, dws + offProj, dws + offNrm, dws + offMX);
+        CLG_HIP(hipGetLastError());
         std::vector<clg_field_s*> Us(S->U.begin(), S->U.end());
-        CLG_TRY(vmm(Us, Us, sla::upper_inverse(R)));
+        CLG_TRY(blas_vmm_dev(ctx, Us.data(), k, Us.data(), k, dws + offMX));
         CLG_TRY(residual(r, x, b));
         return norm(r, dev);
     }
@@ -295,49 +365,37 @@ struct Gcro {
         CLG_TRY(clg_field_copy(X, const_cast<clg_field_s*>(start ? start : b)));
         const bool reuse = S->have_u && (phase == CLG_PHASE_IN_TRAJECTORY || phase == CLG_PHASE_END_TRAJECTORY);
         clg_field_s* r = V[0];  // the reference keeps r in the first Krylov vector
+        CLG_TRY(fill0(dws + offProj, 2 * 2 * 32 * 32));
         if (reuse) {
             CLG_TRY(generate_cu_first(X, r, b));
         } else {
             // one GMRES(m) cycle with its basis in W (FirstTimeGMERESSolve)
-            CLG_TRY(zero_G());
+            CLG_TRY(fill0(dws + offG, 2 * (m + 1) * m));
             clg_field_s* w0 = W(0);
             CLG_TRY(residual(w0, X, b));
-            double beta;
-            CLG_TRY(norm(w0, beta));
-            CLG_TRY(scale(w0, 1.0 / beta));
+            CLG_TRY(blas_norm2_dev(ctx, w0, dws + offR2));
+            CLG_TRY(blas_scale_inv_sqrt(ctx, w0, w0, dws + offR2));
             for (int j = 0; j < m; ++j) {
                 CLG_TRY(apply(W(j + 1), W(j)));
                 CLG_TRY(orthogonalise(W(j + 1), j + 1, j));
-                double wn;
-                CLG_TRY(fetch_doubles(ctx, dws + offS + 1, 1, &wn));
-                CLG_TRY(scale(W(j + 1), 1.0 / std::sqrt(wn)));
             }
             ++cycles;
-            Mat H;
-            CLG_TRY(fetch_G(H));
-            std::vector<cx> rhs(m + 1, cx(0.0, 0.0)), y;
-            rhs[0] = beta;
-            sla::hessenberg_lsq(H, rhs, y);
+            CLG_TRY(small(0));
             std::vector<clg_field_s*> basis;
             for (int j = 0; j < m; ++j) basis.push_back(W(j));
-            CLG_TRY(axpys(X, basis, y, 1.0));
+            CLG_TRY(blas_multiaxpy_dev(ctx, X, basis.data(), m, dws + offY, 1.0, nullptr, nullptr, nullptr, 0.0, nullptr));
             if (p->recalculate_r > 1) {
                 // r = -sum_j (H y - beta e_0)_j W_j
-                std::vector<cx> hy(m + 1, cx(0.0, 0.0));
-                for (int i = 0; i <= m; ++i)
-                    for (int j = 0; j < m; ++j) hy[i] += H(i, j) * y[j];
-                hy[0] -= beta;
                 std::vector<clg_field_s*> full;
                 for (int j = 0; j <= m; ++j) full.push_back(W(j));
                 CLG_TRY(clg_field_zero(Wr));
-                CLG_TRY(axpys(Wr, full, hy, -1.0));
+                CLG_TRY(blas_multiaxpy_dev(ctx, Wr, full.data(), m + 1, dws + offCW, 1.0, nullptr, nullptr, nullptr, 0.0,
+                                           nullptr));
             } else {
                 CLG_TRY(residual(Wr, X, b));
             }
-            CLG_TRY(norm(Wr, dev));
-            Mat P;
-            CLG_TRY(find_pk1(H, P));
-            CLG_TRY(generate_cu(H, P, dev >= tol, true));
+            CLG_TRY(fetch_dev());
+            CLG_TRY(generate_cu(dev >= tol, true));
             CLG_TRY(clg_field_copy(r, Wr));
         }
         if (dev < tol) {
@@ -354,54 +412,35 @@ struct Gcro {
                 CLG_TRY(blas_multiaxpy_dev(ctx, r, Cs.data(), k, dws + offC1, -1.0, nullptr, nullptr, nullptr, 0.0,
                                            nullptr));
             }
-            // NormUkAndSetD: U_i normalised, D = diag(1/|U_i|) in G
-            CLG_TRY(zero_G());
-            std::vector<double> dinv(k);
+            // NormUkAndSetD: U_i normalised (the diagonal block of G is set from the norms on the device)
+            CLG_TRY(fill0(dws + offG, 2 * (m + 1) * m));
             for (int i = 0; i < k; ++i) {
-                double n;
-                CLG_TRY(norm(S->U[i], n));
-                dinv[i] = 1.0 / n;
-                CLG_TRY(scale(S->U[i], dinv[i]));
+                CLG_TRY(blas_norm2_dev(ctx, S->U[i], dws + offNrm + i));
+                CLG_TRY(blas_scale_inv_sqrt(ctx, S->U[i], S->U[i], dws + offNrm + i));
             }
-            CLG_TRY(norm(r, dev));
-            CLG_TRY(scale(r, 1.0 / dev));
+            // Wr = |r| v_0 (the residual before this cycle's correction), v_0 = r / |r|
+            CLG_TRY(blas_norm2_dev(ctx, r, dws + offR2));
+            CLG_TRY(clg_field_copy(Wr, r));
+            CLG_TRY(blas_scale_inv_sqrt(ctx, r, r, dws + offR2));
             for (int j = k; j < m; ++j) {
                 CLG_TRY(apply(W(j + 1), W(j)));
                 CLG_TRY(orthogonalise(W(j + 1), j + 1, j));
-                double wn;
-                CLG_TRY(fetch_doubles(ctx, dws + offS + 1, 1, &wn));
-                CLG_TRY(scale(W(j + 1), 1.0 / std::sqrt(wn)));
             }
             ++cycles;
-            Mat G;
-            CLG_TRY(fetch_G(G));
-            for (int i = 0; i < k; ++i) {
-                for (int j = 0; j < m; ++j)
-                    if (j < k) G(i, j) = cx(0.0, 0.0);
-                G(i, i) = dinv[i];
-            }
-            // Wr = dev * V_0 (the residual before this cycle's correction)
-            CLG_TRY(blas_axpby(ctx, Wr, dev, 0.0, r, 0.0, 0.0));
-            std::vector<cx> rhs(m + 1, cx(0.0, 0.0)), y;
-            rhs[k] = dev;
-            sla::hessenberg_lsq(G, rhs, y);
+            CLG_TRY(small(1));
             std::vector<clg_field_s*> basis;
             for (int j = 0; j < m; ++j) basis.push_back(Vv(j));
-            CLG_TRY(axpys(X, basis, y, 1.0));
+            CLG_TRY(blas_multiaxpy_dev(ctx, X, basis.data(), m, dws + offY, 1.0, nullptr, nullptr, nullptr, 0.0, nullptr));
             if (p->recalculate_r != 0 && it != 0 && it % p->recalculate_r == 0) {
                 CLG_TRY(residual(Wr, X, b));
             } else {
-                std::vector<cx> gy(m + 1, cx(0.0, 0.0));
-                for (int i = 0; i <= m; ++i)
-                    for (int j = 0; j < m; ++j) gy[i] += G(i, j) * y[j];
                 std::vector<clg_field_s*> full;
                 for (int j = 0; j <= m; ++j) full.push_back(W(j));
-                CLG_TRY(axpys(Wr, full, gy, -1.0));
+                CLG_TRY(blas_multiaxpy_dev(ctx, Wr, full.data(), m + 1, dws + offCW, 1.0, nullptr, nullptr, nullptr, 0.0,
+                                           nullptr));
             }
-            CLG_TRY(norm(Wr, dev));
-            Mat P;
-            CLG_TRY(find_pk2(G, P));
-            if (dev >= tol || phase != CLG_PHASE_END_TRAJECTORY) CLG_TRY(generate_cu(G, P, dev >= tol, false));
+            CLG_TRY(fetch_dev());
+            if (dev >= tol || phase != CLG_PHASE_END_TRAJECTORY) CLG_TRY(generate_cu(dev >= tol, false));
             if (dev < tol) {
                 conv = true;
                 return clg_field_copy(xout, X);
@@ -530,13 +569,8 @@ int clg_gcrodr_solve(clg_gcrodr s, clg_field x, clg_field b, clg_field gauge, do
     CLG_TRY(scratch_field(ctx, b, p->kind, CLG_F64, sub, 30200, &g.X));
     CLG_TRY(scratch_field(ctx, b, p->kind, CLG_F64, sub, 30201, &g.Wr));
     if (g.schur) CLG_TRY(scratch_field(ctx, b, CLG_WILSON, CLG_F64, CLG_ODD, 30202, &g.todd));
-    const int m = g.m;
-    g.offG = 0;
-    g.offC1 = g.offG + 2 * (size_t)(m + 1) * m;
-    g.offC2 = g.offC1 + 2 * (size_t)(m + 2) * (m + 2);
-    g.offS = g.offC2 + 2 * (size_t)(m + 2);
-    g.offM = g.offS + 16;
-    CLG_TRY(workspace(ctx, g.offM + 2 * (size_t)(kMaxDimDR + 1) * kMaxDimDR + 16, &g.dws));
+    g.layout();
+    CLG_TRY(workspace(ctx, g.end, &g.dws));
     // |b|^2 of the full system for the reference stop rule
     double bn2;
     CLG_TRY(blas_norm2_dev(ctx, b, g.dws + g.offS));

@@ -208,6 +208,11 @@ int clg_multiaxpy(clg_context ctx, clg_field w, int n, clg_field* V, const doubl
  * a static 2,162,688-byte device array, one per device, whose address clg_init looks up; every call
  * on a device shares it, so calls that overlap on one device (other streams or threads) race. */
 int clg_multidot_noalloc(clg_context ctx, int n, clg_field* V, clg_field w, double* out);
+/* Gram matrix out[2 (i c + j) .. +1] = <A_i, B_j> (host, interleaved complex) of FP64 fermion fields of one
+ * shape, 1 <= r, c <= 32, on the matrix cores (v_mfma_f64_16x16x4f64, FP64 sums in a fixed order).  No
+ * single counterpart in the reference: GCRO-DR forms these matrices with one CField::Dot per entry
+ * (CSolverGCRODR.cpp:700-760); the solver here uses the same kernels on device data.  Blocks. */
+int clg_gram_matrix(clg_context ctx, int r, clg_field* A, int c, clg_field* B, double* out);
 /* CFieldMatrixOperation::VectorMultiplyMatrix (CField.h:372-380; _kernelMatrixMultiply,
  * CFieldFermionWilsonSquareSU3.cu:1533-1560): with W = (res_0..res_{nres-1}, left_0..left_{nleft-1}),
  * res_i <- sum_j W_j M[j*nres + i], M a host (nres+nleft) x nres complex matrix (interleaved
@@ -480,10 +485,12 @@ int clg_solve_batch(clg_context ctx, const clg_solver_params* params, int n, clg
 int clg_autotune_enable(int mode);
 /* fix every single-system hop launch: pin_mode 1|2 (register pinning per half / per direction), 3|4 (the
  * same with non-temporal link loads), 5..8 (as 1..4 with the z and t neighbours staged in LDS; needs
- * block = L2*L3/2 with L3 even, otherwise the launch runs as 1..4), block 64|128|256, xcd 0 (hardware
- * workgroup order) | 8 (each XCD sweeps a contiguous slab of sites; full D interleaves the parities
- * block by block) | 1 (XCD slabs, parities one after the other), sites_per_thread 1|2|4 (0 = 1:
- * consecutive blocks of sites per workgroup); pin_mode 0 clears */
+ * block = L2*L3/2 with L3 even, and a launch that does not fit is refused), block 64|128|256, xcd 0
+ * (hardware workgroup order) | 8 (each XCD sweeps a contiguous slab of sites; full D interleaves the
+ * parities block by block) | 1 (XCD slabs, parities one after the other) | 10*TY + m with TY in 2, 4, 8
+ * (staged modes only: the (x,y) planes cut into 8 tiles, 8/TY along x and TY along y, XCD k sweeping
+ * tile k x-major (m = 0) or y-major (m = 1)), sites_per_thread 1|2|4 (0 = 1: consecutive blocks of
+ * sites per workgroup); pin_mode 0 clears */
 int clg_autotune_force(int pin_mode, int block, int xcd, int sites_per_thread);
 /* Tuned configurations persist across processes when the environment variable CLG_TUNE_CACHE names a
  * file: every tuned key is appended as one line "<device arch> <tuner> <key> : <configuration>", and
