@@ -420,7 +420,10 @@ def rhmc_trajectory(ctx, clg, np):
         mc, err_mc = fit_rational(nf / 8.0, shifts, mass * mass, 20.0)
         flavours.append((mass, md, mc, max(err_md, err_mc)))
     out = {}
-    for tag, prec in (("fp32", clg.F32), ("fp64", clg.F64)):
+    # the shifted BiCGStab (CMultiShiftBiCGStab, the multi-shift solver of the reference's staggered RHMC
+    # configurations) for both force precisions, and the multi-shift FOM with FP32 forces for comparison
+    for tag, prec, ms_kind in (("fp32", clg.F32, clg.MS_BICGSTAB), ("fp64", clg.F64, clg.MS_BICGSTAB),
+                               ("fom_fp32", clg.F32, clg.MS_FOM)):
         g = clg.Field(ctx, clg.GAUGE, clg.F64, HMC_DIMS)
         g.gauge_random(0.4, 1)
         acts = [dict(kind=clg.ACTION_GAUGE_PLAQUETTE, beta=5.5)]
@@ -431,7 +434,7 @@ def rhmc_trajectory(ctx, clg, np):
             sp = clg.solver_params(kind=clg.KS, max_dim=40, restart=200, absolute_accuracy=1,
                                    accuracy=1e-5 if prec == clg.F32 else 1e-8)
             acts.append(dict(kind=clg.ACTION_FERMION_KS, kappa_or_mass=mass, phi=phi, md=md, mc=mc, solver=sp,
-                             ms_kind=clg.MS_FOM, ms_prec=prec, energy_accuracy=1e-9))
+                             ms_kind=ms_kind, ms_prec=prec, energy_accuracy=1e-9))
         hp = clg.hmc_params(integrator=clg.INT_NESTED_FORCE_GRADIENT, step_length=1.0, steps=8, nested_steps=6,
                             exp_precision=8, metropolis=1)
         rep = clg.hmc_trajectory(ctx, g, hp, acts, 1)
@@ -445,7 +448,8 @@ def rhmc_trajectory(ctx, clg, np):
         del g, phis
     out["rhmc_dH_abs_diff_fp32_vs_fp64"] = float(f"{abs(out['rhmc_fp32_dH'] - out['rhmc_fp64_dH']):.3g}")
     out["rhmc_rational_fit_max_rel_err"] = float(f"{max(f[3] for f in flavours):.2g}")
-    out["rhmc_config"] = "24^3x8 KS 2+1 (Mass 0.01, 0.05), 12 poles, NFG 8x6, beta 5.5, eps 0.4"
+    out["rhmc_config"] = ("24^3x8 KS 2+1 (Mass 0.01, 0.05), 12 poles, NFG 8x6, beta 5.5, eps 0.4; force solves: "
+                          "shifted BiCGStab (fom_fp32: multi-shift FOM, MaxDim 40)")
     return out
 
 

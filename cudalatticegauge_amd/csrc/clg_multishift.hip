@@ -12,8 +12,9 @@
 //     x_i = (A + b_i)^-1 in and coefficients [c, a_1..a_n, b_1..b_n] (CRationalApproximation.h).
 // What differs: the per-shift vector updates of a step run as one batched kernel over all
 // active shifts (the shared vectors are read once per step, not once per shift), converged
-// shifts are compacted out of the launches, the FOM small systems are solved on device, and
-// the shift sums accumulate in FP64 whatever the storage precision of the solutions.
+// shifts are left out of the launches, the FOM small systems are solved on device, the shifted
+// BiCGStab keeps every scalar on the device (clg_msbicg.hip), and the shift sums accumulate in
+// FP64 whatever the storage precision of the solutions.
 #include "clg_internal.h"
 #include "clg_smallla.h"
 
@@ -43,6 +44,9 @@ int linop_apply(clg_context ctx, int kind, int op, const clg_field_s* gauge, dou
                 const clg_field_s* in);
 int gauge_convert(clg_context, clg_field_s*, const clg_field_s*);
 int arn_step(clg_context, clg_field_s* const*, int, clg_field_s*, double*, int, bool, clg_field_s*, double*);
+int msbicgstab_dev(clg_context ctx, int kind, int op, const clg_field_s* gauge, double param,
+                   std::vector<clg_field_s*>& X, const std::vector<double>& shifts, const clg_field_s* b, int step_count,
+                   int dev_check, double accuracy, double blen, int& iters, double& maxerr, int& applies);
 }  // namespace clg
 
 namespace {
@@ -86,61 +90,6 @@ __device__ __forceinline__ void cmad(double* acc, const double* c, const double*
     }
 }
 
-// shifted BiCGStab, first update of a step, for every active shift k:
-//   x_k += -beta_k s_k + (chi rho zeta)_k w ;  s_k += -(chi rho zeta / beta)_k w + (chi rho zeta_old / beta)_k r
-// coef[8k .. 8k+7] = {-beta, crz, -crz/beta, crzo/beta} as complex pairs (from the old s_k)
-template <int P>
-__global__ void k_msbicg_update1(long n, int ns, ShiftPtrs sp, const double* __restrict__ coef,
-                                 const typename MS<P>::T* __restrict__ w, const typename MS<P>::T* __restrict__ r) {
-    constexpr int NC = MS<P>::NC;
-    __shared__ double cs[8 * kMaxShift];
-    for (int t = threadIdx.x; t < 8 * ns; t += blockDim.x) cs[t] = coef[t];
-    __syncthreads();
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        double wv[2 * NC], rv[2 * NC];
-        MS<P>::ld(w, i, wv);
-        MS<P>::ld(r, i, rv);
-        for (int k = 0; k < ns; ++k) {
-            const double* c = cs + 8 * k;
-            typename MS<P>::T* xp = (typename MS<P>::T*)sp.x[k];
-            typename MS<P>::T* spp = (typename MS<P>::T*)sp.s[k];
-            double xv[2 * NC], sv[2 * NC], sn[2 * NC];
-            MS<P>::ld(xp, i, xv);
-            MS<P>::ld(spp, i, sv);
-            cmad<NC>(xv, c + 0, sv);
-            cmad<NC>(xv, c + 2, wv);
-#pragma unroll
-            for (int q = 0; q < 2 * NC; ++q) sn[q] = sv[q];
-            cmad<NC>(sn, c + 4, wv);
-            cmad<NC>(sn, c + 6, rv);
-            MS<P>::st(xp, i, xv);
-            MS<P>::st(spp, i, sn);
-        }
-    }
-}
-// second update: s_k = alpha_k s_k + (zeta rho)_k r ; coef[4k .. 4k+3]
-template <int P>
-__global__ void k_msbicg_update2(long n, int ns, ShiftPtrs sp, const double* __restrict__ coef,
-                                 const typename MS<P>::T* __restrict__ r) {
-    constexpr int NC = MS<P>::NC;
-    __shared__ double cs[4 * kMaxShift];
-    for (int t = threadIdx.x; t < 4 * ns; t += blockDim.x) cs[t] = coef[t];
-    __syncthreads();
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        double rv[2 * NC];
-        MS<P>::ld(r, i, rv);
-        for (int k = 0; k < ns; ++k) {
-            typename MS<P>::T* spp = (typename MS<P>::T*)sp.s[k];
-            double sv[2 * NC], o[2 * NC];
-            MS<P>::ld(spp, i, sv);
-#pragma unroll
-            for (int q = 0; q < 2 * NC; ++q) o[q] = 0;
-            cmad<NC>(o, cs + 4 * k, sv);
-            cmad<NC>(o, cs + 4 * k + 2, rv);
-            MS<P>::st(spp, i, o);
-        }
-    }
-}
 // MS-FOM: x_k += sum_j y[k][j] V_j for every active shift k (y[k*2m + 2j])
 template <int P>
 __global__ void k_msfom_update(long n, int ns, int m, ShiftPtrs sp, VPtrs vp, const double* __restrict__ y) {
@@ -239,14 +188,6 @@ long n_packets(const clg_field_s* f) {
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
-
-inline cx cm(cx a, cx b) { return cx{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
-inline cx cdv(cx a, cx b) {
-    const double d = b.x * b.x + b.y * b.y;
-    return cx{(a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d};
-}
-inline cx csb(cx a, cx b) { return cx{a.x - b.x, a.y - b.y}; }
-inline cx cad(cx a, cx b) { return cx{a.x + b.x, a.y + b.y}; }
 
 struct OpRef {
     int kind, op;
@@ -470,161 +411,6 @@ int msgmres(clg_context ctx, OpRef& A, std::vector<clg_field_s*>& X, const std::
     return CLG_NOT_CONVERGED;
 }
 
-// ------------------------------------------------------------------ shifted BiCGStab
-int msbicgstab(clg_context ctx, OpRef& A, std::vector<clg_field_s*>& X, const std::vector<cx>& cn,
-               const clg_field_s* b, int step_count, int dev_check, double accuracy, double blen, int& iters,
-               double& maxerr) {
-    const int ns = (int)cn.size();
-    std::vector<clg_field_s*> S(ns);
-    for (int k = 0; k < ns; ++k) CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 3000 + k, &S[k]));
-    clg_field_s *R, *W, *W0, *Sv, *SA, *WA;
-    CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 120, &R));
-    CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 121, &W));
-    CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 122, &W0));
-    CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 123, &Sv));
-    CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 124, &SA));
-    CLG_TRY(scratch_field(ctx, b, b->kind, b->prec, b->subset, 125, &WA));
-    double* ws = nullptr;
-    CLG_TRY(workspace(ctx, 64 + 8 * (size_t)kMaxShift, &ws));
-    double* d = ws;              // dot results
-    double* coef = ws + 32;      // per-shift coefficients
-    std::vector<cx> betas(ns, cx{1, 0}), zeta(ns, cx{1, 0}), zetaold(ns, cx{1, 0}), rho(ns, cx{1, 0}), chis(ns),
-        alphas(ns);
-    std::vector<double> sl(ns, 1.0);
-    for (int k = 0; k < ns; ++k) {
-        CLG_TRY(clg_field_zero(X[k]));
-        CLG_TRY(blas_axpby(ctx, S[k], 1.0, 0.0, b, 0.0, 0.0));
-    }
-    CLG_TRY(blas_axpby(ctx, R, 1.0, 0.0, b, 0.0, 0.0));
-    CLG_TRY(blas_axpby(ctx, Sv, 1.0, 0.0, b, 0.0, 0.0));
-    CLG_TRY(blas_conj(ctx, W0, b));  // w0 = b^dagger (conjugated field)
-    auto fetch_c = [&](int off, cx& z) -> int {
-        double t[2];
-        CLG_TRY(fetch_doubles(ctx, d + off, 2, t));
-        z = cx{t[0], t[1]};
-        return CLG_OK;
-    };
-    cx delta, phi, tmp;
-    CLG_TRY(blas_dot_dev(ctx, W0, R, d + 0));
-    CLG_TRY(fetch_c(0, delta));
-    CLG_TRY(A.apply(ctx, SA, Sv));
-    CLG_TRY(blas_dot_dev(ctx, W0, SA, d + 2));
-    CLG_TRY(fetch_c(2, tmp));
-    phi = cdv(tmp, delta);
-    cx beta{0, 0}, alpha{0, 0};
-    const long np = n_packets(b);
-    const unsigned g = ew_grid(ctx, np);
-    iters = 0;
-    maxerr = 0;
-    bool done = false;
-    std::vector<double> hc(8 * kMaxShift);
-    for (int i = 0; i < step_count * dev_check; ++i) {
-        ++iters;
-        const cx newbeta = cdv(cx{-1, 0}, phi);
-        for (int k = 0; k < ns; ++k) {
-            if (sl[k] < accuracy * blen) continue;
-            if (i == 0) {
-                zeta[k] = cdv(cx{1, 0}, csb(cx{1, 0}, cm(cn[k], newbeta)));
-                betas[k] = cm(newbeta, zeta[k]);
-            } else {
-                const cx zeta_nm1 = zetaold[k];
-                zetaold[k] = zeta[k];
-                const cx zb = cm(zeta_nm1, beta);
-                zeta[k] = cdv(cm(zetaold[k], zb), cad(cm(alpha, cm(newbeta, csb(zeta_nm1, zetaold[k]))),
-                                                      cm(zb, csb(cx{1, 0}, cm(cn[k], newbeta)))));
-                betas[k] = cdv(cm(newbeta, zeta[k]), zetaold[k]);
-            }
-        }
-        beta = newbeta;
-        // w = r + beta sa ; wa = A w ; chi = (wa . w) / (wa . wa)
-        CLG_TRY(blas_axpby(ctx, W, 1.0, 0.0, R, 0.0, 0.0));
-        CLG_TRY(blas_axpby(ctx, W, beta.x, beta.y, SA, 1.0, 0.0));
-        CLG_TRY(A.apply(ctx, WA, W));
-        CLG_TRY(blas_dot_dev(ctx, WA, W, d + 4));
-        CLG_TRY(blas_norm2_dev(ctx, WA, d + 6));
-        double t3[4];
-        CLG_TRY(fetch_doubles(ctx, d + 4, 3, t3));
-        const cx chi = cx{t3[0] / t3[2], t3[1] / t3[2]};
-        // batched per-shift update 1 over the active shifts
-        std::vector<int> idx;
-        for (int k = 0; k < ns; ++k)
-            if (!(sl[k] < accuracy * blen)) idx.push_back(k);
-        const int na = (int)idx.size();
-        std::vector<cx> dinoms(ns);
-        for (int q = 0; q < na; ++q) {
-            const int k = idx[q];
-            const cx dinom = cad(cx{1, 0}, cm(cn[k], chi));
-            dinoms[k] = dinom;
-            chis[k] = cdv(chi, dinom);
-            const cx chirho = cm(chis[k], rho[k]);
-            const cx crz = cm(chirho, zeta[k]);
-            const cx crzb = cdv(crz, betas[k]);
-            const cx crzob = cdv(cm(chirho, zetaold[k]), betas[k]);
-            const double c8[8] = {-betas[k].x, -betas[k].y, crz.x, crz.y, -crzb.x, -crzb.y, crzob.x, crzob.y};
-            std::memcpy(&hc[8 * q], c8, sizeof(c8));
-        }
-        ShiftPtrs sp;
-        for (int q = 0; q < na; ++q) { sp.x[q] = X[idx[q]]->data; sp.s[q] = S[idx[q]]->data; }
-        if (na > 0) {
-            CLG_HIP(hipMemcpyAsync(coef, hc.data(), sizeof(double) * 8 * na, hipMemcpyHostToDevice, ctx->stream));
-            if (b->prec == CLG_F64)
-                k_msbicg_update1<CLG_F64><<<g, 256, 0, ctx->stream>>>(np, na, sp, coef, (const double2*)W->data,
-                                                                       (const double2*)R->data);
-            else
-                k_msbicg_update1<CLG_F32><<<g, 256, 0, ctx->stream>>>(np, na, sp, coef, (const float4*)W->data,
-                                                                       (const float4*)R->data);
-            CLG_HIP(hipGetLastError());
-        }
-        for (int q = 0; q < na; ++q) rho[idx[q]] = cdv(rho[idx[q]], dinoms[idx[q]]);
-        // r = w - chi wa ; delta' = w0 . r ; alpha = beta (-delta') / (chi delta)
-        CLG_TRY(blas_axpby(ctx, R, 1.0, 0.0, W, 0.0, 0.0));
-        CLG_TRY(blas_axpby(ctx, R, -chi.x, -chi.y, WA, 1.0, 0.0));
-        CLG_TRY(blas_dot_dev(ctx, W0, R, d + 8));
-        cx newdelta;
-        CLG_TRY(fetch_c(8, newdelta));
-        alpha = cdv(cm(beta, cx{-newdelta.x, -newdelta.y}), cm(chi, delta));
-        delta = newdelta;
-        for (int q = 0; q < na; ++q) {
-            const int k = idx[q];
-            alphas[k] = cdv(cm(cm(alpha, zeta[k]), betas[k]), cm(zetaold[k], beta));
-            const cx zr = cm(zeta[k], rho[k]);
-            const double c4[4] = {alphas[k].x, alphas[k].y, zr.x, zr.y};
-            std::memcpy(&hc[4 * q], c4, sizeof(c4));
-        }
-        if (na > 0) {
-            CLG_HIP(hipMemcpyAsync(coef, hc.data(), sizeof(double) * 4 * na, hipMemcpyHostToDevice, ctx->stream));
-            if (b->prec == CLG_F64)
-                k_msbicg_update2<CLG_F64><<<g, 256, 0, ctx->stream>>>(np, na, sp, coef, (const double2*)R->data);
-            else
-                k_msbicg_update2<CLG_F32><<<g, 256, 0, ctx->stream>>>(np, na, sp, coef, (const float4*)R->data);
-            CLG_HIP(hipGetLastError());
-        }
-        // s = alpha (s - chi sa) + r ; sa = A s ; phi = (w0 . sa) / delta
-        CLG_TRY(blas_axpby(ctx, Sv, -chi.x, -chi.y, SA, 1.0, 0.0));
-        CLG_TRY(blas_axpby(ctx, Sv, 1.0, 0.0, R, alpha.x, alpha.y));
-        CLG_TRY(A.apply(ctx, SA, Sv));
-        CLG_TRY(blas_dot_dev(ctx, W0, SA, d + 10));
-        CLG_TRY(fetch_c(10, tmp));
-        phi = cdv(tmp, delta);
-        if ((i + 1) % dev_check == 0) {
-            CLG_TRY(blas_norm2_dev(ctx, Sv, d + 12));
-            double s2;
-            CLG_TRY(fetch_doubles(ctx, d + 12, 1, &s2));
-            maxerr = std::sqrt(s2);
-            for (int q = 0; q < na; ++q) {
-                const int k = idx[q];
-                CLG_TRY(blas_norm2_dev(ctx, S[k], d + 14));
-                double t;
-                CLG_TRY(fetch_doubles(ctx, d + 14, 1, &t));
-                sl[k] = std::sqrt(t);
-                maxerr = std::max(maxerr, sl[k]);
-            }
-            if (maxerr < accuracy * blen) { done = true; break; }
-        }
-    }
-    return done ? CLG_OK : CLG_NOT_CONVERGED;
-}
-
 enum { MS_FOM = 0, MS_BICGSTAB = 1, MS_GMRES = 2, MS_NESTED = 3 };
 
 int run_multishift(clg_context ctx, const clg_solver_params* p, int kind_solver, int prec, std::vector<clg_field_s*>& X,
@@ -665,8 +451,10 @@ int run_multishift(clg_context ctx, const clg_solver_params* p, int kind_solver,
                    err);
     } else {
         // BiCGStab compares |s| with Accuracy * (AbsoluteAccuracy ? 1 : |b|)
-        st = msbicgstab(ctx, A, Xw, shifts, bw, p->max_step, p->deviation_step, p->accuracy,
-                        p->absolute_accuracy ? 1.0 : std::sqrt(bn2), it, err);
+        std::vector<double> sh(2 * shifts.size());
+        for (size_t k = 0; k < shifts.size(); ++k) { sh[2 * k] = shifts[k].x; sh[2 * k + 1] = shifts[k].y; }
+        st = msbicgstab_dev(ctx, A.kind, A.op, A.gauge, A.param, Xw, sh, bw, p->max_step, p->deviation_step,
+                            p->accuracy, p->absolute_accuracy ? 1.0 : std::sqrt(bn2), it, err, A.applies);
     }
     if (st != CLG_OK && st != CLG_NOT_CONVERGED) return st;
     if (prec == CLG_F32)

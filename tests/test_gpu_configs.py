@@ -107,7 +107,8 @@ def fit_rational(power, shifts, lo, hi):
     return np.concatenate([coef, shifts])
 
 
-def test_rhmc_trajectory_fp32_forces_against_fp64(ctx):
+@pytest.mark.parametrize("ms_kind", [clg.MS_BICGSTAB, clg.MS_FOM])
+def test_rhmc_trajectory_fp32_forces_against_fp64(ctx, ms_kind):
     dims = (24, 24, 24, 8)
     shifts = np.logspace(-4, 1, 12)
     flavours = [(mass, fit_rational(-nf / 4.0, shifts, mass * mass, 20.0), fit_rational(nf / 8.0, shifts, mass * mass,
@@ -125,7 +126,7 @@ def test_rhmc_trajectory_fp32_forces_against_fp64(ctx):
             sp = clg.solver_params(kind=clg.KS, max_dim=40, restart=200, absolute_accuracy=1,
                                    accuracy=1e-5 if prec == clg.F32 else 1e-8)
             acts.append(dict(kind=clg.ACTION_FERMION_KS, kappa_or_mass=mass, phi=phi, md=md, mc=mc, solver=sp,
-                             ms_kind=clg.MS_FOM, ms_prec=prec, energy_accuracy=1e-9))
+                             ms_kind=ms_kind, ms_prec=prec, energy_accuracy=1e-9))
         hp = clg.hmc_params(integrator=clg.INT_NESTED_FORCE_GRADIENT, step_length=1.0, steps=8, nested_steps=6,
                             exp_precision=8, metropolis=1)
         rep = clg.hmc_trajectory(ctx, g, hp, acts, 1)
