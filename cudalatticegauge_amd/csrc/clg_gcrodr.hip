@@ -42,6 +42,7 @@ int linop_apply(clg_context, int, int, const clg_field_s*, double, clg_field_s*,
 int wilson_hop_half(clg_context, clg_field_s*, const clg_field_s*, const clg_field_s*, bool, double, double,
                     const clg_field_s*, double, double);
 int wilson_schur_once(clg_context, clg_field_s*, const clg_field_s*, const clg_field_s*, double, bool, clg_field_s*);
+int wilson_full_hop(clg_context, clg_field_s*, const clg_field_s*, const clg_field_s*, double, bool);
 int arn_step(clg_context, clg_field_s* const*, int, clg_field_s*, double*, int, bool, clg_field_s*, double*);
 int blas_gram_dev(clg_context, clg_field_s* const*, int, clg_field_s* const*, int, double*, int);
 int blas_scale_inv_sqrt(clg_context, clg_field_s*, const clg_field_s*, const double*);
@@ -260,6 +261,29 @@ struct Gcro {
         if (schur) return wilson_schur_once(ctx, out, in, gauge, param, op == EFO_F_Ddagger, todd);
         return linop_apply(ctx, p->kind, op, gauge, param, out, in);
     }
+    // Wilson D (D^dagger) and its even-site Schur complement are A = 1 - K with K = kappa H or
+    // kappa^2 H_eo H_oe: the Arnoldi step then runs on K, whose projections cancel nothing (A W_j keeps
+    // most of its norm along W_j, which makes the reorthogonalisation pass fire at every step), and the
+    // column of G follows as e_j - (projections of K W_j): the C rows are exact because the Krylov
+    // vectors are orthogonal to C.
+    bool shiftable() const { return p->kind == CLG_WILSON && (op == EFO_F_D || op == EFO_F_Ddagger); }
+    int apply_k(clg_field_s* out, const clg_field_s* in) {
+        ++applies;
+        const bool dag = op == EFO_F_Ddagger;
+        if (!schur) return wilson_full_hop(ctx, out, in, gauge, param, dag);
+        CLG_TRY(wilson_hop_half(ctx, todd, in, gauge, dag, 1.0, 0.0, nullptr, 0.0, 0.0));
+        return wilson_hop_half(ctx, out, todd, gauge, dag, param * param, 0.0, nullptr, 0.0, 0.0);
+    }
+    // one Arnoldi step of the cycle: W_{j+1} from A W_j, orthogonalised against W_0..W_j, and column j
+    // of G (the reference's B block above its Hessenberg block, CSolverGCRODR.cpp:277-316)
+    int arnoldi(int j) {
+        const bool sh = shiftable();
+        CLG_TRY(sh ? apply_k(W(j + 1), W(j)) : apply(W(j + 1), W(j)));
+        std::vector<clg_field_s*> list(j + 1);
+        for (int i = 0; i <= j; ++i) list[i] = W(i);
+        return arn_step(ctx, list.data(), j + 1, W(j + 1), dws + offG + 2 * (size_t)j * (m + 1), j, sh, nullptr,
+                        dws + offA);
+    }
     // |f| on the host (the stop tests)
     int norm(const clg_field_s* f, double& out) {
         CLG_TRY(blas_norm2_dev(ctx, f, dws + offS));
@@ -272,13 +296,6 @@ struct Gcro {
         k_fill0<<<std::max(1, std::min(64, (n + 255) / 256)), 256, 0, ctx->stream>>>(d, n);
         CLG_HIP(hipGetLastError());
         return CLG_OK;
-    }
-    // w <- (w - sum_i W_i (W_i^H w)) / |.| over W_0..W_{nw-1}, with the conditional second pass, in
-    // place (it is the next basis vector); column j of G on the device
-    int orthogonalise(clg_field_s* w, int nw, int j) {
-        std::vector<clg_field_s*> list(nw);
-        for (int i = 0; i < nw; ++i) list[i] = W(i);
-        return arn_step(ctx, list.data(), nw, w, dws + offG + 2 * (size_t)j * (m + 1), j, false, nullptr, dws + offA);
     }
     // r = b - A x
     int residual(clg_field_s* r, const clg_field_s* x, const clg_field_s* b) {
@@ -375,10 +392,7 @@ struct Gcro {
             CLG_TRY(residual(w0, X, b));
             CLG_TRY(blas_norm2_dev(ctx, w0, dws + offR2));
             CLG_TRY(blas_scale_inv_sqrt(ctx, w0, w0, dws + offR2));
-            for (int j = 0; j < m; ++j) {
-                CLG_TRY(apply(W(j + 1), W(j)));
-                CLG_TRY(orthogonalise(W(j + 1), j + 1, j));
-            }
+            for (int j = 0; j < m; ++j) CLG_TRY(arnoldi(j));
             ++cycles;
             CLG_TRY(small(0));
             std::vector<clg_field_s*> basis;
@@ -422,10 +436,7 @@ struct Gcro {
             CLG_TRY(blas_norm2_dev(ctx, r, dws + offR2));
             CLG_TRY(clg_field_copy(Wr, r));
             CLG_TRY(blas_scale_inv_sqrt(ctx, r, r, dws + offR2));
-            for (int j = k; j < m; ++j) {
-                CLG_TRY(apply(W(j + 1), W(j)));
-                CLG_TRY(orthogonalise(W(j + 1), j + 1, j));
-            }
+            for (int j = k; j < m; ++j) CLG_TRY(arnoldi(j));
             ++cycles;
             CLG_TRY(small(1));
             std::vector<clg_field_s*> basis;

@@ -24,6 +24,7 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int kGramBlock = 256;
 constexpr int kGramMaxGrid = 256;
 constexpr int kGramMax = 32;
+constexpr int kGramU = 4;  // 4-element steps per block of loads
 
 struct GramPtrs {
     const double2* a[kGramMax];
@@ -43,38 +44,39 @@ __global__ __launch_bounds__(kGramBlock) void k_gram(long n, GramPtrs P, int r, 
             acc_re[ti][tj] = v4d{0.0, 0.0, 0.0, 0.0};
             acc_im[ti][tj] = v4d{0.0, 0.0, 0.0, 0.0};
         }
-    // waves of the grid take consecutive 4-element steps
+    // waves of the grid take consecutive blocks of kGramU 4-element steps; a lane issues every load of a
+    // block before the block's first product, so each wave keeps (TR + TC) kGramU loads in flight
     const long nw = (long)gridDim.x * (kGramBlock / 64);
     const long w0 = (long)blockIdx.x * (kGramBlock / 64) + wv;
-    for (long e0 = 4 * w0; e0 < n; e0 += 4 * nw) {
-        const long e = e0 + ke;
-        const bool in = e < n;
-        double ar[TR], ai[TR], br[TC], bi[TC];
+    for (long e0 = 4 * kGramU * w0; e0 < n; e0 += 4 * kGramU * nw) {
+        double2 va[kGramU][TR], vb[kGramU][TC];
 #pragma unroll
-        for (int ti = 0; ti < TR; ++ti) {
-            const int f = ti * 16 + fi;
-            double2 v = {0.0, 0.0};
-            if (in && f < r) v = P.a[f][e];
-            ar[ti] = v.x;
-            ai[ti] = v.y;
-        }
+        for (int u = 0; u < kGramU; ++u) {
+            const long e = e0 + 4 * u + ke;
+            const bool in = e < n;
 #pragma unroll
-        for (int tj = 0; tj < TC; ++tj) {
-            const int f = tj * 16 + fi;
-            double2 v = {0.0, 0.0};
-            if (in && f < c) v = P.b[f][e];
-            br[tj] = v.x;
-            bi[tj] = v.y;
-        }
-#pragma unroll
-        for (int ti = 0; ti < TR; ++ti)
+            for (int ti = 0; ti < TR; ++ti) {
+                const int f = ti * 16 + fi;
+                va[u][ti] = (in && f < r) ? P.a[f][e] : double2{0.0, 0.0};
+            }
 #pragma unroll
             for (int tj = 0; tj < TC; ++tj) {
-                acc_re[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[ti], br[tj], acc_re[ti][tj], 0, 0, 0);
-                acc_re[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[ti], bi[tj], acc_re[ti][tj], 0, 0, 0);
-                acc_im[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[ti], bi[tj], acc_im[ti][tj], 0, 0, 0);
-                acc_im[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[ti], br[tj], acc_im[ti][tj], 0, 0, 0);
+                const int f = tj * 16 + fi;
+                vb[u][tj] = (in && f < c) ? P.b[f][e] : double2{0.0, 0.0};
             }
+        }
+#pragma unroll
+        for (int u = 0; u < kGramU; ++u)
+#pragma unroll
+            for (int ti = 0; ti < TR; ++ti)
+#pragma unroll
+                for (int tj = 0; tj < TC; ++tj) {
+                    const double ar = va[u][ti].x, ai = va[u][ti].y, br = vb[u][tj].x, bi = vb[u][tj].y;
+                    acc_re[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, acc_re[ti][tj], 0, 0, 0);
+                    acc_re[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi, acc_re[ti][tj], 0, 0, 0);
+                    acc_im[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, acc_im[ti][tj], 0, 0, 0);
+                    acc_im[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, br, acc_im[ti][tj], 0, 0, 0);
+                }
     }
     // sum the 4 waves' tiles in LDS, then write the workgroup's partial
     __shared__ double sm[kGramBlock / 64][kGramMax * kGramMax * 2];
@@ -132,7 +134,8 @@ int blas_gram_dev(clg_context ctx, clg_field_s* const* A, int r, clg_field_s* co
     GramPtrs P;
     for (int i = 0; i < r; ++i) P.a[i] = (const double2*)A[i]->data;
     for (int j = 0; j < c; ++j) P.b[j] = (const double2*)B[j]->data;
-    const int G = (int)std::max<long>(1, std::min<long>(kGramMaxGrid, (n + 4 * kGramBlock - 1) / (4 * kGramBlock)));
+    const long per_wg = 4L * kGramU * kGramBlock;  // elements one workgroup takes per pass of its loop
+    const int G = (int)std::max<long>(1, std::min<long>(kGramMaxGrid, (n + per_wg - 1) / per_wg));
     CLG_TRY(ensure_reduction_scratch(ctx, (size_t)kGramMaxGrid * 2 * kGramMax * kGramMax));
     double* part = ctx->red_partial;
     hipStream_t st = ctx->stream;

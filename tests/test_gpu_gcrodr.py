@@ -116,3 +116,35 @@ def test_context_closed_before_its_solver_and_graph():
     sv = C.c_void_p()
     assert L.clg_gcrodr_create(h, C.byref(p), C.byref(sv)) == 0
     assert L.clg_finalize(h) == 0
+
+
+def test_gcrodr_pays_near_critical(ctx):
+    """A smooth field near its critical kappa, where restarted GMRES(12) stalls on the low modes: GCRO-DR's
+    deflated restarts converge with well under half the operator applications, at the oracle's cycle count."""
+    dims, kappa = (8, 8, 8, 8), 0.13
+    U = orc.gauge_random(dims, 0.1, 1)
+    b = orc.wilson_gaussian(dims, 2)
+    acc = 1e-10
+    _, rc_g, cyc_g, _ = orc.gmres(0, orc.D, dims, BC, kappa, U, b, max_dim=12, restart=3000, accuracy=acc, stop_rule=1)
+    _, rc_o, cyc_o, _, _, _ = orc.gcrodr(0, orc.D, dims, BC, kappa, U, b, phase=0, m=20, k=10, restart=3000,
+                                         accuracy=acc * np.linalg.norm(b), absolute=1)
+    assert rc_g == 0 and rc_o == 0
+    g = clg.Field(ctx, clg.GAUGE, clg.F64, dims)
+    g.upload(U)
+    fb = clg.Field(ctx, clg.WILSON, clg.F64, dims, bc=BC)
+    fb.upload(b)
+    x = clg.Field(ctx, clg.WILSON, clg.F64, dims, bc=BC)
+    s = clg.Gcrodr(ctx, m_dim=20, k_dim=10, restart=3000, accuracy=acc * np.linalg.norm(b), absolute_accuracy=1,
+                   deflate=clg.DEFLATE_SVD)
+    st, rep = s.solve(x, fb, g, kappa, clg.EFO_F_D, phase=clg.PHASE_START)
+    s.close()
+    pg = clg.solver_params(max_dim=12, restart=3000, accuracy=acc, stop_rule=clg.STOP_TRUE_RESIDUAL)
+    x2 = clg.Field(ctx, clg.WILSON, clg.F64, dims, bc=BC)
+    st2, rep2 = clg.solve(ctx, pg, x2, fb, g, kappa, clg.EFO_F_InverseD)
+    print(f"GCRO-DR cycles {rep.cycles} (oracle {cyc_o}), applies {rep.operator_applies}; GMRES(12) applies "
+          f"{rep2.operator_applies} (oracle cycles {cyc_g})")
+    assert st == clg.OK and st2 == clg.OK
+    assert abs(rep.cycles - cyc_o) <= 2
+    assert rep.operator_applies < 0.5 * rep2.operator_applies
+    Dx = orc.wilson_op(dims, BC, kappa, U, x.download(), orc.D)
+    assert np.linalg.norm(Dx - b) < 2e-10 * np.linalg.norm(b)
