@@ -73,6 +73,9 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="only run the FP64 D steps plus 5 in-place scalings of one FP64 Wilson field "
                          "(a read+write pass of known size used to calibrate rocprofv3 byte counters)")
+    ap.add_argument("--only", choices=["rhmc_fp32", "rhmc_fp64", "rhmc_fom_fp32", "gcrodr_nc"],
+                    help="run one workload of the extras alone and log it (for a kernel-trace profile): an RHMC "
+                         "trajectory variant or the near-critical GCRO-DR pair; no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -85,6 +88,14 @@ def main():
     V = int(np.prod(DIMS))
     ctx = clg.Context(0)
     log(f"device {ctx.info()}  lattice {DIMS}  V={V}")
+    if args.only:
+        if args.only.startswith("rhmc_"):
+            res = rhmc_trajectory(ctx, clg, np, variants=(args.only[len("rhmc_"):],))
+        else:
+            res = gcrodr_near_critical(ctx, clg, np)
+        log(json.dumps(res))
+        ctx.close()
+        return
 
     def make_inputs(prec, gkind):
         g = clg.Field(ctx, gkind, prec, DIMS)
@@ -452,7 +463,7 @@ def fit_rational(power, shifts, lo, hi):
     return np.concatenate([coef, shifts]), float(err)
 
 
-def rhmc_trajectory(ctx, clg, np):
+def rhmc_trajectory(ctx, clg, np, variants=("fp32", "fp64", "fom_fp32")):
     shifts = np.logspace(-4, 1, 12)
     flavours = []
     for mass, nf in ((0.01, 2), (0.05, 1)):
@@ -464,6 +475,8 @@ def rhmc_trajectory(ctx, clg, np):
     # configurations) for both force precisions, and the multi-shift FOM with FP32 forces for comparison
     for tag, prec, ms_kind in (("fp32", clg.F32, clg.MS_BICGSTAB), ("fp64", clg.F64, clg.MS_BICGSTAB),
                                ("fom_fp32", clg.F32, clg.MS_FOM)):
+        if tag not in variants:
+            continue
         g = clg.Field(ctx, clg.GAUGE, clg.F64, HMC_DIMS)
         g.gauge_random(0.4, 1)
         acts = [dict(kind=clg.ACTION_GAUGE_PLAQUETTE, beta=5.5)]
@@ -486,7 +499,8 @@ def rhmc_trajectory(ctx, clg, np):
         out[f"rhmc_{tag}_solver_cycles"] = rep.solver_iterations
         out[f"rhmc_{tag}_unconverged"] = rep.unconverged_solves
         del g, phis
-    out["rhmc_dH_abs_diff_fp32_vs_fp64"] = float(f"{abs(out['rhmc_fp32_dH'] - out['rhmc_fp64_dH']):.3g}")
+    if "rhmc_fp32_dH" in out and "rhmc_fp64_dH" in out:
+        out["rhmc_dH_abs_diff_fp32_vs_fp64"] = float(f"{abs(out['rhmc_fp32_dH'] - out['rhmc_fp64_dH']):.3g}")
     out["rhmc_rational_fit_max_rel_err"] = float(f"{max(f[3] for f in flavours):.2g}")
     out["rhmc_config"] = ("24^3x8 KS 2+1 (Mass 0.01, 0.05), 12 poles, NFG 8x6, beta 5.5, eps 0.4; force solves: "
                           "shifted BiCGStab (fom_fp32: multi-shift FOM, MaxDim 40)")
