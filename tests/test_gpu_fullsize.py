@@ -67,9 +67,11 @@ def maxrel(a, b):
     return np.abs(a - b).max() / np.abs(b).max()
 
 
-# the benchmark's FP64 launch (LDS-staged (x,y) planes of 256 sites, XCD slabs with the parity
-# interleave, one plane per workgroup) and the other schedules the tuner may pick at this lattice
-C1_LAUNCHES = [(6, 256, 8, 1), (5, 256, 8, 1), (8, 256, 8, 1), (2, 256, 8, 1), (4, 128, 8, 2), (1, 64, 0, 1)]
+# the benchmark's FP64 launch (LDS-staged planes of 256 sites, one plane per workgroup, tiled plane
+# schedule 40: 8 tiles of 16 x 8 planes, one per XCD; the tuner adds non-temporal output stores, which
+# the "tuned" case runs) and the other schedules the tuner may pick at this lattice
+C1_LAUNCHES = [(6, 256, 40, 1), (6, 256, 41, 1), (6, 256, 20, 1), (8, 256, 80, 1), (6, 256, 8, 1), (5, 256, 8, 1),
+               (8, 256, 8, 1), (2, 256, 8, 1), (4, 128, 8, 2), (1, 64, 0, 1)]
 
 
 @pytest.mark.parametrize("launch", C1_LAUNCHES + [None], ids=[str(c) for c in C1_LAUNCHES] + ["tuned"])
