@@ -73,7 +73,7 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="only run the FP64 D steps plus 5 in-place scalings of one FP64 Wilson field "
                          "(a read+write pass of known size used to calibrate rocprofv3 byte counters)")
-    ap.add_argument("--only", choices=["rhmc_fp32", "rhmc_fp64", "rhmc_fom_fp32", "gcrodr_nc"],
+    ap.add_argument("--only", choices=["rhmc_fp32", "rhmc_fp64", "rhmc_fom_fp32", "rhmc_fp16link", "gcrodr_nc"],
                     help="run one workload of the extras alone and log it (for a kernel-trace profile): an RHMC "
                          "trajectory variant or the near-critical GCRO-DR pair; no JSON line")
     args = ap.parse_args()
@@ -463,7 +463,7 @@ def fit_rational(power, shifts, lo, hi):
     return np.concatenate([coef, shifts]), float(err)
 
 
-def rhmc_trajectory(ctx, clg, np, variants=("fp32", "fp64", "fom_fp32")):
+def rhmc_trajectory(ctx, clg, np, variants=("fp32", "fp64", "fom_fp32", "fp16link")):
     shifts = np.logspace(-4, 1, 12)
     flavours = []
     for mass, nf in ((0.01, 2), (0.05, 1)):
@@ -472,9 +472,10 @@ def rhmc_trajectory(ctx, clg, np, variants=("fp32", "fp64", "fom_fp32")):
         flavours.append((mass, md, mc, max(err_md, err_mc)))
     out = {}
     # the shifted BiCGStab (CMultiShiftBiCGStab, the multi-shift solver of the reference's staggered RHMC
-    # configurations) for both force precisions, and the multi-shift FOM with FP32 forces for comparison
+    # configurations) for both force precisions and with FP16 links under FP32 vectors, and the multi-shift
+    # FOM with FP32 forces for comparison
     for tag, prec, ms_kind in (("fp32", clg.F32, clg.MS_BICGSTAB), ("fp64", clg.F64, clg.MS_BICGSTAB),
-                               ("fom_fp32", clg.F32, clg.MS_FOM)):
+                               ("fom_fp32", clg.F32, clg.MS_FOM), ("fp16link", clg.F16, clg.MS_BICGSTAB)):
         if tag not in variants:
             continue
         g = clg.Field(ctx, clg.GAUGE, clg.F64, HMC_DIMS)
@@ -485,7 +486,7 @@ def rhmc_trajectory(ctx, clg, np, variants=("fp32", "fp64", "fom_fp32")):
             phi = clg.Field(ctx, clg.KS, clg.F64, HMC_DIMS, bc=(1, 1, 1, -1))
             phis.append(phi)
             sp = clg.solver_params(kind=clg.KS, max_dim=40, restart=200, absolute_accuracy=1,
-                                   accuracy=1e-5 if prec == clg.F32 else 1e-8)
+                                   accuracy=1e-8 if prec == clg.F64 else 1e-5)
             acts.append(dict(kind=clg.ACTION_FERMION_KS, kappa_or_mass=mass, phi=phi, md=md, mc=mc, solver=sp,
                              ms_kind=ms_kind, ms_prec=prec, energy_accuracy=1e-9))
         hp = clg.hmc_params(integrator=clg.INT_NESTED_FORCE_GRADIENT, step_length=1.0, steps=8, nested_steps=6,
@@ -501,9 +502,12 @@ def rhmc_trajectory(ctx, clg, np, variants=("fp32", "fp64", "fom_fp32")):
         del g, phis
     if "rhmc_fp32_dH" in out and "rhmc_fp64_dH" in out:
         out["rhmc_dH_abs_diff_fp32_vs_fp64"] = float(f"{abs(out['rhmc_fp32_dH'] - out['rhmc_fp64_dH']):.3g}")
+    if "rhmc_fp16link_dH" in out and "rhmc_fp64_dH" in out:
+        out["rhmc_dH_abs_diff_fp16link_vs_fp64"] = float(f"{abs(out['rhmc_fp16link_dH'] - out['rhmc_fp64_dH']):.3g}")
     out["rhmc_rational_fit_max_rel_err"] = float(f"{max(f[3] for f in flavours):.2g}")
     out["rhmc_config"] = ("24^3x8 KS 2+1 (Mass 0.01, 0.05), 12 poles, NFG 8x6, beta 5.5, eps 0.4; force solves: "
-                          "shifted BiCGStab (fom_fp32: multi-shift FOM, MaxDim 40)")
+                          "shifted BiCGStab (fom_fp32: multi-shift FOM, MaxDim 40; fp16link: FP32 vectors, FP16 "
+                          "12-real links)")
     return out
 
 

@@ -726,7 +726,10 @@ int clg_hmc_trajectory(clg_context ctx, clg_field gauge, const clg_hmc_params* h
         } else if (a.kind == CLG_ACTION_FERMION_KS) {
             CLG_CHECK_ARG(a.mc && a.mc_degree >= 1 && a.mc_degree <= 32, "action %d: MC rational needs 1..32 poles", i);
             clg_solve_report r{};
-            const int st = clg_ks_prepare_hmc(ctx, &a.solver, a.ms_kind, a.ms_prec, a.phi, gauge, a.kappa_or_mass, a.mc,
+            // the heat bath draws phi = r_MC(D D^dagger) eta; it keeps full-precision links even when the
+            // force solves take FP16 ones, so phi follows the action the energies measure
+            const int hb_prec = a.ms_prec == CLG_F16 ? CLG_F32 : a.ms_prec;
+            const int st = clg_ks_prepare_hmc(ctx, &a.solver, a.ms_kind, hb_prec, a.phi, gauge, a.kappa_or_mass, a.mc,
                                               a.mc_degree, si, &r);
             if (st != CLG_OK && st != CLG_NOT_CONVERGED) return restore(st);
             add_report(rep, r);

@@ -417,15 +417,17 @@ int run_multishift(clg_context ctx, const clg_solver_params* p, int kind_solver,
                    const std::vector<cx>& shifts, const clg_field_s* b, const clg_field_s* gauge, double param, int op,
                    clg_solve_report* rep) {
     const double t0 = now_s();
-    // working precision: FP64 as given, or FP32 copies of b / gauge / solutions
+    // working precision: FP64 as given, or FP32 copies of b / solutions with FP32 (prec CLG_F32) or FP16
+    // 12-real (prec CLG_F16) copies of the links
     const clg_field_s* bw = b;
     const clg_field_s* gw = gauge;
     std::vector<clg_field_s*> Xw = X;
-    if (prec == CLG_F32) {
+    if (prec == CLG_F32 || prec == CLG_F16) {
         clg_field_s *b32, *g32;
         CLG_TRY(scratch_field(ctx, b, b->kind, CLG_F32, b->subset, 150, &b32));
         CLG_TRY(clg_field_copy(b32, const_cast<clg_field_s*>(b)));
-        CLG_TRY(scratch_field(ctx, gauge, CLG_GAUGE, CLG_F32, CLG_FULL, 0, &g32));
+        if (prec == CLG_F16) CLG_TRY(scratch_field(ctx, gauge, CLG_GAUGE12, CLG_F16, CLG_FULL, 0, &g32));
+        else CLG_TRY(scratch_field(ctx, gauge, CLG_GAUGE, CLG_F32, CLG_FULL, 0, &g32));
         CLG_TRY(gauge_convert(ctx, g32, gauge));
         for (size_t k = 0; k < X.size(); ++k)
             CLG_TRY(scratch_field(ctx, b, b->kind, CLG_F32, b->subset, 4000 + (int)k, &Xw[k]));
@@ -457,7 +459,7 @@ int run_multishift(clg_context ctx, const clg_solver_params* p, int kind_solver,
                             p->accuracy, p->absolute_accuracy ? 1.0 : std::sqrt(bn2), it, err, A.applies);
     }
     if (st != CLG_OK && st != CLG_NOT_CONVERGED) return st;
-    if (prec == CLG_F32)
+    if (prec != CLG_F64)
         for (size_t k = 0; k < X.size(); ++k) CLG_TRY(clg_field_copy(X[k], Xw[k]));
     CLG_HIP(hipStreamSynchronize(ctx->stream));
     if (rep) {
@@ -484,7 +486,9 @@ int clg_multishift_solve(clg_context ctx, const clg_solver_params* p, int kind_s
     CLG_CHECK_ARG(nshift >= 1 && nshift <= kMaxShift, "nshift must be in [1, %d]", kMaxShift);
     CLG_CHECK_ARG(kind_solver >= MS_FOM && kind_solver <= MS_NESTED, "unknown multi-shift solver %d", kind_solver);
     CLG_CHECK_ARG(kind_solver != MS_GMRES || nshift < kMaxShift, "MS-GMRES takes at most %d shifts", kMaxShift - 1);
-    CLG_CHECK_ARG(prec == CLG_F64 || prec == CLG_F32, "multi-shift precision must be FP64 or FP32");
+    CLG_CHECK_ARG(prec == CLG_F64 || prec == CLG_F32 || prec == CLG_F16,
+                  "multi-shift precision must be FP64, FP32, or FP16 links under FP32 vectors");
+    CLG_CHECK_ARG(prec != CLG_F16 || kind_solver != MS_NESTED, "MS-Nested runs in FP64 or FP32");
     CLG_CHECK_ARG(b->kind == p->kind && b->subset == CLG_FULL && b->prec == CLG_F64, "b must be a FULL FP64 field of params.kind");
     CLG_CHECK_ARG(gauge->prec == CLG_F64 && gauge->kind == CLG_GAUGE, "gauge must be the FP64 18-real field");
     CLG_CHECK_ARG(op >= EFO_F_D && op <= EFO_F_DDdagger, "operator %d not supported", op);

@@ -365,9 +365,6 @@ int validate_params(const clg_solver_params* p) {
     if (p->inner_prec == CLG_INNER_F16_VECTORS)
         CLG_CHECK_ARG(p->kind == CLG_WILSON && p->solver == CLG_SOLVER_GMRES && p->max_dim <= 16 && p->even_odd != 2,
                       "FP16-vector inner solves take Wilson GMRES with MaxDim <= 16 (no normal equations)");
-    // the staggered kernels read links in the spinor precision: no FP16 links under FP32 vectors
-    if (p->inner_prec == CLG_F16)
-        CLG_CHECK_ARG(p->kind == CLG_WILSON, "FP16-link inner solves are implemented for Wilson fermions");
     return CLG_OK;
 }
 

@@ -87,11 +87,12 @@ __device__ __forceinline__ void ks_mu(const KSArgs& a, const Site& s, int p, int
 }
 
 // MODE 0: out = c (2am in + hs H in) over both parities; MODE 1/2: the half-lattice hop
-// out_p = a H_{p,1-p} in (+ b x_p), where H = sum_mu eta_mu [U s+ psi(x+mu) - U^dag s- psi(x-mu)]
-template <typename R, int SP, int RECON, int PM, int MODE>
+// out_p = a H_{p,1-p} in (+ b x_p), where H = sum_mu eta_mu [U s+ psi(x+mu) - U^dag s- psi(x-mu)].
+// SP: spinor storage, GP: link storage (FP32 spinors may take FP16 links)
+template <typename R, int SP, int GP, int RECON, int PM, int MODE>
 __global__ __launch_bounds__(256) void k_ks(KSArgs a) {
     typedef KSIO<SP> SIO;
-    typedef LinkIO<SP, RECON> LIO;
+    typedef LinkIO<GP, RECON> LIO;
     int p, h;
     if (!map_site(a.g.Vh, a.nblk, a.pbase, a.xcd, a.pinter, p, h)) return;
     const Site s = site_of(a.g, p, h);
@@ -137,33 +138,37 @@ struct KSCfg {
     int pm;     // 0 or 2 (ks_mu)
 };
 
-template <typename R, int SP, int RECON, int PM>
+template <typename R, int SP, int GP, int RECON, int PM>
 void ks_launch_pm(const KSArgs& a0, hipStream_t st, const KSCfg& c) {
     KSArgs a = a0;
     a.xcd = c.xcd;
     a.pinter = c.xcd > 1 ? 1 : 0;
     const unsigned grid = sched_grid(a.g.Vh, a.nblk, c.block, a.xcd, a.pinter);
-    if (a.mode == 2) k_ks<R, SP, RECON, PM, 2><<<grid, c.block, 0, st>>>(a);
-    else if (a.mode == 1) k_ks<R, SP, RECON, PM, 1><<<grid, c.block, 0, st>>>(a);
-    else k_ks<R, SP, RECON, PM, 0><<<grid, c.block, 0, st>>>(a);
+    if (a.mode == 2) k_ks<R, SP, GP, RECON, PM, 2><<<grid, c.block, 0, st>>>(a);
+    else if (a.mode == 1) k_ks<R, SP, GP, RECON, PM, 1><<<grid, c.block, 0, st>>>(a);
+    else k_ks<R, SP, GP, RECON, PM, 0><<<grid, c.block, 0, st>>>(a);
 }
 
-template <typename R, int SP, int RECON>
+template <typename R, int SP, int GP, int RECON>
 void ks_launch_t(const KSArgs& a, hipStream_t st, const KSCfg& c) {
-    if (c.pm == 2) ks_launch_pm<R, SP, RECON, 2>(a, st, c);
-    else ks_launch_pm<R, SP, RECON, 0>(a, st, c);
+    if (c.pm == 2) ks_launch_pm<R, SP, GP, RECON, 2>(a, st, c);
+    else ks_launch_pm<R, SP, GP, RECON, 0>(a, st, c);
 }
 
-int ks_launch_cfg(const KSArgs& a, int prec, bool r12, hipStream_t st, const KSCfg& c) {
+// spinor precision prec, link precision gprec: equal, or FP32 spinors with FP16 links
+int ks_launch_cfg(const KSArgs& a, int prec, int gprec, bool r12, hipStream_t st, const KSCfg& c) {
     if (prec == CLG_F64) {
-        if (r12) ks_launch_t<double, CLG_F64, 12>(a, st, c);
-        else ks_launch_t<double, CLG_F64, 18>(a, st, c);
+        if (r12) ks_launch_t<double, CLG_F64, CLG_F64, 12>(a, st, c);
+        else ks_launch_t<double, CLG_F64, CLG_F64, 18>(a, st, c);
+    } else if (prec == CLG_F32 && gprec == CLG_F16) {
+        if (r12) ks_launch_t<float, CLG_F32, CLG_F16, 12>(a, st, c);
+        else ks_launch_t<float, CLG_F32, CLG_F16, 18>(a, st, c);
     } else if (prec == CLG_F32) {
-        if (r12) ks_launch_t<float, CLG_F32, 12>(a, st, c);
-        else ks_launch_t<float, CLG_F32, 18>(a, st, c);
+        if (r12) ks_launch_t<float, CLG_F32, CLG_F32, 12>(a, st, c);
+        else ks_launch_t<float, CLG_F32, CLG_F32, 18>(a, st, c);
     } else {
-        if (r12) ks_launch_t<float, CLG_F16, 12>(a, st, c);
-        else ks_launch_t<float, CLG_F16, 18>(a, st, c);
+        if (r12) ks_launch_t<float, CLG_F16, CLG_F16, 12>(a, st, c);
+        else ks_launch_t<float, CLG_F16, CLG_F16, 18>(a, st, c);
     }
     CLG_HIP(hipGetLastError());
     return CLG_OK;
@@ -172,7 +177,7 @@ int ks_launch_cfg(const KSArgs& a, int prec, bool r12, hipStream_t st, const KSC
 // per-shape tuner: the first launch of a (precision, links, lattice) key times each candidate with
 // HIP events on the launch stream and keeps the fastest (every candidate writes the same output)
 struct KSKey {
-    int prec, r12, mode, L0, L1, L2, L3;
+    int prec, gprec, r12, mode, L0, L1, L2, L3;
     bool operator<(const KSKey& o) const { return std::memcmp(this, &o, sizeof(KSKey)) < 0; }
 };
 std::map<KSKey, KSCfg>& ks_table() {
@@ -192,16 +197,18 @@ bool ks_valid(const KSCfg& c) {
     return (c.block == 64 || c.block == 128 || c.block == 256) && (c.xcd == 0 || c.xcd == 8) && (c.pm == 0 || c.pm == 2);
 }
 
-int ks_launch(const KSArgs& a, int prec, bool r12, hipStream_t st) {
+int ks_launch(const KSArgs& a, int prec, int gprec, bool r12, hipStream_t st) {
     KSCfg cfg{256, 8, 0};
     // CLG_TUNE_FORCE_KS="block,xcd,pin" fixes the launch configuration (testing and profiling)
     if (const char* e = std::getenv("CLG_TUNE_FORCE_KS")) {
         KSCfg f{0, 0, 0};
-        if (std::sscanf(e, "%d,%d,%d", &f.block, &f.xcd, &f.pm) == 3 && ks_valid(f)) return ks_launch_cfg(a, prec, r12, st, f);
+        if (std::sscanf(e, "%d,%d,%d", &f.block, &f.xcd, &f.pm) == 3 && ks_valid(f))
+            return ks_launch_cfg(a, prec, gprec, r12, st, f);
     }
     KSKey k;
     std::memset(&k, 0, sizeof(k));
-    k.prec = prec; k.r12 = r12; k.mode = a.mode; k.L0 = a.g.L[0]; k.L1 = a.g.L[1]; k.L2 = a.g.L[2]; k.L3 = a.g.L[3];
+    k.prec = prec; k.gprec = gprec; k.r12 = r12; k.mode = a.mode;
+    k.L0 = a.g.L[0]; k.L1 = a.g.L[1]; k.L2 = a.g.L[2]; k.L3 = a.g.L[3];
     bool need_tune = false;
     const std::string cpath = tune_cache_path();
     {
@@ -214,11 +221,11 @@ int ks_launch(const KSArgs& a, int prec, bool r12, hipStream_t st) {
                 const std::vector<long>& v = e.first;
                 const std::vector<long>& c = e.second;
                 const KSCfg kc{(int)c[0], c.size() > 1 ? (int)c[1] : -1, c.size() > 2 ? (int)c[2] : -1};
-                if (v.size() != 7 || c.size() != 3 || !ks_valid(kc)) continue;
+                if (v.size() != 8 || c.size() != 3 || !ks_valid(kc)) continue;
                 KSKey t;
                 std::memset(&t, 0, sizeof(t));
-                t.prec = (int)v[0]; t.r12 = (int)v[1]; t.mode = (int)v[2];
-                t.L0 = (int)v[3]; t.L1 = (int)v[4]; t.L2 = (int)v[5]; t.L3 = (int)v[6];
+                t.prec = (int)v[0]; t.gprec = (int)v[1]; t.r12 = (int)v[2]; t.mode = (int)v[3];
+                t.L0 = (int)v[4]; t.L1 = (int)v[5]; t.L2 = (int)v[6]; t.L3 = (int)v[7];
                 ks_table()[t] = kc;
             }
             ks_loaded() = cpath;
@@ -238,9 +245,9 @@ int ks_launch(const KSArgs& a, int prec, bool r12, hipStream_t st) {
         CLG_HIP(hipEventCreate(&e1));
         float best = 1e30f;
         for (const KSCfg& c : cands) {
-            CLG_TRY(ks_launch_cfg(a, prec, r12, st, c));  // warm
+            CLG_TRY(ks_launch_cfg(a, prec, gprec, r12, st, c));  // warm
             CLG_HIP(hipEventRecord(e0, st));
-            for (int r = 0; r < 3; ++r) CLG_TRY(ks_launch_cfg(a, prec, r12, st, c));
+            for (int r = 0; r < 3; ++r) CLG_TRY(ks_launch_cfg(a, prec, gprec, r12, st, c));
             CLG_HIP(hipEventRecord(e1, st));
             CLG_HIP(hipEventSynchronize(e1));
             float ms = 0.f;
@@ -251,14 +258,15 @@ int ks_launch(const KSArgs& a, int prec, bool r12, hipStream_t st) {
         (void)hipEventDestroy(e1);
         const char* verbose = std::getenv("CLG_TUNE_VERBOSE");
         if (verbose && verbose[0] == '1')
-            std::fprintf(stderr, "[clg tune] staggered D prec=%d recon=%d mode=%d L=%dx%dx%dx%d -> block=%d xcd=%d pin=%d (%.1f us)\n",
-                         prec, r12 ? 12 : 18, a.mode, a.g.L[0], a.g.L[1], a.g.L[2], a.g.L[3], cfg.block, cfg.xcd, cfg.pm,
-                         best * 1e3 / 3);
+            std::fprintf(stderr, "[clg tune] staggered D prec=%d gprec=%d recon=%d mode=%d L=%dx%dx%dx%d -> block=%d xcd=%d pin=%d (%.1f us)\n",
+                         prec, gprec, r12 ? 12 : 18, a.mode, a.g.L[0], a.g.L[1], a.g.L[2], a.g.L[3], cfg.block, cfg.xcd,
+                         cfg.pm, best * 1e3 / 3);
         std::lock_guard<std::mutex> lk(ks_mutex());
         ks_table()[k] = cfg;
-        tune_cache_append(cpath, "staggered_d", {k.prec, k.r12, k.mode, k.L0, k.L1, k.L2, k.L3}, {cfg.block, cfg.xcd, cfg.pm});
+        tune_cache_append(cpath, "staggered_d", {k.prec, k.gprec, k.r12, k.mode, k.L0, k.L1, k.L2, k.L3},
+                          {cfg.block, cfg.xcd, cfg.pm});
     }
-    return ks_launch_cfg(a, prec, r12, st, cfg);
+    return ks_launch_cfg(a, prec, gprec, r12, st, cfg);
 }
 
 }  // namespace
@@ -276,8 +284,9 @@ int ks_tune_entry(int index, char* buf, int len) {
     auto it = ks_table().begin();
     std::advance(it, index);
     const KSKey& k = it->first;
-    std::snprintf(buf, (size_t)len, "staggered_d prec=%d recon=%d mode=%d dims=%dx%dx%dx%d -> block=%d xcd=%d pin=%d",
-                  k.prec, k.r12 ? 12 : 18, k.mode, k.L0, k.L1, k.L2, k.L3, it->second.block, it->second.xcd,
+    std::snprintf(buf, (size_t)len,
+                  "staggered_d prec=%d gauge_prec=%d recon=%d mode=%d dims=%dx%dx%dx%d -> block=%d xcd=%d pin=%d",
+                  k.prec, k.gprec, k.r12 ? 12 : 18, k.mode, k.L0, k.L1, k.L2, k.L3, it->second.block, it->second.xcd,
                   it->second.pm);
     return CLG_OK;
 }
@@ -306,7 +315,7 @@ int ks_full(clg_context ctx, clg_field_s* out, const clg_field_s* in, const clg_
     a.ci = ci;
     a.pbase = 0;
     a.nblk = 2;
-    return ks_launch(a, out->prec, gauge->kind == CLG_GAUGE12, ctx->stream);
+    return ks_launch(a, out->prec, gauge->prec, gauge->kind == CLG_GAUGE12, ctx->stream);
 }
 
 // out_p = a H_{p,1-p} in + b x_p on half fields (x nullable)
@@ -331,7 +340,7 @@ int ks_hop_half(clg_context ctx, clg_field_s* out, const clg_field_s* in, const 
         a.x = (const char*)x->data;
         a.xn = x->norm;
     }
-    return ks_launch(a, out->prec, gauge->kind == CLG_GAUGE12, ctx->stream);
+    return ks_launch(a, out->prec, gauge->prec, gauge->kind == CLG_GAUGE12, ctx->stream);
 }
 
 // the parity-p block of D D^dagger = (2am)^2 - H^2: out = (2am)^2 in - H_{p,1-p} H_{1-p,p} in
@@ -345,13 +354,16 @@ int ks_schur_once(clg_context ctx, clg_field_s* out, const clg_field_s* in, cons
 
 extern "C" {
 
+// links in the spinor precision, or FP16 links under FP32 spinors
+static bool ks_gauge_prec_ok(int prec, int gprec) { return gprec == prec || (prec == CLG_F32 && gprec == CLG_F16); }
+
 int clg_ks_apply(clg_context ctx, clg_field out, clg_field in, clg_field gauge, double two_am, int op, int coeff_type,
                  double cre, double cim) {
     CLG_CHECK_ARG(ctx && out && in && gauge, "null argument");
     CLG_CHECK_ARG(out != in && out->data != in->data, "out and in must be distinct fields");
     CLG_CHECK_ARG(out->kind == CLG_KS && in->kind == CLG_KS, "need staggered fields");
     CLG_CHECK_ARG(out->subset == CLG_FULL && in->subset == CLG_FULL, "need FULL fields");
-    CLG_CHECK_ARG(out->prec == in->prec && gauge->prec == in->prec, "precision mismatch");
+    CLG_CHECK_ARG(out->prec == in->prec && ks_gauge_prec_ok(in->prec, gauge->prec), "precision mismatch");
     CLG_CHECK_ARG(is_gauge(gauge->kind), "need a gauge field");
     CLG_TRY(check_same_lattice(out, in));
     CLG_TRY(check_same_lattice(out, gauge));
@@ -379,7 +391,7 @@ int clg_ks_hop(clg_context ctx, clg_field out, clg_field in, clg_field gauge, do
     CLG_CHECK_ARG(out->kind == CLG_KS && in->kind == CLG_KS, "need staggered fields");
     CLG_CHECK_ARG(out->subset != CLG_FULL && in->subset != CLG_FULL && out->subset != in->subset,
                   "the hop maps one parity (half field) to the other");
-    CLG_CHECK_ARG(out->prec == in->prec && gauge->prec == in->prec, "precision mismatch");
+    CLG_CHECK_ARG(out->prec == in->prec && ks_gauge_prec_ok(in->prec, gauge->prec), "precision mismatch");
     CLG_CHECK_ARG(is_gauge(gauge->kind), "need a gauge field");
     CLG_CHECK_ARG(out->data != in->data, "out and in must be distinct");
     CLG_TRY(check_same_lattice(out, in));
@@ -395,7 +407,7 @@ int clg_ks_schur(clg_context ctx, clg_field out, clg_field in, clg_field gauge, 
     CLG_CHECK_ARG(ctx && out && in && gauge, "null argument");
     CLG_CHECK_ARG(out->kind == CLG_KS && in->kind == CLG_KS, "need staggered fields");
     CLG_CHECK_ARG(in->subset != CLG_FULL && out->subset == in->subset, "the Schur block acts on one parity");
-    CLG_CHECK_ARG(out->prec == in->prec && gauge->prec == in->prec, "precision mismatch");
+    CLG_CHECK_ARG(out->prec == in->prec && ks_gauge_prec_ok(in->prec, gauge->prec), "precision mismatch");
     CLG_CHECK_ARG(is_gauge(gauge->kind), "need a gauge field");
     CLG_CHECK_ARG(out->data != in->data, "out and in must be distinct");
     CLG_TRY(check_same_lattice(out, in));

@@ -242,8 +242,9 @@ typedef struct {
                               complement (2: its normal equations).  Staggered: D D^dagger is solved
                               as its two parity blocks (2am)^2 - H H, each from a zero guess; D and
                               D^dagger solves become x = D^dagger y or D y with D D^dagger y = b */
-    int inner_prec;        /* CLG_F64: plain solve; CLG_F32 / CLG_F16 (Wilson) / CLG_INNER_F16_VECTORS
-                              (Wilson GMRES): defect correction */
+    int inner_prec;        /* CLG_F64: plain solve; CLG_F32 / CLG_F16 (FP32 vectors, FP16 12-real links;
+                              Wilson and staggered) / CLG_INNER_F16_VECTORS (Wilson GMRES): defect
+                              correction */
     double inner_accuracy; /* relative residual target of each inner solve (e.g. 1e-5) */
     int max_outer;         /* defect-correction outer iterations (mixed precision) */
     int kind;              /* CLG_WILSON or CLG_KS */
@@ -316,7 +317,8 @@ typedef enum {
  * A = op of the params.kind fermion and c_k = shifts[2k] + i shifts[2k+1].  CLG_MS_FOM follows
  * CMultiShiftFOM.cpp:97-270 (params max_dim <= 60, restart, accuracy, absolute_accuracy), CLG_MS_BICGSTAB
  * follows CMultiShiftBiCGStab.cpp:73-270 (max_step, deviation_step, accuracy, absolute_accuracy).
- * prec CLG_F32 runs the whole solve on FP32 copies (solutions written back as FP64).  The xs must
+ * prec CLG_F32 runs the whole solve on FP32 copies (solutions written back as FP64); CLG_F16 the same
+ * with FP16 12-real copies of the links (not for CLG_MS_NESTED).  The xs must
  * not alias b.  Returns CLG_NOT_CONVERGED with the iterates written when the budget runs out. */
 int clg_multishift_solve(clg_context ctx, const clg_solver_params* params, int kind, int prec, clg_field* xs,
                          const double* shifts, int nshift, clg_field b, clg_field gauge, double kappa_or_mass,
@@ -414,7 +416,8 @@ typedef struct {
     double energy_accuracy;   /* accuracy of the energy solves; 0 = solver.accuracy */
     int ms_kind;              /* KS: clg_multishift_kind */
     int ms_prec;              /* KS: CLG_F64, or CLG_F32 multi-shift (FP64 shift sums) in the force and
-                                 heat-bath solves; the energy solves are always FP64 */
+                                 heat-bath solves, or CLG_F16: force solves with FP32 vectors and FP16
+                                 12-real links, heat bath in CLG_F32; the energy solves are always FP64 */
 } clg_action;
 
 /* IntegratorType */

@@ -332,3 +332,18 @@ def gcrodr(kind, op, dims, bc, m_or_kappa, U, b, m=10, k=8, restart=100, recalc=
                           C.c_double(accuracy), absolute, phase, C.byref(cyc), C.byref(dev))
     assert rc in (0, 1)
     return x, rc, cyc.value, dev.value, urec, hu.value
+
+
+def fp16_links_12(U):
+    """The links as FP16 12-real storage holds them, up to the last bit of the rounding: rows 0 and 1 rounded
+    to FP16, row 2 rebuilt as the complex conjugate of their cross product."""
+    m = U.reshape(-1, 9, 2)
+    r = m[:, :6, :].astype(np.float16).astype(np.float64)
+    c = r[..., 0] + 1j * r[..., 1]
+    row2 = np.conj(np.stack([c[:, 1] * c[:, 5] - c[:, 2] * c[:, 4], c[:, 2] * c[:, 3] - c[:, 0] * c[:, 5],
+                             c[:, 0] * c[:, 4] - c[:, 1] * c[:, 3]], axis=1))
+    out = np.empty_like(m)
+    out[:, :6, :] = r
+    out[:, 6:, 0] = row2.real
+    out[:, 6:, 1] = row2.imag
+    return out.reshape(U.shape)

@@ -138,3 +138,35 @@ def test_rhmc_trajectory_fp32_forces_against_fp64(ctx, ms_kind):
     # same start, momenta and pseudofermions: H before agrees to the energy solves' accuracy
     assert abs(dh["fp32"][1] - dh["fp64"][1]) < 1e-6 * abs(dh["fp64"][1])
     assert abs(dh["fp32"][0] - dh["fp64"][0]) < 2e-3
+
+
+def test_rhmc_fp16_link_forces(ctx):
+    """The c4 trajectory with force solves on FP32 vectors and FP16 12-real links: the heat bath keeps
+    full-precision links, so H before equals that of the FP32-force trajectory from the same start; the
+    forces come from an operator rounded at 2^-11, which moves the end point (Delta H) more than FP32 solves
+    do, but the energies are FP64 and the trajectory stays reversible in form."""
+    dims = (24, 24, 24, 8)
+    shifts = np.logspace(-4, 1, 12)
+    flavours = [(mass, fit_rational(-nf / 4.0, shifts, mass * mass, 20.0), fit_rational(nf / 8.0, shifts, mass * mass,
+                                                                                          20.0))
+                for mass, nf in ((0.01, 2), (0.05, 1))]
+    res = {}
+    for prec in (clg.F32, clg.F16):
+        g = clg.Field(ctx, clg.GAUGE, clg.F64, dims)
+        g.gauge_random(0.4, 1)
+        acts = [dict(kind=clg.ACTION_GAUGE_PLAQUETTE, beta=5.5)]
+        phis = []
+        for mass, md, mc in flavours:
+            phi = clg.Field(ctx, clg.KS, clg.F64, dims, bc=BC)
+            phis.append(phi)
+            sp = clg.solver_params(kind=clg.KS, max_dim=40, restart=200, absolute_accuracy=1, accuracy=1e-5)
+            acts.append(dict(kind=clg.ACTION_FERMION_KS, kappa_or_mass=mass, phi=phi, md=md, mc=mc, solver=sp,
+                             ms_kind=clg.MS_BICGSTAB, ms_prec=prec, energy_accuracy=1e-9))
+        hp = clg.hmc_params(integrator=clg.INT_NESTED_FORCE_GRADIENT, step_length=1.0, steps=8, nested_steps=6,
+                            exp_precision=8, metropolis=1)
+        rep = clg.hmc_trajectory(ctx, g, hp, acts, 1)
+        print(f"prec {prec}: dH {rep.h_diff:.6f} (H {rep.h_before:.4f} -> {rep.h_after:.4f}), {rep.seconds:.2f} s")
+        assert rep.unconverged_solves == 0 and np.isfinite(rep.h_diff)
+        res[prec] = rep
+    assert res[clg.F16].h_before == res[clg.F32].h_before
+    assert abs(res[clg.F16].h_diff) < 1.0

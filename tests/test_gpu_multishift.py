@@ -105,3 +105,30 @@ def test_rational_apply_matches_oracle_and_inverts(ctx):
     clg.rational_apply(ctx, p, t, fb, g, 0.3, clg.EFO_F_DDdagger, up)
     clg.rational_apply(ctx, p, t, t, g, 0.3, clg.EFO_F_DDdagger, down)
     assert np.linalg.norm(t.download() - b) / np.linalg.norm(b) < 5e-3
+
+
+@pytest.mark.parametrize("solver", [clg.MS_FOM, clg.MS_BICGSTAB])
+def test_ks_fp16_link_multishift(ctx, solver):
+    """Shifted solves with FP32 vectors and FP16 12-real links solve the shifted systems of the operator with
+    FP16-rounded links: against that operator (links rounded on the host the way the device stores them) the
+    residuals are at the FP32 solve's level, as in the plain FP32 case; against the FP64 links they are at
+    the links' rounding level."""
+    U, b, g, fb = setup(ctx, clg.KS)
+    c = REMEZ["MinusOneOver2"]
+    shifts = c[4:]
+    xs = [clg.Field(ctx, clg.KS, clg.F64, DIMS, bc=orc.BC_FERMION) for _ in shifts]
+    p = clg.solver_params(kind=clg.KS, max_dim=30, restart=50, max_step=400, deviation_step=5, accuracy=1e-5,
+                          absolute_accuracy=1)
+    st, rep = clg.multishift_solve(ctx, p, xs, shifts, fb, g, 0.1, clg.EFO_F_DDdagger, kind=solver, prec=clg.F16)
+    assert st == clg.OK
+    U16 = orc.fp16_links_12(U)
+    worst16 = worst64 = 0.0
+    for k, s in enumerate(shifts):
+        x = xs[k].download()
+        r16 = apply_shifted(clg.KS, orc.DDDAG, U16, x, s, 0.1) - orc.as_complex(b)
+        r64 = apply_shifted(clg.KS, orc.DDDAG, U, x, s, 0.1) - orc.as_complex(b)
+        worst16 = max(worst16, np.linalg.norm(r16) / np.linalg.norm(b))
+        worst64 = max(worst64, np.linalg.norm(r64) / np.linalg.norm(b))
+    print(f"FP16-link shifted solutions: max residual {worst16:.2e} with the FP16-rounded links, {worst64:.2e} "
+          f"with the FP64 links")
+    assert worst16 < 1e-3 and worst64 < 3e-2
