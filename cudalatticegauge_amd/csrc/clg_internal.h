@@ -380,15 +380,18 @@ __device__ __forceinline__ bool map_site(int Vh, int nblk, int pbase, int xcd, i
 // ------------------------------------------------------------------ storage traits
 // Every field is a uniform base pointer plus a set of planes.  An element index e selects
 // one site inside the planes (e = block * NPL * Vh + h for a field of several blocks), and
-// plane k of that site sits at base + k*Vh*CH + e*CH.  The per-lane part is a 32-bit byte
-// offset so the compiler can use the scalar-base + vector-offset load form.
+// plane k of that site sits at base + k*Vh*CH + e*CH.  The element index is 32-bit; the byte
+// offset is formed in 64 bits, so a field may exceed 4 GiB.
+__device__ __forceinline__ size_t plane_off(int Vh, int k, unsigned e, size_t ch) {
+    return ((size_t)k * (size_t)Vh + (size_t)e) * ch;
+}
 template <typename T>
 __device__ __forceinline__ T ldp(const char* base, int Vh, int k, unsigned e) {
-    return *reinterpret_cast<const T*>(base + (size_t)k * (size_t)Vh * sizeof(T) + (unsigned)(e * (unsigned)sizeof(T)));
+    return *reinterpret_cast<const T*>(base + plane_off(Vh, k, e, sizeof(T)));
 }
 template <typename T>
 __device__ __forceinline__ void stp(char* base, int Vh, int k, unsigned e, T v) {
-    *reinterpret_cast<T*>(base + (size_t)k * (size_t)Vh * sizeof(T) + (unsigned)(e * (unsigned)sizeof(T))) = v;
+    *reinterpret_cast<T*>(base + plane_off(Vh, k, e, sizeof(T))) = v;
 }
 
 // Non-temporal form of ldp (the "nt" cache policy: the line is marked for early eviction), for data
@@ -404,8 +407,7 @@ template <typename T, bool NT>
 __device__ __forceinline__ T ldq(const char* base, int Vh, int k, unsigned e) {
     if constexpr (NT) {
         typedef typename NtVec<T>::V V;
-        const V* p = reinterpret_cast<const V*>(base + (size_t)k * (size_t)Vh * sizeof(T) +
-                                                (unsigned)(e * (unsigned)sizeof(T)));
+        const V* p = reinterpret_cast<const V*>(base + plane_off(Vh, k, e, sizeof(T)));
         const V v = __builtin_nontemporal_load(p);
         T out;
         __builtin_memcpy(&out, &v, sizeof(T));
@@ -421,7 +423,7 @@ template <typename T, bool NT>
 __device__ __forceinline__ void stq(char* base, int Vh, int k, unsigned e, T v) {
     if constexpr (NT) {
         typedef typename NtVec<T>::V V;
-        V* p = reinterpret_cast<V*>(base + (size_t)k * (size_t)Vh * sizeof(T) + (unsigned)(e * (unsigned)sizeof(T)));
+        V* p = reinterpret_cast<V*>(base + plane_off(Vh, k, e, sizeof(T)));
         V x;
         __builtin_memcpy(&x, &v, sizeof(T));
         __builtin_nontemporal_store(x, p);

@@ -643,11 +643,8 @@ int clg_field_create(clg_context ctx, int kind, int prec, int subset, const int 
     f->block_bytes = site_bytes(kind, prec) * (size_t)f->Vh;
     const int nblk = is_gauge(kind) ? 8 : (subset == CLG_FULL ? 2 : 1);
     f->bytes = f->block_bytes * nblk;
-    if (f->bytes >= (size_t(1) << 32)) {  // kernels address a field with 32-bit byte offsets
-        delete f;
-        set_error("field of %ld sites exceeds the 4 GiB per-field limit", V);
-        return CLG_BAD_ARG;
-    }
+    // kernels address a field by a 32-bit element index (block, plane and site; below 72 * Vh) and a
+    // 64-bit byte offset, so its size is bounded by the site-count check above, not by 4 GiB
     (void)hipSetDevice(ctx->device);
     hipError_t e = hipMalloc(&f->data, f->bytes);
     if (e != hipSuccess) {
