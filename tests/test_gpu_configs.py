@@ -43,6 +43,17 @@ def c2(ctx):
     return dims, kappa, g, b, g.download(), b.download()
 
 
+@pytest.fixture(scope="module")
+def c2_oracle(c2):
+    """The oracle's FP64 solution of the c2 system (full-lattice GMRES(12) to a true residual below 1e-12 on
+    the host cores, about 20 s)."""
+    dims, kappa, g, b, U, bh = c2
+    xo, rc, cyc, res = orc.gmres(0, orc.D, dims, BC, kappa, U, bh, max_dim=12, restart=400, accuracy=1e-12,
+                                 stop_rule=1)
+    assert rc == 0 and res < 1e-12
+    return xo
+
+
 @pytest.mark.parametrize("inner,inner_acc", [(clg.F32, 1e-4), (clg.F16, 1e-3), (clg.INNER_F16_VECTORS, 1e-2)])
 def test_mixed_solves_at_32x32x32x32_kappa0137(ctx, c2, inner, inner_acc):
     dims, kappa, g, b, U, bh = c2
@@ -57,6 +68,21 @@ def test_mixed_solves_at_32x32x32x32_kappa0137(ctx, c2, inner, inner_acc):
     assert st == clg.OK
     assert res < 1e-10
     assert abs(res - rep.true_residual) < 1e-2 * res
+
+
+def test_c2_solutions_against_the_oracle_solution(ctx, c2, c2_oracle):
+    """c2's error against the FP64 oracle solution: the plain FP64 solve and the FP16-vector mixed solve (the
+    fastest, lowest-precision inner form) to |r|/|b| < 1e-10 are both within 1e-9 of it (|D^-1| at kappa
+    0.137 on this field keeps the error within a few times 1e-10)."""
+    dims, kappa, g, b, U, bh = c2
+    for inner, inner_acc in ((clg.F64, 0.0), (clg.INNER_F16_VECTORS, 1e-2)):
+        x = clg.Field(ctx, clg.WILSON, clg.F64, dims, bc=BC)
+        p = clg.solver_params(max_dim=12, restart=2000, accuracy=1e-10, stop_rule=clg.STOP_TRUE_RESIDUAL,
+                              even_odd=1, inner_prec=inner, inner_accuracy=inner_acc, max_outer=80)
+        st, rep = clg.solve(ctx, p, x, b, g, kappa, clg.EFO_F_InverseD)
+        err = np.linalg.norm(x.download() - c2_oracle) / np.linalg.norm(c2_oracle)
+        print(f"inner {inner}: |x - x_oracle| / |x_oracle| = {err:.2e} (true residual {rep.true_residual:.2e})")
+        assert st == clg.OK and err < 1e-9
 
 
 LATTICES = [((24, 24, 24, 48), 12), ((8, 8, 8, 8), 1), ((12, 12, 12, 12), 1), ((16, 16, 16, 32), 1),
