@@ -200,7 +200,16 @@ def test_precision_conversion(ctx, c0):
     assert rel(h.download(), psi) < 2e-3
 
 
-@pytest.mark.parametrize("dims", [(4, 6, 8, 16), (4, 4, 16, 16)])
+def tiled_fits(dims, xcd):
+    """Whether tiled plane schedule xcd = 10*TY + m fits the lattice: 8/TY tiles along x, TY along y."""
+    TY = xcd // 10
+    return dims[0] % (8 // TY) == 0 and dims[1] % TY == 0
+
+
+TILED = (20, 21, 40, 41, 80, 81)
+
+
+@pytest.mark.parametrize("dims", [(4, 6, 8, 16), (4, 4, 16, 16), (8, 8, 8, 16)])
 def test_lds_staged_launches(ctx, dims):
     """Pin modes 5..8 stage the z and t neighbours in LDS (one workgroup per (x,y) plane of one
     parity, block = L2*L3/2: 64 and 128 here), also with several planes per workgroup in turn
@@ -223,9 +232,10 @@ def test_lds_staged_launches(ctx, dims):
     cases = ((f64, g64, 1e-12), (f32, g32, 1e-6), (f16, g16, 1e-2))
     ref_eo = refs[False].reshape(-1, 24)
     try:
-        for pm, blk, xcd, spt in ((1, plane, 8, 1), (5, plane, 8, 1), (6, plane, 8, 1), (6, plane, 0, 1),
+        tiled = [(5 + i % 4, plane, xcd, 1) for i, xcd in enumerate(TILED) if tiled_fits(dims, xcd)]
+        for pm, blk, xcd, spt in [(1, plane, 8, 1), (5, plane, 8, 1), (6, plane, 8, 1), (6, plane, 0, 1),
                                   (5, plane, 1, 1), (7, plane, 8, 1), (8, plane, 0, 1),
-                                  (6, plane, 8, 2), (8, plane, 8, 4), (5, plane, 0, 4)):
+                                  (6, plane, 8, 2), (8, plane, 8, 4), (5, plane, 0, 4)] + tiled:
             clg.autotune_force(pm, blk, xcd, spt)
             for f, g, tol in cases:
                 for dag in (False, True):
@@ -249,6 +259,14 @@ def test_lds_staged_launches(ctx, dims):
             oute = clg.Field(ctx, clg.WILSON, clg.F64, dims, subset=clg.EVEN, bc=orc.BC_FERMION)
             with pytest.raises(clg.CLGError):
                 clg.wilson_hop(ctx, oute, f64.parity(clg.ODD), g64, dagger=False, a=-KAPPA)
+        # so is a tiled schedule whose tiles do not divide the plane grid
+        for xcd in TILED:
+            if not tiled_fits(dims, xcd):
+                clg.autotune_force(6, plane, xcd, 1)
+                out = clg.Field(ctx, clg.WILSON, clg.F64, dims)
+                with pytest.raises(clg.CLGError):
+                    clg.wilson_apply(ctx, out, f64, g64, KAPPA, clg.EFO_F_D)
+                assert not np.any(out.download())
     finally:
         clg.autotune_force(0)
 
