@@ -9,6 +9,7 @@
 #include <array>
 #include <cerrno>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -114,6 +115,145 @@ void bridge_to_host(const int dims[4], const double* b, double* host) {
                 }
 }
 
+// ---- the compressed format: one SU(3) logarithm per link ---------------------------------------
+// CFieldGaugeSU3::SaveToCompressedFile / InitialWithByteCompressed (CFieldGaugeSU3.cu:1225-1265,
+// 1651-1690) store, per link in the CLG binary link order, 9 reals of A = log U (principal logarithm
+// of each eigenvalue, deviceSU3::Log, SU3.h:1948-1966): Re/Im A01, Re/Im A02, Re/Im A12, Im A00,
+// Im A11, Im A22.  A is anti-Hermitian (A10 = -conj A01, ...; the real parts of the diagonal are 0)
+// and its trace may be +-2 pi i.  Loading rebuilds U = exp(A) exactly (deviceSU3::StrictExp).  Both
+// directions here diagonalise a Hermitian matrix with cyclic complex Jacobi rotations.
+typedef std::complex<double> cplx;
+
+// H (Hermitian, row-major 3x3, overwritten) = V diag(w) V^H; the columns of V are the eigenvectors
+void herm3_eig(cplx H[9], double w[3], cplx V[9]) {
+    for (int i = 0; i < 9; ++i) V[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    double scale = 0.0;
+    for (int i = 0; i < 9; ++i) scale += std::norm(H[i]);
+    for (int sweep = 0; sweep < 64; ++sweep) {
+        const double off = std::norm(H[1]) + std::norm(H[2]) + std::norm(H[5]);
+        if (off <= 1e-34 * scale || off == 0.0) break;
+        static const int PQ[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+        for (const auto& pq : PQ) {
+            const int p = pq[0], q = pq[1];
+            const cplx c = H[3 * p + q];
+            const double ac = std::abs(c);
+            if (ac == 0.0) continue;
+            // J = diag(1, e^{-i phi}) on (p, q), then the real rotation that zeroes |c| (Numerical
+            // Recipes' Jacobi step): J_pp = co, J_pq = si, J_qp = -si e^{-i phi}, J_qq = co e^{-i phi}
+            const cplx ph = std::conj(c) / ac;
+            const double a = H[3 * p + p].real(), b = H[3 * q + q].real();
+            const double th = (b - a) / (2.0 * ac);
+            const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+            const double co = 1.0 / std::sqrt(t * t + 1.0), si = t * co;
+            cplx J[9];
+            for (int i = 0; i < 9; ++i) J[i] = (i % 4 == 0) ? 1.0 : 0.0;
+            J[3 * p + p] = co;
+            J[3 * p + q] = si;
+            J[3 * q + p] = -si * ph;
+            J[3 * q + q] = co * ph;
+            cplx T[9], N[9];
+            for (int i = 0; i < 3; ++i)  // T = H J
+                for (int j = 0; j < 3; ++j) T[3 * i + j] = H[3 * i] * J[j] + H[3 * i + 1] * J[3 + j] + H[3 * i + 2] * J[6 + j];
+            for (int i = 0; i < 3; ++i)  // H = J^H T
+                for (int j = 0; j < 3; ++j)
+                    N[3 * i + j] = std::conj(J[i]) * T[j] + std::conj(J[3 + i]) * T[3 + j] + std::conj(J[6 + i]) * T[6 + j];
+            for (int i = 0; i < 9; ++i) H[i] = N[i];
+            for (int i = 0; i < 3; ++i)  // V = V J
+                for (int j = 0; j < 3; ++j) T[3 * i + j] = V[3 * i] * J[j] + V[3 * i + 1] * J[3 + j] + V[3 * i + 2] * J[6 + j];
+            for (int i = 0; i < 9; ++i) V[i] = T[i];
+        }
+    }
+    for (int k = 0; k < 3; ++k) w[k] = H[4 * k].real();
+}
+
+// M = V diag(d) V^H
+void from_eig(const cplx V[9], const cplx d[3], cplx M[9]) {
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            cplx s = 0.0;
+            for (int k = 0; k < 3; ++k) s += V[3 * i + k] * d[k] * std::conj(V[3 * j + k]);
+            M[3 * i + j] = s;
+        }
+}
+
+// A = log U for unitary U: the eigenvectors of U (a normal matrix) are those of the Hermitian
+// (e^{-i f} U + e^{i f} U^H) / 2, whose eigenvalues cos(theta_k - f) coincide when an eigenvalue of U
+// sits at e^{-2 i f} (theta_j + theta_k = 2 f while the three sum to 0 mod 2 pi).  Near that point
+// the eigenvectors of the pair mix and V^H U V is not diagonal, so up to three angles f are tried and
+// the most diagonal V^H U V is kept; no unitary matrix is near the bad point of all three.
+void su3_log(const cplx U[9], cplx A[9]) {
+    cplx bestV[9], bestB[9];
+    double best = -1.0;
+    for (double f : {0.7236, 2.1817, -1.3109}) {
+        const cplx e = std::polar(1.0, -f);
+        cplx H[9], V[9];
+        double w[3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) H[3 * i + j] = 0.5 * (e * U[3 * i + j] + std::conj(e * U[3 * j + i]));
+        herm3_eig(H, w, V);
+        cplx B[9];  // V^H U V
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                cplx s = 0.0;
+                for (int k = 0; k < 3; ++k)
+                    for (int l = 0; l < 3; ++l) s += std::conj(V[3 * k + i]) * U[3 * k + l] * V[3 * l + j];
+                B[3 * i + j] = s;
+            }
+        const double offd = std::abs(B[1]) + std::abs(B[2]) + std::abs(B[3]) + std::abs(B[5]) + std::abs(B[6]) +
+                            std::abs(B[7]);
+        if (best < 0.0 || offd < best) {
+            best = offd;
+            for (int i = 0; i < 9; ++i) {
+                bestV[i] = V[i];
+                bestB[i] = B[i];
+            }
+        }
+        if (best < 1e-14) break;
+    }
+    cplx d[3];
+    for (int k = 0; k < 3; ++k) d[k] = cplx(0.0, std::arg(bestB[4 * k]));
+    from_eig(bestV, d, A);
+}
+
+// U = exp(A) for anti-Hermitian A: -i A is Hermitian with eigenvalues mu_k, U = V diag(e^{i mu}) V^H
+void ah3_exp(const cplx A[9], cplx U[9]) {
+    cplx H[9], V[9];
+    double w[3];
+    for (int i = 0; i < 9; ++i) H[i] = cplx(A[i].imag(), -A[i].real());
+    herm3_eig(H, w, V);
+    cplx d[3];
+    for (int k = 0; k < 3; ++k) d[k] = std::polar(1.0, w[k]);
+    from_eig(V, d, U);
+}
+
+// one link: 18 host reals (row-major complex 3x3) <-> 9 compressed reals
+void link_compress(const double* u, double* c) {
+    cplx U[9], A[9];
+    for (int i = 0; i < 9; ++i) U[i] = cplx(u[2 * i], u[2 * i + 1]);
+    su3_log(U, A);
+    c[0] = A[1].real(); c[1] = A[1].imag();
+    c[2] = A[2].real(); c[3] = A[2].imag();
+    c[4] = A[5].real(); c[5] = A[5].imag();
+    c[6] = A[0].imag(); c[7] = A[4].imag(); c[8] = A[8].imag();
+}
+void link_expand(const double* c, double* u) {
+    cplx A[9], U[9];
+    A[1] = cplx(c[0], c[1]);
+    A[3] = cplx(-c[0], c[1]);
+    A[2] = cplx(c[2], c[3]);
+    A[6] = cplx(-c[2], c[3]);
+    A[5] = cplx(c[4], c[5]);
+    A[7] = cplx(-c[4], c[5]);
+    A[0] = cplx(0.0, c[6]);
+    A[4] = cplx(0.0, c[7]);
+    A[8] = cplx(0.0, c[8]);
+    ah3_exp(A, U);
+    for (int i = 0; i < 9; ++i) {
+        u[2 * i] = U[i].real();
+        u[2 * i + 1] = U[i].imag();
+    }
+}
+
 }  // namespace
 }  // namespace clg
 
@@ -135,10 +275,6 @@ int clg_file_read(const char* path, int file_type, int kind, const int dims[4], 
     CLG_CHECK_ARG(dims[0] > 0 && dims[1] > 0 && dims[2] > 0 && dims[3] > 0, "bad lattice extents");
     const bool gauge = rps == 72;
     const long nreal = sites_of(dims) * rps;
-    if (file_type == CLG_FILE_CLG_BIN_COMPRESSED) {
-        set_error("CLG compressed files (SU(3) logarithms) are not supported");
-        return CLG_UNSUPPORTED;
-    }
     CLG_CHECK_ARG(file_type >= CLG_FILE_BRIDGEPP_TXT && file_type <= CLG_FILE_CLG_BIN_DOUBLE, "unknown file type %d",
                   file_type);
     CLG_CHECK_ARG(gauge || file_type == CLG_FILE_CLG_BIN,
@@ -168,6 +304,15 @@ int clg_file_read(const char* path, int file_type, int kind, const int dims[4], 
     }
     std::vector<unsigned char> bytes;
     CLG_CHECK_ARG(read_all(path, bytes), "cannot read %s", path);
+    if (file_type == CLG_FILE_CLG_BIN_COMPRESSED) {
+        // 9 doubles per link, rebuilt as U = exp(A) (CFieldGaugeSU3.cu:1225-1265)
+        const long links = sites_of(dims) * 4;
+        CLG_CHECK_ARG(bytes.size() == (size_t)links * 72, "%s holds %zu bytes, the lattice needs %zu (9 doubles a link)",
+                      path, bytes.size(), (size_t)links * 72);
+        const double* c = (const double*)bytes.data();
+        for (long l = 0; l < links; ++l) link_expand(c + 9 * l, host + 18 * l);
+        return CLG_OK;
+    }
     const size_t want = (size_t)nreal * (file_type == CLG_FILE_CLG_BIN_FLOAT ? 4 : 8);
     CLG_CHECK_ARG(bytes.size() == want, "%s holds %zu bytes, the lattice needs %zu", path, bytes.size(), want);
     if (file_type == CLG_FILE_BRIDGEPP_BIN) {
@@ -194,13 +339,22 @@ int clg_file_write(const char* path, int file_type, int kind, const int dims[4],
     const int rps = reals_per_site(kind);
     CLG_CHECK_ARG(rps > 0, "unknown field kind %d", kind);
     CLG_CHECK_ARG(dims[0] > 0 && dims[1] > 0 && dims[2] > 0 && dims[3] > 0, "bad lattice extents");
-    if (file_type != CLG_FILE_CLG_BIN && file_type != CLG_FILE_CLG_BIN_FLOAT && file_type != CLG_FILE_CLG_BIN_DOUBLE) {
+    if (file_type != CLG_FILE_CLG_BIN && file_type != CLG_FILE_CLG_BIN_FLOAT && file_type != CLG_FILE_CLG_BIN_DOUBLE &&
+        file_type != CLG_FILE_CLG_BIN_COMPRESSED) {
         set_error("saving supports the CLG binary formats only (CField.cpp:32-72)");
         return CLG_UNSUPPORTED;
     }
+    CLG_CHECK_ARG(file_type != CLG_FILE_CLG_BIN_COMPRESSED || rps == 72,
+                  "the compressed format holds gauge fields only (CFieldGaugeSU3.cu:1651-1690)");
     const long nreal = sites_of(dims) * rps;
     std::vector<unsigned char> bytes;
-    if (file_type == CLG_FILE_CLG_BIN_FLOAT) {
+    if (file_type == CLG_FILE_CLG_BIN_COMPRESSED) {
+        // 9 doubles of log U per link (CFieldGaugeSU3.cu:1651-1690)
+        const long links = sites_of(dims) * 4;
+        bytes.resize((size_t)links * 72);
+        double* c = (double*)bytes.data();
+        for (long l = 0; l < links; ++l) link_compress(host + 18 * l, c + 9 * l);
+    } else if (file_type == CLG_FILE_CLG_BIN_FLOAT) {
         bytes.resize((size_t)nreal * 4);
         float* f = (float*)bytes.data();
         for (long i = 0; i < nreal; ++i) f[i] = (float)host[i];

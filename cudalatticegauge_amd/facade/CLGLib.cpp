@@ -219,6 +219,9 @@ CField* CFieldGaugeSU3::GetCopy() const {
     CopyTo(c);
     return c;
 }
+std::string CFieldGaugeSU3::SaveToCompressedFile(const std::string& fileName) const {
+    return SaveToFile(fileName, EFFT_CLGBinCompressed);
+}
 
 CFieldFermion::CFieldFermion(CLatticeData* owner, int kind, Real param, const int bc[4])
     : CField(owner, kind), m_fParam(param) {

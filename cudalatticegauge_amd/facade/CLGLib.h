@@ -187,6 +187,9 @@ public:
     // Sum over plaquettes of (1 - ReTr U_p / 3) times 3 * betaOverN (CFieldGaugeSU3.cu:308-371)
     DOUBLE CalculatePlaqutteEnergy(DOUBLE betaOverN) const;
     CField* GetCopy() const override;
+    // the per-link logarithms, 9 reals a link (CFieldGaugeSU3.cu:1651-1690); returns the MD5 string;
+    // InitialFieldWithFile(name, EFFT_CLGBinCompressed) reads them back
+    std::string SaveToCompressedFile(const std::string& fileName) const;
 };
 
 // The fermion calls keep the reference's argument lists (CFieldFermion.h:20-246): the gauge and boson

@@ -14,6 +14,8 @@
 //                   D^{+-1/2}, D^{+-1/4} and of DD^dagger through the section's multi-shift solver
 //          fileio   TestFileIO (TestConfigurationFileIO.cpp:12-37): average plaquette of the gauge
 //                   field the section loads (EFIT_ReadFromFile), against ExpectedRes
+//          fileio_compressed  TestFileIOCLGCompressed (TestConfigurationFileIO.cpp:133-160): the gauge
+//                   field saved compressed and read back, |U' - U|^2
 //          operators    TestOperators (TestCommon.cpp:13-88): Gaussian-source norm estimators and
 //                   <f2, D f1> = <D^dagger f2, f1>
 //          gamma5   TestGamma5Hermiticity (TestCommon.cpp:155-168): the dense D checked for gamma5 D
@@ -200,6 +202,25 @@ static int run_fileio(CParameters& sec) {
     return 0;
 }
 
+// TestFileIOCLGCompressed's pattern (TestConfigurationFileIO.cpp:133-160): save the gauge field
+// compressed, read it into a new field, |U' - U|^2
+static int run_fileio_compressed(CParameters&) {
+    CLatticeData* L = appGetLattice();
+    const CFieldGaugeSU3* g = dynamic_cast<const CFieldGaugeSU3*>(L->m_pGaugeField[0]);
+    if (!g) return 2;
+    const auto t0 = std::chrono::steady_clock::now();
+    const std::string md5 = g->SaveToCompressedFile("testGaugeCompressed.con");
+    CFieldGaugeSU3* n = new CFieldGaugeSU3(L);
+    n->InitialFieldWithFile("testGaugeCompressed.con", EFFT_CLGBinCompressed);
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    n->AxpyMinus(g);
+    const CLGComplex res = n->DotReal(n);
+    delete n;
+    std::printf("{\"test\": \"fileio_compressed\", \"res\": [%.6e, %.6e], \"md5\": \"%s\", \"seconds\": %.3f}\n",
+                res.x, res.y, md5.c_str(), secs);
+    return 0;
+}
+
 static int run_operators(CParameters&) {
     CLatticeData* L = appGetLattice();
     const double V = (double)L->m_dims[0] * L->m_dims[1] * L->m_dims[2] * L->m_dims[3];
@@ -307,7 +328,7 @@ static void dump(const CParameters& p) {
 int main(int argc, char** argv) {
     if (argc < 4) {
         std::fprintf(stderr,
-                     "usage: %s <yaml> <section> <update|update_l|solver|mssolver_ks|fileio|operators|gamma5|"
+                     "usage: %s <yaml> <section> <update|update_l|solver|mssolver_ks|fileio|fileio_compressed|operators|gamma5|"
                      "antihermitian|gauge_invariance|dump>\n",
                      argv[0]);
         return 2;
@@ -331,6 +352,7 @@ int main(int argc, char** argv) {
         else if (t == "update_l") rc = run_update_l(sec);
         else if (t == "solver") rc = run_solver(sec);
         else if (t == "fileio") rc = run_fileio(sec);
+        else if (t == "fileio_compressed") rc = run_fileio_compressed(sec);
         else if (t == "mssolver_ks") rc = run_mssolver_ks(sec);
         else if (t == "operators") rc = run_operators(sec);
         else if (t == "gamma5") rc = run_gamma5(sec);

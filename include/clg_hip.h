@@ -110,7 +110,9 @@ typedef enum {
     CLG_FILE_BRIDGEPP_BIN = 1,       /* the same values as big-endian doubles */
     CLG_FILE_CLG_BIN = 2,            /* the host AoS order above in Real = double (a double build's
                                       * EFFT_CLGBin) */
-    CLG_FILE_CLG_BIN_COMPRESSED = 3, /* SU(3) logarithms: not supported (CLG_UNSUPPORTED) */
+    CLG_FILE_CLG_BIN_COMPRESSED = 3, /* gauge only: per link the 9 doubles of A = log U (principal
+                                      * logarithm; Re/Im A01, A02, A12, Im A00, A11, A22), read back as
+                                      * exp(A) (CFieldGaugeSU3.cu:1225-1265, 1651-1690) */
     CLG_FILE_CLG_BIN_FLOAT = 4,      /* the host AoS order in float */
     CLG_FILE_CLG_BIN_DOUBLE = 5      /* the host AoS order in double */
 } clg_file_type;
@@ -121,7 +123,7 @@ typedef enum {
  * number of text values) does not match the lattice is CLG_BAD_ARG. */
 int clg_file_read(const char* path, int file_type, int kind, const int dims[4], double* host);
 /* Host only.  Writes the host AoS order in one of the CLG binary formats (CField::SaveToFile,
- * CField.cpp:32-72); md5_out (NULL or 33 chars) receives the upper-case hex MD5 of the bytes
+ * CField.cpp:32-72; gauge fields also compressed, CFieldGaugeSU3::SaveToCompressedFile); md5_out (NULL or 33 chars) receives the upper-case hex MD5 of the bytes
  * written, as SaveToFile returns it (CLGMD5.h:566-590). */
 int clg_file_write(const char* path, int file_type, int kind, const int dims[4], const double* host,
                    char md5_out[33]);

@@ -9,6 +9,11 @@ testGaugeSingle.con_, copied unchanged under tests/golden/configs) and its expec
 - The CLG double and float files hold one 8^4 SU(3) configuration: sum |U|^2 = 49152 =
   3 x 4 links x 8^4 (TestFileDOUBLE, TestFileFloat), every link unitary, the two precisions agree.
 - MD5: RFC 1321's test suite and Python's hashlib on the bytes written.
+- The compressed format (CFieldGaugeSU3.cu:1225-1265, 1651-1690) stores 9 reals of the principal
+  logarithm A = log U per link.  The reference ships no compressed file (its test writes one and
+  reads it back), so the logarithm is pinned by closed forms (diagonal links, links conjugated by a
+  known unitary, a trace of 2 pi i) and by scipy's logm / expm; a file written here from scipy's
+  logm must read back as U, and a round trip must meet TestFileIOCLGCompressed's |U' - U|^2 < 1e-8.
 """
 import hashlib
 import os
@@ -107,3 +112,100 @@ def test_file_errors(tmp_path):
         clg.file_write(p, clg.FILE_BRIDGEPP_BIN, clg.GAUGE, (2, 2, 2, 2), np.zeros(16 * 72))
     with pytest.raises(clg.CLGError):
         clg.file_read(str(tmp_path / "missing.con"), clg.FILE_CLG_BIN, clg.GAUGE, (2, 2, 2, 2))
+
+
+def compressed_to_a(c):
+    """9 reals per link (Re/Im A01, A02, A12, Im A00, A11, A22) -> anti-Hermitian 3x3 matrices."""
+    c = c.reshape(-1, 9)
+    A = np.zeros((c.shape[0], 3, 3), dtype=complex)
+    A[:, 0, 1] = c[:, 0] + 1j * c[:, 1]
+    A[:, 0, 2] = c[:, 2] + 1j * c[:, 3]
+    A[:, 1, 2] = c[:, 4] + 1j * c[:, 5]
+    A[:, 1, 0] = -np.conj(A[:, 0, 1])
+    A[:, 2, 0] = -np.conj(A[:, 0, 2])
+    A[:, 2, 1] = -np.conj(A[:, 1, 2])
+    A[:, 0, 0], A[:, 1, 1], A[:, 2, 2] = 1j * c[:, 6], 1j * c[:, 7], 1j * c[:, 8]
+    return A
+
+
+def a_to_compressed(A):
+    return np.stack([A[:, 0, 1].real, A[:, 0, 1].imag, A[:, 0, 2].real, A[:, 0, 2].imag, A[:, 1, 2].real,
+                     A[:, 1, 2].imag, A[:, 0, 0].imag, A[:, 1, 1].imag, A[:, 2, 2].imag], axis=1).ravel()
+
+
+def host_from_links(u):
+    out = np.empty(u.shape[0] * 18)
+    z = u.reshape(-1)
+    out[0::2], out[1::2] = z.real, z.imag
+    return out
+
+
+def random_unitary(rng, n):
+    q, r = np.linalg.qr(rng.normal(size=(n, 3, 3)) + 1j * rng.normal(size=(n, 3, 3)))
+    return q * (np.diagonal(r, axis1=1, axis2=2) / np.abs(np.diagonal(r, axis1=1, axis2=2)))[:, None, :]
+
+
+def test_compressed_closed_forms(tmp_path):
+    dims = (1, 1, 1, 3)  # 12 links
+    rng = np.random.default_rng(21)
+    # the last four put an eigenvalue at e^{-2 i f} for the library's first, second and third angle f
+    # (0.7236, 2.1817, -1.3109), where that angle's Hermitian combination has a double eigenvalue, and
+    # one at two of them at once
+    th = np.array([[0.3, -0.5, 0.2], [2.5, 2.5, 2 * np.pi - 5.0], [-3.0, 1.0, 2.0 - 2 * np.pi + 2 * np.pi],
+                   [1e-9, -2e-9, 1e-9], [0.0, 0.0, 0.0], [1.2, 1.2, -2.4], [3.1, -1.5, -1.6], [-2.0, -2.0, 4.0 - 2 * np.pi],
+                   [-1.4472, 0.3, 1.1472], [-4.3634, -0.5, 4.8634], [2.6218, -1.0, -1.6218], [-1.4472, -4.3634, 5.8106]])
+    th = np.angle(np.exp(1j * th))  # principal values in (-pi, pi]
+    assert abs(th[1].sum() - 2 * np.pi) < 1e-12  # a link whose logarithm has trace 2 pi i
+    W = random_unitary(rng, 12)
+    W[:4] = np.eye(3)  # the first four links diagonal
+    A = np.einsum("nij,nj,nkj->nik", W, 1j * th, W.conj())
+    U = np.einsum("nij,nj,nkj->nik", W, np.exp(1j * th), W.conj())
+    p = str(tmp_path / "c.con")
+    md5 = clg.file_write(p, clg.FILE_CLG_BIN_COMPRESSED, clg.GAUGE, dims, host_from_links(U))
+    raw = np.fromfile(p, dtype="<f8")
+    assert raw.size == 12 * 9 and md5 == hashlib.md5(raw.tobytes()).hexdigest().upper()
+    # diagonal links store exactly their phases, zero off the diagonal
+    assert np.abs(raw.reshape(12, 9)[:4, :6]).max() < 1e-15
+    assert np.abs(raw.reshape(12, 9)[:4, 6:] - th[:4]).max() < 1e-14
+    assert np.abs(compressed_to_a(raw) - A).max() < 1e-13
+    back = links(clg.file_read(p, clg.FILE_CLG_BIN_COMPRESSED, clg.GAUGE, dims))
+    assert np.abs(back - U).max() < 1e-14
+
+
+@pytest.mark.parametrize("eps", [0.5, 2.0])
+def test_compressed_against_scipy_logm_expm(tmp_path, eps):
+    from scipy.linalg import expm, logm
+    dims = (2, 2, 4, 4)
+    U = orc.gauge_random(dims, eps, 4)
+    u = links(U)
+    p = str(tmp_path / "c.con")
+    clg.file_write(p, clg.FILE_CLG_BIN_COMPRESSED, clg.GAUGE, dims, U)
+    A = compressed_to_a(np.fromfile(p, dtype="<f8"))
+    assert A.shape[0] == u.shape[0]
+    ev = np.linalg.eigvals(u)
+    far = np.abs(np.angle(ev)).max(axis=1) < np.pi - 1e-6  # principal logarithm unambiguous
+    assert far.sum() > 0.9 * len(far)
+    for i in range(len(u)):
+        assert np.abs(expm(A[i]) - u[i]).max() < 1e-13, i
+        if far[i]:
+            assert np.abs(A[i] - logm(u[i])).max() < 1e-11, i
+    back = clg.file_read(p, clg.FILE_CLG_BIN_COMPRESSED, clg.GAUGE, dims)
+    assert np.abs(back - U).max() < 1e-13
+    # a file written independently from scipy's logm reads back as U
+    q = str(tmp_path / "s.con")
+    a_to_compressed(np.stack([logm(x) for x in u])).astype("<f8").tofile(q)
+    assert np.abs(clg.file_read(q, clg.FILE_CLG_BIN_COMPRESSED, clg.GAUGE, dims) - U).max() < 1e-13
+
+
+def test_compressed_round_trip_criterion(tmp_path):
+    # TestFileIOCLGCompressed (TestConfigurationFileIO.cpp:133-160): save compressed, load, |U' - U|^2 < 1e-8
+    dims = (8, 8, 8, 8)
+    U = orc.gauge_random(dims, 1.0, 9)
+    p = str(tmp_path / "c.con")
+    clg.file_write(p, clg.FILE_CLG_BIN_COMPRESSED, clg.GAUGE, dims, U)
+    d = clg.file_read(p, clg.FILE_CLG_BIN_COMPRESSED, clg.GAUGE, dims) - U
+    assert np.dot(d, d) < 1e-8 and np.abs(d).max() < 1e-14
+    with pytest.raises(clg.CLGError, match="gauge"):
+        clg.file_write(p, clg.FILE_CLG_BIN_COMPRESSED, clg.WILSON, dims, np.zeros(int(np.prod(dims)) * 24))
+    with pytest.raises(clg.CLGError, match="bytes"):
+        clg.file_read(p, clg.FILE_CLG_BIN_COMPRESSED, clg.GAUGE, (8, 8, 8, 4))

@@ -1,5 +1,6 @@
 """CLGTest sections of the reference's test suite replayed through the C++ facade (CLGLib class
 names over the HIP library), checked by the reference tests' own criteria and expected values."""
+import hashlib
 import json
 import os
 import subprocess
@@ -55,6 +56,16 @@ def test_fileio_sections(section, tmp_path):
     (tmp_path / "Release").mkdir()
     r = run(section, "fileio", cwd=str(tmp_path / "Release"))
     assert abs(r["plaquette"] - r["expected"]) < 1e-6, r
+
+
+@pytest.mark.parametrize("section", ["TestFileIOCLGCompressedDebug", "TestFileIOCLGCompressed"])
+def test_fileio_compressed_sections(section, tmp_path):
+    # TestFileIOCLGCompressed (TestConfigurationFileIO.cpp:133-160): a random gauge field (4^4, and 32^4
+    # in the release section) saved as per-link logarithms and read back differs by |U' - U|^2 < 1e-8
+    r = run(section, "fileio_compressed", cwd=str(tmp_path))
+    assert abs(complex(*r["res"])) < 1e-8, r
+    raw = (tmp_path / "testGaugeCompressed.con").read_bytes()
+    assert len(raw) % 72 == 0 and r["md5"] == hashlib.md5(raw).hexdigest().upper()
 
 
 @pytest.mark.parametrize("section", ["TestMSKSSolverFOM", "TestMSKSSolverBiCGStab", "TestMSKSSolverNested"])
