@@ -6,6 +6,7 @@
 // CFieldFermionWilsonSquareSU3.cu:602-633, 1144-1220).
 #include "clg_internal.h"
 
+#include <algorithm>
 #include <array>
 #include <cerrno>
 #include <cmath>
@@ -14,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace clg {
@@ -131,7 +133,7 @@ void herm3_eig(cplx H[9], double w[3], cplx V[9]) {
     for (int i = 0; i < 9; ++i) scale += std::norm(H[i]);
     for (int sweep = 0; sweep < 64; ++sweep) {
         const double off = std::norm(H[1]) + std::norm(H[2]) + std::norm(H[5]);
-        if (off <= 1e-34 * scale || off == 0.0) break;
+        if (off <= 1e-30 * scale) break;  // off-diagonal below 1e-15 of the matrix: rounding level
         static const int PQ[3][2] = {{0, 1}, {0, 2}, {1, 2}};
         for (const auto& pq : PQ) {
             const int p = pq[0], q = pq[1];
@@ -254,6 +256,23 @@ void link_expand(const double* c, double* u) {
     }
 }
 
+// f(l) for l in [0, n), split over up to 16 host threads (the compressed format's per-link
+// logarithms and exponentials are independent)
+template <class F>
+void parallel_links(long n, F f) {
+    const long nt = std::max(1L, std::min<long>({16L, (long)std::thread::hardware_concurrency(), n / 4096}));
+    if (nt <= 1) {
+        for (long l = 0; l < n; ++l) f(l);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (long t = 0; t < nt; ++t)
+        th.emplace_back([=]() {
+            for (long l = n * t / nt; l < n * (t + 1) / nt; ++l) f(l);
+        });
+    for (auto& x : th) x.join();
+}
+
 }  // namespace
 }  // namespace clg
 
@@ -310,7 +329,7 @@ int clg_file_read(const char* path, int file_type, int kind, const int dims[4], 
         CLG_CHECK_ARG(bytes.size() == (size_t)links * 72, "%s holds %zu bytes, the lattice needs %zu (9 doubles a link)",
                       path, bytes.size(), (size_t)links * 72);
         const double* c = (const double*)bytes.data();
-        for (long l = 0; l < links; ++l) link_expand(c + 9 * l, host + 18 * l);
+        parallel_links(links, [&](long l) { link_expand(c + 9 * l, host + 18 * l); });
         return CLG_OK;
     }
     const size_t want = (size_t)nreal * (file_type == CLG_FILE_CLG_BIN_FLOAT ? 4 : 8);
@@ -353,7 +372,7 @@ int clg_file_write(const char* path, int file_type, int kind, const int dims[4],
         const long links = sites_of(dims) * 4;
         bytes.resize((size_t)links * 72);
         double* c = (double*)bytes.data();
-        for (long l = 0; l < links; ++l) link_compress(host + 18 * l, c + 9 * l);
+        parallel_links(links, [&](long l) { link_compress(host + 18 * l, c + 9 * l); });
     } else if (file_type == CLG_FILE_CLG_BIN_FLOAT) {
         bytes.resize((size_t)nreal * 4);
         float* f = (float*)bytes.data();
