@@ -296,7 +296,10 @@ int clg_file_read(const char* path, int file_type, int kind, const int dims[4], 
     const long nreal = sites_of(dims) * rps;
     CLG_CHECK_ARG(file_type >= CLG_FILE_BRIDGEPP_TXT && file_type <= CLG_FILE_CLG_BIN_DOUBLE, "unknown file type %d",
                   file_type);
-    CLG_CHECK_ARG(gauge || file_type == CLG_FILE_CLG_BIN,
+    // fermion files are the CLG binary format in the build's Real (CFieldFermionWilsonSquareSU3.cu:
+    // 604-613, CFieldFermionKSSU3.cu:1056-1078): double, or float for a single-precision build's files
+    CLG_CHECK_ARG(gauge || file_type == CLG_FILE_CLG_BIN || file_type == CLG_FILE_CLG_BIN_FLOAT ||
+                      file_type == CLG_FILE_CLG_BIN_DOUBLE,
                   "fermion fields are read from CLG binary files only (CFieldFermionWilsonSquareSU3.cu:604-608)");
     if (file_type == CLG_FILE_BRIDGEPP_TXT) {
         FILE* fp = std::fopen(path, "r");

@@ -118,8 +118,9 @@ typedef enum {
 } clg_file_type;
 /* Host only (no device needed).  Reads a full-lattice field of `kind` (CLG_GAUGE[12], CLG_WILSON,
  * CLG_KS) on `dims` into the host AoS order.  Bridge++ files are gauge only
- * (CFieldGaugeSU3::InitialFieldWithFile, CFieldGaugeSU3.cu:1080-1185); fermion fields read
- * CLG_FILE_CLG_BIN only (CFieldFermionWilsonSquareSU3.cu:602-613).  A file whose size (or
+ * (CFieldGaugeSU3::InitialFieldWithFile, CFieldGaugeSU3.cu:1080-1185); fermion fields read the CLG
+ * binary format only (CFieldFermionWilsonSquareSU3.cu:602-613), in double (CLG_FILE_CLG_BIN, _DOUBLE)
+ * or float (CLG_FILE_CLG_BIN_FLOAT: a single-precision build's files).  A file whose size (or
  * number of text values) does not match the lattice is CLG_BAD_ARG. */
 int clg_file_read(const char* path, int file_type, int kind, const int dims[4], double* host);
 /* Host only.  Writes the host AoS order in one of the CLG binary formats (CField::SaveToFile,

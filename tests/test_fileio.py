@@ -9,6 +9,10 @@ testGaugeSingle.con_, copied unchanged under tests/golden/configs) and its expec
 - The CLG double and float files hold one 8^4 SU(3) configuration: sum |U|^2 = 49152 =
   3 x 4 links x 8^4 (TestFileDOUBLE, TestFileFloat), every link unitary, the two precisions agree.
 - MD5: RFC 1321's test suite and Python's hashlib on the bytes written.
+- TestFileIOConsistency (TestConfigurationFileIO.cpp:174-204, section TestSaveLoad): the reference's
+  __f2.con_ (an SU(3) gauge-type field) and __f18.con_ (a CFieldFermionKSSU3 field), 8^4, written by
+  its single-precision build, load with Dot(f, f) / V equal to its Expected entries 2508152.599227
+  and 1189.146596 within the test's 1e-5 relative bound.
 - The compressed format (CFieldGaugeSU3.cu:1225-1265, 1651-1690) stores 9 reals of the principal
   logarithm A = log U per link.  The reference ships no compressed file (its test writes one and
   reads it back), so the logarithm is pinned by closed forms (diagonal links, links conjugated by a
@@ -90,9 +94,23 @@ def test_clg_binary_round_trip_and_md5(tmp_path, kind, per_site):
         raw = open(p, "rb").read()
         assert md5 == hashlib.md5(raw).hexdigest().upper()
         assert np.array_equal(np.frombuffer(raw, dtype=dt), host.astype(dt))
-        if kind == clg.GAUGE or ftype == clg.FILE_CLG_BIN:
-            back = clg.file_read(p, ftype, kind, dims)
-            assert np.array_equal(back, host.astype(dt).astype(np.float64))
+        back = clg.file_read(p, ftype, kind, dims)
+        assert np.array_equal(back, host.astype(dt).astype(np.float64))
+
+
+@pytest.mark.parametrize("fname,kind,expected", [("__f2.con_", clg.GAUGE, 2508152.599227),
+                                                  ("__f18.con_", clg.KS, 1189.146596)])
+def test_reference_save_load_fixtures(fname, kind, expected):
+    # the single-precision build's EFFT_CLGBin files are float; TestFileIOConsistency's amplitude
+    # Dot(f, f).x / V against its Expected list
+    dims = (8, 8, 8, 8)
+    V = int(np.prod(dims))
+    f = clg.file_read(os.path.join(HERE, fname), clg.FILE_CLG_BIN_FLOAT, kind, dims)
+    amp = np.dot(f, f) / V
+    assert abs(amp - expected) <= 1e-5 * expected, amp
+    # read as doubles, the same bytes are the wrong size for the lattice
+    with pytest.raises(clg.CLGError, match="bytes"):
+        clg.file_read(os.path.join(HERE, fname), clg.FILE_CLG_BIN, kind, dims)
 
 
 def test_file_errors(tmp_path):

@@ -73,3 +73,20 @@ def test_fermion_save_load_round_trip(ctx, tmp_path, kind):
         f2.load_file(p, clg.FILE_CLG_BIN_FLOAT)
     with pytest.raises(clg.CLGError):                     # files hold full-lattice fields
         f.parity(clg.EVEN).save_file(p)
+
+
+@pytest.mark.parametrize("fname,kind,expected", [("__f2.con_", clg.GAUGE, 2508152.599227),
+                                                  ("__f18.con_", clg.KS, 1189.146596)])
+def test_reference_save_load_fixtures_on_the_device(ctx, fname, kind, expected):
+    # TestFileIOConsistency (TestConfigurationFileIO.cpp:174-204): the single-precision build's 8^4
+    # files loaded into device fields, Dot(f, f).x / V against the section's Expected entries (1e-5)
+    dims = (8, 8, 8, 8)
+    f = clg.Field(ctx, kind, clg.F64, dims) if kind == clg.GAUGE else clg.Field(ctx, kind, clg.F64, dims,
+                                                                               bc=(1, 1, 1, -1))
+    f.load_file(os.path.join(HERE, fname), clg.FILE_CLG_BIN_FLOAT)
+    if kind == clg.GAUGE:
+        d = f.download()
+        amp = float(np.dot(d, d)) / np.prod(dims)
+    else:
+        amp = clg.norm2(ctx, f) / np.prod(dims)
+    assert abs(amp - expected) <= 1e-5 * expected, amp
