@@ -14,6 +14,8 @@
 //                   D^{+-1/2}, D^{+-1/4} and of DD^dagger through the section's multi-shift solver
 //          fileio   TestFileIO (TestConfigurationFileIO.cpp:12-37): average plaquette of the gauge
 //                   field the section loads (EFIT_ReadFromFile), against ExpectedRes
+//          fileio_save  TestFileIOCLG (TestConfigurationFileIO.cpp:55-99): gauge field and fermion field 2
+//                   saved and read back, <U', U> / (V Dir) and |psi' - psi|^2
 //          fileio_compressed  TestFileIOCLGCompressed (TestConfigurationFileIO.cpp:133-160): the gauge
 //                   field saved compressed and read back, |U' - U|^2
 //          operators    TestOperators (TestCommon.cpp:13-88): Gaussian-source norm estimators and
@@ -202,6 +204,33 @@ static int run_fileio(CParameters& sec) {
     return 0;
 }
 
+// TestFileIOCLG's pattern (TestConfigurationFileIO.cpp:55-99): save the gauge field and fermion field 2
+// in the CLG binary format, read them into new fields; <U', U> / (V Dir) and |psi' - psi|^2
+static int run_fileio_save(CParameters&) {
+    CLatticeData* L = appGetLattice();
+    const CField* g = L->m_pGaugeField[0];
+    const CField* f = L->GetFieldById(2);
+    if (!g || !f) return 2;
+    const std::string md5g = g->SaveToFile("testGauge.con");
+    const std::string md5f = f->SaveToFile("testFermion.con");
+    CField* ng = g->GetCopy();
+    CField* nf = f->GetCopy();
+    ng->Zero();
+    nf->Zero();
+    ng->InitialFieldWithFile("testGauge.con", EFFT_CLGBin);
+    nf->InitialFieldWithFile("testFermion.con", EFFT_CLGBin);
+    const double V = (double)L->m_dims[0] * L->m_dims[1] * L->m_dims[2] * L->m_dims[3];
+    const CLGComplex r1 = ng->DotReal(g);
+    nf->AxpyMinus(f);
+    const CLGComplex r2 = nf->DotReal(nf);
+    delete ng;
+    delete nf;
+    std::printf("{\"test\": \"fileio_save\", \"gauge\": [%.12f, %.12f], \"fermion\": [%.6e, %.6e], "
+                "\"md5_gauge\": \"%s\", \"md5_fermion\": \"%s\"}\n",
+                r1.x / (V * 4), r1.y / (V * 4), r2.x, r2.y, md5g.c_str(), md5f.c_str());
+    return 0;
+}
+
 // TestFileIOCLGCompressed's pattern (TestConfigurationFileIO.cpp:133-160): save the gauge field
 // compressed, read it into a new field, |U' - U|^2
 static int run_fileio_compressed(CParameters&) {
@@ -328,7 +357,7 @@ static void dump(const CParameters& p) {
 int main(int argc, char** argv) {
     if (argc < 4) {
         std::fprintf(stderr,
-                     "usage: %s <yaml> <section> <update|update_l|solver|mssolver_ks|fileio|fileio_compressed|operators|gamma5|"
+                     "usage: %s <yaml> <section> <update|update_l|solver|mssolver_ks|fileio|fileio_save|fileio_compressed|operators|gamma5|"
                      "antihermitian|gauge_invariance|dump>\n",
                      argv[0]);
         return 2;
@@ -353,6 +382,7 @@ int main(int argc, char** argv) {
         else if (t == "solver") rc = run_solver(sec);
         else if (t == "fileio") rc = run_fileio(sec);
         else if (t == "fileio_compressed") rc = run_fileio_compressed(sec);
+        else if (t == "fileio_save") rc = run_fileio_save(sec);
         else if (t == "mssolver_ks") rc = run_mssolver_ks(sec);
         else if (t == "operators") rc = run_operators(sec);
         else if (t == "gamma5") rc = run_gamma5(sec);

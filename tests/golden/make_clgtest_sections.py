@@ -15,8 +15,8 @@ SECTIONS = {
     "TestSuit_FermionUpdator.yaml": ["TestFermionUpdatorNestedForceGradient", "TestFermionUpdatorLargeScale",
                                      "TestFermionUpdatorLargeScaleFloat"],
     "TestSuit_Common.yaml": ["TestOperators", "TestGamm5Hermiticity", "TestAnitiHermiticity", "TestGaugeInvarience"],
-    "TestSuit_FileIO.yaml": ["TestFileIOBridgePPText", "TestFileIOBridgePPBin", "TestFileIOCLGCompressed",
-                             "TestFileIOCLGCompressedDebug"],
+    "TestSuit_FileIO.yaml": ["TestFileIOBridgePPText", "TestFileIOBridgePPBin", "TestSaveConfiguration",
+                             "TestFileIOCLGCompressed", "TestFileIOCLGCompressedDebug"],
 }
 
 

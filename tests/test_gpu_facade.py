@@ -58,6 +58,16 @@ def test_fileio_sections(section, tmp_path):
     assert abs(r["plaquette"] - r["expected"]) < 1e-6, r
 
 
+def test_fileio_save_section(tmp_path):
+    # TestFileIOCLG (TestConfigurationFileIO.cpp:55-99): the identity gauge field and a Gaussian Wilson
+    # field saved as EFFT_CLGBin and read back: <U', U> / (V Dir) = 3 and |psi' - psi|^2 = 0 within 1e-6
+    r = run("TestSaveConfiguration", "fileio_save", cwd=str(tmp_path))
+    assert abs(r["gauge"][0] - 3.0) <= 1e-6 and abs(r["gauge"][1]) <= 1e-6, r
+    assert abs(r["fermion"][0]) <= 1e-6 and abs(r["fermion"][1]) <= 1e-6, r
+    for key, name in (("md5_gauge", "testGauge.con"), ("md5_fermion", "testFermion.con")):
+        assert r[key] == hashlib.md5((tmp_path / name).read_bytes()).hexdigest().upper(), r
+
+
 @pytest.mark.parametrize("section", ["TestFileIOCLGCompressedDebug", "TestFileIOCLGCompressed"])
 def test_fileio_compressed_sections(section, tmp_path):
     # TestFileIOCLGCompressed (TestConfigurationFileIO.cpp:133-160): a random gauge field (4^4, and 32^4
