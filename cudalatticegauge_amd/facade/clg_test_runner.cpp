@@ -14,6 +14,8 @@
 //                   D^{+-1/2}, D^{+-1/4} and of DD^dagger through the section's multi-shift solver
 //          fileio   TestFileIO (TestConfigurationFileIO.cpp:12-37): average plaquette of the gauge
 //                   field the section loads (EFIT_ReadFromFile), against ExpectedRes
+//          fileio_double / fileio_float  TestFileDOUBLE / TestFileFloat (TestConfigurationFileIO.cpp:
+//                   101-162): Dot(U, U) of the reference's 8^4 configuration read in double or float
 //          fileio_save  TestFileIOCLG (TestConfigurationFileIO.cpp:55-99): gauge field and fermion field 2
 //                   saved and read back, <U', U> / (V Dir) and |psi' - psi|^2
 //          fileio_compressed  TestFileIOCLGCompressed (TestConfigurationFileIO.cpp:133-160): the gauge
@@ -204,6 +206,20 @@ static int run_fileio(CParameters& sec) {
     return 0;
 }
 
+// TestFileDOUBLE / TestFileFloat (TestConfigurationFileIO.cpp:101-162): the reference's 8^4 gauge
+// configuration in double or float read into the lattice's gauge field; Dot(U, U).x (49152 expected)
+static int run_fileio_dot(bool dbl) {
+    CLatticeData* L = appGetLattice();
+    CField* g = L->m_pGaugeField[0];
+    if (!g) return 2;
+    const std::string file = std::string("../Debug/") + (dbl ? "testGaugeDouble.con_" : "testGaugeSingle.con_");
+    g->InitialFieldWithFile(file, dbl ? EFFT_CLGBinDouble : EFFT_CLGBinFloat);
+    const CLGComplex r = g->Dot(g);
+    std::printf("{\"test\": \"%s\", \"dot\": [%.10f, %.10f], \"expected\": 49152.0}\n",
+                dbl ? "fileio_double" : "fileio_float", r.x, r.y);
+    return 0;
+}
+
 // TestFileIOCLG's pattern (TestConfigurationFileIO.cpp:55-99): save the gauge field and fermion field 2
 // in the CLG binary format, read them into new fields; <U', U> / (V Dir) and |psi' - psi|^2
 static int run_fileio_save(CParameters&) {
@@ -357,7 +373,8 @@ static void dump(const CParameters& p) {
 int main(int argc, char** argv) {
     if (argc < 4) {
         std::fprintf(stderr,
-                     "usage: %s <yaml> <section> <update|update_l|solver|mssolver_ks|fileio|fileio_save|fileio_compressed|operators|gamma5|"
+                     "usage: %s <yaml> <section> <update|update_l|solver|mssolver_ks|fileio|fileio_double|fileio_float|fileio_save|fileio_compressed|"
+                     "operators|gamma5|"
                      "antihermitian|gauge_invariance|dump>\n",
                      argv[0]);
         return 2;
@@ -383,6 +400,7 @@ int main(int argc, char** argv) {
         else if (t == "fileio") rc = run_fileio(sec);
         else if (t == "fileio_compressed") rc = run_fileio_compressed(sec);
         else if (t == "fileio_save") rc = run_fileio_save(sec);
+        else if (t == "fileio_double" || t == "fileio_float") rc = run_fileio_dot(t == "fileio_double");
         else if (t == "mssolver_ks") rc = run_mssolver_ks(sec);
         else if (t == "operators") rc = run_operators(sec);
         else if (t == "gamma5") rc = run_gamma5(sec);

@@ -16,7 +16,8 @@ SECTIONS = {
                                      "TestFermionUpdatorLargeScaleFloat"],
     "TestSuit_Common.yaml": ["TestOperators", "TestGamm5Hermiticity", "TestAnitiHermiticity", "TestGaugeInvarience"],
     "TestSuit_FileIO.yaml": ["TestFileIOBridgePPText", "TestFileIOBridgePPBin", "TestSaveConfiguration",
-                             "TestFileIOCLGCompressed", "TestFileIOCLGCompressedDebug"],
+                             "TestFileIOCLGCompressed", "TestFileIOCLGCompressedDebug", "TestSaveConfigurationDouble",
+                             "TestSaveConfigurationFloat"],
 }
 
 

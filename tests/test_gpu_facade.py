@@ -58,6 +58,17 @@ def test_fileio_sections(section, tmp_path):
     assert abs(r["plaquette"] - r["expected"]) < 1e-6, r
 
 
+@pytest.mark.parametrize("section,test", [("TestSaveConfigurationDouble", "fileio_double"),
+                                          ("TestSaveConfigurationFloat", "fileio_float")])
+def test_fileio_double_float_sections(section, test, tmp_path):
+    # TestFileDOUBLE / TestFileFloat (TestConfigurationFileIO.cpp:101-162): Dot(U, U).x of the reference's
+    # 8^4 configuration within 1e-6 of 49152; the files sit in ../Debug of the working directory
+    (tmp_path / "Debug").symlink_to(os.path.join(ROOT, "tests", "golden", "configs"))
+    (tmp_path / "Release").mkdir()
+    r = run(section, test, cwd=str(tmp_path / "Release"))
+    assert abs(r["dot"][0] - r["expected"]) <= 1e-6 and abs(r["dot"][1]) <= 1e-6, r
+
+
 def test_fileio_save_section(tmp_path):
     # TestFileIOCLG (TestConfigurationFileIO.cpp:55-99): the identity gauge field and a Gaussian Wilson
     # field saved as EFFT_CLGBin and read back: <U', U> / (V Dir) = 3 and |psi' - psi|^2 = 0 within 1e-6
