@@ -413,13 +413,56 @@ __global__ __launch_bounds__(256) void k_vmm(VmmArgs a, const double* __restrict
 // ---- fused Gram-Schmidt step of the Arnoldi process (see blas_gs_step)
 // w' = w - sum_k h_k V_k (h = the first-pass projections, on device) is stored, and the same pass
 // yields the partials of V_k^H w' (2K values) and |w'|^2 (the last one).
+// With vnext given, the pass first takes |w'|^2 from the first pass as |w|^2 - |h|^2 (V orthonormal).
+// When that keeps at least half of |w|^2 (no second pass will be needed), the step's fast form
+// applies: w' / |w'| goes straight to vnext and no partials are taken.  Workgroup 0 records the choice
+// in sc ([1] |w'|^2, [2] 1/|w'|, [3] 2 for the fast form, else 0) for the finish and finalize.
 template <int P, int K>
 __global__ __launch_bounds__(kRedBlock) void k_gs_project(long n, Ptrs<P> V, const double* __restrict__ h,
-                                                           typename Pk<P>::T* w, double* partial) {
+                                                           typename Pk<P>::T* w, double* partial, double* sc,
+                                                           typename Pk<P>::T* vnext) {
     constexpr int NC = Pk<P>::NC;
     double hr[K], hi[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) { hr[k] = h[2 * k]; hi[k] = h[2 * k + 1]; }
+    if (vnext) {
+        const double w2 = sc[0];
+        double b2 = w2;
+#pragma unroll
+        for (int k = 0; k < K; ++k) b2 -= hr[k] * hr[k] + hi[k] * hi[k];
+        const bool fast = b2 >= 0.5 * w2 && b2 > 1e-300;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            sc[3] = fast ? 2.0 : 0.0;
+            if (fast) {
+                sc[1] = b2;
+                sc[2] = 1.0 / sqrt(b2);
+            }
+        }
+        if (fast) {
+            const double inv = 1.0 / sqrt(b2);
+            GRID_STRIDE(i, n) {
+                typename Pk<P>::T raw[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) raw[k] = V.p[k][i];
+                double y[2 * NC];
+                Pk<P>::ld(w, i, y);
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    double x[2 * NC];
+                    Pk<P>::cvt(raw[k], x);
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        y[2 * c] -= hr[k] * x[2 * c] - hi[k] * x[2 * c + 1];
+                        y[2 * c + 1] -= hr[k] * x[2 * c + 1] + hi[k] * x[2 * c];
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < 2 * NC; ++c) y[c] *= inv;
+                Pk<P>::st(vnext, i, y);
+            }
+            return;
+        }
+    }
     double v[2 * K + 1];
 #pragma unroll
     for (int k = 0; k < 2 * K + 1; ++k) v[k] = 0;
@@ -467,8 +510,22 @@ __global__ __launch_bounds__(kRedBlock) void k_gs_project(long n, Ptrs<P> V, con
 // [4..4+2K) c2.
 template <int K>
 __global__ __launch_bounds__(kRedBlock) void k_gs_finish(const double* __restrict__ partial, int nblk, double* hcol,
-                                                          int j, int shifted, double* sc) {
+                                                          int j, int shifted, double* sc, int fast_ok) {
     constexpr int NR = 2 * K + 1;
+    if (fast_ok && sc[3] == 2.0) {
+        // the projection pass took the fast form: H(i, j) = h_i, H(j+1, j) = |w'| from sc
+        const int i = threadIdx.x;
+        if (i < K && shifted) {
+            hcol[2 * i] = (i == j ? 1.0 : 0.0) - hcol[2 * i];
+            hcol[2 * i + 1] = -hcol[2 * i + 1];
+        }
+        if (i == 0) {
+            const double f = sqrt(sc[1]);
+            hcol[2 * (j + 1)] = shifted ? -f : f;
+            hcol[2 * (j + 1) + 1] = 0.0;
+        }
+        return;
+    }
     __shared__ double sm[kRedBlock / 64][NR];
     __shared__ double tot[NR];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -529,6 +586,7 @@ template <int P, int K>
 __global__ __launch_bounds__(kRedBlock) void k_gs_finalize(long n, Ptrs<P> V, const typename Pk<P>::T* __restrict__ w,
                                                             typename Pk<P>::T* vnext, const double* __restrict__ sc) {
     constexpr int NC = Pk<P>::NC;
+    if (sc[3] == 2.0) return;  // the projection pass wrote vnext (fast form)
     const bool flag = sc[3] != 0.0;
     const double inv = sc[2];
     if (flag) {
@@ -941,17 +999,17 @@ static int gs_step_p(clg_context ctx, clg_field_s* const* V, int nv, clg_field_s
 #undef CALL
     CLG_HIP(hipGetLastError());
     CLG_TRY(red_finish(ctx, g, 2 * nv + 1, hcol, 2 * nv, sc, nullptr, nullptr, 0.0, nullptr));
-    // (2) w' = w - V h with V^H w' and |w'|^2, then the column
+    // (2) w' = w - V h with V^H w' and |w'|^2, then the column (or, in the fast form, vnext directly)
+    typename Pk<P>::T* vp = vnext ? (typename Pk<P>::T*)vnext->data : nullptr;
 #define CALL(KK)                                                                                     \
-    k_gs_project<P, KK><<<g, kRedBlock, 0, ctx->stream>>>(n, Pp, hcol, wp, ctx->red_partial);       \
-    k_gs_finish<KK><<<1, kRedBlock, 0, ctx->stream>>>(ctx->red_partial, (int)g, hcol, j, shifted ? 1 : 0, sc);
+    k_gs_project<P, KK><<<g, kRedBlock, 0, ctx->stream>>>(n, Pp, hcol, wp, ctx->red_partial, sc, vp);  \
+    k_gs_finish<KK><<<1, kRedBlock, 0, ctx->stream>>>(ctx->red_partial, (int)g, hcol, j, shifted ? 1 : 0, sc, vp ? 1 : 0);
     SWITCH_K(nv, CALL)
 #undef CALL
     CLG_HIP(hipGetLastError());
     // (3) the next basis vector
     if (vnext) {
         const unsigned ge = ew_grid(ctx, n);
-        typename Pk<P>::T* vp = (typename Pk<P>::T*)vnext->data;
 #define CALL(KK) k_gs_finalize<P, KK><<<ge, kRedBlock, 0, ctx->stream>>>(n, Pp, wc, vp, sc);
         SWITCH_K(nv, CALL)
 #undef CALL
@@ -966,7 +1024,7 @@ int red_finish_partials(clg_context ctx, unsigned nblk, int nr, double* out, int
     return red_finish(ctx, nblk, nr, out, nout, out2, cond, ref, eta, flag);
 }
 int gs_finish_launch(clg_context ctx, unsigned nblk, int nv, double* hcol, int j, bool shifted, double* sc) {
-#define CALL(KK) k_gs_finish<KK><<<1, kRedBlock, 0, ctx->stream>>>(ctx->red_partial, (int)nblk, hcol, j, shifted ? 1 : 0, sc);
+#define CALL(KK) k_gs_finish<KK><<<1, kRedBlock, 0, ctx->stream>>>(ctx->red_partial, (int)nblk, hcol, j, shifted ? 1 : 0, sc, 0);
     SWITCH_K(nv, CALL)
 #undef CALL
     CLG_HIP(hipGetLastError());
@@ -976,7 +1034,9 @@ int gs_finish_launch(clg_context ctx, unsigned nblk, int nv, double* hcol, int j
 // One Arnoldi orthogonalisation: w against the orthonormal V_0..V_{nv-1} (nv <= 16) by classical
 // Gram-Schmidt with the DGKS-conditioned second pass, in three streaming passes (projections;
 // subtraction fused with the second-pass projections; normalisation, which applies the second pass
-// when it is needed) plus two one-workgroup finishes.  hcol: column j of the Hessenberg matrix
+// when it is needed) plus two one-workgroup finishes.  With vnext given and no second pass needed
+// (|w|^2 - |h|^2 >= |w|^2 / 2, decided on the device), the subtraction pass writes vnext normalised
+// by that norm and the other passes return at once; w then keeps w (not w').  hcol: column j of the Hessenberg matrix
 // (device, interleaved complex, rows 0..j+1), written with h + [flag] c2 (shifted: delta_ij minus
 // that) and H(j+1, j) = +-|w''|.  vnext (nullable) receives w'' / |w''|; w keeps w'.  sc: device
 // scratch of 4 + 32 doubles.
