@@ -850,10 +850,14 @@ __global__ __launch_bounds__(kRedBlock) void k_bh_multidot(const BTab* T, Act ac
     block_partial<2 * K + 1>(acc, partial + (size_t)blockIdx.y * gridDim.x * (2 * K + 1));
 }
 // w' = w - sum_k h_k V_k, stored to vector osel (w itself or the next basis vector), with the
-// partials of V_k^H w' and |w'|^2; h at ws[s*stride + offh]
+// partials of V_k^H w' and |w'|^2; h at ws[s*stride + offh].  fast_ok (osel is the next basis
+// vector): per system, when |w|^2 - |h|^2 keeps half of |w|^2 (no second pass needed), w'/|w'| is
+// stored and no partials are taken; the system's workgroup 0 records the choice in its Gram-Schmidt
+// block (gs[1] |w'|^2, gs[2] 1/|w'|, gs[3] 2 for the fast form, else 0).
 template <int K>
-__global__ __launch_bounds__(kRedBlock) void k_bh_gs_project(const BTab* T, Act act, int wsel, int osel, const double* ws,
-                                                              int stride, int offh, double* partial) {
+__global__ __launch_bounds__(kRedBlock) void k_bh_gs_project(const BTab* T, Act act, int wsel, int osel, double* ws,
+                                                              int stride, int offh, double* partial, int offgs,
+                                                              int fast_ok) {
     const int s = act.idx[blockIdx.y];
     const unsigned n = T->vh[s];
     const double* h = ws + (size_t)s * stride + offh;
@@ -864,6 +868,54 @@ __global__ __launch_bounds__(kRedBlock) void k_bh_gs_project(const BTab* T, Act 
     float hr[K], hi[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) { hr[k] = (float)h[2 * k]; hi[k] = (float)h[2 * k + 1]; }
+    if (fast_ok) {
+        double* gs = ws + (size_t)s * stride + offgs;
+        const double w2 = gs[0];
+        double b2 = w2;
+#pragma unroll
+        for (int k = 0; k < K; ++k) b2 -= h[2 * k] * h[2 * k] + h[2 * k + 1] * h[2 * k + 1];
+        const bool fast = b2 >= 0.5 * w2 && b2 > 1e-300;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            gs[3] = fast ? 2.0 : 0.0;
+            if (fast) {
+                gs[1] = b2;
+                gs[2] = 1.0 / sqrt(b2);
+            }
+        }
+        if (fast) {
+            const float inv = (float)(1.0 / sqrt(b2));
+            BSITE_LOOP(i, n) {
+                float sv[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) sv[k] = T->Vn[s][k][i];
+                const float sw = wn[i];
+                float y[24];
+#pragma unroll
+                for (int ch = 0; ch < 3; ++ch) {
+                    const size_t ci = h16_chunk(i, n, ch);
+                    uint4 raw[K];
+#pragma unroll
+                    for (int k = 0; k < K; ++k) raw[k] = ((const uint4*)T->V[s][k])[ci];
+                    float* yc = y + 8 * ch;
+                    h16_unpack8(wc[ci], sw, yc);
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        float x[8];
+                        h16_unpack8(raw[k], sv[k], x);
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            yc[2 * q] -= hr[k] * x[2 * q] - hi[k] * x[2 * q + 1];
+                            yc[2 * q + 1] -= hr[k] * x[2 * q + 1] + hi[k] * x[2 * q];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 24; ++q) y[q] *= inv;
+                h16_store_site(oc, on, n, i, y);
+            }
+            return;
+        }
+    }
     double acc[2 * K + 1];
 #pragma unroll
     for (int r = 0; r < 2 * K + 1; ++r) acc[r] = 0.0;
@@ -918,13 +970,26 @@ __global__ __launch_bounds__(kRedBlock) void k_bh_gs_project(const BTab* T, Act 
 // gs (per system, 36 doubles): [0] |w|^2 (in), [1] |w''|^2, [2] 1/|w''|, [3] flag, [4..) c2.
 template <int K>
 __global__ __launch_bounds__(kRedBlock) void k_bgs_finish(Act act, const double* __restrict__ partial, int nblk,
-                                                           double* ws, int stride, int offh, int offgs) {
+                                                           double* ws, int stride, int offh, int offgs, int fast_ok) {
     constexpr int NR = 2 * K + 1;
     const int s = act.idx[blockIdx.y];
     const double* pp = partial + (size_t)blockIdx.y * nblk * NR;
     double* hcol = ws + (size_t)s * stride + offh;
     double* gs = ws + (size_t)s * stride + offgs;
     const int j = K - 1;
+    if (fast_ok && gs[3] == 2.0) {
+        // the projection pass took the fast form: H(i, j) = delta_ij - h_i, H(j+1, j) = -|w'|
+        const int i = threadIdx.x;
+        if (i < K) {
+            hcol[2 * i] = (i == j ? 1.0 : 0.0) - hcol[2 * i];
+            hcol[2 * i + 1] = -hcol[2 * i + 1];
+        }
+        if (i == 0) {
+            hcol[2 * (j + 1)] = -sqrt(gs[1]);
+            hcol[2 * (j + 1) + 1] = 0.0;
+        }
+        return;
+    }
     __shared__ double sm[kRedBlock / 64][NR];
     __shared__ double tot[NR];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -983,6 +1048,7 @@ __global__ __launch_bounds__(256) void k_bh_gs_finalize(const BTab* T, Act act, 
     const int s = act.idx[blockIdx.y];
     const unsigned n = T->vh[s];
     const double* gs = ws + (size_t)s * stride + offgs;
+    if (gs[3] == 2.0) return;  // the projection pass stored w'/|w'| (fast form)
     const bool flag = gs[3] != 0.0;
     const float inv = (float)gs[2];
     uint4* dc = hvec(T, s, dsel);
@@ -1378,9 +1444,9 @@ struct BatchSolver {
                                                                ws + offh, 2 * KK, ws + L.offGS, nullptr, nullptr, \
                                                                0.0, nullptr, stride);                            \
     k_bh_gs_project<KK><<<rgrid, kRedBlock, 0, ctx->stream>>>(dtab, A, m, j < m - 1 ? j + 1 : m, ws, stride, offh, \
-                                                              ctx->red_partial);                                 \
+                                                              ctx->red_partial, (int)L.offGS, j < m - 1 ? 1 : 0); \
     k_bgs_finish<KK><<<dim3(1, A.n), kRedBlock, 0, ctx->stream>>>(A, ctx->red_partial, (int)rg, ws, stride, offh, \
-                                                                  (int)L.offGS);                                 \
+                                                                  (int)L.offGS, j < m - 1 ? 1 : 0);              \
     if (j < m - 1)                                                                                               \
         k_bh_gs_finalize<KK><<<egrid, 256, 0, ctx->stream>>>(dtab, A, j + 1, ws, stride, (int)L.offGS);
                 switch (nv) {

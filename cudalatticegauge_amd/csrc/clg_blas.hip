@@ -1023,8 +1023,9 @@ int red_finish_partials(clg_context ctx, unsigned nblk, int nr, double* out, int
                         const double* ref, double eta, int* flag) {
     return red_finish(ctx, nblk, nr, out, nout, out2, cond, ref, eta, flag);
 }
-int gs_finish_launch(clg_context ctx, unsigned nblk, int nv, double* hcol, int j, bool shifted, double* sc) {
-#define CALL(KK) k_gs_finish<KK><<<1, kRedBlock, 0, ctx->stream>>>(ctx->red_partial, (int)nblk, hcol, j, shifted ? 1 : 0, sc, 0);
+int gs_finish_launch(clg_context ctx, unsigned nblk, int nv, double* hcol, int j, bool shifted, double* sc,
+                     bool fast_ok) {
+#define CALL(KK) k_gs_finish<KK><<<1, kRedBlock, 0, ctx->stream>>>(ctx->red_partial, (int)nblk, hcol, j, shifted ? 1 : 0, sc, fast_ok ? 1 : 0);
     SWITCH_K(nv, CALL)
 #undef CALL
     CLG_HIP(hipGetLastError());

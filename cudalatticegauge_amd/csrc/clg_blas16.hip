@@ -21,7 +21,7 @@ using namespace clgd;
 
 namespace clg {
 int red_finish_partials(clg_context, unsigned, int, double*, int, double*, const int*, const double*, double, int*);
-int gs_finish_launch(clg_context, unsigned, int, double*, int, bool, double*);
+int gs_finish_launch(clg_context, unsigned, int, double*, int, bool, double*, bool);
 int ensure_reduction_scratch(clg_context, size_t);
 }  // namespace clg
 
@@ -133,14 +133,63 @@ __global__ __launch_bounds__(kRedBlock) void k_h_multidot(unsigned n, unsigned V
 }
 
 // w' = w - sum_k h_k V_k, stored to (oc, os) (w itself or the next basis vector), with the
-// partials of V_k^H w' and |w'|^2 of the unrounded w'
+// partials of V_k^H w' and |w'|^2 of the unrounded w'.  fast_ok (oc is the next basis vector): the
+// fast form of the FP64 / FP32 step (k_gs_project) -- when |w|^2 - |h|^2 keeps half of |w|^2, w'/|w'|
+// is stored and no partials are taken; workgroup 0 records the choice in sc[1..3].
 template <int K>
 __global__ __launch_bounds__(kRedBlock) void k_h_gs_project(unsigned n, unsigned Vh, HTab V, const double* __restrict__ h,
                                                              const uint4* wc, const float* ws, uint4* oc, float* os,
-                                                             double* partial) {
+                                                             double* partial, double* sc, int fast_ok) {
     float hr[K], hi[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) { hr[k] = (float)h[2 * k]; hi[k] = (float)h[2 * k + 1]; }
+    if (fast_ok) {
+        const double w2 = sc[0];
+        double b2 = w2;
+#pragma unroll
+        for (int k = 0; k < K; ++k) b2 -= h[2 * k] * h[2 * k] + h[2 * k + 1] * h[2 * k + 1];
+        const bool fast = b2 >= 0.5 * w2 && b2 > 1e-300;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            sc[3] = fast ? 2.0 : 0.0;
+            if (fast) {
+                sc[1] = b2;
+                sc[2] = 1.0 / sqrt(b2);
+            }
+        }
+        if (fast) {
+            const float inv = (float)(1.0 / sqrt(b2));
+            SITE_LOOP(i, n) {
+                float sv[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) sv[k] = V.v[k].s[i];
+                const float sw = ws[i];
+                float y[24];
+#pragma unroll
+                for (int ch = 0; ch < 3; ++ch) {
+                    const size_t ci = h16_chunk(i, Vh, ch);
+                    uint4 raw[K];
+#pragma unroll
+                    for (int k = 0; k < K; ++k) raw[k] = V.v[k].c[ci];
+                    float* yc = y + 8 * ch;
+                    h16_unpack8(wc[ci], sw, yc);
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        float x[8];
+                        h16_unpack8(raw[k], sv[k], x);
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            yc[2 * q] -= hr[k] * x[2 * q] - hi[k] * x[2 * q + 1];
+                            yc[2 * q + 1] -= hr[k] * x[2 * q + 1] + hi[k] * x[2 * q];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 24; ++q) y[q] *= inv;
+                h16_store_site(oc, os, Vh, i, y);
+            }
+            return;
+        }
+    }
     double acc[2 * K + 1];
 #pragma unroll
     for (int r = 0; r < 2 * K + 1; ++r) acc[r] = 0.0;
@@ -198,6 +247,7 @@ __global__ __launch_bounds__(kRedBlock) void k_h_gs_project(unsigned n, unsigned
 template <int K>
 __global__ __launch_bounds__(256) void k_h_gs_finalize(unsigned n, unsigned Vh, HTab V, uint4* vc, float* vs,
                                 const double* __restrict__ sc) {
+    if (sc[3] == 2.0) return;  // the projection pass stored w'/|w'| (fast form)
     const bool flag = sc[3] != 0.0;
     const float inv = (float)sc[2];
     if (!flag) {
@@ -379,11 +429,11 @@ int h16_gs_step(clg_context ctx, clg_field_s* const* V, int nv, clg_field_s* w, 
     clg_field_s* o = vnext ? vnext : w;
 #define CALL(KK)                                                                                             \
     k_h_gs_project<KK><<<g, kRedBlock, 0, ctx->stream>>>(n, Vh, t, hcol, (const uint4*)w->data, w->norm, \
-                                                         (uint4*)o->data, o->norm, ctx->red_partial);
+                                                         (uint4*)o->data, o->norm, ctx->red_partial, sc, vnext ? 1 : 0);
     SWITCH_V(nv, CALL)
 #undef CALL
     CLG_HIP(hipGetLastError());
-    CLG_TRY(gs_finish_launch(ctx, g, nv, hcol, j, shifted, sc));
+    CLG_TRY(gs_finish_launch(ctx, g, nv, hcol, j, shifted, sc, vnext != nullptr));
     if (vnext) {
         const unsigned ge = ew_grid16(ctx, n);
 #define CALL(KK) k_h_gs_finalize<KK><<<ge, 256, 0, ctx->stream>>>(n, Vh, t, (uint4*)vnext->data, vnext->norm, sc);
