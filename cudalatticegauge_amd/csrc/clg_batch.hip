@@ -489,10 +489,12 @@ __global__ __launch_bounds__(kRedBlock) void k_bnorm2(const BTab* T, Act act, in
     block_partial<1>(acc, partial + (size_t)blockIdx.y * gridDim.x);
 }
 
-// dst = src / sqrt(n2[s * nstride])
+// dst = src / sqrt(n2[s * nstride]); cond (nullable): systems with cond[s * nstride] == 0 skip
 template <int P>
-__global__ void k_bscale(const BTab* T, Act act, int dsel, int ssel, const double* n2, int nstride) {
+__global__ void k_bscale(const BTab* T, Act act, int dsel, int ssel, const double* n2, int nstride,
+                         const double* cond = nullptr) {
     const int s = act.idx[blockIdx.y];
+    if (cond && cond[(size_t)s * nstride] == 0.0) return;
     typedef typename BP<P>::T PT;
     const PT* src = (const PT*)vec_of(T, s, ssel);
     PT* dst = (PT*)vec_of(T, s, dsel);
@@ -542,6 +544,60 @@ __global__ __launch_bounds__(kRedBlock) void k_bmultidot(const BTab* T, Act act,
         }
     }
     block_partial<NR>(acc, partial + (size_t)blockIdx.y * gridDim.x * NR);
+}
+
+// The fast form of a batched Arnoldi step, per system: with h = V^H w (K values at ws[s*stride + offh])
+// and |w|^2 = sc[4] from the projections, |w'|^2 is taken as |w|^2 - |h|^2 (V orthonormal).  When that
+// keeps at least half of |w|^2 (no second pass needed), w'/|w'| goes straight to V[s][dsel], sc[1] =
+// |w'|^2, sc[8] = 0 (no second pass) and sc[9] = 0 (the subtraction and normalisation passes skip the
+// system); otherwise sc[9] = 1 and nothing else is written.  sc = ws + s*stride + offs.
+template <int P, int K>
+__global__ __launch_bounds__(kRedBlock) void k_bgs_fast(const BTab* T, Act act, int wsel, int dsel, double* ws,
+                                                        int stride, int offh, int offs) {
+    const int s = act.idx[blockIdx.y];
+    typedef typename BP<P>::T PT;
+    constexpr int NC = BP<P>::NC;
+    double* sc = ws + (size_t)s * stride + offs;
+    const double* h = ws + (size_t)s * stride + offh;
+    const double w2 = sc[4];
+    double b2 = w2;
+    double hr[K], hi[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        hr[k] = h[2 * k];
+        hi[k] = h[2 * k + 1];
+        b2 -= hr[k] * hr[k] + hi[k] * hi[k];
+    }
+    const bool fast = b2 >= 0.5 * w2 && b2 > 1e-300;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc[9] = fast ? 0.0 : 1.0;
+        if (fast) {
+            sc[1] = b2;
+            sc[8] = 0.0;
+        }
+    }
+    if (!fast) return;
+    const double inv = 1.0 / sqrt(b2);
+    const PT* w = (const PT*)T->V[s][wsel];
+    PT* d = (PT*)T->V[s][dsel];
+    const long n = T->npk[s];
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        double y[2 * NC];
+        BP<P>::ld(w, i, y);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double x[2 * NC];
+            BP<P>::ld((const PT*)T->V[s][k], i, x);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                y[2 * c] -= hr[k] * x[2 * c] - hi[k] * x[2 * c + 1];
+                y[2 * c + 1] -= hr[k] * x[2 * c + 1] + hi[k] * x[2 * c];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 2 * NC; ++c) y[c] *= inv;
+        BP<P>::st(d, i, y);
+    }
 }
 
 // y = target (V[s][tsel] or X), y -= sgn * sum_j c_j V_j with c at c[s*cstride + 2j]; NORM: |y|^2 into
@@ -666,7 +722,8 @@ struct WsLayout {
     }
 };
 // sc: [0] |r|^2, [1] |w'|^2, [2] dev, [3] |dev|, [4] |w|^2 before GS, [5] |b|^2, [6] |r_outer|^2,
-// [7] |b - D x|^2, [8] second Gram-Schmidt pass flag (1.0 / 0.0)
+// [7] |b - D x|^2, [8] second Gram-Schmidt pass flag (1.0 / 0.0), [9] 0.0 when the step took the fast
+// form (k_bgs_fast), else 1.0
 
 __global__ void k_bhess_update(double* ws, size_t stride, size_t offH, size_t offC, size_t offS, int ld, int j,
                                Act act) {
@@ -1311,6 +1368,24 @@ struct BatchSolver {
         }
         return CLG_OK;
     }
+    template <int P, int K>
+    int gs_fast(const Act& A, int dsel, size_t offh) {
+        const unsigned g = red_grid_b(ctx, max_npk, A.n);
+        k_bgs_fast<P, K><<<dim3(g, A.n), kRedBlock, 0, ctx->stream>>>(dtab, A, m, dsel, ws, (int)L.stride, (int)offh,
+                                                                      (int)L.offS);
+        CLG_HIP(hipGetLastError());
+        return CLG_OK;
+    }
+    template <int P>
+    int gs_fast_k(int k, const Act& A, int dsel, size_t offh) {
+        switch (k) {
+#define GFK(N) case N: return gs_fast<P, N>(A, dsel, offh);
+            GFK(1) GFK(2) GFK(3) GFK(4) GFK(5) GFK(6) GFK(7) GFK(8) GFK(9) GFK(10) GFK(11) GFK(12) GFK(13) GFK(14)
+            GFK(15) GFK(16)
+#undef GFK
+        }
+        return CLG_BAD_ARG;
+    }
     // dispatch on the runtime vector count k (1..kMaxM)
     template <int P>
     int multidot_k(int k, const Act& A, double* base, size_t off, double* wn, const double* cond) {
@@ -1373,13 +1448,19 @@ struct BatchSolver {
                 CLG_TRY(hop(act, W.data(), tlo, nullptr, k2, 0.0, true));
                 double* Hj0 = ws + L.offH + 2 * (size_t)j * L.ld;
                 double* reorth = ws + L.offS + 8;
+                // systems whose step takes the fast form (sc[9] == 0) have V_{j+1} written after the
+                // projections and skip the subtraction and normalisation passes; the last step keeps
+                // w' in place (no next vector) and runs them for every system
+                const double* regular = j < m - 1 ? ws + L.offS + 9 : nullptr;
                 CLG_TRY(multidot_k<P>(j + 1, A, ws, L.offH + 2 * (size_t)j * L.ld, ws + L.offS + 4, nullptr));
-                CLG_TRY(multiaxpy_k<P>(j + 1, A, m, Hj0, 1.0, ws + L.offS + 1, nullptr, ws + L.offS + 4, 0.5, reorth));
+                if (j < m - 1) CLG_TRY(gs_fast_k<P>(j + 1, A, j + 1, L.offH + 2 * (size_t)j * L.ld));
+                CLG_TRY(multiaxpy_k<P>(j + 1, A, m, Hj0, 1.0, ws + L.offS + 1, regular, ws + L.offS + 4, 0.5, reorth));
                 CLG_TRY(multidot_k<P>(j + 1, A, ws, L.offC, nullptr, reorth));
                 CLG_TRY(multiaxpy_k<P>(j + 1, A, m, ws + L.offC, 1.0, ws + L.offS + 1, reorth, nullptr, 0.0, nullptr));
                 k_bhess_update<<<dim3(1, A.n), 64, 0, ctx->stream>>>(ws, L.stride, L.offH, L.offC, L.offS, L.ld, j, A);
                 if (j < m - 1)
-                    k_bscale<P><<<egrid, 256, 0, ctx->stream>>>(dtab, A, j + 1, m, ws + L.offS + 1, (int)L.stride);
+                    k_bscale<P><<<egrid, 256, 0, ctx->stream>>>(dtab, A, j + 1, m, ws + L.offS + 1, (int)L.stride,
+                                                                regular);
                 CLG_HIP(hipGetLastError());
             }
             k_bgmres_lsq<<<dim3(1, A.n), 64, 0, ctx->stream>>>(ws, L.stride, L, A);
