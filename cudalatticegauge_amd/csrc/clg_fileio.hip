@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -266,10 +267,16 @@ void parallel_links(long n, F f) {
         return;
     }
     std::vector<std::thread> th;
-    for (long t = 0; t < nt; ++t)
-        th.emplace_back([=]() {
-            for (long l = n * t / nt; l < n * (t + 1) / nt; ++l) f(l);
-        });
+    long started = 0;
+    try {
+        for (long t = 0; t < nt; ++t, ++started)
+            th.emplace_back([=]() {
+                for (long l = n * t / nt; l < n * (t + 1) / nt; ++l) f(l);
+            });
+    } catch (const std::system_error&) {
+        // no more threads: this thread takes the chunks that were not started
+        for (long l = n * started / nt; l < n; ++l) f(l);
+    }
     for (auto& x : th) x.join();
 }
 
