@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 #include <string>
 #include <system_error>
 #include <thread>
@@ -294,7 +295,7 @@ int clg_md5(const void* data, unsigned long n, char md5_out[33]) {
     return CLG_OK;
 }
 
-int clg_file_read(const char* path, int file_type, int kind, const int dims[4], double* host) {
+static int clg_file_read_impl(const char* path, int file_type, int kind, const int dims[4], double* host) {
     CLG_CHECK_ARG(path && dims && host, "null argument");
     const int rps = reals_per_site(kind);
     CLG_CHECK_ARG(rps > 0, "unknown field kind %d", kind);
@@ -362,7 +363,7 @@ int clg_file_read(const char* path, int file_type, int kind, const int dims[4], 
     return CLG_OK;
 }
 
-int clg_file_write(const char* path, int file_type, int kind, const int dims[4], const double* host,
+static int clg_file_write_impl(const char* path, int file_type, int kind, const int dims[4], const double* host,
                    char md5_out[33]) {
     CLG_CHECK_ARG(path && dims && host, "null argument");
     const int rps = reals_per_site(kind);
@@ -403,7 +404,7 @@ int clg_file_write(const char* path, int file_type, int kind, const int dims[4],
     return CLG_OK;
 }
 
-int clg_field_load_file(clg_field f, const char* path, int file_type) {
+static int clg_field_load_file_impl(clg_field f, const char* path, int file_type) {
     CLG_CHECK_ARG(f && path, "null argument");
     CLG_CHECK_ARG(f->subset == CLG_FULL, "files hold full-lattice fields");
     std::vector<double> host((size_t)clg_field_host_doubles(f));
@@ -432,12 +433,36 @@ int clg_field_load_file(clg_field f, const char* path, int file_type) {
     return st;
 }
 
-int clg_field_save_file(clg_field f, const char* path, int file_type, char md5_out[33]) {
+static int clg_field_save_file_impl(clg_field f, const char* path, int file_type, char md5_out[33]) {
     CLG_CHECK_ARG(f && path, "null argument");
     CLG_CHECK_ARG(f->subset == CLG_FULL, "files hold full-lattice fields");
     std::vector<double> host((size_t)clg_field_host_doubles(f));
     CLG_TRY(clg_field_download(f, host.data()));
     return clg_file_write(path, file_type, f->kind, f->dims, host.data(), md5_out);
+}
+
+// host allocations of a whole field (or file) can fail for very large lattices: reported as
+// CLG_OUT_OF_MEMORY instead of an exception crossing the C interface
+#define CLG_HOST_GUARD(call)                                                  \
+    try {                                                                     \
+        return call;                                                          \
+    } catch (const std::bad_alloc&) {                                         \
+        set_error("host memory exhausted");                                   \
+        return CLG_OUT_OF_MEMORY;                                             \
+    }
+
+int clg_file_read(const char* path, int file_type, int kind, const int dims[4], double* host) {
+    CLG_HOST_GUARD(clg_file_read_impl(path, file_type, kind, dims, host))
+}
+int clg_file_write(const char* path, int file_type, int kind, const int dims[4], const double* host,
+                   char md5_out[33]) {
+    CLG_HOST_GUARD(clg_file_write_impl(path, file_type, kind, dims, host, md5_out))
+}
+int clg_field_load_file(clg_field f, const char* path, int file_type) {
+    CLG_HOST_GUARD(clg_field_load_file_impl(f, path, file_type))
+}
+int clg_field_save_file(clg_field f, const char* path, int file_type, char md5_out[33]) {
+    CLG_HOST_GUARD(clg_field_save_file_impl(f, path, file_type, md5_out))
 }
 
 }  // extern "C"
