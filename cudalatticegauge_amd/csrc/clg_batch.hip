@@ -471,10 +471,9 @@ __device__ __forceinline__ void* vec_of(const BTab* T, int s, int sel) {
     return sel >= 0 ? T->V[s][sel] : T->X[s][-sel - 1];
 }
 
-// per-system |v|^2 -> out[s * ostride]
+// partials of the per-system |v|^2 (k_bred_finish adds them up)
 template <int P>
-__global__ __launch_bounds__(kRedBlock) void k_bnorm2(const BTab* T, Act act, int sel, double* partial,
-                                                      unsigned* counter, double* out, int ostride) {
+__global__ __launch_bounds__(kRedBlock) void k_bnorm2(const BTab* T, Act act, int sel, double* partial) {
     const int s = act.idx[blockIdx.y];
     typedef typename BP<P>::T PT;
     const PT* v = (const PT*)vec_of(T, s, sel);
@@ -509,12 +508,11 @@ __global__ void k_bscale(const BTab* T, Act act, int dsel, int ssel, const doubl
     }
 }
 
-// out[s*ostride + 2j..] = V_j^dagger w for j < K (w = V[s][wsel]); optionally |w|^2 into wn[s*wstride].
-// cond (nullable): systems with cond[s] == 0 skip the pass entirely.
+// partials of V_j^dagger w for j < K (w = V[s][wsel]) and, with WN, of |w|^2 (k_bred_finish adds
+// them up).  cond (nullable): systems with cond[s * cstride] == 0 skip the pass entirely.
 template <int P, int K, bool WN>
 __global__ __launch_bounds__(kRedBlock) void k_bmultidot(const BTab* T, Act act, int wsel, double* partial,
-                                                         unsigned* counter, double* out, int ostride, double* wn,
-                                                         int wstride, const double* cond, int cstride) {
+                                                         const double* cond, int cstride) {
     const int s = act.idx[blockIdx.y];
     if (cond && cond[(size_t)s * cstride] == 0.0) return;
     typedef typename BP<P>::T PT;
@@ -600,15 +598,15 @@ __global__ __launch_bounds__(kRedBlock) void k_bgs_fast(const BTab* T, Act act, 
     }
 }
 
-// y = target (V[s][tsel] or X), y -= sgn * sum_j c_j V_j with c at c[s*cstride + 2j]; NORM: |y|^2 into
-// nrm[s*nstride] and flag[s*nstride] = |y|^2 < eta * ref[s*nstride]
+// y = target (V[s][tsel] or X), y -= sgn * sum_j c_j V_j with c at c[s*cstride + 2j]; NORM: partials
+// of |y|^2 (k_bred_finish forms |y|^2 and the second-pass flag from them).  cond (nullable): systems
+// with cond[s * cstride] == 0 skip the pass.
 template <int P, int K, bool NORM>
 __global__ __launch_bounds__(kRedBlock) void k_bmultiaxpy(const BTab* T, Act act, int tsel, const double* cc,
-                                                          int cstride, double sgn, double* partial, unsigned* counter,
-                                                          double* nrm, int nstride, const double* cond,
-                                                          const double* ref, double eta, double* flag) {
+                                                          int cstride, double sgn, double* partial,
+                                                          const double* cond) {
     const int s = act.idx[blockIdx.y];
-    if (cond && cond[(size_t)s * nstride] == 0.0) return;
+    if (cond && cond[(size_t)s * cstride] == 0.0) return;
     typedef typename BP<P>::T PT;
     constexpr int NC = BP<P>::NC;
     PT* y = (PT*)vec_of(T, s, tsel);
@@ -666,7 +664,7 @@ __global__ __launch_bounds__(kRedBlock) void k_bred_finish(Act act, const double
         if (lane == 0) sm[wv][r] = a;
     }
     __syncthreads();
-    if (threadIdx.x < nr) {
+    if ((int)threadIdx.x < nr) {
         const int r = threadIdx.x;
         double t = 0;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sm[w][r];
@@ -1333,13 +1331,11 @@ struct BatchSolver {
         const unsigned g = red_grid_b(ctx, max_npk, A.n);
         dim3 grid(g, A.n);
         if (wn)
-            k_bmultidot<P, K, true><<<grid, kRedBlock, 0, ctx->stream>>>(
-                dtab, A, m, ctx->red_partial, ctx->red_counter, out_base + ostride_off, (int)L.stride, wn,
-                (int)L.stride, cond, (int)L.stride);
+            k_bmultidot<P, K, true><<<grid, kRedBlock, 0, ctx->stream>>>(dtab, A, m, ctx->red_partial, cond,
+                                                                         (int)L.stride);
         else
-            k_bmultidot<P, K, false><<<grid, kRedBlock, 0, ctx->stream>>>(
-                dtab, A, m, ctx->red_partial, ctx->red_counter, out_base + ostride_off, (int)L.stride, nullptr,
-                (int)L.stride, cond, (int)L.stride);
+            k_bmultidot<P, K, false><<<grid, kRedBlock, 0, ctx->stream>>>(dtab, A, m, ctx->red_partial, cond,
+                                                                          (int)L.stride);
         CLG_HIP(hipGetLastError());
         k_bred_finish<<<dim3(1, A.n), kRedBlock, 0, ctx->stream>>>(A, ctx->red_partial, (int)g, 2 * K + (wn ? 1 : 0),
                                                                    out_base + ostride_off, 2 * K, wn, cond, nullptr,
@@ -1354,12 +1350,10 @@ struct BatchSolver {
         dim3 grid(g, A.n);
         if (nrm)
             k_bmultiaxpy<P, K, true><<<grid, kRedBlock, 0, ctx->stream>>>(dtab, A, tsel, c, (int)L.stride, sgn,
-                                                                          ctx->red_partial, ctx->red_counter, nrm,
-                                                                          (int)L.stride, cond, ref, eta, flag);
+                                                                          ctx->red_partial, cond);
         else
             k_bmultiaxpy<P, K, false><<<grid, kRedBlock, 0, ctx->stream>>>(dtab, A, tsel, c, (int)L.stride, sgn,
-                                                                           ctx->red_partial, ctx->red_counter, nullptr,
-                                                                           (int)L.stride, cond, ref, eta, flag);
+                                                                           ctx->red_partial, cond);
         CLG_HIP(hipGetLastError());
         if (nrm) {
             k_bred_finish<<<dim3(1, A.n), kRedBlock, 0, ctx->stream>>>(A, ctx->red_partial, (int)g, 1, nrm, 1, nullptr,
@@ -1433,8 +1427,7 @@ struct BatchSolver {
                 k_baxpby<P><<<egrid, 256, 0, ctx->stream>>>(dtab, A, 0, 1.0, -2, 1.0);
             }
             CLG_HIP(hipGetLastError());
-            k_bnorm2<P><<<rgrid, kRedBlock, 0, ctx->stream>>>(dtab, A, 0, ctx->red_partial, ctx->red_counter,
-                                                              ws + L.offS, (int)L.stride);
+            k_bnorm2<P><<<rgrid, kRedBlock, 0, ctx->stream>>>(dtab, A, 0, ctx->red_partial);
             k_bred_finish<<<dim3(1, A.n), kRedBlock, 0, ctx->stream>>>(A, ctx->red_partial, (int)rg, 1, ws + L.offS, 1,
                                                                        nullptr, nullptr, nullptr, 0.0, nullptr,
                                                                        (int)L.stride);
