@@ -127,6 +127,22 @@ def test_not_converged_writes_best_iterate(ctx, c0):
     assert r1 < r0   # x moved from x0 = b towards the solution
 
 
+@pytest.mark.parametrize("eo", [0, 1])
+def test_gmres_when_the_projections_cancel(ctx, c0, eo):
+    # DD^dagger at kappa 0.005 is 1 + O(kappa): each Arnoldi vector A v_j is v_j up to a few 1e-2, so
+    # the first Gram-Schmidt pass cancels nearly all of |w|^2 and every step takes the second pass
+    # (the DGKS branch, not the single-pass fast form); the solution must still meet 1e-12
+    U, b, _ = c0
+    kappa = 0.005
+    g, fb, fx = fields(ctx, U, b)
+    p = clg.solver_params(max_dim=12, restart=50, accuracy=1e-13, stop_rule=clg.STOP_TRUE_RESIDUAL, even_odd=eo)
+    st, rep = clg.solve(ctx, p, fx, fb, g, kappa, clg.EFO_F_InverseDDdagger)
+    assert st == clg.OK and rep.converged
+    x = fx.download()
+    Ax = orc.wilson_op(DIMS0, orc.BC_FERMION, kappa, U, x, clg.EFO_F_DDdagger - clg.EFO_F_D)
+    assert np.linalg.norm(b - Ax) / np.linalg.norm(b) < 1e-12
+
+
 @pytest.mark.parametrize("op", [clg.EFO_F_InverseDdagger, clg.EFO_F_InverseDDdagger, clg.EFO_F_InverseDD])
 def test_inverse_composites(ctx, c0, op):
     U, b, _ = c0
